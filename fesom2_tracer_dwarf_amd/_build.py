@@ -1,0 +1,65 @@
+"""Build libfesom_tracer_amd.so (HIP for gfx950 + host C++) in-tree.
+
+Run as ``python -m fesom2_tracer_dwarf_amd._build`` or through
+``__graft_entry__.build()``. hipcc cross-compiles for gfx950 without a GPU.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+BUILD = PKG.parent / "_build"
+LIBDIR = PKG / "lib"
+LIBNAME = "libfesom_tracer_amd.so"
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+ARCH = os.environ.get("FTA_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off keeps the FP64 path's operand order identical to the
+# reference Fortran (no fused multiply-add), so parity is to rounding.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+             f"-I{ROCM / 'include'}", "-Wall"]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build(verbose: bool = False) -> Path:
+    hipcc = shutil.which("hipcc") or str(ROCM / "bin" / "hipcc")
+    cxx = shutil.which("g++") or "g++"
+    BUILD.mkdir(exist_ok=True)
+    LIBDIR.mkdir(exist_ok=True)
+    jobs = []
+    for src in sorted(CSRC.glob("*.hip")):
+        jobs.append([hipcc, *HIP_FLAGS, "-c", str(src), "-o", str(BUILD / (src.stem + ".o"))])
+    for src in sorted(CSRC.glob("*.cpp")):
+        jobs.append([cxx, *CXX_FLAGS, "-c", str(src), "-o", str(BUILD / (src.stem + ".o"))])
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for j, _ in zip(jobs, ex.map(_run, jobs)):
+            if verbose:
+                print(" ".join(j))
+    objs = [str(BUILD / (s.stem + ".o")) for s in sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))]
+    out = LIBDIR / LIBNAME
+    tmp = LIBDIR / (LIBNAME + ".tmp")
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs,
+          f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
+    os.replace(tmp, out)
+    return out
+
+
+if __name__ == "__main__":
+    p = build(verbose="-v" in sys.argv)
+    print(p)

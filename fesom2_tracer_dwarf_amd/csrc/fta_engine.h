@@ -1,0 +1,96 @@
+// fta_engine.h -- device engine for FESOM2's tracer-advection step on one
+// MI355X (gfx950). Owns the device copy of the mesh, the velocity fields,
+// the tracers and the work arrays, and launches the step on its own stream.
+//
+// Device layout: every level-indexed array is stored column-major with a
+// padded column stride S (>= nl, multiple of 8), so one node/edge/element
+// column is one contiguous, 64-byte-aligned run of S values; a wavefront
+// handles one column with lane k = level k+1 (requires nl <= 64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fta_mesh.h"
+
+namespace fta {
+
+enum Hor { HOR_UPW1 = 0, HOR_MUSCL = 1, HOR_MFCT = 2 };
+enum Ver { VER_UPW1 = 0, VER_QR4C = 1, VER_PPM = 2, VER_CDIFF = 3 };
+enum Lim { LIM_NONE = 0, LIM_FCT = 1 };
+// FP64: everything in double (parity path). FP32: fluxes, gradients and the
+// limiter in float, mesh and velocity stored as float, node tracer, LO
+// solution and divergence accumulation kept in double. FP32_H16: FP32 with
+// the elemental/nodal gradients stored as half.
+enum Prec { PREC_FP64 = 0, PREC_FP32 = 1, PREC_FP32_H16 = 2 };
+
+struct TracerSpec {
+    int hor = HOR_UPW1, ver = VER_UPW1, lim = LIM_NONE;
+    double ph = 1.0, pv = 1.0;  // tra_adv_ph / tra_adv_pv
+};
+
+struct Diag {
+    double vmin, vmax, sum;          // over all (nl-1) x nod2D values, as tracer_get_stats
+    double amin, amax, mass;         // over active cells; mass = sum T*hnode_new*areasvol
+    double nan_count;
+};
+
+class Engine {
+public:
+    Engine(const HostMesh &m, int ntracers, int precision);
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    int ntracers() const { return ntr_; }
+    int precision() const { return prec_; }
+    int stride() const { return S_; }
+
+    // (2,nl-1,elem2D), (nl,nod2D), (nl,nod2D) unpadded double arrays
+    void set_velocity(const double *uv, const double *w, const double *we);
+    void set_values(int tr, const double *v);    // (nl-1,nod2D)
+    void set_valuesAB(int tr, const double *v);  // (nl-1,nod2D)
+    void get_values(int tr, double *v);
+    void set_spec(int tr, const TracerSpec &s) { spec_.at(tr) = s; graph_valid_ = false; }
+    const TracerSpec &spec(int tr) const { return spec_.at(tr); }
+
+    // refresh_ab: valuesAB := values at the start of each step (AB1)
+    void set_refresh_ab(bool r) { refresh_ab_ = r; graph_valid_ = false; }
+
+    // enqueue one step for tracers [first, first+count) on the stream
+    void enqueue_step(double dt, int first, int count);
+    // nsteps steps of all tracers, replayed from a captured hipGraph
+    void run_steps(int nsteps, double dt, bool use_graph);
+    void synchronize();
+    hipStream_t stream() const { return stream_; }
+
+    Diag diagnostics(int tr);
+    // copy a device work array out in the unpadded reference layout
+    // names: fct_LO, adv_flux_hor, adv_flux_ver, fct_plus, fct_minus, tr_xy, gnod
+    int get_work(const std::string &name, int tr_unused, double *out, size_t count);
+    size_t work_count(const std::string &name) const;
+
+    // per-kernel timing of one step with hip events on the engine stream
+    // (names and milliseconds, averaged over reps)
+    void time_kernels(double dt, int reps, std::vector<std::string> &names, std::vector<double> &ms);
+
+    struct Impl;  // device buffers (defined in fta_engine.hip)
+
+private:
+    Impl *p_;
+    int N_, T_, E_, nl_, S_, ntr_, prec_;
+    std::vector<TracerSpec> spec_;
+    bool refresh_ab_ = true;
+    bool graph_valid_ = false;
+    double graph_dt_ = 0.0;
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipStream_t stream_ = nullptr;
+};
+
+const char *last_error();
+void set_error(const std::string &s);
+
+}  // namespace fta

@@ -1,0 +1,529 @@
+// fta_mesh.cpp -- host-side mesh generation and the setup-time derivations
+// the tracer-advection step reads (geometry, levels, adjacency, MUSCL
+// up/down-wind triangles). Setup only; nothing here runs inside a step.
+#include "fta_mesh.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace fta {
+
+namespace {
+
+uint64_t splitmix64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint32_t morton2(uint32_t i, uint32_t j)
+{
+    auto spread = [](uint32_t v) {
+        v &= 0xFFFF;
+        v = (v | (v << 8)) & 0x00FF00FF;
+        v = (v | (v << 4)) & 0x0F0F0F0F;
+        v = (v | (v << 2)) & 0x33333333;
+        v = (v | (v << 1)) & 0x55555555;
+        return v;
+    };
+    return spread(i) | (spread(j) << 1);
+}
+
+// minimum-image trim of a coordinate difference (trim_cyclic,
+// lib/gen_modules_config.F90:207-223, generalised to both axes)
+inline double trim(double d, double cyc)
+{
+    if (cyc > 0.0) {
+        if (d > cyc / 2.0) d -= cyc;
+        if (d < -cyc / 2.0) d += cyc;
+    }
+    return d;
+}
+
+// Edges from clockwise elements, in order of first appearance. A clockwise
+// element lies to the right of each of its directed sides a->b, i.e. to the
+// left of b->a, so the first element met on a side becomes edge_tri(1) with
+// edges = (b, a): FESOM's convention (lib/MOD_MESH.F90:35-37).
+void build_edges(HostMesh &m)
+{
+    std::unordered_map<uint64_t, int> idx;
+    idx.reserve((size_t)m.elem2D * 2);
+    m.edges.clear();
+    m.edge_tri.clear();
+    for (int el = 0; el < m.elem2D; ++el) {
+        for (int s = 0; s < 3; ++s) {
+            int a = m.elem2D_nodes[3 * el + s], b = m.elem2D_nodes[3 * el + (s + 1) % 3];
+            uint64_t key = ((uint64_t)std::min(a, b) << 32) | (uint32_t)std::max(a, b);
+            auto it = idx.find(key);
+            if (it == idx.end()) {
+                int e = (int)(m.edges.size() / 2);
+                idx.emplace(key, e);
+                m.edges.push_back(b);
+                m.edges.push_back(a);
+                m.edge_tri.push_back(el);
+                m.edge_tri.push_back(-1);
+            } else {
+                int e = it->second;
+                if (m.edge_tri[2 * e + 1] != -1)
+                    throw std::runtime_error("mesh: edge shared by more than two elements");
+                m.edge_tri[2 * e + 1] = el;
+            }
+        }
+    }
+    m.edge2D = (int)(m.edges.size() / 2);
+}
+
+void build_nod_in_elem(HostMesh &m)
+{
+    // element loop in index order, appended per node (find_neighbors order)
+    m.nod_in_elem2D_num.assign(m.nod2D, 0);
+    for (int el = 0; el < m.elem2D; ++el)
+        for (int s = 0; s < 3; ++s) m.nod_in_elem2D_num[m.elem2D_nodes[3 * el + s]]++;
+    m.maxAdj = *std::max_element(m.nod_in_elem2D_num.begin(), m.nod_in_elem2D_num.end());
+    m.nod_in_elem2D.assign((size_t)m.maxAdj * m.nod2D, -1);
+    std::vector<int> cnt(m.nod2D, 0);
+    for (int el = 0; el < m.elem2D; ++el)
+        for (int s = 0; s < 3; ++s) {
+            int n = m.elem2D_nodes[3 * el + s];
+            m.nod_in_elem2D[(size_t)n * m.maxAdj + cnt[n]++] = el;
+        }
+}
+
+void finish_topology(HostMesh &m)
+{
+    m.coord_nod2D.resize(2 * (size_t)m.nod2D);
+    for (size_t i = 0; i < m.coord_nod2D.size(); ++i) m.coord_nod2D[i] = m.coord[i] / kREarth;
+    // clockwise check (test_tri, lib/oce_mesh.F90:1565-1610)
+    for (int el = 0; el < m.elem2D; ++el) {
+        int *en = &m.elem2D_nodes[3 * el];
+        double bx = trim(m.coord[2 * en[1]] - m.coord[2 * en[0]], m.cyc_x);
+        double by = trim(m.coord[2 * en[1] + 1] - m.coord[2 * en[0] + 1], m.cyc_y);
+        double cx = trim(m.coord[2 * en[2]] - m.coord[2 * en[0]], m.cyc_x);
+        double cy = trim(m.coord[2 * en[2] + 1] - m.coord[2 * en[0] + 1], m.cyc_y);
+        if (bx * cy - by * cx > 0.0) std::swap(en[1], en[2]);
+    }
+    build_edges(m);
+    build_nod_in_elem(m);
+}
+
+void rect_nodes(HostMesh &m, int nx, int ny, double Lx, double Ly, int order, bool periodic,
+                std::vector<int> &node_id)
+{
+    const double dx = periodic ? Lx / nx : Lx / (nx - 1);
+    const double dy = periodic ? Ly / ny : Ly / (ny - 1);
+    m.nod2D = nx * ny;
+    std::vector<int> perm(m.nod2D);
+    std::iota(perm.begin(), perm.end(), 0);
+    if (order == 1) {
+        std::vector<uint32_t> key(m.nod2D);
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) key[j * nx + i] = morton2(i, j);
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    }
+    node_id.assign(m.nod2D, 0);
+    for (int r = 0; r < m.nod2D; ++r) node_id[perm[r]] = r;
+    m.coord.assign(2 * (size_t)m.nod2D, 0.0);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            int n = node_id[j * nx + i];
+            m.coord[2 * n] = i * dx;
+            m.coord[2 * n + 1] = j * dy;
+        }
+}
+
+void rect_elements(HostMesh &m, int ncx, int ncy, int nx, int order, bool periodic,
+                   const std::vector<int> &node_id)
+{
+    int ny_nodes = m.nod2D / nx;
+    auto nid = [&](int i, int j) {
+        if (periodic) { i = (i + nx) % nx; j = (j + ny_nodes) % ny_nodes; }
+        return node_id[j * nx + i];
+    };
+    std::vector<int> cells(ncx * ncy);
+    std::iota(cells.begin(), cells.end(), 0);
+    if (order == 1) {
+        std::vector<uint32_t> key(cells.size());
+        for (int j = 0; j < ncy; ++j)
+            for (int i = 0; i < ncx; ++i) key[j * ncx + i] = morton2(i, j);
+        std::stable_sort(cells.begin(), cells.end(), [&](int a, int b) { return key[a] < key[b]; });
+    }
+    m.elem2D = 2 * ncx * ncy;
+    m.elem2D_nodes.assign(3 * (size_t)m.elem2D, 0);
+    int el = 0;
+    for (int c : cells) {
+        int i = c % ncx, j = c / ncx;
+        // A: (i,j),(i,j+1),(i+1,j+1)   B: (i,j),(i+1,j+1),(i+1,j)   both clockwise
+        int A[3] = {nid(i, j), nid(i, j + 1), nid(i + 1, j + 1)};
+        int B[3] = {nid(i, j), nid(i + 1, j + 1), nid(i + 1, j)};
+        for (int s = 0; s < 3; ++s) m.elem2D_nodes[3 * el + s] = A[s];
+        ++el;
+        for (int s = 0; s < 3; ++s) m.elem2D_nodes[3 * el + s] = B[s];
+        ++el;
+    }
+}
+
+}  // namespace
+
+void generate_periodic(HostMesh &m, int nx, int ny, double Lx, double Ly, int order)
+{
+    if (nx < 3 || ny < 3) throw std::runtime_error("periodic mesh needs nx, ny >= 3");
+    m = HostMesh();
+    m.cartesian = true;
+    m.cyc_x = Lx;
+    m.cyc_y = Ly;
+    std::vector<int> node_id;
+    rect_nodes(m, nx, ny, Lx, Ly, order, true, node_id);
+    rect_elements(m, nx, ny, nx, order, true, node_id);
+    finish_topology(m);
+}
+
+void generate_closed(HostMesh &m, int nx, int ny, double Lx, double Ly, int order)
+{
+    if (nx < 2 || ny < 2) throw std::runtime_error("closed mesh needs nx, ny >= 2");
+    m = HostMesh();
+    m.cartesian = true;
+    std::vector<int> node_id;
+    rect_nodes(m, nx, ny, Lx, Ly, order, false, node_id);
+    rect_elements(m, nx - 1, ny - 1, nx, order, false, node_id);
+    finish_topology(m);
+}
+
+void set_vertical_grid(HostMesh &m, int nl, double dz, double stretch)
+{
+    if (nl < 3) throw std::runtime_error("need nl >= 3 level interfaces");
+    m.nl = nl;
+    m.zbar.assign(nl, 0.0);
+    m.Z.assign(nl - 1, 0.0);
+    for (int k = 1; k < nl; ++k) {
+        double t = (nl > 2) ? (double)(k - 1) / (double)(nl - 2) : 0.0;
+        m.zbar[k] = m.zbar[k - 1] - dz * (1.0 + stretch * t);
+    }
+    for (int k = 0; k < nl - 1; ++k) m.Z[k] = 0.5 * (m.zbar[k] + m.zbar[k + 1]);
+}
+
+static void node_levels_from_elements(HostMesh &m)
+{
+    m.ulevels.assign(m.elem2D, 1);
+    m.nlevels_nod2D.assign(m.nod2D, 0);
+    m.ulevels_nod2D.assign(m.nod2D, 1);
+    for (int n = 0; n < m.nod2D; ++n) {
+        int mx = 0;
+        for (int k = 0; k < m.nod_in_elem2D_num[n]; ++k)
+            mx = std::max(mx, m.nlevels[m.nod_in_elem2D[(size_t)n * m.maxAdj + k]]);
+        m.nlevels_nod2D[n] = mx;
+    }
+}
+
+void set_levels_random(HostMesh &m, int lo, int hi, uint64_t seed)
+{
+    if (m.nl == 0) throw std::runtime_error("set the vertical grid first");
+    lo = std::max(2, std::min(lo, m.nl));
+    hi = std::max(lo, std::min(hi, m.nl));
+    std::vector<int> raw(m.nod2D);
+    uint64_t s = seed;
+    for (int n = 0; n < m.nod2D; ++n) raw[n] = lo + (int)(splitmix64(s) % (uint64_t)(hi - lo + 1));
+    m.nlevels.assign(m.elem2D, 0);
+    for (int el = 0; el < m.elem2D; ++el) {
+        const int *en = &m.elem2D_nodes[3 * el];
+        m.nlevels[el] = std::min(raw[en[0]], std::min(raw[en[1]], raw[en[2]]));
+    }
+    node_levels_from_elements(m);
+}
+
+void set_levels_from_elements(HostMesh &m, const int *nlevels_elem)
+{
+    m.nlevels.assign(nlevels_elem, nlevels_elem + m.elem2D);
+    node_levels_from_elements(m);
+}
+
+void derive_all(HostMesh &m)
+{
+    const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
+    if (nl < 3) throw std::runtime_error("vertical grid not set");
+    if ((int)m.nlevels.size() != T) throw std::runtime_error("levels not set");
+
+    // find_levels_min_e2n (lib/oce_mesh.F90:1522-1560)
+    m.nlevels_nod2D_min.assign(N, 0);
+    m.ulevels_nod2D_max.assign(N, 0);
+    for (int n = 0; n < N; ++n) {
+        int mn = 1 << 30, mx = 0;
+        for (int k = 0; k < m.nod_in_elem2D_num[n]; ++k) {
+            int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + k];
+            mn = std::min(mn, m.nlevels[el]);
+            mx = std::max(mx, m.ulevels[el]);
+        }
+        m.nlevels_nod2D_min[n] = mn;
+        m.ulevels_nod2D_max[n] = mx;
+    }
+
+    // edge orientation check: edge_tri(1) must lie left of edges(1)->edges(2)
+    auto centroid = [&](int el, double &cx, double &cy) {
+        const int *en = &m.elem2D_nodes[3 * el];
+        double x0 = m.coord[2 * en[0]], y0 = m.coord[2 * en[0] + 1];
+        cx = x0 + (trim(m.coord[2 * en[1]] - x0, m.cyc_x) + trim(m.coord[2 * en[2]] - x0, m.cyc_x)) / 3.0;
+        cy = y0 + (trim(m.coord[2 * en[1] + 1] - y0, m.cyc_y) + trim(m.coord[2 * en[2] + 1] - y0, m.cyc_y)) / 3.0;
+    };
+    for (int e = 0; e < E; ++e) {
+        int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
+        double ex = trim(m.coord[2 * n2] - m.coord[2 * n1], m.cyc_x);
+        double ey = trim(m.coord[2 * n2 + 1] - m.coord[2 * n1 + 1], m.cyc_y);
+        double cx, cy;
+        centroid(m.edge_tri[2 * e], cx, cy);
+        double rx = trim(cx - m.coord[2 * n1], m.cyc_x), ry = trim(cy - m.coord[2 * n1 + 1], m.cyc_y);
+        if (ex * ry - ey * rx <= 0.0) throw std::runtime_error("mesh: edge_tri(1) is not left of its edge");
+    }
+
+    // mesh_areas (lib/oce_mesh.F90:2043-2309), cartesian, no cavity
+    m.elem_area.assign(T, 0.0);
+    m.elem_cos.assign(T, 1.0);
+    for (int el = 0; el < T; ++el) {
+        const int *en = &m.elem2D_nodes[3 * el];
+        double ax = trim(m.coord[2 * en[1]] - m.coord[2 * en[0]], m.cyc_x);
+        double ay = trim(m.coord[2 * en[1] + 1] - m.coord[2 * en[0] + 1], m.cyc_y);
+        double bx = trim(m.coord[2 * en[2]] - m.coord[2 * en[0]], m.cyc_x);
+        double by = trim(m.coord[2 * en[2] + 1] - m.coord[2 * en[0] + 1], m.cyc_y);
+        m.elem_area[el] = 0.5 * std::fabs(ax * by - bx * ay);
+    }
+    m.area.assign((size_t)nl * N, 0.0);
+    m.areasvol.assign((size_t)nl * N, 0.0);
+    for (int n = 0; n < N; ++n)
+        for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) {
+            int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
+            for (int nz = m.ulevels[el]; nz <= m.nlevels[el] - 1; ++nz)
+                m.area[(size_t)n * nl + nz - 1] += m.elem_area[el] / 3.0;
+        }
+    for (int n = 0; n < N; ++n)
+        for (int nz = m.ulevels_nod2D[n]; nz <= m.nlevels_nod2D[n] - 1; ++nz)
+            m.areasvol[(size_t)n * nl + nz - 1] = m.area[(size_t)n * nl + nz - 1];
+
+    // mesh_auxiliary_arrays (lib/oce_mesh.F90:2313-2534), cartesian
+    m.edge_dxdy.assign(2 * (size_t)E, 0.0);
+    m.edge_cross_dxdy.assign(4 * (size_t)E, 0.0);
+    for (int e = 0; e < E; ++e) {
+        int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
+        double ex = trim(m.coord[2 * n2] - m.coord[2 * n1], m.cyc_x);
+        double ey = trim(m.coord[2 * n2 + 1] - m.coord[2 * n1 + 1], m.cyc_y);
+        m.edge_dxdy[2 * e] = ex / kREarth;
+        m.edge_dxdy[2 * e + 1] = ey / kREarth;
+        double mx = m.coord[2 * n1] + 0.5 * ex, my = m.coord[2 * n1 + 1] + 0.5 * ey;
+        for (int s = 0; s < 2; ++s) {
+            int el = m.edge_tri[2 * e + s];
+            if (el < 0) continue;
+            double cx, cy;
+            centroid(el, cx, cy);
+            m.edge_cross_dxdy[4 * e + 2 * s] = trim(cx - mx, m.cyc_x) * m.elem_cos[el];
+            m.edge_cross_dxdy[4 * e + 2 * s + 1] = trim(cy - my, m.cyc_y);
+        }
+    }
+    m.gradient_sca.assign(6 * (size_t)T, 0.0);
+    for (int el = 0; el < T; ++el) {
+        const int *en = &m.elem2D_nodes[3 * el];
+        double dX31 = trim(m.coord[2 * en[2]] - m.coord[2 * en[0]], m.cyc_x) * m.elem_cos[el];
+        double dX21 = trim(m.coord[2 * en[1]] - m.coord[2 * en[0]], m.cyc_x) * m.elem_cos[el];
+        double dY31 = trim(m.coord[2 * en[2] + 1] - m.coord[2 * en[0] + 1], m.cyc_y);
+        double dY21 = trim(m.coord[2 * en[1] + 1] - m.coord[2 * en[0] + 1], m.cyc_y);
+        double dfactor = -0.5 / m.elem_area[el];
+        double *g = &m.gradient_sca[6 * el];
+        g[0] = (-dY31 + dY21) * dfactor;
+        g[1] = dY31 * dfactor;
+        g[2] = -dY21 * dfactor;
+        g[3] = (dX31 - dX21) * dfactor;
+        g[4] = -dX31 * dfactor;
+        g[5] = dX21 * dfactor;
+    }
+
+    // ALE thicknesses and 3-D depth arrays (src/fesom_analytic.F90:226-244)
+    m.hnode.assign((size_t)(nl - 1) * N, 0.0);
+    m.hnode_new.assign((size_t)(nl - 1) * N, 0.0);
+    m.helem.assign((size_t)(nl - 1) * T, 0.0);
+    m.Z_3d_n.assign((size_t)(nl - 1) * N, 0.0);
+    m.zbar_3d_n.assign((size_t)nl * N, 0.0);
+    for (int n = 0; n < N; ++n) {
+        for (int k = 0; k < nl - 1; ++k) {
+            m.hnode[(size_t)n * (nl - 1) + k] = m.zbar[k] - m.zbar[k + 1];
+            m.hnode_new[(size_t)n * (nl - 1) + k] = m.hnode[(size_t)n * (nl - 1) + k];
+            m.Z_3d_n[(size_t)n * (nl - 1) + k] = m.Z[k];
+        }
+        for (int k = 0; k < nl; ++k) m.zbar_3d_n[(size_t)n * nl + k] = m.zbar[k];
+    }
+    for (int el = 0; el < T; ++el)
+        for (int k = 0; k < nl - 1; ++k) m.helem[(size_t)el * (nl - 1) + k] = m.zbar[k] - m.zbar[k + 1];
+
+    // find_up_downwind_triangles (lib/oce_muscl_adv.F90:166-357) on the
+    // radian coordinates; both axes trimmed; the last matching element wins
+    {
+        const double cx = m.cyc_x / kREarth, cy = m.cyc_y / kREarth;
+        const double *X = m.coord_nod2D.data();
+        m.edge_up_dn_tri.assign(2 * (size_t)E, -1);
+        for (int n = 0; n < E; ++n) {
+            int ed[2] = {m.edges[2 * n], m.edges[2 * n + 1]};
+            double x0 = trim(X[2 * ed[1]] - X[2 * ed[0]], cx), x1 = trim(X[2 * ed[1] + 1] - X[2 * ed[0] + 1], cy);
+            for (int side = 0; side < 2; ++side) {
+                int node = ed[side];
+                double xs0 = side == 0 ? -x0 : x0, xs1 = side == 0 ? -x1 : x1;
+                for (int k = 0; k < m.nod_in_elem2D_num[node]; ++k) {
+                    int elem = m.nod_in_elem2D[(size_t)node * m.maxAdj + k];
+                    const int *en = &m.elem2D_nodes[3 * elem];
+                    int i0, i1, i2;
+                    if (en[0] == node) { i0 = 0; i1 = 1; i2 = 2; }
+                    else if (en[1] == node) { i0 = 1; i1 = 0; i2 = 2; }
+                    else { i0 = 2; i1 = 0; i2 = 1; }
+                    double b0 = trim(X[2 * en[i1]] - X[2 * en[i0]], cx);
+                    double b1 = trim(X[2 * en[i1] + 1] - X[2 * en[i0] + 1], cy);
+                    double c0 = trim(X[2 * en[i2]] - X[2 * en[i0]], cx);
+                    double c1 = trim(X[2 * en[i2] + 1] - X[2 * en[i0] + 1], cy);
+                    double cr = c0 * c0 + c1 * c1;
+                    double bx = (b0 * c0 + b1 * c1) / cr;
+                    double by = (-b0 * c1 + b1 * c0) / cr;
+                    double xx = (xs0 * c0 + xs1 * c1) / cr;
+                    double xy = (-xs0 * c1 + xs1 * c0) / cr;
+                    double ab = std::atan2(by, bx), ax = std::atan2(xy, xx);
+                    if ((ab > 0.0) && (ax > 0.0) && (ax < ab)) { m.edge_up_dn_tri[2 * n + side] = elem; continue; }
+                    if ((ab < 0.0) && (ax < 0.0) && (ax > ab)) { m.edge_up_dn_tri[2 * n + side] = elem; continue; }
+                    if ((ab == ax) || (ax == 0.0)) { m.edge_up_dn_tri[2 * n + side] = elem; continue; }
+                }
+            }
+        }
+    }
+
+    // nboundary_lay (lib/oce_muscl_adv.F90:68-104)
+    m.nboundary_lay.assign(N, nl - 1);
+    for (int e = 0; e < E; ++e) {
+        int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
+        int t1 = m.edge_tri[2 * e], t2 = m.edge_tri[2 * e + 1];
+        if (t1 < 0 || t2 < 0) {
+            m.nboundary_lay[n1] = 0;
+            m.nboundary_lay[n2] = 0;
+        } else {
+            int mn = std::min(m.nlevels[t1], m.nlevels[t2]) - 1;
+            m.nboundary_lay[n1] = std::min(m.nboundary_lay[n1], mn);
+            m.nboundary_lay[n2] = std::min(m.nboundary_lay[n2], mn);
+        }
+    }
+
+    // node -> incident edges, sorted by edge index (the order of the
+    // reference's edge-loop scatters)
+    m.n2e_ptr.assign(N + 1, 0);
+    for (int e = 0; e < E; ++e) {
+        m.n2e_ptr[m.edges[2 * e] + 1]++;
+        m.n2e_ptr[m.edges[2 * e + 1] + 1]++;
+    }
+    for (int n = 0; n < N; ++n) m.n2e_ptr[n + 1] += m.n2e_ptr[n];
+    m.n2e.assign(m.n2e_ptr[N], 0);
+    std::vector<int> fill(m.n2e_ptr.begin(), m.n2e_ptr.end() - 1);
+    for (int e = 0; e < E; ++e) {
+        m.n2e[fill[m.edges[2 * e]]++] = e + 1;
+        m.n2e[fill[m.edges[2 * e + 1]]++] = -(e + 1);
+    }
+}
+
+double make_velocity(const HostMesh &m, double u_bg, double v_bg, double cell_amp, double shear,
+                     double cfl, std::vector<double> &uv, std::vector<double> &w)
+{
+    const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl, L = nl - 1;
+    const double PI = 3.14159265358979323846;
+    const double Lx = m.cyc_x > 0 ? m.cyc_x : 1.0, Ly = m.cyc_y > 0 ? m.cyc_y : 1.0;
+    // reference column depth: mean over elements
+    double Href = 0.0;
+    std::vector<double> H(T, 0.0);
+    for (int el = 0; el < T; ++el) {
+        for (int k = 0; k < m.nlevels[el] - 1; ++k) H[el] += m.helem[(size_t)el * L + k];
+        Href += H[el];
+    }
+    Href /= T;
+    // transport streamfunction: rotation cells; closed meshes get psi = 0 on the rim
+    std::vector<double> psi(N, 0.0);
+    double xm = 0, ym = 0;
+    for (int q = 0; q < N; ++q) { xm = std::max(xm, m.coord[2 * q]); ym = std::max(ym, m.coord[2 * q + 1]); }
+    for (int n = 0; n < N; ++n) {
+        double x = m.coord[2 * n], y = m.coord[2 * n + 1];
+        double sx, sy;
+        if (m.cyc_x > 0) { sx = std::sin(2.0 * PI * x / Lx); sy = std::sin(2.0 * PI * y / Ly); }
+        else { sx = std::sin(PI * x / xm); sy = std::sin(PI * y / ym); }
+        psi[n] = cell_amp * Href * (Lx / (2.0 * PI)) * sx * sy;
+    }
+    uv.assign(2 * (size_t)L * T, 0.0);
+    double umax = 0.0;
+    for (int el = 0; el < T; ++el) {
+        const int *en = &m.elem2D_nodes[3 * el];
+        const double *g = &m.gradient_sca[6 * el];
+        double px = g[0] * psi[en[0]] + g[1] * psi[en[1]] + g[2] * psi[en[2]];
+        double py = g[3] * psi[en[0]] + g[4] * psi[en[1]] + g[5] * psi[en[2]];
+        double bg = (m.cyc_x > 0) ? Href : 0.0;  // a uniform current only on periodic meshes
+        double tx = u_bg * bg - py, ty = v_bg * bg + px;
+        int nk = m.nlevels[el] - 1;
+        // vertical shear profile with zero thickness-weighted mean
+        std::vector<double> c(nk, 0.0);
+        double depth = 0.0, mean = 0.0;
+        for (int k = 0; k < nk; ++k) {
+            double h = m.helem[(size_t)el * L + k];
+            c[k] = std::cos(PI * (depth + 0.5 * h) / H[el]);
+            depth += h;
+            mean += h * c[k];
+        }
+        mean /= H[el];
+        for (int k = 0; k < nk; ++k) {
+            double f = (1.0 + shear * (c[k] - mean)) / H[el];
+            uv[((size_t)el * L + k) * 2] = tx * f;
+            uv[((size_t)el * L + k) * 2 + 1] = ty * f;
+            umax = std::max(umax, std::hypot(tx * f, ty * f));
+        }
+    }
+    // W from discrete continuity of the flux-form update (T == 1 is preserved)
+    std::vector<double> inflow((size_t)L * N, 0.0);
+    for (int e = 0; e < E; ++e) {
+        int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
+        int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
+        int nl1 = m.nlevels[el1] - 1, nl2 = el2 >= 0 ? m.nlevels[el2] - 1 : 0;
+        int nl12 = std::min(nl1, nl2);
+        const double *cd = &m.edge_cross_dxdy[4 * e];
+        for (int nz = 1; nz <= std::max(nl1, nl2); ++nz) {
+            double vf = 0.0;
+            if (nz <= nl1) {
+                const double *u = &uv[((size_t)el1 * L + nz - 1) * 2];
+                vf += (-u[1] * cd[0] + u[0] * cd[1]) * m.helem[(size_t)el1 * L + nz - 1];
+            }
+            if (el2 >= 0 && (nz <= nl12 || nz > nl1)) {
+                const double *u = &uv[((size_t)el2 * L + nz - 1) * 2];
+                vf += (u[1] * cd[2] - u[0] * cd[3]) * m.helem[(size_t)el2 * L + nz - 1];
+            }
+            inflow[(size_t)n1 * L + nz - 1] -= vf;
+            inflow[(size_t)n2 * L + nz - 1] += vf;
+        }
+    }
+    w.assign((size_t)nl * N, 0.0);
+    for (int n = 0; n < N; ++n) {
+        int nzmax = m.nlevels_nod2D[n];
+        double *wc = &w[(size_t)n * nl];
+        const double *ar = &m.area[(size_t)n * nl];
+        wc[nzmax - 1] = 0.0;
+        for (int nz = nzmax - 1; nz >= 1; --nz)
+            wc[nz - 1] = (wc[nz] * ar[nz] + inflow[(size_t)n * L + nz - 1]) / ar[nz - 1];
+        wc[0] = 0.0;  // rigid lid: the column-integrated divergence vanishes
+    }
+    double dxmin = 1e300;
+    for (int e = 0; e < E; ++e)
+        dxmin = std::min(dxmin, std::hypot(m.edge_dxdy[2 * e], m.edge_dxdy[2 * e + 1]) * kREarth);
+    return umax > 0.0 ? cfl * dxmin / umax : 0.0;
+}
+
+int64_t active_node_cells(const HostMesh &m)
+{
+    int64_t c = 0;
+    for (int n = 0; n < m.nod2D; ++n) c += m.nlevels_nod2D[n] - m.ulevels_nod2D[n];
+    return c;
+}
+
+int64_t active_elem_cells(const HostMesh &m)
+{
+    int64_t c = 0;
+    for (int el = 0; el < m.elem2D; ++el) c += m.nlevels[el] - m.ulevels[el];
+    return c;
+}
+
+}  // namespace fta

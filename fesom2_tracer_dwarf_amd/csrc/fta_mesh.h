@@ -1,0 +1,87 @@
+// fta_mesh.h -- host-side mesh, geometry and configuration for the tracer
+// advection step. Arrays use the Fortran data model of FESOM2 (column-major,
+// no padding) with 0-based entity indices and -1 for "no element"; level
+// numbers (nlevels, ulevels, nboundary_lay) keep FESOM's 1-based values.
+//
+// Reference fields mirrored: T_MESH (lib/MOD_MESH.F90:24-179) for the
+// members the hot path reads, and the work arrays edge_up_dn_tri and
+// nboundary_lay (lib/MOD_TRACER.F90:30-62).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fta {
+
+constexpr double kREarth = 6367500.0;  // lib/oce_modules.F90:24
+
+struct HostMesh {
+    int nl = 0;        // number of level interfaces (layers = nl-1)
+    int nod2D = 0, elem2D = 0, edge2D = 0, maxAdj = 0;
+    bool cartesian = true;
+    double cyc_x = 0.0, cyc_y = 0.0;  // periodic lengths in metres (0 = closed)
+
+    std::vector<double> coord;        // (2,N) metres (cartesian) -- generator space
+    std::vector<double> coord_nod2D;  // (2,N) radians, as FESOM stores it
+    std::vector<int> elem2D_nodes;    // (3,T), clockwise
+    std::vector<int> edges;           // (2,E)
+    std::vector<int> edge_tri;        // (2,E), edge_tri(1) left of edges(1)->edges(2)
+
+    std::vector<int> nlevels, ulevels;                 // (T)
+    std::vector<int> nlevels_nod2D, ulevels_nod2D;     // (N)
+    std::vector<int> nlevels_nod2D_min, ulevels_nod2D_max;
+    std::vector<int> nod_in_elem2D_num;                // (N)
+    std::vector<int> nod_in_elem2D;                    // (maxAdj,N)
+
+    std::vector<double> zbar, Z;                       // (nl), (nl-1)
+    std::vector<double> elem_cos, elem_area;           // (T)
+    std::vector<double> gradient_sca;                  // (6,T)
+    std::vector<double> edge_dxdy;                     // (2,E) radians
+    std::vector<double> edge_cross_dxdy;               // (4,E) metres
+    std::vector<double> area, areasvol;                // (nl,N)
+    std::vector<double> hnode, hnode_new;              // (nl-1,N)
+    std::vector<double> helem;                         // (nl-1,T)
+    std::vector<double> Z_3d_n;                        // (nl-1,N)
+    std::vector<double> zbar_3d_n;                     // (nl,N)
+    std::vector<int> edge_up_dn_tri;                   // (2,E)
+    std::vector<int> nboundary_lay;                    // (N)
+
+    // node -> incident edges, sorted by edge index; entry = +(e+1) when the
+    // node is edges(1,e), -(e+1) when it is edges(2,e)
+    std::vector<int> n2e_ptr, n2e;
+};
+
+// Doubly periodic nx x ny node rectangle of Lx x Ly metres split into right
+// triangles (a re-design of generate_analytic_mesh, lib/analytic_mesh.F90:114,
+// with FESOM's edge orientation). order: 0 = row-major, 1 = Morton (Z) curve.
+void generate_periodic(HostMesh &m, int nx, int ny, double Lx, double Ly, int order);
+
+// Closed (non-periodic) rectangle: boundary edges have edge_tri(2) = -1.
+void generate_closed(HostMesh &m, int nx, int ny, double Lx, double Ly, int order);
+
+// Vertical grid: zbar(1)=0, layer k thickness dz*(1+stretch*(k-1)/(nl-2)).
+void set_vertical_grid(HostMesh &m, int nl, double dz, double stretch);
+
+// Bottom levels: raw node levels drawn uniformly from [lo, hi] (seeded), element
+// levels = min over its 3 nodes, node levels = max over adjacent elements
+// (FESOM's find_levels invariant). lo=hi=nl gives a flat bottom.
+void set_levels_random(HostMesh &m, int lo, int hi, uint64_t seed);
+// Element levels given, node levels derived (max over adjacent elements).
+void set_levels_from_elements(HostMesh &m, const int *nlevels_elem);
+
+// Everything derived from coordinates, topology and levels: nod_in_elem2D,
+// level min/max, mesh_areas, mesh_auxiliary_arrays, ALE thicknesses,
+// up/down-wind triangles, nboundary_lay and the node->edge lists.
+void derive_all(HostMesh &m);
+
+// Analytic divergence-free velocity on elements, (2,nl-1,T), plus W (nl,N)
+// from discrete continuity. Returns the dt giving the requested CFL.
+double make_velocity(const HostMesh &m, double u_bg, double v_bg, double cell_amp,
+                     double shear, double cfl, std::vector<double> &uv, std::vector<double> &w);
+
+// counts of active node-cells and element-cells
+int64_t active_node_cells(const HostMesh &m);
+int64_t active_elem_cells(const HostMesh &m);
+
+}  // namespace fta

@@ -1,0 +1,650 @@
+// tracer_c_interface.cpp -- the library's C ABI (declared in
+// include/fesom_tracer_amd.h). One module-level instance, as the reference
+// module keeps its global_mesh/global_tracers/global_dyn singletons
+// (lib/tracer_c_interface.F90:28-31); not thread-safe.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "../../include/fesom_tracer_amd.h"
+#include "fta_engine.h"
+#include "fta_mesh.h"
+
+using namespace fta;
+
+namespace {
+
+struct State {
+    bool initialized = false;
+    int mpi_comm = 0;
+    int nl = 0, nod2D = 0, edge2D = 0, elem2D = 0, myDim = 0, eDim = 0;
+    HostMesh mesh;
+    bool mesh_ready = false;
+    int ntr = 0, ab_order = 1;
+    std::vector<std::vector<double>> values, valuesAB;
+    std::vector<TracerSpec> specs;
+    std::vector<double> uv, w, we;
+    std::unique_ptr<Engine> eng;
+    int precision = 64, refresh = 1, use_graph = 1;
+};
+
+State &S()
+{
+    static State s;
+    return s;
+}
+
+int fail(const std::string &msg, int code = -1)
+{
+    set_error(msg);
+    std::fprintf(stderr, "### error: %s\n", msg.c_str());
+    return code;
+}
+
+bool dir_exists(const std::string &p)
+{
+    struct stat st;
+    return ::stat(p.c_str(), &st) == 0 && S_ISDIR(st.st_mode);
+}
+
+int prec_code(int p) { return p == 32 ? PREC_FP32 : (p == 16 ? PREC_FP32_H16 : PREC_FP64); }
+
+void drop_engine() { S().eng.reset(); }
+
+// make sure host-side arrays exist with the current mesh dimensions
+void ensure_host_arrays()
+{
+    State &s = S();
+    const size_t L = s.mesh.nl - 1;
+    if (s.uv.size() != 2 * L * s.mesh.elem2D) s.uv.assign(2 * L * s.mesh.elem2D, 0.0);
+    if (s.w.size() != (size_t)s.mesh.nl * s.mesh.nod2D) s.w.assign((size_t)s.mesh.nl * s.mesh.nod2D, 0.0);
+    if (s.we.size() != (size_t)s.mesh.nl * s.mesh.nod2D) s.we.assign((size_t)s.mesh.nl * s.mesh.nod2D, 0.0);
+}
+
+// create the GPU engine from the host state (mesh, velocity, tracers)
+int ensure_engine()
+{
+    State &s = S();
+    if (s.eng) return 0;
+    if (!s.mesh_ready) return fail("mesh topology missing: generate or load a mesh first");
+    if (s.ntr < 1) return fail("no tracers allocated");
+    try {
+        ensure_host_arrays();
+        s.eng.reset(new Engine(s.mesh, s.ntr, prec_code(s.precision)));
+        s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
+        s.eng->set_refresh_ab(s.refresh != 0);
+        for (int t = 0; t < s.ntr; ++t) {
+            s.eng->set_spec(t, s.specs[t]);
+            s.eng->set_values(t, s.values[t].data());
+            s.eng->set_valuesAB(t, s.valuesAB[t].data());
+        }
+    } catch (const std::exception &e) {
+        s.eng.reset();
+        return fail(e.what());
+    }
+    return 0;
+}
+
+void adopt_mesh_dims()
+{
+    State &s = S();
+    s.nl = s.mesh.nl;
+    s.nod2D = s.mesh.nod2D;
+    s.edge2D = s.mesh.edge2D;
+    s.elem2D = s.mesh.elem2D;
+    s.myDim = s.mesh.nod2D;
+    s.eDim = 0;
+}
+
+// check a QR4C/PPM column needs >= 2 layers (the reference reads outside
+// the column otherwise, lib/oce_adv_tra_ver.F90:405-411, :508-510)
+int check_vertical_scheme(int ver)
+{
+    State &s = S();
+    if (!s.mesh_ready || (ver != VER_QR4C && ver != VER_PPM)) return 0;
+    for (int n = 0; n < s.mesh.nod2D; ++n)
+        if (s.mesh.nlevels_nod2D[n] - s.mesh.ulevels_nod2D[n] < 2)
+            return fail("QR4C/PPM need at least two layers in every column");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tracer_init_mpi(int mpi_comm)
+{
+    S().mpi_comm = mpi_comm;
+    return 0;
+}
+
+int tracer_finalize_mpi(void) { return 0; }
+
+int tracer_init(const char *restart_dir)
+{
+    std::string p = std::string(restart_dir ? restart_dir : "") + "/np1";
+    if (!dir_exists(p)) return fail("could not find restart directory: " + p);
+    return fail("binary derived-type restarts are not readable by this build: " + p);
+}
+
+int tracer_advect_step(double dt)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized. Call tracer_init first.");
+    if (ensure_engine()) return -1;
+    try {
+        s.eng->enqueue_step(dt, 0, 1);
+        s.eng->synchronize();
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int tracer_advect_steps(int nsteps, double dt)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (nsteps < 0) return fail("nsteps must be >= 0");
+    if (ensure_engine()) return -1;
+    try {
+        s.eng->run_steps(nsteps, dt, s.use_graph != 0);
+        s.eng->synchronize();
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int tracer_get_stats(double *tmin, double *tmax, double *tsum)
+{
+    State &s = S();
+    if (!s.initialized || s.ntr < 1) {
+        if (tmin) *tmin = 0.0;
+        if (tmax) *tmax = 0.0;
+        if (tsum) *tsum = 0.0;
+        return fail("tracer not initialized.");
+    }
+    double d[7];
+    if (tracer_get_diagnostics(1, d)) return -1;
+    *tmin = d[0];
+    *tmax = d[1];
+    *tsum = d[2];
+    return 0;
+}
+
+int tracer_get_size(int *nz, int *nn)
+{
+    State &s = S();
+    if (!s.initialized) {
+        *nz = 0;
+        *nn = 0;
+        return fail("tracer not initialized.");
+    }
+    *nz = s.nl - 1;
+    *nn = s.myDim;
+    return 0;
+}
+
+int tracer_get_values_id(int tracer_id, double *values, int nz, int nn)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id");
+    if (nz != s.nl - 1 || nn > s.nod2D || nn < 0) return fail("tracer_get_values: size mismatch");
+    const std::vector<double> *src = &s.values[tracer_id - 1];
+    std::vector<double> tmp;
+    if (s.eng) {
+        tmp.resize((size_t)(s.nl - 1) * s.nod2D);
+        try {
+            s.eng->get_values(tracer_id - 1, tmp.data());
+        } catch (const std::exception &e) {
+            return fail(e.what());
+        }
+        src = &tmp;
+    }
+    std::memcpy(values, src->data(), sizeof(double) * (size_t)nz * nn);
+    return 0;
+}
+
+int tracer_get_values(double *values, int nz, int nn) { return tracer_get_values_id(1, values, nz, nn); }
+
+int tracer_finalize(const char *output_dir)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    s.initialized = false;
+    (void)output_dir;
+    return fail("binary derived-type restarts are not written by this build");
+}
+
+int tracer_run_workflow(const char *restart_dir, int nsteps, double dt)
+{
+    int st = tracer_init(restart_dir);
+    if (st) return st;
+    for (int i = 0; i < nsteps; ++i) {
+        st = tracer_advect_step(dt);
+        if (st) return st;
+    }
+    return tracer_finalize(restart_dir);
+}
+
+int tracer_cleanup(void)
+{
+    State &s = S();
+    drop_engine();
+    s.initialized = false;
+    return 0;
+}
+
+int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D)
+{
+    State &s = S();
+    s.nl = nl;
+    s.nod2D = nod2D;
+    s.edge2D = edge2D;
+    s.elem2D = elem2D;
+    return 0;
+}
+
+int tracer_set_levels(const int *nlevels, const int *ulevels, int nod2D)
+{
+    State &s = S();
+    if (s.mesh_ready && nod2D == s.mesh.nod2D) {
+        // node levels must stay consistent with the element levels (FESOM's
+        // nlevels_nod2D = max over adjacent elements); accept only that
+        for (int n = 0; n < nod2D; ++n)
+            if (nlevels[n] != s.mesh.nlevels_nod2D[n] || ulevels[n] != s.mesh.ulevels_nod2D[n])
+                return fail("tracer_set_levels: levels disagree with the mesh's element levels");
+    }
+    return 0;
+}
+
+int tracer_set_thickness(const double *hnode, int nl, int nod2D)
+{
+    State &s = S();
+    if (!s.mesh_ready || nl != s.mesh.nl || nod2D != s.mesh.nod2D) return fail("tracer_set_thickness: mesh mismatch");
+    std::memcpy(s.mesh.hnode_new.data(), hnode, sizeof(double) * (size_t)(nl - 1) * nod2D);
+    drop_engine();
+    return 0;
+}
+
+int tracer_set_partition(int myDim_nod2D, int eDim_nod2D)
+{
+    State &s = S();
+    if (eDim_nod2D != 0) return fail("one GPU, one partition: eDim_nod2D must be 0");
+    s.myDim = myDim_nod2D;
+    s.eDim = eDim_nod2D;
+    return 0;
+}
+
+int tracer_allocate_tracers(int num_tracers, int nl, int nod2D, int AB_order)
+{
+    State &s = S();
+    if (num_tracers < 1) return fail("num_tracers must be >= 1");
+    if (s.mesh_ready && (nl != s.mesh.nl || nod2D != s.mesh.nod2D)) return fail("tracer_allocate_tracers: mesh mismatch");
+    s.ntr = num_tracers;
+    s.ab_order = AB_order;
+    s.values.assign(num_tracers, std::vector<double>((size_t)(nl - 1) * nod2D, 0.0));
+    s.valuesAB = s.values;
+    s.specs.assign(num_tracers, TracerSpec());
+    drop_engine();
+    return 0;
+}
+
+int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D)
+{
+    State &s = S();
+    if (tracer_id < 1 || tracer_id > s.ntr) {
+        return fail("invalid tracer_id " + std::to_string(tracer_id) + ", num_tracers=" + std::to_string(s.ntr));
+    }
+    const size_t n = (size_t)(nl - 1) * nod2D;
+    if (s.values[tracer_id - 1].size() != n) return fail("tracer_set_values: size mismatch");
+    std::memcpy(s.values[tracer_id - 1].data(), values, sizeof(double) * n);
+    std::memcpy(s.valuesAB[tracer_id - 1].data(), values, sizeof(double) * n);
+    if (s.eng) {
+        try {
+            s.eng->set_values(tracer_id - 1, values);
+            s.eng->set_valuesAB(tracer_id - 1, values);
+        } catch (const std::exception &e) {
+            return fail(e.what());
+        }
+    }
+    return 0;
+}
+
+int tracer_set_values_ab(int tracer_id, const double *values, int nl, int nod2D)
+{
+    State &s = S();
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id");
+    const size_t n = (size_t)(nl - 1) * nod2D;
+    if (s.valuesAB[tracer_id - 1].size() != n) return fail("tracer_set_values_ab: size mismatch");
+    std::memcpy(s.valuesAB[tracer_id - 1].data(), values, sizeof(double) * n);
+    if (s.eng) {
+        try {
+            s.eng->set_valuesAB(tracer_id - 1, values);
+        } catch (const std::exception &e) {
+            return fail(e.what());
+        }
+    }
+    return 0;
+}
+
+int tracer_set_velocity_zero(int nl, int nod2D, int edge2D)
+{
+    State &s = S();
+    (void)edge2D;
+    if (!s.mesh_ready || nl != s.mesh.nl || nod2D != s.mesh.nod2D) return fail("tracer_set_velocity_zero: mesh mismatch");
+    s.uv.assign(2 * (size_t)(nl - 1) * s.mesh.elem2D, 0.0);
+    s.w.assign((size_t)nl * nod2D, 0.0);
+    s.we.assign((size_t)nl * nod2D, 0.0);
+    if (s.eng) s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
+    return 0;
+}
+
+int tracer_set_velocity(const double *uv, const double *w, int nl, int nod2D, int edge2D)
+{
+    State &s = S();
+    if (!s.mesh_ready || nl != s.mesh.nl || nod2D != s.mesh.nod2D) return fail("tracer_set_velocity: mesh mismatch");
+    if (edge2D < s.mesh.elem2D) return fail("tracer_set_velocity: buffer smaller than vel(2,nl-1,elem2D)");
+    const size_t nv = 2 * (size_t)(nl - 1) * s.mesh.elem2D;
+    s.uv.assign(uv, uv + nv);
+    s.w.assign(w, w + (size_t)nl * nod2D);
+    s.we.assign((size_t)nl * nod2D, 0.0);
+    if (s.eng) s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
+    return 0;
+}
+
+int tracer_set_velocity_elem(const double *uv, const double *w, const double *w_e, int nl, int nod2D, int elem2D)
+{
+    State &s = S();
+    if (!s.mesh_ready || nl != s.mesh.nl || nod2D != s.mesh.nod2D || elem2D != s.mesh.elem2D)
+        return fail("tracer_set_velocity_elem: mesh mismatch");
+    s.uv.assign(uv, uv + 2 * (size_t)(nl - 1) * elem2D);
+    s.w.assign(w, w + (size_t)nl * nod2D);
+    s.we.assign(w_e, w_e + (size_t)nl * nod2D);
+    if (s.eng) s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
+    return 0;
+}
+
+int tracer_make_velocity(double u_bg, double v_bg, double cell_amp, double shear, double cfl, double *dt_out)
+{
+    State &s = S();
+    if (!s.mesh_ready) return fail("mesh topology missing");
+    double dt = make_velocity(s.mesh, u_bg, v_bg, cell_amp, shear, cfl, s.uv, s.w);
+    s.we = s.w;
+    if (dt_out) *dt_out = dt;
+    if (s.eng) s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
+    return 0;
+}
+
+int tracer_init_complete(void)
+{
+    S().initialized = true;
+    return 0;
+}
+
+int tracer_load_mesh_partition(const char *restart_dir)
+{
+    std::string p = std::string(restart_dir ? restart_dir : "") + "/np1";
+    if (!dir_exists(p)) return fail("could not find mesh partition directory: " + p);
+    return fail("binary derived-type restarts are not readable by this build: " + p);
+}
+
+int tracer_load_mesh_from_files(const char *mesh_path)
+{
+    std::string p = mesh_path ? mesh_path : "";
+    if (!dir_exists(p)) return fail("could not find mesh directory: " + p);
+    return fail("FESOM mesh-file reader is not part of this build: " + p);
+}
+
+int tracer_init_ale_arrays(void)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer_init_ale_arrays called before initialization");
+    if (!s.mesh_ready) return fail("mesh topology missing");
+    HostMesh &m = s.mesh;
+    for (int n = 0; n < m.nod2D; ++n)
+        for (int k = 0; k < m.nl - 1; ++k) {
+            m.hnode[(size_t)n * (m.nl - 1) + k] = m.zbar[k] - m.zbar[k + 1];
+            m.hnode_new[(size_t)n * (m.nl - 1) + k] = m.hnode[(size_t)n * (m.nl - 1) + k];
+        }
+    for (int el = 0; el < m.elem2D; ++el)
+        for (int k = 0; k < m.nl - 1; ++k) m.helem[(size_t)el * (m.nl - 1) + k] = m.zbar[k] - m.zbar[k + 1];
+    drop_engine();
+    return 0;
+}
+
+int tracer_init_for_advection(void)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer_init_for_advection called before mesh loaded");
+    if (s.ntr < 1) tracer_allocate_tracers(1, s.mesh.nl, s.mesh.nod2D, 1);
+    return ensure_engine();
+}
+
+int tracer_generate_mesh(int nx, int ny, int periodic, double Lx, double Ly, int nl, double dz, double stretch,
+                         int lev_lo, int lev_hi, unsigned long long seed, int sfc)
+{
+    State &s = S();
+    try {
+        drop_engine();
+        HostMesh m;
+        if (periodic) generate_periodic(m, nx, ny, Lx, Ly, sfc ? 1 : 0);
+        else generate_closed(m, nx, ny, Lx, Ly, sfc ? 1 : 0);
+        set_vertical_grid(m, nl, dz, stretch);
+        set_levels_random(m, lev_lo, lev_hi, seed);
+        derive_all(m);
+        s.mesh = std::move(m);
+    } catch (const std::exception &e) {
+        s.mesh_ready = false;
+        return fail(e.what());
+    }
+    s.mesh_ready = true;
+    s.initialized = true;
+    adopt_mesh_dims();
+    s.ntr = 0;
+    s.values.clear();
+    s.valuesAB.clear();
+    s.specs.clear();
+    s.uv.clear();
+    s.w.clear();
+    s.we.clear();
+    ensure_host_arrays();
+    return 0;
+}
+
+int tracer_set_schemes(int tracer_id, const char *hor, const char *ver, const char *lim, double ph, double pv)
+{
+    State &s = S();
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id");
+    TracerSpec sp;
+    std::string h = hor ? hor : "", v = ver ? ver : "", l = lim ? lim : "";
+    if (h == "UPW1") sp.hor = HOR_UPW1;
+    else if (h == "MUSCL") sp.hor = HOR_MUSCL;
+    else if (h == "MFCT") sp.hor = HOR_MFCT;
+    else return fail("Unknown horizontal advection type " + h);
+    if (v == "UPW1") sp.ver = VER_UPW1;
+    else if (v == "QR4C") sp.ver = VER_QR4C;
+    else if (v == "PPM") sp.ver = VER_PPM;
+    else if (v == "CDIFF") sp.ver = VER_CDIFF;
+    else return fail("Unknown vertical advection type " + v);
+    if (l == "NONE") sp.lim = LIM_NONE;
+    else if (l == "FCT") sp.lim = LIM_FCT;
+    else return fail("Unknown limiter type " + l);
+    if (check_vertical_scheme(sp.ver)) return -1;
+    sp.ph = ph;
+    sp.pv = pv;
+    s.specs[tracer_id - 1] = sp;
+    if (s.eng) s.eng->set_spec(tracer_id - 1, sp);
+    return 0;
+}
+
+int tracer_set_option(const char *name, int value)
+{
+    State &s = S();
+    std::string n = name ? name : "";
+    if (n == "precision") {
+        if (value != 64 && value != 32 && value != 16) return fail("precision must be 64, 32 or 16");
+        if (s.eng) {
+            // carry the current tracer state over to the new engine
+            for (int t = 0; t < s.ntr; ++t) s.eng->get_values(t, s.values[t].data());
+        }
+        s.precision = value;
+        drop_engine();
+        return 0;
+    }
+    if (n == "refresh_valuesAB") {
+        s.refresh = value ? 1 : 0;
+        if (s.eng) s.eng->set_refresh_ab(value != 0);
+        return 0;
+    }
+    if (n == "use_graph") {
+        s.use_graph = value ? 1 : 0;
+        return 0;
+    }
+    if (n == "accumulate_del_ttf") {
+        if (value) return fail("accumulate_del_ttf is not supported by the device path");
+        return 0;
+    }
+    return fail("unknown option " + n);
+}
+
+int tracer_get_diagnostics(int tracer_id, double *out)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id");
+    if (ensure_engine()) return -1;
+    try {
+        Diag d = s.eng->diagnostics(tracer_id - 1);
+        out[0] = d.vmin; out[1] = d.vmax; out[2] = d.sum;
+        out[3] = d.amin; out[4] = d.amax; out[5] = d.mass; out[6] = d.nan_count;
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+namespace {
+struct ArrRef { const void *p; long long n; int kind; };
+
+bool mesh_array(const std::string &name, ArrRef &r)
+{
+    State &s = S();
+    HostMesh &m = s.mesh;
+    static std::vector<int> dims;
+    static std::vector<double> zb;
+#define IA(NAME, VEC) if (name == NAME) { r = {VEC.data(), (long long)VEC.size(), 0}; return true; }
+#define DA(NAME, VEC) if (name == NAME) { r = {VEC.data(), (long long)VEC.size(), 1}; return true; }
+    if (name == "dims") {
+        dims = {m.nl, m.nod2D, m.elem2D, m.edge2D, m.maxAdj};
+        r = {dims.data(), (long long)dims.size(), 0};
+        return true;
+    }
+    IA("edges", m.edges) IA("edge_tri", m.edge_tri) IA("elem2D_nodes", m.elem2D_nodes)
+    IA("nlevels", m.nlevels) IA("ulevels", m.ulevels) IA("nlevels_nod2D", m.nlevels_nod2D)
+    IA("ulevels_nod2D", m.ulevels_nod2D) IA("nlevels_nod2D_min", m.nlevels_nod2D_min)
+    IA("ulevels_nod2D_max", m.ulevels_nod2D_max) IA("nod_in_elem2D_num", m.nod_in_elem2D_num)
+    IA("nod_in_elem2D", m.nod_in_elem2D) IA("edge_up_dn_tri", m.edge_up_dn_tri)
+    IA("nboundary_lay", m.nboundary_lay)
+    DA("coord", m.coord) DA("coord_nod2D", m.coord_nod2D) DA("zbar", m.zbar) DA("Z", m.Z)
+    DA("elem_cos", m.elem_cos) DA("elem_area", m.elem_area) DA("gradient_sca", m.gradient_sca)
+    DA("edge_dxdy", m.edge_dxdy) DA("edge_cross_dxdy", m.edge_cross_dxdy) DA("area", m.area)
+    DA("areasvol", m.areasvol) DA("hnode", m.hnode) DA("hnode_new", m.hnode_new) DA("helem", m.helem)
+    DA("Z_3d_n", m.Z_3d_n) DA("zbar_3d_n", m.zbar_3d_n)
+    DA("uv", s.uv) DA("w", s.w) DA("w_e", s.we)
+#undef IA
+#undef DA
+    (void)zb;
+    return false;
+}
+}  // namespace
+
+int tracer_mesh_array_info(const char *name, long long *count, int *kind)
+{
+    if (!S().mesh_ready) return fail("mesh topology missing");
+    ArrRef r;
+    if (!mesh_array(name ? name : "", r)) return fail(std::string("unknown mesh array ") + (name ? name : ""));
+    *count = r.n;
+    *kind = r.kind;
+    return 0;
+}
+
+int tracer_get_mesh_array(const char *name, void *out, long long count)
+{
+    if (!S().mesh_ready) return fail("mesh topology missing");
+    ArrRef r;
+    if (!mesh_array(name ? name : "", r)) return fail(std::string("unknown mesh array ") + (name ? name : ""));
+    if (count < r.n) return fail("tracer_get_mesh_array: buffer too small");
+    std::memcpy(out, r.p, (size_t)r.n * (r.kind ? sizeof(double) : sizeof(int)));
+    return 0;
+}
+
+int tracer_get_work_array(const char *name, double *out, long long count)
+{
+    State &s = S();
+    if (!s.eng) return fail("no device state yet");
+    try {
+        if (s.eng->get_work(name ? name : "", 0, out, (size_t)count)) return fail("unknown work array or short buffer");
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int tracer_time_steps(int nsteps, double dt, double *ms_out)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (ensure_engine()) return -1;
+    try {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return fail("hipEventCreate failed");
+        if (s.use_graph) s.eng->run_steps(0, dt, true);  // instantiate outside the timed region
+        hipEventRecord(a, s.eng->stream());
+        s.eng->run_steps(nsteps, dt, s.use_graph != 0);
+        hipEventRecord(b, s.eng->stream());
+        hipEventSynchronize(b);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, a, b);
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        *ms_out = ms;
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int tracer_time_kernels(double dt, int reps, char *names, int names_len, double *ms, int max_kernels)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (ensure_engine()) return -1;
+    try {
+        std::vector<std::string> nm;
+        std::vector<double> t;
+        s.eng->time_kernels(dt, reps, nm, t);
+        std::string joined;
+        for (size_t i = 0; i < nm.size(); ++i) {
+            if (i) joined += ";";
+            joined += nm[i];
+            if ((int)i < max_kernels) ms[i] = t[i];
+        }
+        std::snprintf(names, (size_t)names_len, "%s", joined.c_str());
+        return (int)nm.size();
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+}
+
+const char *tracer_last_error(void) { return last_error(); }
+
+}  // extern "C"

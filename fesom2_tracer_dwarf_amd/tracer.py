@@ -1,0 +1,143 @@
+"""Python interface of the tracer-advection library.
+
+``FESOMTracerAdvection`` keeps the method names, arguments and return values
+of the reference's ctypes wrapper class (python/demo_tracer.py:40-208); the
+extra methods below it drive the parts of the step the reference wrapper
+cannot reach (mesh generation, schemes, element velocity, batched steps).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class FESOMTracerAdvection:
+    """Handle on the library's single instance (python/demo_tracer.py:40)."""
+
+    def __init__(self, lib_path=None):
+        self.lib = _lib.load(lib_path)
+
+    # ---- reference methods (python/demo_tracer.py:101-208) -------------
+    def initialize_mpi(self, mpi_comm=0):
+        return self.lib.tracer_init_mpi(mpi_comm)
+
+    def initialize(self, restart_path="./fesom_bin_restart"):
+        return self.lib.tracer_init(restart_path.encode())
+
+    def advect_step(self, dt=1.0e-3):
+        return self.lib.tracer_advect_step(dt)
+
+    def get_tracer_values(self, tracer_id=1):
+        nz, nn = ctypes.c_int(), ctypes.c_int()
+        ret = self.lib.tracer_get_size(ctypes.byref(nz), ctypes.byref(nn))
+        if ret != 0:
+            raise RuntimeError(f"tracer_get_size failed with code {ret}")
+        values = np.zeros((nz.value, nn.value), dtype=np.float64, order="F")
+        ret = self.lib.tracer_get_values_id(tracer_id, values, nz, nn)
+        if ret != 0:
+            raise RuntimeError(f"tracer_get_values failed with code {ret}")
+        return values
+
+    def get_statistics(self):
+        tmin, tmax, tsum = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        ret = self.lib.tracer_get_stats(ctypes.byref(tmin), ctypes.byref(tmax), ctypes.byref(tsum))
+        if ret != 0:
+            raise RuntimeError(f"tracer_get_stats failed with code {ret}")
+        return {"min": tmin.value, "max": tmax.value, "sum": tsum.value}
+
+    def finalize(self, output_path="./fesom_bin_restart"):
+        return self.lib.tracer_finalize(output_path.encode())
+
+    def finalize_mpi(self):
+        return self.lib.tracer_finalize_mpi()
+
+    # ---- extensions -----------------------------------------------------
+    def generate_mesh(self, nx, ny, nl, *, periodic=True, Lx=1.0e5, Ly=1.0e5, dz=10.0, stretch=0.0,
+                      lev_lo=None, lev_hi=None, seed=0, sfc=True):
+        lev_lo = nl if lev_lo is None else lev_lo
+        lev_hi = nl if lev_hi is None else lev_hi
+        _lib.check(self.lib.tracer_generate_mesh(nx, ny, int(periodic), Lx, Ly, nl, dz, stretch, lev_lo, lev_hi,
+                                                 seed, int(sfc)), "tracer_generate_mesh", self.lib)
+        return self.dims()
+
+    def dims(self):
+        d = self.mesh_array("dims")
+        return dict(nl=int(d[0]), nod2D=int(d[1]), elem2D=int(d[2]), edge2D=int(d[3]), maxAdj=int(d[4]))
+
+    def mesh_array(self, name):
+        cnt, kind = ctypes.c_longlong(), ctypes.c_int()
+        _lib.check(self.lib.tracer_mesh_array_info(name.encode(), ctypes.byref(cnt), ctypes.byref(kind)),
+                   f"mesh array {name}", self.lib)
+        out = np.empty(cnt.value, dtype=np.float64 if kind.value else np.int32)
+        _lib.check(self.lib.tracer_get_mesh_array(name.encode(), out.ctypes.data, cnt.value),
+                   f"mesh array {name}", self.lib)
+        return out
+
+    def allocate_tracers(self, num_tracers, AB_order=1):
+        d = self.dims()
+        _lib.check(self.lib.tracer_allocate_tracers(num_tracers, d["nl"], d["nod2D"], AB_order),
+                   "tracer_allocate_tracers", self.lib)
+
+    def set_values(self, tracer_id, values):
+        v = np.asfortranarray(values, dtype=np.float64)
+        _lib.check(self.lib.tracer_set_values(tracer_id, v, v.shape[0] + 1, v.shape[1]), "tracer_set_values",
+                   self.lib)
+
+    def set_values_ab(self, tracer_id, values):
+        v = np.asfortranarray(values, dtype=np.float64)
+        _lib.check(self.lib.tracer_set_values_ab(tracer_id, v, v.shape[0] + 1, v.shape[1]),
+                   "tracer_set_values_ab", self.lib)
+
+    def set_schemes(self, tracer_id, hor="MUSCL", ver="QR4C", lim="FCT", ph=1.0, pv=1.0):
+        _lib.check(self.lib.tracer_set_schemes(tracer_id, hor.encode(), ver.encode(), lim.encode(), ph, pv),
+                   "tracer_set_schemes", self.lib)
+
+    def make_velocity(self, u_bg=0.1, v_bg=0.05, cell_amp=0.2, shear=0.5, cfl=0.3):
+        dt = ctypes.c_double()
+        _lib.check(self.lib.tracer_make_velocity(u_bg, v_bg, cell_amp, shear, cfl, ctypes.byref(dt)),
+                   "tracer_make_velocity", self.lib)
+        return dt.value
+
+    def set_velocity_elem(self, uv, w, w_e):
+        d = self.dims()
+        _lib.check(self.lib.tracer_set_velocity_elem(np.asfortranarray(uv, dtype=np.float64),
+                                                     np.asfortranarray(w, dtype=np.float64),
+                                                     np.asfortranarray(w_e, dtype=np.float64),
+                                                     d["nl"], d["nod2D"], d["elem2D"]),
+                   "tracer_set_velocity_elem", self.lib)
+
+    def set_option(self, name, value):
+        _lib.check(self.lib.tracer_set_option(name.encode(), int(value)), f"option {name}", self.lib)
+
+    def advect_steps(self, nsteps, dt):
+        _lib.check(self.lib.tracer_advect_steps(nsteps, dt), "tracer_advect_steps", self.lib)
+
+    def diagnostics(self, tracer_id=1):
+        out = np.zeros(7)
+        _lib.check(self.lib.tracer_get_diagnostics(tracer_id, out), "tracer_get_diagnostics", self.lib)
+        keys = ("min", "max", "sum", "active_min", "active_max", "mass", "nan_count")
+        return dict(zip(keys, out.tolist()))
+
+    def work_array(self, name, count):
+        out = np.zeros(count)
+        _lib.check(self.lib.tracer_get_work_array(name.encode(), out, count), f"work array {name}", self.lib)
+        return out
+
+    def time_steps(self, nsteps, dt):
+        ms = ctypes.c_double()
+        _lib.check(self.lib.tracer_time_steps(nsteps, dt, ctypes.byref(ms)), "tracer_time_steps", self.lib)
+        return ms.value
+
+    def time_kernels(self, dt, reps=3):
+        names = ctypes.create_string_buffer(4096)
+        ms = np.zeros(32)
+        n = self.lib.tracer_time_kernels(dt, reps, names, 4096, ms, 32)
+        if n < 0:
+            raise RuntimeError(f"tracer_time_kernels failed: {_lib.last_error(self.lib)}")
+        return dict(zip(names.value.decode().split(";"), ms[:n].tolist()))
+
+    def cleanup(self):
+        return self.lib.tracer_cleanup()

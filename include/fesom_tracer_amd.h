@@ -1,0 +1,193 @@
+/*
+ * fesom_tracer_amd.h -- C ABI of the MI355X tracer-advection library
+ * (libfesom_tracer_amd.so).
+ *
+ * Part 1 keeps the 23 entry points of the reference's tracer_c_interface
+ * (lib/tracer_c_interface.F90) with the same names, argument meaning and
+ * return codes (0 = ok, -1 = not initialised / invalid id / missing
+ * directory), writing errors to stderr as "### error: ...". Like the
+ * reference, the library holds one module-level instance and is not
+ * thread-safe. Arrays are Fortran column-major float64, (nl-1, nod2D) for
+ * tracers.
+ *
+ * Part 2 adds what the reference's array-initialisation path lacks to run a
+ * step (mesh topology, schemes, element velocity, multi-tracer stepping,
+ * diagnostics). These have no counterpart in the reference; each says so.
+ */
+#ifndef FESOM_TRACER_AMD_H
+#define FESOM_TRACER_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------
+ * Part 1: the reference C ABI
+ * ------------------------------------------------------------------- */
+
+/* lib/tracer_c_interface.F90:46-84. One process drives one GPU; no MPI is
+ * linked. Records the communicator and returns 0. */
+int tracer_init_mpi(int mpi_comm);
+
+/* lib/tracer_c_interface.F90:89-106. Returns 0. */
+int tracer_finalize_mpi(void);
+
+/* lib/tracer_c_interface.F90:113-157. -1 if <restart_dir>/np1 is missing;
+ * the derived-type restart reader is not part of this build, so an
+ * existing directory also returns -1 with an explanatory error. */
+int tracer_init(const char *restart_dir);
+
+/* lib/tracer_c_interface.F90:163-220. One step of tracer 1 on the GPU:
+ * zero the advective tendencies, do_oce_adv_tra, del_ttf += dttf_h+dttf_v,
+ * values += del_ttf/hnode_new. del_ttf is zeroed every step unless the
+ * option "accumulate_del_ttf" is set (the reference never zeroes it). */
+int tracer_advect_step(double dt);
+
+/* lib/tracer_c_interface.F90:228-248: min, max and sum of all
+ * (nl-1) x nod2D values of tracer 1 (inactive levels included). */
+int tracer_get_stats(double *tmin, double *tmax, double *tsum);
+
+/* lib/tracer_c_interface.F90:255-272: nz = nl-1, nn = myDim_nod2D */
+int tracer_get_size(int *nz, int *nn);
+
+/* lib/tracer_c_interface.F90:280-302: tracer 1 values, (nz, nn) column-major */
+int tracer_get_values(double *values, int nz, int nn);
+
+/* lib/tracer_c_interface.F90:308-348. The restart writer is not part of
+ * this build: marks the library uninitialised and returns -1. */
+int tracer_finalize(const char *output_dir);
+
+/* lib/tracer_c_interface.F90:356-386: init + nsteps + finalize */
+int tracer_run_workflow(const char *restart_dir, int nsteps, double dt);
+
+/* lib/tracer_c_interface.F90:391-402: resets the flags (and frees the GPU state) */
+int tracer_cleanup(void);
+
+/* lib/tracer_c_interface.F90:415-433 */
+int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D);
+
+/* lib/tracer_c_interface.F90:441-469: nlevels_nod2D / ulevels_nod2D */
+int tracer_set_levels(const int *nlevels, const int *ulevels, int nod2D);
+
+/* lib/tracer_c_interface.F90:477-503: sets hnode_new only, (nl-1, nod2D) */
+int tracer_set_thickness(const double *hnode, int nl, int nod2D);
+
+/* lib/tracer_c_interface.F90:510-530 */
+int tracer_set_partition(int myDim_nod2D, int eDim_nod2D);
+
+/* lib/tracer_c_interface.F90:539-604: schemes default to UPW1/UPW1/NONE,
+ * tra_adv_ph = tra_adv_pv = 1 */
+int tracer_allocate_tracers(int num_tracers, int nl, int nod2D, int AB_order);
+
+/* lib/tracer_c_interface.F90:613-644: tracer_id is 1-based; copies into
+ * values and valuesAB */
+int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D);
+
+/* lib/tracer_c_interface.F90:652-684 */
+int tracer_set_velocity_zero(int nl, int nod2D, int edge2D);
+
+/* lib/tracer_c_interface.F90:694-744. The buffer holds (nl-1)*edge2D*2
+ * doubles; as in the reference (which passes dynamics%uv to
+ * do_oce_adv_tra's explicit-shape vel(2, nl-1, elem2D) dummy), the first
+ * 2*(nl-1)*elem2D of them are read as vel(2, nl-1, elem2D). w is
+ * (nl, nod2D); w_e and w_i are set to zero as the reference does. */
+int tracer_set_velocity(const double *uv, const double *w, int nl, int nod2D, int edge2D);
+
+/* lib/tracer_c_interface.F90:749-759 */
+int tracer_init_complete(void);
+
+/* lib/tracer_c_interface.F90:769-821: -1 (restart-based, see tracer_init) */
+int tracer_load_mesh_partition(const char *restart_dir);
+
+/* lib/tracer_c_interface.F90:830-868: FESOM mesh files (nod2d.out,
+ * elem2d.out, aux3d.out, edges.out, edge_tri.out, nlvls.out, elvls.out);
+ * -1 when the directory or a file is missing */
+int tracer_load_mesh_from_files(const char *mesh_path);
+
+/* lib/tracer_c_interface.F90:875-921: hnode = hnode_new = helem = zbar(k)-zbar(k+1) */
+int tracer_init_ale_arrays(void);
+
+/* lib/tracer_c_interface.F90:931-997: allocates dynamics/work arrays; here
+ * also uploads mesh, velocity and tracers to the GPU */
+int tracer_init_for_advection(void);
+
+/* ---------------------------------------------------------------------
+ * Part 2: extensions (no counterpart in the reference)
+ * ------------------------------------------------------------------- */
+
+/* No reference counterpart. Builds a doubly periodic (periodic != 0) or
+ * closed nx x ny node rectangle of Lx x Ly metres split into right
+ * triangles with FESOM's orientation (edge_tri(1) left of the edge,
+ * clockwise elements), nl interfaces of thickness dz (stretched by
+ * 'stretch'), per-node bottom levels drawn from [lev_lo, lev_hi] with
+ * 'seed' (elements = min of nodes, nodes = max of elements), nodes and
+ * elements ordered along a Morton curve when sfc != 0. Replaces the
+ * reference's generate_analytic_mesh (lib/analytic_mesh.F90:114). */
+int tracer_generate_mesh(int nx, int ny, int periodic, double Lx, double Ly, int nl, double dz,
+                         double stretch, int lev_lo, int lev_hi, unsigned long long seed, int sfc);
+
+/* No reference counterpart. Scheme strings as tra_adv_hor/ver/lim
+ * (lib/MOD_TRACER.F90:18-20): hor 'UPW1'|'MUSCL'|'MFCT', ver
+ * 'UPW1'|'QR4C'|'PPM'|'CDIFF', lim 'NONE'|'FCT'. -1 on an unknown scheme
+ * (the reference aborts: lib/oce_adv_tra_driver.F90:253-254). */
+int tracer_set_schemes(int tracer_id, const char *hor, const char *ver, const char *lim,
+                       double tra_adv_ph, double tra_adv_pv);
+
+/* No reference counterpart: velocity in do_oce_adv_tra's layout,
+ * uv(2, nl-1, elem2D), w and w_e (nl, nod2D) */
+int tracer_set_velocity_elem(const double *uv, const double *w, const double *w_e, int nl,
+                             int nod2D, int elem2D);
+
+/* No reference counterpart: analytic divergence-free velocity (background
+ * current + rotation cells + vertical shear) on the current mesh, w from
+ * discrete continuity, w_e = w. Writes the dt giving the requested CFL. */
+int tracer_make_velocity(double u_bg, double v_bg, double cell_amp, double shear, double cfl,
+                         double *dt_out);
+
+/* No reference counterpart: caller-supplied valuesAB (1-based id) */
+int tracer_set_values_ab(int tracer_id, const double *values, int nl, int nod2D);
+
+/* No reference counterpart: values of any tracer (1-based id) */
+int tracer_get_values_id(int tracer_id, double *values, int nz, int nn);
+
+/* No reference counterpart. Options:
+ *   "precision"          64 (default), 32, 16 (FP32 with FP16 gradients)
+ *   "refresh_valuesAB"   1 (default): valuesAB := values each step; 0: as set
+ *   "accumulate_del_ttf" 0 (default); 1: never zero del_ttf (reference C ABI)
+ *   "use_graph"          1 (default): replay multi-step runs from a hipGraph */
+int tracer_set_option(const char *name, int value);
+
+/* No reference counterpart: nsteps steps of ALL tracers */
+int tracer_advect_steps(int nsteps, double dt);
+
+/* No reference counterpart: out[7] = {min, max, sum over all values,
+ * min and max over active cells, mass = sum T*hnode_new*areasvol, NaN count} */
+int tracer_get_diagnostics(int tracer_id, double *out);
+
+/* No reference counterpart: size (elements) and kind (0 int32, 1 float64)
+ * of a named mesh array in the reference layout; -1 if unknown */
+int tracer_mesh_array_info(const char *name, long long *count, int *kind);
+/* No reference counterpart: copy a named mesh array out (0-based indices,
+ * -1 for no element) */
+int tracer_get_mesh_array(const char *name, void *out, long long count);
+
+/* No reference counterpart: a named GPU work array of the last step in
+ * the reference layout (fct_LO, adv_flux_hor, adv_flux_lo, adv_flux_ver,
+ * fct_plus, fct_minus, tr_xy, gnod) */
+int tracer_get_work_array(const char *name, double *out, long long count);
+
+/* No reference counterpart: wall time of nsteps steps of all tracers on the
+ * library's stream (hipEvent), milliseconds */
+int tracer_time_steps(int nsteps, double dt, double *ms_out);
+
+/* No reference counterpart: per-kernel mean time (ms) of one step, measured
+ * with hipEvents on the library's stream; names written ';'-separated */
+int tracer_time_kernels(double dt, int reps, char *names, int names_len, double *ms, int max_kernels);
+
+/* No reference counterpart: the last error message */
+const char *tracer_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,144 @@
+"""GPU FP64 path against the CPU oracle, step by step, through the C ABI.
+
+Bar (FP64): relative max-norm |T_gpu - T_oracle|_inf / |T_oracle|_inf <= 1e-12
+after every step, volume-weighted tracer mass conserved to 1e-13 relative,
+and with FCT no value outside the initial global range.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import mesh_dict
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+MASS_TOL = 1e-13
+
+MESHES = {
+    "periodic_24x20x12_random_bottom": dict(nx=24, ny=20, nl=12, periodic=True, Lx=1.2e5, Ly=1.0e5, dz=10.0,
+                                            stretch=0.5, lev_lo=3, lev_hi=12, seed=3),
+    "closed_16x14x10_random_bottom": dict(nx=16, ny=14, nl=10, periodic=False, Lx=8.0e4, Ly=7.0e4, dz=12.0,
+                                          stretch=0.3, lev_lo=3, lev_hi=10, seed=5),
+    "periodic_20x20x10_flat": dict(nx=20, ny=20, nl=10, periodic=True, Lx=1.0e5, Ly=1.0e5, dz=10.0,
+                                   stretch=0.0, lev_lo=10, lev_hi=10, seed=0),
+}
+
+SCHEMES = [("UPW1", "UPW1", "NONE"), ("UPW1", "UPW1", "FCT"), ("MUSCL", "QR4C", "FCT"), ("MFCT", "PPM", "FCT"),
+           ("MUSCL", "CDIFF", "FCT"), ("MUSCL", "QR4C", "NONE"), ("MFCT", "UPW1", "FCT"), ("MUSCL", "PPM", "NONE")]
+
+
+def setup_case(api, mesh_kw, schemes, ntr=4, seed=1, ph=1.0, pv=1.0):
+    from fesom2_tracer_dwarf_amd import configs
+    kw = dict(mesh_kw)
+    nx, ny, nl = kw.pop("nx"), kw.pop("ny"), kw.pop("nl")
+    d = api.generate_mesh(nx, ny, nl, sfc=True, **kw)
+    api.set_option("precision", 64)
+    api.allocate_tracers(ntr)
+    dt = api.make_velocity(0.1, 0.05, 0.2, 0.5, 0.3)
+    coord = api.mesh_array("coord")
+    trs = configs.initial_tracers(coord, nl, kw["Lx"], kw["Ly"], ntr, seed=seed, periodic=kw["periodic"])
+    for i, t in enumerate(trs):
+        api.set_values(i + 1, t)
+        api.set_schemes(i + 1, *schemes, ph=ph, pv=pv)
+    return d, dt, trs
+
+
+def rel_err(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("mesh_name", list(MESHES))
+@pytest.mark.parametrize("schemes", SCHEMES, ids=lambda s: "-".join(s))
+def test_fp64_matches_oracle(api, mesh_name, schemes):
+    import oracle as orc
+    print(f"[gpu] parity {mesh_name} {schemes}", flush=True)
+    sys.stdout.flush()
+    d, dt, trs = setup_case(api, MESHES[mesh_name], schemes)
+    m = mesh_dict(api)
+    o = orc.Oracle(m)
+    uv, w, we = api.mesh_array("uv"), api.mesh_array("w"), api.mesh_array("w_e")
+    L, N = d["nl"] - 1, d["nod2D"]
+    hn = m["hnode_new"].reshape(N, L).T
+    av = m["areasvol"].reshape(N, d["nl"]).T[:L]
+    act = np.arange(L)[:, None] < (m["nlevels_nod2D"] - 1)[None, :]
+    ref = [np.array(t, order="F", copy=True) for t in trs]
+    mass0 = [(t * hn * av)[act].sum() for t in ref]
+    rng0 = [(t[act].min(), t[act].max()) for t in ref]
+    for step in range(3):
+        api.advect_steps(1, dt)
+        for i in range(len(ref)):
+            o.step(dt, uv, w, we, ref[i], hor=schemes[0], ver=schemes[1], lim=schemes[2])
+            g = api.get_tracer_values(i + 1)
+            err = rel_err(g, ref[i])
+            assert err <= RTOL, f"tracer {i + 1} step {step + 1}: rel err {err:.3e}"
+            mass = (g * hn * av)[act].sum()
+            assert abs(mass - mass0[i]) <= MASS_TOL * abs(mass0[i]), f"mass drift {abs(mass - mass0[i]) / abs(mass0[i]):.3e}"
+            if schemes[2] == "FCT":
+                lo, hi = rng0[i]
+                span = hi - lo
+                assert g[act].min() >= lo - 1e-12 * span and g[act].max() <= hi + 1e-12 * span
+    if schemes[2] == "FCT":
+        lo_gpu = api.work_array("fct_LO", L * N)
+        lo_ref = o.work_arrays["fct_LO"]
+        # the last tracer's LO solution, levels of the wet column only
+        mask = act.T.ravel()
+        assert rel_err(lo_gpu[mask], lo_ref[mask]) <= RTOL
+
+
+def test_determinism_bitwise(api):
+    print("[gpu] determinism", flush=True)
+    d, dt, trs = setup_case(api, MESHES["periodic_24x20x12_random_bottom"], ("MUSCL", "QR4C", "FCT"))
+    api.advect_steps(3, dt)
+    a = [api.get_tracer_values(i + 1) for i in range(len(trs))]
+    for i, t in enumerate(trs):
+        api.set_values(i + 1, t)
+    api.advect_steps(3, dt)
+    b = [api.get_tracer_values(i + 1) for i in range(len(trs))]
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_graph_replay_equals_eager(api):
+    print("[gpu] graph vs eager", flush=True)
+    d, dt, trs = setup_case(api, MESHES["closed_16x14x10_random_bottom"], ("MUSCL", "PPM", "FCT"))
+    api.set_option("use_graph", 1)
+    outs = []
+    for rep in range(3):  # replays with eager work and syncs in between
+        api.advect_steps(2, dt)
+        outs.append([api.get_tracer_values(i + 1) for i in range(len(trs))])
+        api.diagnostics(1)
+    graph_vals = outs[-1]
+    for i, t in enumerate(trs):
+        api.set_values(i + 1, t)
+    api.set_option("use_graph", 0)
+    for rep in range(3):
+        api.advect_steps(2, dt)
+    eager_vals = [api.get_tracer_values(i + 1) for i in range(len(trs))]
+    api.set_option("use_graph", 1)
+    for x, y in zip(graph_vals, eager_vals):
+        assert np.array_equal(x, y)
+
+
+def test_diagnostics_match_host(api):
+    print("[gpu] diagnostics", flush=True)
+    d, dt, trs = setup_case(api, MESHES["periodic_24x20x12_random_bottom"], ("MUSCL", "QR4C", "FCT"))
+    api.advect_steps(2, dt)
+    m = mesh_dict(api)
+    L, N = d["nl"] - 1, d["nod2D"]
+    hn = m["hnode_new"].reshape(N, L).T
+    av = m["areasvol"].reshape(N, d["nl"]).T[:L]
+    act = np.arange(L)[:, None] < (m["nlevels_nod2D"] - 1)[None, :]
+    for i in range(len(trs)):
+        g = api.get_tracer_values(i + 1)
+        dg = api.diagnostics(i + 1)
+        assert dg["min"] == g.min() and dg["max"] == g.max()
+        assert abs(dg["sum"] - g.sum()) <= 1e-13 * abs(g).sum()
+        assert dg["active_min"] == g[act].min() and dg["active_max"] == g[act].max()
+        mass = (g * hn * av)[act].sum()
+        assert abs(dg["mass"] - mass) <= 1e-13 * abs(mass)
+        assert dg["nan_count"] == 0
+    st = api.get_statistics()
+    g1 = api.get_tracer_values(1)
+    assert st["min"] == g1.min() and st["max"] == g1.max()
