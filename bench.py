@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark of the tracer-advection step on one MI355X.
+
+Workload (default): doubly periodic 1000 x 1000 node mesh (3.0M edges, 2.0M
+triangles), 64 level interfaces with per-node bottom level drawn from
+U{3..64} (seed 0), tracers T and S, MUSCL + QR4C + FCT (Zalesak), analytic
+divergence-free velocity at CFL 0.3. One "step" advances both tracers once.
+
+Metric: node-level tracer updates per second = n_tracers * C * steps / t,
+C = active (node, layer) cells of the generated mesh. The timed region is K
+graph-replayed steps bracketed by hipEvents on the library's stream (inputs
+already resident in HBM). FP64 is the headline (the parity path); the FP32
+form (FP64 node tracer/accumulation) is reported beside it.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "node-level tracer updates/s at 1M nodes×64 lvls, FP64 vs FP32; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def counts(api):
+    m = {k: api.mesh_array(k) for k in ("nlevels_nod2D", "nlevels", "edge_tri")}
+    d = api.dims()
+    C = int((m["nlevels_nod2D"] - 1).sum())
+    Ce = int((m["nlevels"] - 1).sum())
+    et = m["edge_tri"].reshape(-1, 2)
+    nl1 = m["nlevels"][et[:, 0]] - 1
+    nl2 = np.where(et[:, 1] >= 0, m["nlevels"][np.maximum(et[:, 1], 0)] - 1, 0)
+    Ced = int(np.maximum(nl1, nl2).sum())
+    return dict(C=C, Ce=Ce, Ced=Ced, N=d["nod2D"], T=d["elem2D"], E=d["edge2D"], nl=d["nl"], maxAdj=d["maxAdj"])
+
+
+def step_alg_bytes(c, ntr, s_t, s_v, s_m):
+    """Compulsory HBM bytes of one step: each semantic input read once, each output written once."""
+    return (ntr * 2 * s_t * c["C"]                       # tracer read + write
+            + s_v * 2 * c["Ce"] + s_m * c["Ce"]           # uv(2,.,elem) + helem
+            + s_v * (c["C"] + c["N"])                     # w on interfaces
+            + s_m * 5 * c["C"]                            # hnode, hnode_new, areasvol, Z_3d_n, zbar_3d_n
+            + c["E"] * (16 + 6 * s_m) + c["T"] * (20 + 7 * s_m) + 8 * c["N"] + 4 * c["maxAdj"] * c["N"])
+
+
+def kernel_alg_bytes(name, c, s_t, s_r, s_g):
+    """Compulsory bytes of one launch (one tracer) of each kernel: every array it
+    reads or writes counted once over its active cells."""
+    C, Ce, Ced, N, T, E = c["C"], c["Ce"], c["Ced"], c["N"], c["T"], c["E"]
+    if name == "k_elem_grad":     # tracer, gradient_sca, elem nodes -> tr_xy
+        return s_t * C + T * (6 * s_r + 12 + 4) + 2 * s_g * Ce
+    if name == "k_node_grad":     # tr_xy, elem_area, node->elem -> nodal gradient
+        return 2 * s_g * Ce + s_r * T + 4 * (N + 3 * T) + 4 * N + 2 * s_g * C
+    if name == "k_edge_flux":     # tracer (T and AB share storage), vel, helem, grads, edge geometry -> LO, HO-LO
+        return (s_t * C + 3 * s_r * Ce + 2 * s_g * (Ce + C) + E * (7 * s_r + 24) + 2 * s_r * Ced)
+    if name == "k_node_lo":       # LO flux, HO-LO flux, w, w_e, tracer, area, hnode(2), areasvol, Z, zbar -> LO, adf_v, P+, P-
+        return (2 * s_r * Ced + 2 * s_r * (C + N) + s_t * C + s_r * (C + N) + 5 * s_r * C + 4 * (N + 2 * E)
+                + 8 * C + s_r * (C + N) + 2 * s_r * C)
+    if name == "k_node_limit":    # LO, tracer, P+, P-, areasvol -> R+, R-
+        return 8 * C + s_t * C + 2 * s_r * C + s_r * C + 4 * (N + 3 * T) + 2 * s_r * C
+    if name == "k_node_update":   # adf_h, adf_v, R+, R-, LO, tracer, hnode(2), areasvol -> tracer
+        return s_r * Ced + s_r * (C + N) + 2 * s_r * C + 8 * C + s_t * C + 3 * s_r * C + 4 * (N + 2 * E) + s_t * C
+    return 0
+
+
+def run_precision(api, prec, dt, steps, warmup, ntr):
+    api.set_option("precision", prec)
+    api.advect_steps(warmup, dt)
+    t0 = time.perf_counter()
+    ms = api.time_steps(steps, dt)
+    wall = time.perf_counter() - t0
+    kern = api.time_kernels(dt, reps=3)
+    return ms, wall, kern
+
+
+def cpu_baseline(api, dt, ntr, c, max_seconds=20.0):
+    log = globals()["log"]
+    """Oracle (the CPU FP64 restatement of the reference path), 1 thread, on the
+    same mesh: as many single-tracer steps as fit in ~max_seconds (at least one)."""
+    import oracle as orc
+    from fesom2_tracer_dwarf_amd import configs
+    names = orc.MESH_INT + orc.MESH_DBL
+    mesh = {k: api.mesh_array(k) for k in names}
+    mesh.update(api.dims())
+    o = orc.Oracle(mesh)
+    uv, w, we = api.mesh_array("uv"), api.mesh_array("w"), api.mesh_array("w_e")
+    cfg = configs.WORKLOADS["periodic_1000x1000x64"]
+    trs = configs.initial_tracers(api.mesh_array("coord"), mesh["nl"], cfg["Lx"], cfg["Ly"], ntr, seed=0)
+    done, t_total = 0, 0.0
+    while True:
+        v = trs[done % ntr]
+        t0 = time.perf_counter()
+        o.step(dt, uv, w, we, v, hor="MUSCL", ver="QR4C", lim="FCT")
+        t_total += time.perf_counter() - t0
+        done += 1
+        log(f"cpu step {done}: {t_total:.1f} s")
+        if t_total >= max_seconds or done >= 4:
+            break
+    return dict(value=done * c["C"] / t_total, unit="updates/s", cores=1, kind="port",
+                sample=f"{done} single-tracer MUSCL+QR4C+FCT steps of the same 1000x1000x64 mesh "
+                       f"({t_total:.1f} s, 1 thread, {platform.processor() or platform.machine()})")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=1000)
+    ap.add_argument("--ny", type=int, default=1000)
+    ap.add_argument("--nl", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the mesh would shard by "
+              "space-filling-curve ranges with halo exchanges; see the design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    from fesom2_tracer_dwarf_amd import FESOMTracerAdvection, configs
+    api = FESOMTracerAdvection()
+    cfg = dict(configs.WORKLOADS["periodic_1000x1000x64"])
+    cfg["nx"], cfg["ny"], cfg["nl"] = args.nx, args.ny, args.nl
+    ntr = cfg["ntr"]
+    t_setup = time.perf_counter()
+    api.generate_mesh(cfg["nx"], cfg["ny"], cfg["nl"], periodic=True, Lx=cfg["Lx"], Ly=cfg["Ly"], dz=cfg["dz"],
+                      lev_lo=cfg["lev"][0], lev_hi=cfg["lev"][1], seed=0, sfc=True)
+    api.allocate_tracers(ntr)
+    dt = api.make_velocity(0.1, 0.05, 0.2, 0.5, 0.3)
+    trs = configs.initial_tracers(api.mesh_array("coord"), cfg["nl"], cfg["Lx"], cfg["Ly"], ntr, seed=0)
+    for i, t in enumerate(trs):
+        api.set_values(i + 1, t)
+        api.set_schemes(i + 1, "MUSCL", "QR4C", "FCT")
+    c = counts(api)
+    t_setup = time.perf_counter() - t_setup
+    log(f"mesh ready in {t_setup:.1f} s: {c}")
+
+    res = {}
+    for prec, tag in ((64, "fp64"), (32, "fp32")):
+        log(f"timing {tag}")
+        ms, wall, kern = run_precision(api, prec, dt, args.steps, args.warmup, ntr)
+        log(f"{tag}: {ms / args.steps:.3f} ms/step, kernels {kern}")
+        s_r = 8 if prec == 64 else 4
+        upd = ntr * c["C"] * args.steps / (ms * 1e-3)
+        balg = step_alg_bytes(c, ntr, 8, s_r, s_r)
+        kb = {k: kernel_alg_bytes(k, c, 8, s_r, s_r) for k in kern}
+        dom = max(kern, key=kern.get)
+        res[tag] = dict(updates_per_s=upd, ms_per_step=ms / args.steps, wall_s=wall,
+                        step_alg_bytes=balg, step_gbs=balg / (ms / args.steps * 1e-3) / 1e9,
+                        step_hbm_frac=balg / (ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        kernels_ms=kern,
+                        kernels_gbs={k: kb[k] / (kern[k] * 1e-3) / 1e9 for k in kern if kern[k] > 0},
+                        dominant=dom)
+        diag = api.diagnostics(1)
+        res[tag]["T_mass"] = diag["mass"]
+        res[tag]["nan"] = diag["nan_count"]
+
+    f64 = res["fp64"]
+    dom = f64["dominant"]
+    achieved = f64["kernels_gbs"][dom]
+    line = {
+        "metric": METRIC,
+        "value": f64["updates_per_s"],
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"periodic_{cfg['nx']}x{cfg['ny']}x{cfg['nl']}", "nodes": c["N"],
+                   "edges": c["E"], "elements": c["T"], "level_interfaces": cfg["nl"],
+                   "bottom_levels": "U{3..64} seed 0", "tracers": ntr, "schemes": "MUSCL+QR4C+FCT",
+                   "active_node_cells": c["C"], "dense_node_cells": c["N"] * (cfg["nl"] - 1),
+                   "dt_s": dt, "cfl": 0.3},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "step_roofline": {"alg_bytes": f64["step_alg_bytes"], "achieved_gbs": f64["step_gbs"],
+                          "frac": f64["step_hbm_frac"]},
+        "variants": {k: {kk: vv for kk, vv in v.items() if kk not in ("wall_s",)} for k, v in res.items()},
+        "fp32_over_fp64": res["fp32"]["updates_per_s"] / f64["updates_per_s"],
+        "setup_s": t_setup,
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle, 1 thread)")
+        api.set_option("precision", 64)
+        line["cpu_baseline"] = cpu_baseline(api, dt, ntr, c, args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
