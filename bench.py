@@ -54,23 +54,28 @@ def step_alg_bytes(c, ntr, s_t, s_v, s_m):
             + c["E"] * (16 + 6 * s_m) + c["T"] * (20 + 7 * s_m) + 8 * c["N"] + 4 * c["maxAdj"] * c["N"])
 
 
-def kernel_alg_bytes(name, c, s_t, s_r, s_g):
-    """Compulsory bytes of one launch (one tracer) of each kernel: every array it
-    reads or writes counted once over its active cells."""
+def kernel_alg_bytes(name, c, ntr, s_t, s_r, s_g):
+    """Compulsory bytes of one step's launches of a kernel (all ntr tracers
+    batched into it): every array it reads or writes counted once over its
+    active cells; mesh/velocity terms once, tracer terms ntr times."""
     C, Ce, Ced, N, T, E = c["C"], c["Ce"], c["Ced"], c["N"], c["T"], c["E"]
-    if name == "k_elem_grad":     # tracer, gradient_sca, elem nodes -> tr_xy
-        return s_t * C + T * (6 * s_r + 12 + 4) + 2 * s_g * Ce
-    if name == "k_node_grad":     # tr_xy, elem_area, node->elem -> nodal gradient
-        return 2 * s_g * Ce + s_r * T + 4 * (N + 3 * T) + 4 * N + 2 * s_g * C
-    if name == "k_edge_flux":     # tracer (T and AB share storage), vel, helem, grads, edge geometry -> LO, HO-LO
-        return (s_t * C + 3 * s_r * Ce + 2 * s_g * (Ce + C) + E * (7 * s_r + 24) + 2 * s_r * Ced)
-    if name == "k_node_lo":       # LO flux, HO-LO flux, w, w_e, tracer, area, hnode(2), areasvol, Z, zbar -> LO, adf_v, P+, P-
-        return (2 * s_r * Ced + 2 * s_r * (C + N) + s_t * C + s_r * (C + N) + 5 * s_r * C + 4 * (N + 2 * E)
-                + 8 * C + s_r * (C + N) + 2 * s_r * C)
-    if name == "k_node_limit":    # LO, tracer, P+, P-, areasvol -> R+, R-
-        return 8 * C + s_t * C + 2 * s_r * C + s_r * C + 4 * (N + 3 * T) + 2 * s_r * C
-    if name == "k_node_update":   # adf_h, adf_v, R+, R-, LO, tracer, hnode(2), areasvol -> tracer
-        return s_r * Ced + s_r * (C + N) + 2 * s_r * C + 8 * C + s_t * C + 3 * s_r * C + 4 * (N + 2 * E) + s_t * C
+    D = 8  # padded node degree of the packed node records
+    node_rec = N * (4 + 3 * 4 * D)          # nlevels + edge list + edge levels + element list
+    if name == "k_grad":          # valuesAB, element records -> nodal mean gradient (2 comps)
+        return ntr * (s_t * C + 2 * s_g * C) + T * (16 + 8 * s_r) + N * (4 + 4 * D)
+    if name == "k_edge":          # tracer (T and AB share storage), gradients, vel, helem, edge records -> LO, HO-LO
+        return (ntr * (s_t * C + 2 * s_g * C + 2 * s_r * Ced)
+                + 3 * s_r * Ce + E * (32 + 8 * s_r) + T * (16 + 8 * s_r))
+    if name == "k_node_lo":       # LO and HO-LO edge fluxes, tracer -> LO, adf_v, P+, P-; w, w_e, 7 node geometry arrays
+        return (ntr * (2 * s_r * Ced + s_t * C + 8 * C + s_r * (C + N) + 2 * s_r * C)
+                + 2 * s_r * (C + N) + s_r * (C + N) * 2 + 5 * s_r * C + node_rec)
+    if name == "k_node_limit":    # LO, tracer, P+/P- -> R+/R-; areasvol, element records
+        return ntr * (8 * C + s_t * C + 2 * s_r * C + 2 * s_r * C) + s_r * C + node_rec + T * 16
+    if name == "k_node_update":   # HO-LO edge flux, adf_v, R+/R-, LO, tracer -> tracer; hnode(2), areasvol
+        return (ntr * (s_r * Ced + s_r * (C + N) + 2 * s_r * C + 8 * C + 2 * s_t * C)
+                + 3 * s_r * C + node_rec + 4 * D * N)
+    if name == "k_node_nolim":
+        return ntr * (s_r * Ced + 2 * s_t * C) + s_r * (C + N) * 2 + 3 * s_r * C + node_rec
     return 0
 
 
@@ -126,7 +131,18 @@ def main():
     ap.add_argument("--nl", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": a.numel() * 8, "launches": 5}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the mesh would shard by "
               "space-filling-curve ranges with halo exchanges; see the design notes).", file=sys.stderr)
@@ -158,7 +174,7 @@ def main():
         s_r = 8 if prec == 64 else 4
         upd = ntr * c["C"] * args.steps / (ms * 1e-3)
         balg = step_alg_bytes(c, ntr, 8, s_r, s_r)
-        kb = {k: kernel_alg_bytes(k, c, 8, s_r, s_r) for k in kern}
+        kb = {k: kernel_alg_bytes(k, c, ntr, 8, s_r, s_r) for k in kern}
         dom = max(kern, key=kern.get)
         res[tag] = dict(updates_per_s=upd, ms_per_step=ms / args.steps, wall_s=wall,
                         step_alg_bytes=balg, step_gbs=balg / (ms / args.steps * 1e-3) / 1e9,

@@ -68,8 +68,8 @@ public:
 
     Diag diagnostics(int tr);
     // copy a device work array out in the unpadded reference layout
-    // names: fct_LO, adv_flux_hor, adv_flux_ver, fct_plus, fct_minus, tr_xy, gnod
-    int get_work(const std::string &name, int tr_unused, double *out, size_t count);
+    // names: fct_LO, adv_flux_lo, adv_flux_hor, adv_flux_ver, fct_plus, fct_minus, gnod
+    int get_work(const std::string &name, int tr, double *out, size_t count);
     size_t work_count(const std::string &name) const;
 
     // per-kernel timing of one step with hip events on the engine stream

@@ -1,17 +1,20 @@
 // fta_engine.hip -- the tracer-advection step on gfx950 and its host driver.
 //
-// One step of one tracer (reference: do_oce_adv_tra, lib/oce_adv_tra_driver.F90:23-294,
-// plus the caller's update, src/fesom_analytic.F90:405-437) is five launches:
-//   k_elem_grad   elemental gradient of valuesAB (MUSCL/MFCT only)
-//   k_node_grad   area-weighted nodal mean gradient (MUSCL/MFCT only)
-//   k_edge_flux   per edge-level: LO upwind flux and HO-LO antidiffusive flux
+// One step (reference: do_oce_adv_tra, lib/oce_adv_tra_driver.F90:23-294,
+// plus the caller's update, src/fesom_analytic.F90:405-437) for a batch of
+// NT tracers that share their schemes is five launches:
+//   k_grad        nodal mean of the elemental gradients of valuesAB (MUSCL/MFCT)
+//   k_edge        per edge-level: LO upwind flux and HO-LO antidiffusive flux
 //   k_node_lo     per node-level: LO solution, vertical HO-LO flux, P+/P- sums
 //   k_node_limit  per node-level: Zalesak bounds (a1-a3) and R+/R- (b2)
 //   k_node_update per node-level: limited fluxes (b3), divergence, T update
-// (without FCT the last three collapse into k_node_nolim).
+// (without FCT the last three collapse into k_node_nolim). Elemental
+// gradients are recomputed from their three nodes where needed, never stored.
 #include "fta_engine.h"
 #include "fta_kernels.h"
+#include "fta_vertical.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -33,205 +36,56 @@ void set_error(const std::string &s) { g_last_error = s; }
                                      " at " #call);                                           \
     } while (0)
 
-// ===========================================================================
-// Vertical fluxes at interface nz of a node column (lane = interface).
-// A is the tracer the scheme reads, W the vertical velocity it uses and
-// fin the flux already stored there (o_init_zero=.false. semantics:
-// the routine returns new - fin). All lanes must call (PPM shuffles).
-// ===========================================================================
+// the elemental gradient of tracer A at layer k (tracer_gradient_elements:
+// gradient_sca(1:3)*T(nodes) and gradient_sca(4:6)*T(nodes), node order 1,2,3)
 template <class R>
-struct VCol {
-    const double *A;  // tracer column (double storage)
-    const R *W, *area, *Z, *zbar, *h, *hn;
-    int nlev;         // nlevels_nod2D: interfaces 1..nlev, layers 1..nlev-1
-    int nz;           // this lane's interface / layer (1-based)
-};
-
-template <class R>
-__device__ __forceinline__ R ldA(const VCol<R> &c, int nz)
+__device__ __forceinline__ void elem_gradient(const ElemMeta &em, const R *g, const double *A, int S, int k,
+                                              R &gx, R &gy)
 {
-    return (nz >= 1 && nz <= c.nlev - 1) ? (R)c.A[nz - 1] : (R)0;
-}
-
-// adv_tra_ver_upw1 (lib/oce_adv_tra_ver.F90:248-335)
-template <class R>
-__device__ __forceinline__ R vflux_upw1(const VCol<R> &c, R fin)
-{
-    const int nz = c.nz;
-    if (nz == 1) return -c.W[0] * ldA(c, 1) * c.area[0] - fin;
-    if (nz == c.nlev) return (R)0 - fin;
-    if (nz < c.nlev) {
-        R w = c.W[nz - 1];
-        return (R)(-0.5) * (ldA(c, nz) * (w + fabs(w)) + ldA(c, nz - 1) * (w - fabs(w))) * c.area[nz - 1] - fin;
-    }
-    return fin;
-}
-
-// adv_tra_ver_qr4c (lib/oce_adv_tra_ver.F90:339-444) including the double
-// write of interface nzmin+1 on two-layer columns
-template <class R>
-__device__ __forceinline__ R vflux_qr4c(const VCol<R> &c, R num_ord, R fin)
-{
-    const int nz = c.nz, nlev = c.nlev;
-    if (nz == 1) return -ldA(c, 1) * c.W[0] * c.area[0] - fin;
-    if (nz == nlev) return (R)0 - fin;
-    if (nz > nlev) return fin;
-    if (nz == 2 || nz == nlev - 1) {
-        R cen = (R)(-0.5) * (ldA(c, nz - 1) + ldA(c, nz)) * c.W[nz - 1] * c.area[nz - 1];
-        if (nz == 2 && nz == nlev - 1) return cen - (cen - fin);
-        return cen - fin;
-    }
-    // 3 <= nz <= nlev-2
-    R t0 = ldA(c, nz), tm1 = ldA(c, nz - 1), tm2 = ldA(c, nz - 2), tp1 = ldA(c, nz + 1);
-    R Z0 = c.Z[nz - 1], Zm1 = c.Z[nz - 2], Zm2 = c.Z[nz - 3], Zp1 = c.Z[nz];
-    R qc = (tm1 - t0) / (Zm1 - Z0);
-    R qu = (t0 - tp1) / (Z0 - Zp1);
-    R qd = (tm2 - tm1) / (Zm2 - Zm1);
-    R zb = c.zbar[nz - 1];
-    R Tmean1 = t0 + ((R)2 * qc + qu) * (zb - Z0) / (R)3;
-    R Tmean2 = tm1 + ((R)2 * qc + qd) * (zb - Zm1) / (R)3;
-    R w = c.W[nz - 1];
-    R Tmean = (w + fabs(w)) * Tmean1 + (w - fabs(w)) * Tmean2;
-    return ((R)(-0.5) * ((R)1 - num_ord) * Tmean - num_ord * ((R)0.5 * (Tmean1 + Tmean2)) * w) * c.area[nz - 1] - fin;
-}
-
-// adv_tra_ver_cdiff (lib/oce_adv_tra_ver.F90:648-712); interface nlev untouched
-template <class R>
-__device__ __forceinline__ R vflux_cdiff(const VCol<R> &c, R fin)
-{
-    const int nz = c.nz;
-    if (nz == 1) return -c.W[0] * ldA(c, 1) * c.area[0] - fin;
-    if (nz <= c.nlev - 1) {
-        R tv = (R)0.5 * (ldA(c, nz - 1) + ldA(c, nz));
-        return -tv * c.W[nz - 1] * c.area[nz - 1] - fin;
-    }
-    return fin;
-}
-
-// adv_tra_vert_ppm (lib/oce_adv_tra_ver.F90:448-644). Lane nz first builds
-// the interface value tv(nz), then the limited parabola of layer nz; the
-// flux through interface nz comes from its upwind layer (nz if W>0, nz-1 if
-// W<0), which is exactly what the reference's per-layer writes produce
-// (its CYCLE only skips layers that would write nothing).
-template <class R>
-__device__ __forceinline__ R vflux_ppm(const VCol<R> &c, R dt, R fin)
-{
-    const int nz = c.nz, nlev = c.nlev;
-    const int nzmax = nlev;
-    // --- interface values tv(nz)
-    R tv = (R)0;
-    if (nz == 1) tv = ldA(c, 1);
-    else if (nz == nzmax) tv = ldA(c, nzmax - 1);
-    else if (nz == nzmax - 1) {
-        R sw = copysign((R)1, c.W[nzmax - 2]);
-        tv = -ldA(c, nzmax - 2) * fmn(sw, (R)0) + ldA(c, nzmax - 1) * fmx(sw, (R)0);
-    } else if (nz == 2) tv = (R)0.5 * (ldA(c, 1) + ldA(c, 2));
-    else if (nz >= 3 && nz <= nzmax - 2) {
-        const int j = nz - 1;  // reference loop index: tv(j+1)
-        R dzjm1 = c.hn[j - 2], dzj = c.hn[j - 1], dzjp1 = c.hn[j], dzjp2 = c.hn[j + 1];
-        R Tm1 = ldA(c, j - 1), T0 = ldA(c, j), Tp1 = ldA(c, j + 1), Tp2 = ldA(c, j + 2);
-        R deltaj = dzj / (dzjm1 + dzj + dzjp1) *
-                   (((R)2 * dzjm1 + dzj) / (dzjp1 + dzj) * (Tp1 - T0) + (dzj + (R)2 * dzjp1) / (dzjm1 + dzj) * (T0 - Tm1));
-        R deltajp1 = dzjp1 / (dzj + dzjp1 + dzjp2) *
-                     (((R)2 * dzj + dzjp1) / (dzjp2 + dzjp1) * (Tp2 - Tp1) + (dzjp1 + (R)2 * dzjp2) / (dzj + dzjp1) * (Tp1 - T0));
-        if ((Tp1 - T0) * (T0 - Tm1) > (R)0)
-            deltaj = fmn(fmn(fabs(deltaj), (R)2 * fabs(Tp1 - T0)), (R)2 * fabs(T0 - Tm1)) * copysign((R)1, deltaj);
-        else
-            deltaj = (R)0;
-        if ((Tp2 - Tp1) * (Tp1 - T0) > (R)0)
-            deltajp1 = fmn(fmn(fabs(deltajp1), (R)2 * fabs(Tp2 - Tp1)), (R)2 * fabs(Tp1 - T0)) * copysign((R)1, deltajp1);
-        else
-            deltajp1 = (R)0;
-        tv = T0 + dzj / (dzj + dzjp1) * (Tp1 - T0) +
-             (R)1 / (dzjm1 + dzj + dzjp1 + dzjp2) *
-             ((R)2 * dzjp1 * dzj / (dzj + dzjp1) *
-              ((dzjm1 + dzj) / ((R)2 * dzj + dzjp1) - (dzjp2 + dzjp1) / ((R)2 * dzjp1 + dzj)) * (Tp1 - T0) -
-              dzj * (dzjm1 + dzj) / ((R)2 * dzj + dzjp1) * deltajp1 +
-              dzjp1 * (dzjp1 + dzjp2) / (dzj + (R)2 * dzjp1) * deltaj);
-    }
-    // --- limited parabola of layer nz: aL = tv(nz), aR = tv(nz+1)
-    R aL = tv, aR = shdn(tv);
-    R a0 = ldA(c, nz);
-    if ((aR - a0) * (a0 - aL) <= (R)0) { aL = a0; aR = a0; }
-    if ((aR - aL) * (a0 - (R)0.5 * (aL + aR)) > (aR - aL) * (aR - aL) / (R)6) aL = (R)3 * a0 - (R)2 * aR;
-    if ((aR - aL) * (a0 - (R)0.5 * (aR + aL)) < -(aR - aL) * (aR - aL) / (R)6) aR = (R)3 * a0 - (R)2 * aL;
-    R aj = (R)6 * (a0 - (R)0.5 * (aL + aR));
-    // parabola of layer nz-1 for downward flow through interface nz
-    R aLu = shup(aL), aRu = shup(aR), aju = shup(aj);
-    if (nz > nzmax) return fin;
-    R tvert = (R)0;
-    R w = c.W[nz - 1];
-    if (nz <= nzmax - 1 && w > (R)0) {
-        R x = fmn(w * dt / c.h[nz - 1], (R)1);
-        tvert = (-aL - (R)0.5 * x * (aR - aL + ((R)1 - (R)2 / (R)3 * x) * aj));
-        tvert = tvert * c.area[nz - 1] * w;
-    }
-    if (nz >= 2 && w < (R)0) {
-        R x = fmn(-w * dt / c.h[nz - 2], (R)1);
-        tvert = (-aRu + (R)0.5 * x * (aRu - aLu - ((R)1 - (R)2 / (R)3 * x) * aju));
-        tvert = tvert * c.area[nz - 1] * w;
-    }
-    if (nz == 1) tvert = -tv * c.W[0] * c.area[0];  // surface: -tv(nzmin)*W*area
-    if (nz == nzmax) tvert = (R)0;
-    return tvert - fin;
-}
-
-template <class R, int VER>
-__device__ __forceinline__ R ver_flux(const VCol<R> &c, R num_ord, R dt, R fin)
-{
-    if constexpr (VER == VER_QR4C) return vflux_qr4c(c, num_ord, fin);
-    else if constexpr (VER == VER_PPM) return vflux_ppm(c, dt, fin);
-    else if constexpr (VER == VER_CDIFF) return vflux_cdiff(c, fin);
-    else return vflux_upw1(c, fin);
+    const R A0 = (R)A[(size_t)em.n0 * S + k], A1 = (R)A[(size_t)em.n1 * S + k], A2 = (R)A[(size_t)em.n2 * S + k];
+    gx = g[0] * A0 + g[1] * A1 + g[2] * A2;
+    gy = g[3] * A0 + g[4] * A1 + g[5] * A2;
 }
 
 // ===========================================================================
-// MUSCL prerequisites: elemental gradient and nodal mean gradient
+// MUSCL prerequisite: area-weighted mean over the node's elements that have
+// layer nz of their elemental gradients (fill_up_dn_grad, lib/oce_muscl_adv.F90:397-407)
 // ===========================================================================
-template <class R, class G>
-__global__ __launch_bounds__(kBlock) void k_elem_grad(DMesh<R> m, const double *__restrict__ ttf,
-                                                      G *__restrict__ trxy, float gscale)
-{
-    const int el = wave_column();
-    if (el >= m.T) return;
-    const int k = lane_id(), nz = k + 1;
-    const int nlev = m.nlev_e[el];
-    if (nz > nlev - 1) return;
-    const int n0 = m.en[3 * el], n1 = m.en[3 * el + 1], n2 = m.en[3 * el + 2];
-    const R *g = m.gsca + 6 * el;
-    const R A0 = (R)ttf[(size_t)n0 * m.S + k], A1 = (R)ttf[(size_t)n1 * m.S + k], A2 = (R)ttf[(size_t)n2 * m.S + k];
-    R gx = g[0] * A0 + g[1] * A1 + g[2] * A2;
-    R gy = g[3] * A0 + g[4] * A1 + g[5] * A2;
-    G *o = trxy + ((size_t)el * m.S + k) * 2;
-    GradIO<G>::st(o, gx, gscale);
-    GradIO<G>::st(o + 1, gy, gscale);
-}
-
-// area-weighted mean over the node's elements that have layer nz
-// (inner loop of fill_up_dn_grad, lib/oce_muscl_adv.F90:397-407)
-template <class R, class G>
-__global__ __launch_bounds__(kBlock) void k_node_grad(DMesh<R> m, const G *__restrict__ trxy,
-                                                      G *__restrict__ gnod, float gscale, float ginv)
+template <class R, class G, int NT>
+__global__ __launch_bounds__(kBlock) void k_grad(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
+                                                 G *__restrict__ gnod, size_t tsg, float gscale)
 {
     const int n = wave_column();
     if (n >= m.N) return;
     const int k = lane_id(), nz = k + 1;
-    const int nlev = m.nlev_n[n];
-    if (nz > nlev - 1) return;
-    R tvol = 0, tx = 0, ty = 0;
-    const int j0 = m.n2t_ptr[n], j1 = m.n2t_ptr[n + 1];
-    for (int j = j0; j < j1; ++j) {
-        const int el = m.n2t[j];
-        if (m.nlev_e[el] - 1 < nz) continue;
-        const R a = m.earea[el];
-        const G *t = trxy + ((size_t)el * m.S + k) * 2;
+    if (nz > m.nlev_n[n] - 1) return;
+    const NodeElems ne = m.nelem[n];
+    R tvol = 0, tx[NT], ty[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) { tx[t] = 0; ty[t] = 0; }
+#pragma unroll
+    for (int j = 0; j < kDeg; ++j) {
+        const int el = ne.t[j];
+        if (el < 0) break;
+        const ElemMeta em = m.tmeta[el];
+        if (em.nlev - 1 < nz) continue;
+        const R *g = m.tgeo + 8 * (size_t)el;
+        const R a = g[6];
         tvol = tvol + a;
-        tx = tx + (R)GradIO<G>::ld(t, ginv) * a;
-        ty = ty + (R)GradIO<G>::ld(t + 1, ginv) * a;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            R gx, gy;
+            elem_gradient(em, g, ab + t * tsn, m.S, k, gx, gy);
+            tx[t] = tx[t] + gx * a;
+            ty[t] = ty[t] + gy * a;
+        }
     }
-    G *o = gnod + ((size_t)n * m.S + k) * 2;
-    GradIO<G>::st(o, tx / tvol, gscale);
-    GradIO<G>::st(o + 1, ty / tvol, gscale);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        G *o = gnod + t * tsg + ((size_t)n * m.S + k) * 2;
+        GradIO<G>::st(o, tx[t] / tvol, gscale);
+        GradIO<G>::st(o + 1, ty[t] / tvol, gscale);
+    }
 }
 
 // ===========================================================================
@@ -239,92 +93,126 @@ __global__ __launch_bounds__(kBlock) void k_node_grad(DMesh<R> m, const G *__res
 // MUSCL (:265-550), MFCT (:554-846). Segments C (both elements), D (el1
 // only) and E (el2 only); no cavities, so segments A/B are empty.
 // FCT: writes the LO flux (from ttf) and HO-LO (HO from ttfAB) per level.
-// No limiter: writes the HO flux.
+// No limiter: writes the HO flux. Gradients follow fill_up_dn_grad
+// (lib/oce_muscl_adv.F90:361-530): up/down-wind triangle on levels shared by
+// all elements around both nodes, nodal means below.
 // ===========================================================================
-template <class R, class G, int HOR, bool FCT>
-__global__ __launch_bounds__(kBlock) void k_edge_flux(DMesh<R> m, const R *__restrict__ vel,
-                                                      const double *__restrict__ ttf,
-                                                      const double *__restrict__ ttfAB,
-                                                      const G *__restrict__ trxy, const G *__restrict__ gnod,
-                                                      R num_ord, float ginv,
-                                                      R *__restrict__ flux_lo, R *__restrict__ flux)
+template <class R, class G, int HOR, bool FCT, int NT>
+__global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict__ vel,
+                                                 const double *__restrict__ ttf,
+                                                 const double *__restrict__ ttfAB, size_t tsn,
+                                                 const G *__restrict__ gnod, size_t tsg,
+                                                 R num_ord, float ginv,
+                                                 R *__restrict__ flux_lo, R *__restrict__ flux, size_t tse)
 {
     const int e = wave_column();
     if (e >= m.E) return;
     const int k = lane_id(), nz = k + 1;
-    const int2 nn = m.edges[e], tt = m.etri[e];
-    const int nl1 = m.nlev_e[tt.x] - 1;
-    const int nl2 = tt.y >= 0 ? m.nlev_e[tt.y] - 1 : 0;
-    const int nl12 = min(nl1, nl2), nlmax = max(nl1, nl2);
+    const EdgeMeta em = m.emeta[e];
+    const int nl1 = em.nl & 255, nl2 = (em.nl >> 8) & 255, nzmax_e = (em.nl >> 16) & 255;
+    const int nlmax = (int)((unsigned)em.nl >> 24);
+    const int nl12 = min(nl1, nl2);
     if (nz > nlmax) return;
-    const R dx1 = m.ecross[4 * e], dy1 = m.ecross[4 * e + 1];
-    const R dx2 = m.ecross[4 * e + 2], dy2 = m.ecross[4 * e + 3];
+    const R *geo = m.egeo + 8 * (size_t)e;
+    const R dx1 = geo[0], dy1 = geo[1], dx2 = geo[2], dy2 = geo[3];
     const bool has1 = nz <= nl1;
-    const bool has2 = tt.y >= 0 && (nz <= nl12 || nz > nl1);
+    const bool has2 = em.el2 >= 0 && (nz <= nl12 || nz > nl1);
     R vf1 = 0, vf2 = 0;
     if (has1) {
-        const R *u = vel + ((size_t)tt.x * m.S + k) * 2;
-        vf1 = (-u[1] * dx1 + u[0] * dy1) * m.helem[(size_t)tt.x * m.S + k];
+        const R *u = vel + ((size_t)em.el1 * m.S + k) * 2;
+        vf1 = (-u[1] * dx1 + u[0] * dy1) * m.helem[(size_t)em.el1 * m.S + k];
     }
     if (has2) {
-        const R *u = vel + ((size_t)tt.y * m.S + k) * 2;
-        vf2 = (u[1] * dx2 - u[0] * dy2) * m.helem[(size_t)tt.y * m.S + k];
+        const R *u = vel + ((size_t)em.el2 * m.S + k) * 2;
+        vf2 = (u[1] * dx2 - u[0] * dy2) * m.helem[(size_t)em.el2 * m.S + k];
     }
     const R vflux = (has1 && has2) ? vf1 + vf2 : (has1 ? vf1 : vf2);
     const size_t o = (size_t)e * m.S + k;
+    const size_t o1 = (size_t)em.n1 * m.S + k, o2 = (size_t)em.n2 * m.S + k;
 
-    R lo = 0;
-    if constexpr (FCT) {
-        const R T1 = (R)ttf[(size_t)nn.x * m.S + k], T2 = (R)ttf[(size_t)nn.y * m.S + k];
-        lo = (R)(-0.5) * (T1 * (vflux + fabs(vflux)) + T2 * (vflux - fabs(vflux))) - (R)0;
-        flux_lo[o] = lo;
-    }
-    const R A1 = (R)ttfAB[(size_t)nn.x * m.S + k], A2 = (R)ttfAB[(size_t)nn.y * m.S + k];
-    R ho;
-    if constexpr (HOR == HOR_UPW1) {
-        ho = (R)(-0.5) * (A1 * (vflux + fabs(vflux)) + A2 * (vflux - fabs(vflux)));
-    } else {
-        // edge_up_dn_grad as fill_up_dn_grad builds it (lib/oce_muscl_adv.F90:361-530)
-        const int2 ud = m.updn[e];
-        const int nzmax_e = min(m.nlev_n_min[nn.x], m.nlev_n_min[nn.y]);
-        R g1x, g1y, g2x, g2y;
-        if (ud.x >= 0 && ud.y >= 0 && nz <= nzmax_e - 1) {
-            const G *a = trxy + ((size_t)ud.x * m.S + k) * 2;
-            const G *b = trxy + ((size_t)ud.y * m.S + k) * 2;
-            g1x = (R)GradIO<G>::ld(a, ginv); g1y = (R)GradIO<G>::ld(a + 1, ginv);
-            g2x = (R)GradIO<G>::ld(b, ginv); g2y = (R)GradIO<G>::ld(b + 1, ginv);
-        } else {
-            const G *a = gnod + ((size_t)nn.x * m.S + k) * 2;
-            const G *b = gnod + ((size_t)nn.y * m.S + k) * 2;
-            g1x = (R)GradIO<G>::ld(a, ginv); g1y = (R)GradIO<G>::ld(a + 1, ginv);
-            g2x = (R)GradIO<G>::ld(b, ginv); g2y = (R)GradIO<G>::ld(b + 1, ginv);
-        }
-        R clo1 = 1, clo2 = 1;
+    // MUSCL/MFCT: reconstruction geometry, gradient source per level
+    R ex = 0, ey = 0, a = 0, clo1 = 1, clo2 = 1;
+    bool shared = false;
+    ElemMeta tu{}, td{};
+    const R *gu = nullptr, *gd = nullptr;
+    if constexpr (HOR != HOR_UPW1) {
+        ex = geo[4]; ey = geo[5]; a = geo[6];
         if constexpr (HOR == HOR_MUSCL) {
-            clo1 = (m.nbl[nn.x] - nz) >= 0 ? (R)1 : (R)0;
-            clo2 = (m.nbl[nn.y] - nz) >= 0 ? (R)1 : (R)0;
+            clo1 = ((em.nb & 0xFFFF) - nz) >= 0 ? (R)1 : (R)0;
+            clo2 = ((em.nb >> 16) - nz) >= 0 ? (R)1 : (R)0;
         }
-        const R a = m.edge_a[e], ex = m.edxdy[2 * e], ey = m.edxdy[2 * e + 1];
-        const R Tmean2 = A2 - ((R)2 * (A2 - A1) + ex * a * g2x + ey * (R)kRE * g2y) / (R)6 * clo2;
-        const R Tmean1 = A1 + ((R)2 * (A2 - A1) + ex * a * g1x + ey * (R)kRE * g1y) / (R)6 * clo1;
-        const R cHO = (vflux + fabs(vflux)) * Tmean1 + (vflux - fabs(vflux)) * Tmean2;
-        ho = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux * num_ord * (R)0.5 * (Tmean1 + Tmean2);
+        const bool updn = em.up >= 0 && em.dn >= 0;
+        shared = updn && nz <= nzmax_e - 1;
+        const int up = updn ? em.up : 0, dn = updn ? em.dn : 0;
+        tu = m.tmeta[up];
+        td = m.tmeta[dn];
+        gu = m.tgeo + 8 * (size_t)up;
+        gd = m.tgeo + 8 * (size_t)dn;
     }
-    flux[o] = ho - lo;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        R lo = 0;
+        if constexpr (FCT) {
+            const R T1 = (R)ttf[t * tsn + o1], T2 = (R)ttf[t * tsn + o2];
+            lo = (R)(-0.5) * (T1 * (vflux + fabs(vflux)) + T2 * (vflux - fabs(vflux))) - (R)0;
+            flux_lo[t * tse + o] = lo;
+        }
+        const double *AB = ttfAB + t * tsn;
+        const R A1 = (R)AB[o1], A2 = (R)AB[o2];
+        R ho;
+        if constexpr (HOR == HOR_UPW1) {
+            ho = (R)(-0.5) * (A1 * (vflux + fabs(vflux)) + A2 * (vflux - fabs(vflux)));
+        } else {
+            R g1x, g1y, g2x, g2y;
+            if (shared) {
+                elem_gradient(tu, gu, AB, m.S, k, g1x, g1y);
+                elem_gradient(td, gd, AB, m.S, k, g2x, g2y);
+            } else {
+                const G *p1 = gnod + t * tsg + o1 * 2, *p2 = gnod + t * tsg + o2 * 2;
+                g1x = (R)GradIO<G>::ld(p1, ginv); g1y = (R)GradIO<G>::ld(p1 + 1, ginv);
+                g2x = (R)GradIO<G>::ld(p2, ginv); g2y = (R)GradIO<G>::ld(p2 + 1, ginv);
+            }
+            const R Tmean2 = A2 - ((R)2 * (A2 - A1) + ex * a * g2x + ey * (R)kRE * g2y) / (R)6 * clo2;
+            const R Tmean1 = A1 + ((R)2 * (A2 - A1) + ex * a * g1x + ey * (R)kRE * g1y) / (R)6 * clo1;
+            const R cHO = (vflux + fabs(vflux)) * Tmean1 + (vflux - fabs(vflux)) * Tmean2;
+            ho = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux * num_ord * (R)0.5 * (Tmean1 + Tmean2);
+        }
+        flux[t * tse + o] = ho - lo;
+    }
+}
+
+// decoded incident-edge list of a node, valid for this lane's layer
+struct NodeEdgeList {
+    int eid[kDeg];
+    bool pos[kDeg];
+    bool on[kDeg];
+};
+
+template <class R>
+__device__ __forceinline__ void load_edge_list(const DMesh<R> &m, int n, int nz, bool layer, NodeEdgeList &L)
+{
+    const NodeEdges ne = m.nedge[n];
+    const NodeEdgeLev nl = m.nedgel[n];
+#pragma unroll
+    for (int j = 0; j < kDeg; ++j) {
+        const int se = ne.e[j];
+        L.eid[j] = (se > 0 ? se : -se) - 1;
+        L.pos[j] = se > 0;
+        L.on[j] = se != 0 && layer && nz <= nl.nl[j];
+    }
 }
 
 // ===========================================================================
-// FCT node kernels
-// ===========================================================================
 // LO solution (lib/oce_adv_tra_driver.F90:95-211), HO-LO vertical flux
 // (:257-283), and the raw antidiffusive sums P+/P- (lib/oce_adv_tra_fct.F90:232-346)
-template <class R, int VER>
+// ===========================================================================
+template <class R, int VER, int NT>
 __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restrict__ Wv,
                                                     const R *__restrict__ WEv,
                                                     const double *__restrict__ ttf,
-                                                    const double *__restrict__ ttfAB,
+                                                    const double *__restrict__ ttfAB, size_t tsn,
                                                     const R *__restrict__ flux_lo,
-                                                    const R *__restrict__ adf_h,
+                                                    const R *__restrict__ adf_h, size_t tse,
                                                     double dt, R num_ord,
                                                     double *__restrict__ LO, R *__restrict__ adf_v,
                                                     R *__restrict__ Pp, R *__restrict__ Pm)
@@ -335,54 +223,58 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
     const int nlev = m.nlev_n[n];
     const size_t c0 = (size_t)n * m.S;
     const bool layer = nz <= nlev - 1;
+    NodeEdgeList L;
+    load_edge_list(m, n, nz, layer, L);
 
-    // LO horizontal sum over incident edges, edge-index order
-    double slo = 0.0;
-    const int j0 = m.n2e_ptr[n], j1 = m.n2e_ptr[n + 1];
-    if (layer)
-        for (int j = j0; j < j1; ++j) {
-            const int se = m.n2e[j];
-            const int e = (se > 0 ? se : -se) - 1;
-            if (nz > m.edge_nlmax[e]) continue;
-            const double f = (double)flux_lo[(size_t)e * m.S + k];
-            slo = se > 0 ? slo + f : slo - f;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const size_t tn = t * tsn, te = t * tse;
+        // LO horizontal sum over incident edges, edge-index order
+        double slo = 0.0;
+        R fadf[kDeg];
+#pragma unroll
+        for (int j = 0; j < kDeg; ++j) {
+            fadf[j] = 0;
+            if (L.on[j]) {
+                const size_t oe = te + (size_t)L.eid[j] * m.S + k;
+                const double f = (double)flux_lo[oe];
+                fadf[j] = adf_h[oe];
+                slo = L.pos[j] ? slo + f : slo - f;
+            }
         }
-
-    // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
-    VCol<R> lc{ttf + c0, WEv + c0, m.area + c0, m.Zn + c0, m.zbarn + c0, m.hnode + c0, m.hnode_new + c0, nlev, nz};
-    const R fv_lo = vflux_upw1(lc, (R)0);
-    const R fv_lo_next = shdn(fv_lo);
-    VCol<R> hc{ttfAB + c0, Wv + c0, m.area + c0, m.Zn + c0, m.zbarn + c0, m.hnode + c0, m.hnode_new + c0, nlev, nz};
-    const R adfv = ver_flux<R, VER>(hc, num_ord, (R)dt, fv_lo);
-    const R adfv_next = shdn(adfv);
-
-    if (nz <= nlev) adf_v[c0 + k] = adfv;
-    if (!layer) return;
-
-    const double T = ttf[c0 + k];
-    const double lo = (T * (double)m.hnode[c0 + k] +
-                       (slo + (double)(fv_lo - fv_lo_next)) * dt / (double)m.areasvol[c0 + k]) /
-                      (double)m.hnode_new[c0 + k];
-    LO[c0 + k] = lo;
-
-    R pp = (R)0 + (fmx((R)0, adfv) + fmx((R)0, -adfv_next));
-    R pm = (R)0 + (fmn((R)0, adfv) + fmn((R)0, -adfv_next));
-    for (int j = j0; j < j1; ++j) {
-        const int se = m.n2e[j];
-        const int e = (se > 0 ? se : -se) - 1;
-        if (nz > m.edge_nlmax[e]) continue;
-        const R a = adf_h[(size_t)e * m.S + k];
-        if (se > 0) { pp = pp + fmx((R)0, a); pm = pm + fmn((R)0, a); }
-        else        { pp = pp + fmx((R)0, -a); pm = pm + fmn((R)0, -a); }
+        // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
+        VCol<R> lc{ttf + tn + c0, WEv + c0, m.area + c0, m.Zn + c0, m.zbarn + c0, m.hnode + c0, m.hnode_new + c0, nlev, nz};
+        const R fv_lo = vflux_upw1(lc, (R)0);
+        const R fv_lo_next = shdn(fv_lo);
+        VCol<R> hc{ttfAB + tn + c0, Wv + c0, m.area + c0, m.Zn + c0, m.zbarn + c0, m.hnode + c0, m.hnode_new + c0, nlev, nz};
+        const R adfv = ver_flux<R, VER>(hc, num_ord, (R)dt, fv_lo);
+        const R adfv_next = shdn(adfv);
+        if (nz <= nlev) adf_v[tn + c0 + k] = adfv;
+        if (layer) {
+            const double T = ttf[tn + c0 + k];
+            LO[tn + c0 + k] = (T * (double)m.hnode[c0 + k] +
+                               (slo + (double)(fv_lo - fv_lo_next)) * dt / (double)m.areasvol[c0 + k]) /
+                              (double)m.hnode_new[c0 + k];
+            R pp = (R)0 + (fmx((R)0, adfv) + fmx((R)0, -adfv_next));
+            R pm = (R)0 + (fmn((R)0, adfv) + fmn((R)0, -adfv_next));
+#pragma unroll
+            for (int j = 0; j < kDeg; ++j) {
+                if (!L.on[j]) continue;
+                const R a = fadf[j];
+                if (L.pos[j]) { pp = pp + fmx((R)0, a); pm = pm + fmn((R)0, a); }
+                else          { pp = pp + fmx((R)0, -a); pm = pm + fmn((R)0, -a); }
+            }
+            Pp[tn + c0 + k] = pp;
+            Pm[tn + c0 + k] = pm;
+        }
     }
-    Pp[c0 + k] = pp;
-    Pm[c0 + k] = pm;
 }
 
-// Zalesak bounds a1-a3 and limiting factors b2 (lib/oce_adv_tra_fct.F90:94-217,361-374)
-template <class R>
+// Zalesak bounds a1-a3 and limiting factors b2 (lib/oce_adv_tra_fct.F90:94-217,361-374);
+// R+/R- overwrite P+/P- in place
+template <class R, int NT>
 __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double *__restrict__ ttf,
-                                                       const double *__restrict__ LO,
+                                                       const double *__restrict__ LO, size_t tsn,
                                                        double dt, R *__restrict__ Pp_Rp,
                                                        R *__restrict__ Pm_Rm)
 {
@@ -393,64 +285,71 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
     const size_t c0 = (size_t)n * m.S;
     const bool layer = nz <= nlev - 1;
     const R bignumber = (R)1000;
+    const NodeElems ne = m.nelem[n];
 
-    R tvmax = 0, tvmin = 0;
-    if (layer) {
-        const int j0 = m.n2t_ptr[n], j1 = m.n2t_ptr[n + 1];
-        for (int j = j0; j < j1; ++j) {
-            const int el = m.n2t[j];
-            R emax, emin;
-            if (nz <= m.nlev_e[el] - 1) {
-                R mx[3], mi[3];
+    R tvmax[NT], tvmin[NT];
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const size_t o = (size_t)m.en[3 * el + q] * m.S + k;
-                    const R l = (R)LO[o], t = (R)ttf[o];
-                    mx[q] = fmx(l, t);
-                    mi[q] = fmn(l, t);
+    for (int t = 0; t < NT; ++t) { tvmax[t] = 0; tvmin[t] = 0; }
+    if (layer) {
+#pragma unroll
+        for (int j = 0; j < kDeg; ++j) {
+            const int el = ne.t[j];
+            if (el < 0) break;
+            const ElemMeta em = m.tmeta[el];
+            const bool wet = nz <= em.nlev - 1;
+            const size_t o0 = (size_t)em.n0 * m.S + k, o1 = (size_t)em.n1 * m.S + k, o2 = (size_t)em.n2 * m.S + k;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                R emax = -bignumber, emin = bignumber;
+                if (wet) {
+                    const size_t tn = t * tsn;
+                    const R l0 = (R)LO[tn + o0], l1 = (R)LO[tn + o1], l2 = (R)LO[tn + o2];
+                    const R t0 = (R)ttf[tn + o0], t1 = (R)ttf[tn + o1], t2 = (R)ttf[tn + o2];
+                    emax = fmx(fmx(fmx(l0, t0), fmx(l1, t1)), fmx(l2, t2));
+                    emin = fmn(fmn(fmn(l0, t0), fmn(l1, t1)), fmn(l2, t2));
                 }
-                emax = fmx(fmx(mx[0], mx[1]), mx[2]);
-                emin = fmn(fmn(mi[0], mi[1]), mi[2]);
-            } else {
-                emax = -bignumber;
-                emin = bignumber;
+                tvmax[t] = (j == 0) ? emax : fmx(tvmax[t], emax);
+                tvmin[t] = (j == 0) ? emin : fmn(tvmin[t], emin);
             }
-            tvmax = (j == j0) ? emax : fmx(tvmax, emax);
-            tvmin = (j == j0) ? emin : fmn(tvmin, emin);
         }
     }
-    const R up_max = shup(tvmax), dn_max = shdn(tvmax);
-    const R up_min = shup(tvmin), dn_min = shdn(tvmin);
-    if (!layer) return;
-    const R lo = (R)LO[c0 + k];
-    R qp, qm;
-    if (nz == 1 || nz == nlev - 1) {
-        qp = tvmax - lo;
-        qm = tvmin - lo;
-    } else {
-        qp = fmx(fmx(up_max, tvmax), dn_max) - lo;
-        qm = fmn(fmn(up_min, tvmin), dn_min) - lo;
-    }
     const R flux_eps = (R)1e-16;
-    const R av = m.areasvol[c0 + k];
-    R flux = Pp_Rp[c0 + k] * (R)dt / av + flux_eps;
-    const R rp = fmn((R)1, qp / flux);
-    flux = Pm_Rm[c0 + k] * (R)dt / av - flux_eps;
-    const R rm = fmn((R)1, qm / flux);
-    Pp_Rp[c0 + k] = rp;
-    Pm_Rm[c0 + k] = rm;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const R up_max = shup(tvmax[t]), dn_max = shdn(tvmax[t]);
+        const R up_min = shup(tvmin[t]), dn_min = shdn(tvmin[t]);
+        if (!layer) continue;
+        const size_t tn = t * tsn;
+        const R lo = (R)LO[tn + c0 + k];
+        R qp, qm;
+        if (nz == 1 || nz == nlev - 1) {
+            qp = tvmax[t] - lo;
+            qm = tvmin[t] - lo;
+        } else {
+            qp = fmx(fmx(up_max, tvmax[t]), dn_max) - lo;
+            qm = fmn(fmn(up_min, tvmin[t]), dn_min) - lo;
+        }
+        const R av = m.areasvol[c0 + k];
+        R flux = Pp_Rp[tn + c0 + k] * (R)dt / av + flux_eps;
+        const R rp = fmn((R)1, qp / flux);
+        flux = Pm_Rm[tn + c0 + k] * (R)dt / av - flux_eps;
+        const R rm = fmn((R)1, qm / flux);
+        Pp_Rp[tn + c0 + k] = rp;
+        Pm_Rm[tn + c0 + k] = rm;
+    }
 }
 
 // limiting b3 (lib/oce_adv_tra_fct.F90:394-471), flux divergence
 // (lib/oce_adv_tra_driver.F90:298-450) and the update
 // values += (dttf_h + dttf_v)/hnode_new (src/fesom_analytic.F90:419-437)
-template <class R>
+template <class R, int NT>
 __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__restrict__ ttf,
-                                                        const double *__restrict__ LO,
-                                                        const R *__restrict__ adf_h,
+                                                        const double *__restrict__ LO, size_t tsn,
+                                                        const R *__restrict__ adf_h, size_t tse,
                                                         const R *__restrict__ adf_v,
                                                         const R *__restrict__ Rp,
-                                                        const R *__restrict__ Rm, double dt)
+                                                        const R *__restrict__ Rm,
+                                                        const int *__restrict__ nother, double dt)
 {
     const int n = wave_column();
     if (n >= m.N) return;
@@ -458,58 +357,64 @@ __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__re
     const int nlev = m.nlev_n[n];
     const size_t c0 = (size_t)n * m.S;
     const bool layer = nz <= nlev - 1;
+    NodeEdgeList L;
+    load_edge_list(m, n, nz, layer, L);
+    int oth[kDeg];
+#pragma unroll
+    for (int j = 0; j < kDeg; ++j) oth[j] = nother[(size_t)n * kDeg + j];
 
-    R rp = 0, rm = 0, fv = 0;
-    if (nz <= nlev) fv = adf_v[c0 + k];
-    if (layer) { rp = Rp[c0 + k]; rm = Rm[c0 + k]; }
-    const R rp_up = shup(rp), rm_up = shup(rm);
-    R fvl = fv;
-    if (nz == 1) {
-        R ae = (R)1;
-        ae = fv >= (R)0 ? fmn(ae, rp) : fmn(ae, rm);
-        fvl = ae * fv;
-    } else if (layer) {
-        R ae = (R)1;
-        if (fv >= (R)0) { ae = fmn(ae, rm_up); ae = fmn(ae, rp); }
-        else            { ae = fmn(ae, rp_up); ae = fmn(ae, rm); }
-        fvl = ae * fv;
-    }
-    const R fvl_next = shdn(fvl);
-    if (!layer) return;
-
-    const double av = (double)m.areasvol[c0 + k];
-    double sh = 0.0;
-    const int j0 = m.n2e_ptr[n], j1 = m.n2e_ptr[n + 1];
-    for (int j = j0; j < j1; ++j) {
-        const int se = m.n2e[j];
-        const int e = (se > 0 ? se : -se) - 1;
-        if (nz > m.edge_nlmax[e]) continue;
-        const int2 nn = m.edges[e];
-        const R f = adf_h[(size_t)e * m.S + k];
-        R ae = (R)1;
-        if (f >= (R)0) {
-            ae = fmn(ae, Rp[(size_t)nn.x * m.S + k]);
-            ae = fmn(ae, Rm[(size_t)nn.y * m.S + k]);
-        } else {
-            ae = fmn(ae, Rm[(size_t)nn.x * m.S + k]);
-            ae = fmn(ae, Rp[(size_t)nn.y * m.S + k]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const size_t tn = t * tsn, te = t * tse;
+        R rp = 0, rm = 0, fv = 0;
+        if (nz <= nlev) fv = adf_v[tn + c0 + k];
+        if (layer) { rp = Rp[tn + c0 + k]; rm = Rm[tn + c0 + k]; }
+        const R rp_up = shup(rp), rm_up = shup(rm);
+        R fvl = fv;
+        if (nz == 1) {
+            R ae = (R)1;
+            ae = fv >= (R)0 ? fmn(ae, rp) : fmn(ae, rm);
+            fvl = ae * fv;
+        } else if (layer) {
+            R ae = (R)1;
+            if (fv >= (R)0) { ae = fmn(ae, rm_up); ae = fmn(ae, rp); }
+            else            { ae = fmn(ae, rp_up); ae = fmn(ae, rm); }
+            fvl = ae * fv;
         }
-        const double fl = (double)(ae * f);
-        sh = se > 0 ? sh + fl * dt / av : sh - fl * dt / av;
+        const R fvl_next = shdn(fvl);
+        if (!layer) continue;
+
+        const double av = (double)m.areasvol[c0 + k];
+        double sh = 0.0;
+#pragma unroll
+        for (int j = 0; j < kDeg; ++j) {
+            if (!L.on[j]) continue;
+            const R f = adf_h[te + (size_t)L.eid[j] * m.S + k];
+            const size_t oo = tn + (size_t)oth[j] * m.S + k;
+            const R rpo = Rp[oo], rmo = Rm[oo];
+            // edge orientation: this node is edges(1) when pos
+            const R rp1 = L.pos[j] ? rp : rpo, rm1 = L.pos[j] ? rm : rmo;
+            const R rp2 = L.pos[j] ? rpo : rp, rm2 = L.pos[j] ? rmo : rm;
+            R ae = (R)1;
+            if (f >= (R)0) { ae = fmn(ae, rp1); ae = fmn(ae, rm2); }
+            else           { ae = fmn(ae, rm1); ae = fmn(ae, rp2); }
+            const double fl = (double)(ae * f);
+            sh = L.pos[j] ? sh + fl * dt / av : sh - fl * dt / av;
+        }
+        const double T = ttf[tn + c0 + k];
+        double dv = 0.0 - T * (double)m.hnode[c0 + k] + LO[tn + c0 + k] * (double)m.hnode_new[c0 + k];
+        dv = dv + (double)(fvl - fvl_next) * dt / av;
+        const double del = (0.0 + sh) + dv;
+        ttf[tn + c0 + k] = T + del / (double)m.hnode_new[c0 + k];
     }
-    const double T = ttf[c0 + k];
-    double dv = 0.0 - T * (double)m.hnode[c0 + k] + LO[c0 + k] * (double)m.hnode_new[c0 + k];
-    dv = dv + (double)(fvl - fvl_next) * dt / av;
-    const double del = (0.0 + sh) + dv;
-    ttf[c0 + k] = T + del / (double)m.hnode_new[c0 + k];
 }
 
 // lim = 'NONE': HO fluxes straight into the divergence (driver :291)
-template <class R, int VER>
+template <class R, int VER, int NT>
 __global__ __launch_bounds__(kBlock) void k_node_nolim(DMesh<R> m, const R *__restrict__ WEv,
                                                        double *__restrict__ ttf,
-                                                       const double *__restrict__ ttfAB,
-                                                       const R *__restrict__ flux_h,
+                                                       const double *__restrict__ ttfAB, size_t tsn,
+                                                       const R *__restrict__ flux_h, size_t tse,
                                                        double dt, R num_ord)
 {
     const int n = wave_column();
@@ -518,24 +423,28 @@ __global__ __launch_bounds__(kBlock) void k_node_nolim(DMesh<R> m, const R *__re
     const int nlev = m.nlev_n[n];
     const size_t c0 = (size_t)n * m.S;
     const bool layer = nz <= nlev - 1;
-    VCol<R> hc{ttfAB + c0, WEv + c0, m.area + c0, m.Zn + c0, m.zbarn + c0, m.hnode + c0, m.hnode_new + c0, nlev, nz};
-    const R fv = ver_flux<R, VER>(hc, num_ord, (R)dt, (R)0);
-    const R fv_next = shdn(fv);
-    if (!layer) return;
-    const double av = (double)m.areasvol[c0 + k];
-    double sh = 0.0;
-    const int j0 = m.n2e_ptr[n], j1 = m.n2e_ptr[n + 1];
-    for (int j = j0; j < j1; ++j) {
-        const int se = m.n2e[j];
-        const int e = (se > 0 ? se : -se) - 1;
-        if (nz > m.edge_nlmax[e]) continue;
-        const double f = (double)flux_h[(size_t)e * m.S + k];
-        sh = se > 0 ? sh + f * dt / av : sh - f * dt / av;
+    NodeEdgeList L;
+    load_edge_list(m, n, nz, layer, L);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const size_t tn = t * tsn, te = t * tse;
+        VCol<R> hc{ttfAB + tn + c0, WEv + c0, m.area + c0, m.Zn + c0, m.zbarn + c0, m.hnode + c0, m.hnode_new + c0, nlev, nz};
+        const R fv = ver_flux<R, VER>(hc, num_ord, (R)dt, (R)0);
+        const R fv_next = shdn(fv);
+        if (!layer) continue;
+        const double av = (double)m.areasvol[c0 + k];
+        double sh = 0.0;
+#pragma unroll
+        for (int j = 0; j < kDeg; ++j) {
+            if (!L.on[j]) continue;
+            const double f = (double)flux_h[te + (size_t)L.eid[j] * m.S + k];
+            sh = L.pos[j] ? sh + f * dt / av : sh - f * dt / av;
+        }
+        const double dv = 0.0 + (double)(fv - fv_next) * dt / av;
+        const double del = (0.0 + sh) + dv;
+        const double T = ttf[tn + c0 + k];
+        ttf[tn + c0 + k] = T + del / (double)m.hnode_new[c0 + k];
     }
-    const double dv = 0.0 + (double)(fv - fv_next) * dt / av;
-    const double del = (0.0 + sh) + dv;
-    const double T = ttf[c0 + k];
-    ttf[c0 + k] = T + del / (double)m.hnode_new[c0 + k];
 }
 
 // ===========================================================================
@@ -623,12 +532,6 @@ std::vector<T> pad_cols(const double *src, int L, int ncol, int S, int comps = 1
     return out;
 }
 
-template <class T>
-std::vector<T> conv(const std::vector<double> &v)
-{
-    return std::vector<T>(v.begin(), v.end());
-}
-
 inline unsigned grid_for(int ncols)
 {
     unsigned nb = (unsigned)((ncols + kWavesPerBlock - 1) / kWavesPerBlock);
@@ -641,22 +544,22 @@ struct Engine::Impl {
     std::vector<void *> allocs;
     template <class T> T *track(T *p) { allocs.push_back((void *)p); return p; }
 
-    // mesh (typed by precision)
-    void *dm = nullptr;  // DMesh<R>* host copy
     DMesh<double> m64{};
     DMesh<float> m32{};
+    int *nother = nullptr;  // (kDeg,N) other endpoint of each incident edge
     // velocity
     void *vel = nullptr, *W = nullptr, *WE = nullptr;
-    // tracers
-    std::vector<double *> values, valuesAB;
-    // work
-    void *flux_lo = nullptr, *adf_h = nullptr, *adf_v = nullptr, *Pp = nullptr, *Pm = nullptr;
-    double *LO = nullptr;
-    void *trxy = nullptr, *gnod = nullptr;
+    // tracers [ntr][S*N]
+    double *values = nullptr, *valuesAB = nullptr;
+    // work, per tracer
+    void *flux_lo = nullptr, *adf_h = nullptr;   // [ntr][S*E]
+    void *adf_v = nullptr, *Pp = nullptr, *Pm = nullptr;  // [ntr][S*N]
+    double *LO = nullptr;                        // [ntr][S*N]
+    void *gnod = nullptr;                        // [ntr][2*S*N]
     DiagAcc *dpart = nullptr, *dout = nullptr;
     int ndiag = 1024;
     float gscale = 1.0f, ginv = 1.0f;
-    const HostMesh *host = nullptr;
+    size_t tsn = 0, tse = 0, tsg = 0;
 };
 
 template <class R>
@@ -664,43 +567,66 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, int S)
 {
     const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
     d.N = N; d.T = T; d.E = E; d.nl = nl; d.S = S;
-    d.en = I.track(dupload(m.elem2D_nodes));
-    std::vector<int2> ed(E), et(E), ud(E);
-    std::vector<int> nlmax(E), ea_dummy;
-    std::vector<double> ea(E);
+    std::vector<EdgeMeta> em(E);
+    std::vector<R> eg(8 * (size_t)E, (R)0);
     for (int e = 0; e < E; ++e) {
-        ed[e] = make_int2(m.edges[2 * e], m.edges[2 * e + 1]);
-        et[e] = make_int2(m.edge_tri[2 * e], m.edge_tri[2 * e + 1]);
-        ud[e] = make_int2(m.edge_up_dn_tri[2 * e], m.edge_up_dn_tri[2 * e + 1]);
-        int nl1 = m.nlevels[et[e].x] - 1, nl2 = et[e].y >= 0 ? m.nlevels[et[e].y] - 1 : 0;
-        nlmax[e] = std::max(nl1, nl2);
-        double a = kRE * m.elem_cos[et[e].x];
-        if (et[e].y >= 0) a = 0.5 * (a + kRE * m.elem_cos[et[e].y]);
-        ea[e] = a;
+        EdgeMeta &r = em[e];
+        r.n1 = m.edges[2 * e]; r.n2 = m.edges[2 * e + 1];
+        r.el1 = m.edge_tri[2 * e]; r.el2 = m.edge_tri[2 * e + 1];
+        r.up = m.edge_up_dn_tri[2 * e]; r.dn = m.edge_up_dn_tri[2 * e + 1];
+        int nl1 = m.nlevels[r.el1] - 1, nl2 = r.el2 >= 0 ? m.nlevels[r.el2] - 1 : 0;
+        int nzmax_e = std::min(m.nlevels_nod2D_min[r.n1], m.nlevels_nod2D_min[r.n2]);
+        int nlmax = std::max(nl1, nl2);
+        r.nl = nl1 | (nl2 << 8) | (nzmax_e << 16) | (nlmax << 24);
+        r.nb = (m.nboundary_lay[r.n1] & 0xFFFF) | ((m.nboundary_lay[r.n2] & 0xFFFF) << 16);
+        double a = kRE * m.elem_cos[r.el1];
+        if (r.el2 >= 0) a = 0.5 * (a + kRE * m.elem_cos[r.el2]);
+        R *g = &eg[8 * (size_t)e];
+        for (int q = 0; q < 4; ++q) g[q] = (R)m.edge_cross_dxdy[4 * e + q];
+        g[4] = (R)m.edge_dxdy[2 * e];
+        g[5] = (R)m.edge_dxdy[2 * e + 1];
+        g[6] = (R)a;
     }
-    d.edges = I.track(dupload(ed));
-    d.etri = I.track(dupload(et));
-    d.updn = I.track(dupload(ud));
-    d.nlev_e = I.track(dupload(m.nlevels));
+    d.emeta = I.track(dupload(em));
+    d.egeo = I.track(dupload(eg));
+    std::vector<ElemMeta> tm(T);
+    std::vector<R> tg(8 * (size_t)T, (R)0);
+    for (int el = 0; el < T; ++el) {
+        tm[el] = ElemMeta{m.elem2D_nodes[3 * el], m.elem2D_nodes[3 * el + 1], m.elem2D_nodes[3 * el + 2], m.nlevels[el]};
+        for (int q = 0; q < 6; ++q) tg[8 * (size_t)el + q] = (R)m.gradient_sca[6 * el + q];
+        tg[8 * (size_t)el + 6] = (R)m.elem_area[el];
+    }
+    d.tmeta = I.track(dupload(tm));
+    d.tgeo = I.track(dupload(tg));
     d.nlev_n = I.track(dupload(m.nlevels_nod2D));
-    d.nlev_n_min = I.track(dupload(m.nlevels_nod2D_min));
-    d.nbl = I.track(dupload(m.nboundary_lay));
-    d.edge_nlmax = I.track(dupload(nlmax));
-    d.n2e_ptr = I.track(dupload(m.n2e_ptr));
-    d.n2e = I.track(dupload(m.n2e));
-    std::vector<int> tp(N + 1, 0), tl;
-    tl.reserve((size_t)N * m.maxAdj);
+    std::vector<NodeEdges> ne(N);
+    std::vector<NodeEdgeLev> nel(N);
+    std::vector<NodeElems> nt(N);
+    std::vector<int> oth((size_t)N * kDeg, 0);
     for (int n = 0; n < N; ++n) {
-        for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) tl.push_back(m.nod_in_elem2D[(size_t)n * m.maxAdj + j]);
-        tp[n + 1] = (int)tl.size();
+        const int deg = m.n2e_ptr[n + 1] - m.n2e_ptr[n];
+        if (deg > kDeg || m.nod_in_elem2D_num[n] > kDeg)
+            throw std::runtime_error("node degree above " + std::to_string(kDeg) + " is not supported");
+        for (int j = 0; j < kDeg; ++j) {
+            ne[n].e[j] = 0;
+            nel[n].nl[j] = 0;
+            nt[n].t[j] = -1;
+        }
+        for (int j = 0; j < deg; ++j) {
+            int se = m.n2e[m.n2e_ptr[n] + j];
+            int e = (se > 0 ? se : -se) - 1;
+            ne[n].e[j] = se;
+            nel[n].nl[j] = (int)((unsigned)em[e].nl >> 24);
+            oth[(size_t)n * kDeg + j] = se > 0 ? em[e].n2 : em[e].n1;
+        }
+        for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) nt[n].t[j] = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
     }
-    d.n2t_ptr = I.track(dupload(tp));
-    d.n2t = I.track(dupload(tl));
-    d.ecross = I.track(dupload(conv<R>(m.edge_cross_dxdy)));
-    d.edxdy = I.track(dupload(conv<R>(m.edge_dxdy)));
-    d.edge_a = I.track(dupload(conv<R>(ea)));
-    d.earea = I.track(dupload(conv<R>(m.elem_area)));
-    d.gsca = I.track(dupload(conv<R>(m.gradient_sca)));
+    d.nedge = I.track(dupload(ne));
+    d.nedgel = I.track(dupload(nel));
+    d.nelem = I.track(dupload(nt));
+    if (!I.nother) I.nother = I.track(dupload(oth));
+    auto cv = [](const std::vector<double> &v) { return std::vector<R>(v.begin(), v.end()); };
+    (void)cv;
     d.area = I.track(dupload(pad_cols<R>(m.area.data(), nl, N, S)));
     d.areasvol = I.track(dupload(pad_cols<R>(m.areasvol.data(), nl, N, S)));
     d.hnode = I.track(dupload(pad_cols<R>(m.hnode.data(), nl - 1, N, S)));
@@ -720,33 +646,31 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     S_ = std::max(8, (nl_ + 7) / 8 * 8);
     spec_.assign(ntr_, TracerSpec());
     Impl &I = *p_;
-    I.host = &m;
     HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
     const size_t gs = prec_ == PREC_FP64 ? sizeof(double) : (prec_ == PREC_FP32 ? sizeof(float) : sizeof(__half));
     if (prec_ == PREC_FP64) build_dmesh(I, I.m64, m, S_);
     else build_dmesh(I, I.m32, m, S_);
-    auto ralloc = [&](size_t n) { return I.track((void *)dalloc<char>(n * rs)); };
-    I.vel = ralloc(2 * (size_t)S_ * T_);
-    I.W = ralloc((size_t)S_ * N_);
-    I.WE = ralloc((size_t)S_ * N_);
-    HIP_OK(hipMemset(I.vel, 0, 2 * (size_t)S_ * T_ * rs));
-    HIP_OK(hipMemset(I.W, 0, (size_t)S_ * N_ * rs));
-    HIP_OK(hipMemset(I.WE, 0, (size_t)S_ * N_ * rs));
-    for (int t = 0; t < ntr_; ++t) {
-        I.values.push_back(I.track(dalloc<double>((size_t)S_ * N_)));
-        I.valuesAB.push_back(I.track(dalloc<double>((size_t)S_ * N_)));
-        HIP_OK(hipMemset(I.values.back(), 0, (size_t)S_ * N_ * sizeof(double)));
-        HIP_OK(hipMemset(I.valuesAB.back(), 0, (size_t)S_ * N_ * sizeof(double)));
-    }
-    I.flux_lo = ralloc((size_t)S_ * E_);
-    I.adf_h = ralloc((size_t)S_ * E_);
-    I.adf_v = ralloc((size_t)S_ * N_);
-    I.Pp = ralloc((size_t)S_ * N_);
-    I.Pm = ralloc((size_t)S_ * N_);
-    I.LO = I.track(dalloc<double>((size_t)S_ * N_));
-    I.trxy = I.track((void *)dalloc<char>(2 * (size_t)S_ * T_ * gs));
-    I.gnod = I.track((void *)dalloc<char>(2 * (size_t)S_ * N_ * gs));
+    I.tsn = (size_t)S_ * N_;
+    I.tse = (size_t)S_ * E_;
+    I.tsg = 2 * (size_t)S_ * N_;
+    auto zalloc = [&](size_t bytes) {
+        void *p = I.track((void *)dalloc<char>(bytes));
+        HIP_OK(hipMemset(p, 0, bytes));
+        return p;
+    };
+    I.vel = zalloc(2 * (size_t)S_ * T_ * rs);
+    I.W = zalloc((size_t)S_ * N_ * rs);
+    I.WE = zalloc((size_t)S_ * N_ * rs);
+    I.values = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
+    I.valuesAB = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
+    I.flux_lo = zalloc(I.tse * ntr_ * rs);
+    I.adf_h = zalloc(I.tse * ntr_ * rs);
+    I.adf_v = zalloc(I.tsn * ntr_ * rs);
+    I.Pp = zalloc(I.tsn * ntr_ * rs);
+    I.Pm = zalloc(I.tsn * ntr_ * rs);
+    I.LO = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
+    I.gnod = zalloc(I.tsg * ntr_ * gs);
     I.dpart = I.track(dalloc<DiagAcc>(I.ndiag));
     I.dout = I.track(dalloc<DiagAcc>(1));
     // FP16 gradient scale: mean edge length (gradient * length = tracer increment)
@@ -769,40 +693,40 @@ void Engine::set_velocity(const double *uv, const double *w, const double *we)
 {
     Impl &I = *p_;
     const int L = nl_ - 1;
-    if (prec_ == PREC_FP64) {
-        auto v = pad_cols<double>(uv, L, T_, S_, 2);
-        HIP_OK(hipMemcpy(I.vel, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
-        auto a = pad_cols<double>(w, nl_, N_, S_);
-        HIP_OK(hipMemcpy(I.W, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
-        auto b = pad_cols<double>(we, nl_, N_, S_);
-        HIP_OK(hipMemcpy(I.WE, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
-    } else {
-        auto v = pad_cols<float>(uv, L, T_, S_, 2);
-        HIP_OK(hipMemcpy(I.vel, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice));
-        auto a = pad_cols<float>(w, nl_, N_, S_);
-        HIP_OK(hipMemcpy(I.W, a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice));
-        auto b = pad_cols<float>(we, nl_, N_, S_);
-        HIP_OK(hipMemcpy(I.WE, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
-    }
+    auto put = [&](void *dst, const double *src, int Lc, int ncol, int comps) {
+        if (prec_ == PREC_FP64) {
+            auto v = pad_cols<double>(src, Lc, ncol, S_, comps);
+            HIP_OK(hipMemcpy(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            auto v = pad_cols<float>(src, Lc, ncol, S_, comps);
+            HIP_OK(hipMemcpy(dst, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+    };
+    put(I.vel, uv, L, T_, 2);
+    put(I.W, w, nl_, N_, 1);
+    put(I.WE, we, nl_, N_, 1);
 }
 
 void Engine::set_values(int tr, const double *v)
 {
+    if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     auto a = pad_cols<double>(v, nl_ - 1, N_, S_);
-    HIP_OK(hipMemcpy(p_->values.at(tr), a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(p_->values + tr * p_->tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
 void Engine::set_valuesAB(int tr, const double *v)
 {
+    if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     auto a = pad_cols<double>(v, nl_ - 1, N_, S_);
-    HIP_OK(hipMemcpy(p_->valuesAB.at(tr), a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(p_->valuesAB + tr * p_->tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
 void Engine::get_values(int tr, double *v)
 {
+    if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     std::vector<double> a((size_t)S_ * N_);
     HIP_OK(hipStreamSynchronize(stream_));
-    HIP_OK(hipMemcpy(a.data(), p_->values.at(tr), a.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(a.data(), p_->values + tr * p_->tsn, a.size() * sizeof(double), hipMemcpyDeviceToHost));
     const int L = nl_ - 1;
     for (int n = 0; n < N_; ++n)
         for (int k = 0; k < L; ++k) v[(size_t)n * L + k] = a[(size_t)n * S_ + k];
@@ -820,25 +744,27 @@ struct LaunchHook {
     }
 };
 
-template <class R, class G>
-static void launch_step(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &sp, double dt,
-                        double *vals, const double *ab, hipStream_t st, LaunchHook hk = LaunchHook())
+// one step of the NT tracers [t0, t0+NT), all with schemes sp
+template <class R, class G, int NT>
+static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &sp, double dt, int t0, bool refresh,
+                         hipStream_t st, LaunchHook &hk)
 {
-    hk.mark("start", st);
     const R *vel = (const R *)I.vel, *W = (const R *)I.W, *WE = (const R *)I.WE;
-    R *flux_lo = (R *)I.flux_lo, *adf_h = (R *)I.adf_h, *adf_v = (R *)I.adf_v, *Pp = (R *)I.Pp, *Pm = (R *)I.Pm;
-    G *trxy = (G *)I.trxy, *gnod = (G *)I.gnod;
-    const dim3 blk(kBlock);
+    const size_t tsn = I.tsn, tse = I.tse, tsg = I.tsg;
+    double *vals = I.values + t0 * tsn;
+    const double *ab = refresh ? vals : I.valuesAB + t0 * tsn;
+    R *flux_lo = (R *)I.flux_lo + t0 * tse, *adf_h = (R *)I.adf_h + t0 * tse;
+    R *adf_v = (R *)I.adf_v + t0 * tsn, *Pp = (R *)I.Pp + t0 * tsn, *Pm = (R *)I.Pm + t0 * tsn;
+    double *LO = I.LO + t0 * tsn;
+    G *gnod = (G *)I.gnod + t0 * tsg;
+    const dim3 blk(kBlock), ge(grid_for(m.E)), gn(grid_for(m.N));
     const bool fct = sp.lim == LIM_FCT;
-    if (sp.hor == HOR_MUSCL || sp.hor == HOR_MFCT) {
-        hipLaunchKernelGGL((k_elem_grad<R, G>), dim3(grid_for(m.T)), blk, 0, st, m, ab, trxy, I.gscale);
-        hk.mark("k_elem_grad", st);
-        hipLaunchKernelGGL((k_node_grad<R, G>), dim3(grid_for(m.N)), blk, 0, st, m, trxy, gnod, I.gscale, I.ginv);
-        hk.mark("k_node_grad", st);
-    }
-    const dim3 ge(grid_for(m.E)), gn(grid_for(m.N));
     const R ph = (R)sp.ph, pv = (R)sp.pv;
-#define EDGE(HOR, F) hipLaunchKernelGGL((k_edge_flux<R, G, HOR, F>), ge, blk, 0, st, m, vel, vals, ab, trxy, gnod, ph, I.ginv, flux_lo, adf_h)
+    if (sp.hor == HOR_MUSCL || sp.hor == HOR_MFCT) {
+        hipLaunchKernelGGL((k_grad<R, G, NT>), gn, blk, 0, st, m, ab, tsn, gnod, tsg, I.gscale);
+        hk.mark("k_grad", st);
+    }
+#define EDGE(HOR, F) hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT>), ge, blk, 0, st, m, vel, vals, ab, tsn, gnod, tsg, ph, I.ginv, flux_lo, adf_h, tse)
     if (fct) {
         if (sp.hor == HOR_MUSCL) EDGE(HOR_MUSCL, true);
         else if (sp.hor == HOR_MFCT) EDGE(HOR_MFCT, true);
@@ -849,9 +775,9 @@ static void launch_step(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &sp
         else EDGE(HOR_UPW1, false);
     }
 #undef EDGE
-    hk.mark("k_edge_flux", st);
+    hk.mark("k_edge", st);
     if (fct) {
-#define NLO(V) hipLaunchKernelGGL((k_node_lo<R, V>), gn, blk, 0, st, m, W, WE, vals, ab, flux_lo, adf_h, dt, pv, I.LO, adf_v, Pp, Pm)
+#define NLO(V) hipLaunchKernelGGL((k_node_lo<R, V, NT>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, flux_lo, adf_h, tse, dt, pv, LO, adf_v, Pp, Pm)
         switch (sp.ver) {
         case VER_QR4C: NLO(VER_QR4C); break;
         case VER_PPM: NLO(VER_PPM); break;
@@ -860,12 +786,13 @@ static void launch_step(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &sp
         }
 #undef NLO
         hk.mark("k_node_lo", st);
-        hipLaunchKernelGGL((k_node_limit<R>), gn, blk, 0, st, m, vals, I.LO, dt, Pp, Pm);
+        hipLaunchKernelGGL((k_node_limit<R, NT>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm);
         hk.mark("k_node_limit", st);
-        hipLaunchKernelGGL((k_node_update<R>), gn, blk, 0, st, m, vals, I.LO, adf_h, adf_v, Pp, Pm, dt);
+        hipLaunchKernelGGL((k_node_update<R, NT>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v, Pp, Pm,
+                           I.nother, dt);
         hk.mark("k_node_update", st);
     } else {
-#define NNL(V) hipLaunchKernelGGL((k_node_nolim<R, V>), gn, blk, 0, st, m, WE, vals, ab, adf_h, dt, pv)
+#define NNL(V) hipLaunchKernelGGL((k_node_nolim<R, V, NT>), gn, blk, 0, st, m, WE, vals, ab, tsn, adf_h, tse, dt, pv)
         switch (sp.ver) {
         case VER_QR4C: NNL(VER_QR4C); break;
         case VER_PPM: NNL(VER_PPM); break;
@@ -878,17 +805,42 @@ static void launch_step(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &sp
     HIP_OK(hipGetLastError());
 }
 
+static bool same_spec(const TracerSpec &a, const TracerSpec &b)
+{
+    return a.hor == b.hor && a.ver == b.ver && a.lim == b.lim && a.ph == b.ph && a.pv == b.pv;
+}
+
+template <class R, class G>
+static void launch_range(Engine::Impl &I, const DMesh<R> &m, const std::vector<TracerSpec> &spec, double dt, int first,
+                         int count, bool refresh, hipStream_t st, LaunchHook &hk)
+{
+    // runs of consecutive tracers with identical schemes, in batches of 4/2/1
+    int t = first;
+    const int end = first + count;
+    while (t < end) {
+        int run = 1;
+        while (t + run < end && same_spec(spec[t], spec[t + run])) ++run;
+        while (run > 0) {
+            hk.mark("start", st);
+            if (run >= 4) { launch_batch<R, G, 4>(I, m, spec[t], dt, t, refresh, st, hk); t += 4; run -= 4; }
+            else if (run >= 2) { launch_batch<R, G, 2>(I, m, spec[t], dt, t, refresh, st, hk); t += 2; run -= 2; }
+            else { launch_batch<R, G, 1>(I, m, spec[t], dt, t, refresh, st, hk); t += 1; run -= 1; }
+        }
+    }
+}
+
+static void enqueue(Engine::Impl &I, int prec, const std::vector<TracerSpec> &spec, double dt, int first, int count,
+                    bool refresh, hipStream_t st, LaunchHook hk = LaunchHook())
+{
+    if (prec == PREC_FP64) launch_range<double, double>(I, I.m64, spec, dt, first, count, refresh, st, hk);
+    else if (prec == PREC_FP32) launch_range<float, float>(I, I.m32, spec, dt, first, count, refresh, st, hk);
+    else launch_range<float, __half>(I, I.m32, spec, dt, first, count, refresh, st, hk);
+}
+
 void Engine::enqueue_step(double dt, int first, int count)
 {
-    Impl &I = *p_;
-    for (int t = first; t < first + count; ++t) {
-        const TracerSpec &sp = spec_.at(t);
-        double *vals = I.values[t];
-        const double *ab = refresh_ab_ ? vals : I.valuesAB[t];
-        if (prec_ == PREC_FP64) launch_step<double, double>(I, I.m64, sp, dt, vals, ab, stream_);
-        else if (prec_ == PREC_FP32) launch_step<float, float>(I, I.m32, sp, dt, vals, ab, stream_);
-        else launch_step<float, __half>(I, I.m32, sp, dt, vals, ab, stream_);
-    }
+    if (first < 0 || count < 0 || first + count > ntr_) throw std::runtime_error("tracer range out of bounds");
+    enqueue(*p_, prec_, spec_, dt, first, count, refresh_ab_, stream_);
 }
 
 void Engine::run_steps(int nsteps, double dt, bool use_graph)
@@ -915,10 +867,12 @@ void Engine::synchronize() { HIP_OK(hipStreamSynchronize(stream_)); }
 Diag Engine::diagnostics(int tr)
 {
     Impl &I = *p_;
+    if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
+    const double *v = I.values + tr * I.tsn;
     if (prec_ == PREC_FP64)
-        hipLaunchKernelGGL((k_diag_partial<double>), dim3(I.ndiag), dim3(256), 0, stream_, I.m64, I.values.at(tr), I.dpart);
+        hipLaunchKernelGGL((k_diag_partial<double>), dim3(I.ndiag), dim3(256), 0, stream_, I.m64, v, I.dpart);
     else
-        hipLaunchKernelGGL((k_diag_partial<float>), dim3(I.ndiag), dim3(256), 0, stream_, I.m32, I.values.at(tr), I.dpart);
+        hipLaunchKernelGGL((k_diag_partial<float>), dim3(I.ndiag), dim3(256), 0, stream_, I.m32, v, I.dpart);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, stream_, I.dpart, I.ndiag, I.dout);
     HIP_OK(hipGetLastError());
     DiagAcc a;
@@ -931,33 +885,34 @@ size_t Engine::work_count(const std::string &name) const
 {
     const size_t L = nl_ - 1;
     if (name == "fct_LO" || name == "fct_plus" || name == "fct_minus") return L * N_;
-    if (name == "adv_flux_hor") return L * E_;
-    if (name == "adv_flux_lo") return L * E_;
+    if (name == "adv_flux_hor" || name == "adv_flux_lo") return L * E_;
     if (name == "adv_flux_ver") return (size_t)nl_ * N_;
-    if (name == "tr_xy") return 2 * L * T_;
     if (name == "gnod") return 2 * L * N_;
     return 0;
 }
 
-int Engine::get_work(const std::string &name, int, double *out, size_t count)
+int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
 {
     Impl &I = *p_;
     const size_t need = work_count(name);
-    if (need == 0 || count < need) return -1;
+    if (need == 0 || count < need || tr < 0 || tr >= ntr_) return -1;
     HIP_OK(hipStreamSynchronize(stream_));
-    const void *src = nullptr;
+    const char *src = nullptr;
     int L = nl_ - 1, ncol = N_, comps = 1;
     size_t es = prec_ == PREC_FP64 ? 8 : 4;
-    if (name == "fct_LO") { src = I.LO; es = 8; }
-    else if (name == "fct_plus") src = I.Pp;
-    else if (name == "fct_minus") src = I.Pm;
-    else if (name == "adv_flux_hor") { src = I.adf_h; ncol = E_; }
-    else if (name == "adv_flux_lo") { src = I.flux_lo; ncol = E_; }
-    else if (name == "adv_flux_ver") { src = I.adf_v; L = nl_; }
-    else if (name == "tr_xy") { src = I.trxy; ncol = T_; comps = 2; es = prec_ == PREC_FP64 ? 8 : (prec_ == PREC_FP32 ? 4 : 2); }
-    else if (name == "gnod") { src = I.gnod; comps = 2; es = prec_ == PREC_FP64 ? 8 : (prec_ == PREC_FP32 ? 4 : 2); }
+    size_t ts = I.tsn;
+    if (name == "fct_LO") { src = (const char *)I.LO; es = 8; }
+    else if (name == "fct_plus") src = (const char *)I.Pp;
+    else if (name == "fct_minus") src = (const char *)I.Pm;
+    else if (name == "adv_flux_hor") { src = (const char *)I.adf_h; ncol = E_; ts = I.tse; }
+    else if (name == "adv_flux_lo") { src = (const char *)I.flux_lo; ncol = E_; ts = I.tse; }
+    else if (name == "adv_flux_ver") { src = (const char *)I.adf_v; L = nl_; }
+    else if (name == "gnod") {
+        src = (const char *)I.gnod; comps = 2; ts = I.tsg;
+        es = prec_ == PREC_FP64 ? 8 : (prec_ == PREC_FP32 ? 4 : 2);
+    }
     std::vector<char> buf((size_t)S_ * ncol * comps * es);
-    HIP_OK(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(buf.data(), src + tr * ts * es, buf.size(), hipMemcpyDeviceToHost));
     for (int c = 0; c < ncol; ++c)
         for (int k = 0; k < L; ++k)
             for (int q = 0; q < comps; ++q) {
@@ -965,9 +920,7 @@ int Engine::get_work(const std::string &name, int, double *out, size_t count)
                 double v;
                 if (es == 8) v = ((const double *)buf.data())[i];
                 else if (es == 4) v = ((const float *)buf.data())[i];
-                else {
-                    v = (double)__half2float(((const __half *)buf.data())[i]) * (double)I.ginv;
-                }
+                else v = (double)__half2float(((const __half *)buf.data())[i]) * (double)I.ginv;
                 out[((size_t)c * L + k) * comps + q] = v;
             }
     return 0;
@@ -978,33 +931,27 @@ void Engine::time_kernels(double dt, int reps, std::vector<std::string> &names, 
     // one event after every launch of a step on the engine stream; the time
     // of a kernel is the gap to the previous event (launches are serialised
     // on the one stream, so the gap is that kernel's execution)
-    Impl &I = *p_;
     names.clear();
     ms.clear();
     for (int r = 0; r < reps; ++r) {
-        for (int t = 0; t < ntr_; ++t) {
-            std::vector<std::pair<std::string, hipEvent_t>> ev;
-            LaunchHook hk;
-            hk.ev = &ev;
-            const TracerSpec &sp = spec_.at(t);
-            double *vals = I.values[t];
-            const double *ab = refresh_ab_ ? vals : I.valuesAB[t];
-            if (prec_ == PREC_FP64) launch_step<double, double>(I, I.m64, sp, dt, vals, ab, stream_, hk);
-            else if (prec_ == PREC_FP32) launch_step<float, float>(I, I.m32, sp, dt, vals, ab, stream_, hk);
-            else launch_step<float, __half>(I, I.m32, sp, dt, vals, ab, stream_, hk);
-            HIP_OK(hipStreamSynchronize(stream_));
-            for (size_t i = 1; i < ev.size(); ++i) {
-                float e = 0.0f;
-                HIP_OK(hipEventElapsedTime(&e, ev[i - 1].second, ev[i].second));
-                size_t j = 0;
-                while (j < names.size() && names[j] != ev[i].first) ++j;
-                if (j == names.size()) { names.push_back(ev[i].first); ms.push_back(0.0); }
-                ms[j] += e;
-            }
-            for (auto &p : ev) HIP_OK(hipEventDestroy(p.second));
+        std::vector<std::pair<std::string, hipEvent_t>> ev;
+        LaunchHook hk;
+        hk.ev = &ev;
+        enqueue(*p_, prec_, spec_, dt, 0, ntr_, refresh_ab_, stream_, hk);
+        HIP_OK(hipStreamSynchronize(stream_));
+        for (size_t i = 1; i < ev.size(); ++i) {
+            if (ev[i].first == "start") continue;
+            float e = 0.0f;
+            HIP_OK(hipEventElapsedTime(&e, ev[i - 1].second, ev[i].second));
+            size_t j = 0;
+            while (j < names.size() && names[j] != ev[i].first) ++j;
+            if (j == names.size()) { names.push_back(ev[i].first); ms.push_back(0.0); }
+            ms[j] += e;
         }
+        for (auto &p : ev) HIP_OK(hipEventDestroy(p.second));
     }
-    for (double &v : ms) v /= (double)(reps * ntr_);
+    // per step (all tracers), averaged over reps
+    for (double &v : ms) v /= (double)reps;
 }
 
 }  // namespace fta

@@ -1,11 +1,17 @@
-// fta_kernels.h -- gfx950 device kernels of the tracer-advection step.
+// fta_kernels.h -- gfx950 device data structures of the tracer-advection step.
 //
-// Mapping: one 64-lane wavefront per column (node, edge or element), lane k
-// handles level k+1 (layers 1..nl-1, interfaces 1..nl). Column indices are
-// made wave-uniform with readfirstlane so that all per-column mesh data
-// (indices, level counts, geometry) is fetched with scalar loads, and blocks
-// are remapped so that each of the 8 XCDs sweeps one contiguous range of
-// columns (its own L2 then holds the neighbourhood the gathers revisit).
+// Mapping: one 64-lane wavefront per column (node or edge), lane k handles
+// level k+1 (layers 1..nl-1, interfaces 1..nl). Column indices are made
+// wave-uniform with readfirstlane so that all per-column metadata comes in
+// through scalar loads of small fixed-width records (one s_load per record,
+// no chains of dependent index loads), and blocks are remapped so that each
+// of the 8 XCDs sweeps one contiguous range of columns (its own L2 then holds
+// the neighbourhood that the gathers revisit).
+//
+// Tracers that share their advection schemes go through the kernels
+// together (NT per launch): the mesh records and velocity are fetched once
+// per column for all of them, and each wave keeps NT times more independent
+// loads in flight.
 //
 // Scatter-free: every node-side sum over incident edges is a gather over a
 // node->edge list sorted by edge index, which is the order the reference's
@@ -20,29 +26,49 @@ namespace fta {
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
+constexpr int kDeg = 8;  // padded node degree (incident edges / elements)
 constexpr double kRE = 6367500.0;  // r_earth, lib/oce_modules.F90:24
+
+// per-edge record (32 bytes): nodes, elements, up/down-wind triangles, and
+// packed level counts: nl1 | nl2 << 8 | nzmax_e << 16 | nlmax << 24 where
+// nzmax_e = min(nlevels_nod2D_min(n1), nlevels_nod2D_min(n2)) bounds the
+// levels shared by all elements around both nodes (fill_up_dn_grad), and
+// nb = nboundary_lay(n1) | nboundary_lay(n2) << 16
+struct alignas(32) EdgeMeta {
+    int n1, n2, el1, el2, up, dn, nl, nb;
+};
+
+// per-element record: nodes and nlevels
+struct alignas(16) ElemMeta {
+    int n0, n1, n2, nlev;
+};
+
+// per-node record: nlevels_nod2D and incident-edge / element lists padded to
+// kDeg. edge entry = +(e+1) if the node is edges(1,e), -(e+1) if edges(2,e),
+// 0 = empty; enl = max(nlevels(el1), nlevels(el2)) - 1 of that edge.
+struct alignas(32) NodeEdges {
+    int e[kDeg];
+};
+struct alignas(32) NodeEdgeLev {
+    int nl[kDeg];
+};
+struct alignas(32) NodeElems {
+    int t[kDeg];  // -1 = empty, nod_in_elem2D order
+};
 
 template <class R>
 struct DMesh {
     int N, T, E, nl, S;
-    const int *en;          // (3,T) element nodes
-    const int2 *edges;      // (E) node pairs
-    const int2 *etri;       // (E) element pairs, .y = -1 on boundary
-    const int2 *updn;       // (E) up/down-wind triangles
-    const int *nlev_e;      // nlevels (T)
-    const int *nlev_n;      // nlevels_nod2D (N)
-    const int *nlev_n_min;  // nlevels_nod2D_min (N)
-    const int *nbl;         // nboundary_lay (N)
-    const int *edge_nlmax;  // max(nlevels(el1),nlevels(el2))-1 (E)
-    const int *n2e_ptr, *n2e;  // node -> signed edge (+(e+1): node is edges(1))
-    const int *n2t_ptr, *n2t;  // node -> elements (nod_in_elem2D order)
-    const R *ecross;        // (4,E) edge_cross_dxdy
-    const R *edxdy;         // (2,E) edge_dxdy
-    const R *edge_a;        // (E) r_earth*elem_cos averaged over the edge's elements
-    const R *earea;         // (T) elem_area
-    const R *gsca;          // (6,T) gradient_sca
+    const EdgeMeta *emeta;      // (E)
+    const R *egeo;              // (8,E): dx1 dy1 dx2 dy2 (edge_cross_dxdy) ex ey (edge_dxdy) a
+    const ElemMeta *tmeta;      // (T)
+    const R *tgeo;              // (8,T): gradient_sca(1:6), elem_area, 0
+    const int *nlev_n;          // nlevels_nod2D (N)
+    const NodeEdges *nedge;     // (N)
+    const NodeEdgeLev *nedgel;  // (N)
+    const NodeElems *nelem;     // (N)
     const R *area, *areasvol, *hnode, *hnode_new, *Zn, *zbarn;  // (S,N)
-    const R *helem;         // (S,T)
+    const R *helem;             // (S,T)
 };
 
 __device__ __forceinline__ int wave_column()

@@ -587,12 +587,14 @@ int tracer_get_mesh_array(const char *name, void *out, long long count)
     return 0;
 }
 
-int tracer_get_work_array(const char *name, double *out, long long count)
+int tracer_get_work_array(int tracer_id, const char *name, double *out, long long count)
 {
     State &s = S();
     if (!s.eng) return fail("no device state yet");
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id");
     try {
-        if (s.eng->get_work(name ? name : "", 0, out, (size_t)count)) return fail("unknown work array or short buffer");
+        if (s.eng->get_work(name ? name : "", tracer_id - 1, out, (size_t)count))
+            return fail("unknown work array or short buffer");
     } catch (const std::exception &e) {
         return fail(e.what());
     }

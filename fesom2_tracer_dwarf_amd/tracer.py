@@ -121,9 +121,10 @@ class FESOMTracerAdvection:
         keys = ("min", "max", "sum", "active_min", "active_max", "mass", "nan_count")
         return dict(zip(keys, out.tolist()))
 
-    def work_array(self, name, count):
+    def work_array(self, name, count, tracer_id=1):
         out = np.zeros(count)
-        _lib.check(self.lib.tracer_get_work_array(name.encode(), out, count), f"work array {name}", self.lib)
+        _lib.check(self.lib.tracer_get_work_array(tracer_id, name.encode(), out, count), f"work array {name}",
+                   self.lib)
         return out
 
     def time_steps(self, nsteps, dt):

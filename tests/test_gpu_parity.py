@@ -80,7 +80,7 @@ def test_fp64_matches_oracle(api, mesh_name, schemes):
                 span = hi - lo
                 assert g[act].min() >= lo - 1e-12 * span and g[act].max() <= hi + 1e-12 * span
     if schemes[2] == "FCT":
-        lo_gpu = api.work_array("fct_LO", L * N)
+        lo_gpu = api.work_array("fct_LO", L * N, tracer_id=len(ref))
         lo_ref = o.work_arrays["fct_LO"]
         # the last tracer's LO solution, levels of the wet column only
         mask = act.T.ravel()
