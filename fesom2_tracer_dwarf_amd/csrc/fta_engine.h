@@ -2,10 +2,11 @@
 // MI355X (gfx950). Owns the device copy of the mesh, the velocity fields,
 // the tracers and the work arrays, and launches the step on its own stream.
 //
-// Device layout: every level-indexed array is stored column-major with a
-// padded column stride S (>= nl, multiple of 8), so one node/edge/element
-// column is one contiguous, 64-byte-aligned run of S values; a wavefront
-// handles one column with lane k = level k+1 (requires nl <= 64).
+// Device layout: every level-indexed array stores only the wet levels of
+// each column, columns back to back in mesh order (nodes, elements and edges
+// follow a space-filling curve when the mesh was generated with one), so a
+// sweep over consecutive columns streams through contiguous memory. A
+// wavefront handles one column with lane k = level k+1 (requires nl <= 64).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,7 +47,6 @@ public:
 
     int ntracers() const { return ntr_; }
     int precision() const { return prec_; }
-    int stride() const { return S_; }
 
     // (2,nl-1,elem2D), (nl,nod2D), (nl,nod2D) unpadded double arrays
     void set_velocity(const double *uv, const double *w, const double *we);

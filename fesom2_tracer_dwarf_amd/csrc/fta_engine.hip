@@ -39,10 +39,9 @@ void set_error(const std::string &s) { g_last_error = s; }
 // the elemental gradient of tracer A at layer k (tracer_gradient_elements:
 // gradient_sca(1:3)*T(nodes) and gradient_sca(4:6)*T(nodes), node order 1,2,3)
 template <class R>
-__device__ __forceinline__ void elem_gradient(const ElemMeta &em, const R *g, const double *A, int S, int k,
-                                              R &gx, R &gy)
+__device__ __forceinline__ void elem_gradient(const ElemMeta &em, const R *g, const double *A, int k, R &gx, R &gy)
 {
-    const R A0 = (R)A[(size_t)em.n0 * S + k], A1 = (R)A[(size_t)em.n1 * S + k], A2 = (R)A[(size_t)em.n2 * S + k];
+    const R A0 = (R)A[em.o0 + k], A1 = (R)A[em.o1 + k], A2 = (R)A[em.o2 + k];
     gx = g[0] * A0 + g[1] * A1 + g[2] * A2;
     gy = g[3] * A0 + g[4] * A1 + g[5] * A2;
 }
@@ -51,14 +50,38 @@ __device__ __forceinline__ void elem_gradient(const ElemMeta &em, const R *g, co
 // MUSCL prerequisite: area-weighted mean over the node's elements that have
 // layer nz of their elemental gradients (fill_up_dn_grad, lib/oce_muscl_adv.F90:397-407)
 // ===========================================================================
+// value of neighbourhood slot s (wave-uniform): incident-edge neighbour s, or the node itself
+template <class R>
+__device__ __forceinline__ R pick(const R (&nb)[kDeg], R self, int s)
+{
+    R v = self;  // branch-free select chain on the uniform slot
+#pragma unroll
+    for (int j = 0; j < kDeg; ++j) v = (s == j) ? nb[j] : v;
+    return v;
+}
+
+// the 1-ring neighbours' values of array A at this level (each distinct
+// column once; a neighbour shallower than this level is not read)
+template <class R>
+__device__ __forceinline__ void load_ring(const NodeEdgeRec &ne, const double *A, int k, R (&nb)[kDeg])
+{
+    const int nz = k + 1;
+#pragma unroll
+    for (int j = 0; j < kDeg; ++j) {
+        const bool ok = ne.oe[j] >= 0 && nz <= ((ne.info[j] >> 16) & 255);
+        nb[j] = ok ? (R)A[(size_t)ne.oo[j] + k] : (R)0;
+    }
+}
+
 template <class R, class G, int NT>
 __global__ __launch_bounds__(kBlock) void k_grad(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
-                                                 G *__restrict__ gnod, size_t tsg, float gscale)
+                                                 G *__restrict__ gnod, float gscale)
 {
     const int n = wave_column();
     if (n >= m.N) return;
     const int k = lane_id(), nz = k + 1;
-    if (nz > m.nlev_n[n] - 1) return;
+    const NodeRec nr = m.nrec[n];
+    if (nz > nr.nlev - 1) return;
     const NodeElems ne = m.nelem[n];
     R tvol = 0, tx[NT], ty[NT];
 #pragma unroll
@@ -75,14 +98,14 @@ __global__ __launch_bounds__(kBlock) void k_grad(DMesh<R> m, const double *__res
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             R gx, gy;
-            elem_gradient(em, g, ab + t * tsn, m.S, k, gx, gy);
+            elem_gradient(em, g, ab + t * tsn, k, gx, gy);
             tx[t] = tx[t] + gx * a;
             ty[t] = ty[t] + gy * a;
         }
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        G *o = gnod + t * tsg + ((size_t)n * m.S + k) * 2;
+        G *o = gnod + 2 * (t * tsn + nr.off + k);
         GradIO<G>::st(o, tx[t] / tvol, gscale);
         GradIO<G>::st(o + 1, ty[t] / tvol, gscale);
     }
@@ -97,11 +120,11 @@ __global__ __launch_bounds__(kBlock) void k_grad(DMesh<R> m, const double *__res
 // (lib/oce_muscl_adv.F90:361-530): up/down-wind triangle on levels shared by
 // all elements around both nodes, nodal means below.
 // ===========================================================================
-template <class R, class G, int HOR, bool FCT, int NT>
+template <class R, class G, int HOR, bool FCT, int NT, bool SAME>
 __global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict__ vel,
                                                  const double *__restrict__ ttf,
                                                  const double *__restrict__ ttfAB, size_t tsn,
-                                                 const G *__restrict__ gnod, size_t tsg,
+                                                 const G *__restrict__ gnod,
                                                  R num_ord, float ginv,
                                                  R *__restrict__ flux_lo, R *__restrict__ flux, size_t tse)
 {
@@ -119,16 +142,16 @@ __global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict
     const bool has2 = em.el2 >= 0 && (nz <= nl12 || nz > nl1);
     R vf1 = 0, vf2 = 0;
     if (has1) {
-        const R *u = vel + ((size_t)em.el1 * m.S + k) * 2;
-        vf1 = (-u[1] * dx1 + u[0] * dy1) * m.helem[(size_t)em.el1 * m.S + k];
+        const R *u = vel + 2 * ((size_t)em.t1 + k);
+        vf1 = (-u[1] * dx1 + u[0] * dy1) * m.helem[em.t1 + k];
     }
     if (has2) {
-        const R *u = vel + ((size_t)em.el2 * m.S + k) * 2;
-        vf2 = (u[1] * dx2 - u[0] * dy2) * m.helem[(size_t)em.el2 * m.S + k];
+        const R *u = vel + 2 * ((size_t)em.t2 + k);
+        vf2 = (u[1] * dx2 - u[0] * dy2) * m.helem[em.t2 + k];
     }
     const R vflux = (has1 && has2) ? vf1 + vf2 : (has1 ? vf1 : vf2);
-    const size_t o = (size_t)e * m.S + k;
-    const size_t o1 = (size_t)em.n1 * m.S + k, o2 = (size_t)em.n2 * m.S + k;
+    const size_t o = (size_t)em.oe + k;
+    const size_t o1 = (size_t)em.o1 + k, o2 = (size_t)em.o2 + k;
 
     // MUSCL/MFCT: reconstruction geometry, gradient source per level
     R ex = 0, ey = 0, a = 0, clo1 = 1, clo2 = 1;
@@ -158,17 +181,24 @@ __global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict
             flux_lo[t * tse + o] = lo;
         }
         const double *AB = ttfAB + t * tsn;
-        const R A1 = (R)AB[o1], A2 = (R)AB[o2];
+        R A1, A2;
+        if constexpr (FCT && SAME) {
+            A1 = (R)ttf[t * tsn + o1];  // the same loads as T1, T2 above
+            A2 = (R)ttf[t * tsn + o2];
+        } else {
+            A1 = (R)AB[o1];
+            A2 = (R)AB[o2];
+        }
         R ho;
         if constexpr (HOR == HOR_UPW1) {
             ho = (R)(-0.5) * (A1 * (vflux + fabs(vflux)) + A2 * (vflux - fabs(vflux)));
         } else {
             R g1x, g1y, g2x, g2y;
             if (shared) {
-                elem_gradient(tu, gu, AB, m.S, k, g1x, g1y);
-                elem_gradient(td, gd, AB, m.S, k, g2x, g2y);
+                elem_gradient(tu, gu, AB, k, g1x, g1y);
+                elem_gradient(td, gd, AB, k, g2x, g2y);
             } else {
-                const G *p1 = gnod + t * tsg + o1 * 2, *p2 = gnod + t * tsg + o2 * 2;
+                const G *p1 = gnod + 2 * (t * tsn + o1), *p2 = gnod + 2 * (t * tsn + o2);
                 g1x = (R)GradIO<G>::ld(p1, ginv); g1y = (R)GradIO<G>::ld(p1 + 1, ginv);
                 g2x = (R)GradIO<G>::ld(p2, ginv); g2y = (R)GradIO<G>::ld(p2 + 1, ginv);
             }
@@ -183,7 +213,8 @@ __global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict
 
 // decoded incident-edge list of a node, valid for this lane's layer
 struct NodeEdgeList {
-    int eid[kDeg];
+    int oe[kDeg];
+    int oo[kDeg];
     bool pos[kDeg];
     bool on[kDeg];
 };
@@ -191,26 +222,32 @@ struct NodeEdgeList {
 template <class R>
 __device__ __forceinline__ void load_edge_list(const DMesh<R> &m, int n, int nz, bool layer, NodeEdgeList &L)
 {
-    const NodeEdges ne = m.nedge[n];
-    const NodeEdgeLev nl = m.nedgel[n];
+    const NodeEdgeRec ne = m.nedge[n];
 #pragma unroll
     for (int j = 0; j < kDeg; ++j) {
-        const int se = ne.e[j];
-        L.eid[j] = (se > 0 ? se : -se) - 1;
-        L.pos[j] = se > 0;
-        L.on[j] = se != 0 && layer && nz <= nl.nl[j];
+        L.oe[j] = ne.oe[j];
+        L.oo[j] = ne.oo[j];
+        L.pos[j] = (ne.info[j] & 256) != 0;
+        L.on[j] = ne.oe[j] >= 0 && layer && nz <= (ne.info[j] & 255);
     }
+}
+
+template <class R>
+__device__ __forceinline__ VCol<R> vcol(const DMesh<R> &m, const NodeRec &nr, const double *A, const R *W, int nz)
+{
+    return VCol<R>{A + nr.off, W + nr.offi, m.area + nr.offi, m.Zn + nr.off, m.zbarn + nr.offi,
+                   m.hnode + nr.off, m.hnode_new + nr.off, nr.nlev, nz};
 }
 
 // ===========================================================================
 // LO solution (lib/oce_adv_tra_driver.F90:95-211), HO-LO vertical flux
 // (:257-283), and the raw antidiffusive sums P+/P- (lib/oce_adv_tra_fct.F90:232-346)
 // ===========================================================================
-template <class R, int VER, int NT>
+template <class R, int VER, int NT, bool SAME>
 __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restrict__ Wv,
                                                     const R *__restrict__ WEv,
                                                     const double *__restrict__ ttf,
-                                                    const double *__restrict__ ttfAB, size_t tsn,
+                                                    const double *__restrict__ ttfAB, size_t tsn, size_t tsi,
                                                     const R *__restrict__ flux_lo,
                                                     const R *__restrict__ adf_h, size_t tse,
                                                     double dt, R num_ord,
@@ -220,9 +257,10 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
     const int n = wave_column();
     if (n >= m.N) return;
     const int k = lane_id(), nz = k + 1;
-    const int nlev = m.nlev_n[n];
-    const size_t c0 = (size_t)n * m.S;
+    const NodeRec nr = m.nrec[n];
+    const int nlev = nr.nlev;
     const bool layer = nz <= nlev - 1;
+    const size_t cl = (size_t)nr.off + k, ci = (size_t)nr.offi + k;
     NodeEdgeList L;
     load_edge_list(m, n, nz, layer, L);
 
@@ -236,25 +274,25 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
         for (int j = 0; j < kDeg; ++j) {
             fadf[j] = 0;
             if (L.on[j]) {
-                const size_t oe = te + (size_t)L.eid[j] * m.S + k;
+                const size_t oe = te + (size_t)L.oe[j] + k;
                 const double f = (double)flux_lo[oe];
                 fadf[j] = adf_h[oe];
                 slo = L.pos[j] ? slo + f : slo - f;
             }
         }
         // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
-        VCol<R> lc{ttf + tn + c0, WEv + c0, m.area + c0, m.Zn + c0, m.zbarn + c0, m.hnode + c0, m.hnode_new + c0, nlev, nz};
+        const VCol<R> lc = vcol(m, nr, ttf + tn, WEv, nz);
         const R fv_lo = vflux_upw1(lc, (R)0);
         const R fv_lo_next = shdn(fv_lo);
-        VCol<R> hc{ttfAB + tn + c0, Wv + c0, m.area + c0, m.Zn + c0, m.zbarn + c0, m.hnode + c0, m.hnode_new + c0, nlev, nz};
+        const VCol<R> hc = vcol(m, nr, (SAME ? ttf : ttfAB) + tn, Wv, nz);
         const R adfv = ver_flux<R, VER>(hc, num_ord, (R)dt, fv_lo);
         const R adfv_next = shdn(adfv);
-        if (nz <= nlev) adf_v[tn + c0 + k] = adfv;
+        if (nz <= nlev) adf_v[t * tsi + ci] = adfv;
         if (layer) {
-            const double T = ttf[tn + c0 + k];
-            LO[tn + c0 + k] = (T * (double)m.hnode[c0 + k] +
-                               (slo + (double)(fv_lo - fv_lo_next)) * dt / (double)m.areasvol[c0 + k]) /
-                              (double)m.hnode_new[c0 + k];
+            const double T = ttf[tn + cl];
+            LO[tn + cl] = (T * (double)m.hnode[cl] +
+                           (slo + (double)(fv_lo - fv_lo_next)) * dt / (double)m.areasvol[cl]) /
+                          (double)m.hnode_new[cl];
             R pp = (R)0 + (fmx((R)0, adfv) + fmx((R)0, -adfv_next));
             R pm = (R)0 + (fmn((R)0, adfv) + fmn((R)0, -adfv_next));
 #pragma unroll
@@ -264,14 +302,15 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
                 if (L.pos[j]) { pp = pp + fmx((R)0, a); pm = pm + fmn((R)0, a); }
                 else          { pp = pp + fmx((R)0, -a); pm = pm + fmn((R)0, -a); }
             }
-            Pp[tn + c0 + k] = pp;
-            Pm[tn + c0 + k] = pm;
+            Pp[tn + cl] = pp;
+            Pm[tn + cl] = pm;
         }
     }
 }
 
 // Zalesak bounds a1-a3 and limiting factors b2 (lib/oce_adv_tra_fct.F90:94-217,361-374);
-// R+/R- overwrite P+/P- in place
+// R+/R- overwrite P+/P- in place. a1 (max/min of LO and T) is formed once per
+// distinct node of the 1-ring, a2 per element from its three nodes' a1.
 template <class R, int NT>
 __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double *__restrict__ ttf,
                                                        const double *__restrict__ LO, size_t tsn,
@@ -281,36 +320,44 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
     const int n = wave_column();
     if (n >= m.N) return;
     const int k = lane_id(), nz = k + 1;
-    const int nlev = m.nlev_n[n];
-    const size_t c0 = (size_t)n * m.S;
+    const NodeRec nr = m.nrec[n];
+    const int nlev = nr.nlev;
     const bool layer = nz <= nlev - 1;
+    const size_t cl = (size_t)nr.off + k;
     const R bignumber = (R)1000;
-    const NodeElems ne = m.nelem[n];
 
     R tvmax[NT], tvmin[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) { tvmax[t] = 0; tvmin[t] = 0; }
     if (layer) {
+        const NodeEdgeRec ne = m.nedge[n];
+        const NodeElems nel = m.nelem[n];
 #pragma unroll
-        for (int j = 0; j < kDeg; ++j) {
-            const int el = ne.t[j];
-            if (el < 0) break;
-            const ElemMeta em = m.tmeta[el];
-            const bool wet = nz <= em.nlev - 1;
-            const size_t o0 = (size_t)em.n0 * m.S + k, o1 = (size_t)em.n1 * m.S + k, o2 = (size_t)em.n2 * m.S + k;
+        for (int t = 0; t < NT; ++t) {
+            const size_t tn = t * tsn;
+            R lnb[kDeg], tnb[kDeg], mxn[kDeg], mnn[kDeg];
+            load_ring<R>(ne, LO + tn, k, lnb);
+            load_ring<R>(ne, ttf + tn, k, tnb);
+            const R ls = (R)LO[tn + cl], ts = (R)ttf[tn + cl];
+            const R mxs = fmx(ls, ts), mns = fmn(ls, ts);
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
+            for (int j = 0; j < kDeg; ++j) { mxn[j] = fmx(lnb[j], tnb[j]); mnn[j] = fmn(lnb[j], tnb[j]); }
+            R vmax = 0, vmin = 0;
+#pragma unroll
+            for (int j = 0; j < kDeg; ++j) {
+                if (nel.t[j] < 0) break;
+                const int sl = nel.sl[j];
                 R emax = -bignumber, emin = bignumber;
-                if (wet) {
-                    const size_t tn = t * tsn;
-                    const R l0 = (R)LO[tn + o0], l1 = (R)LO[tn + o1], l2 = (R)LO[tn + o2];
-                    const R t0 = (R)ttf[tn + o0], t1 = (R)ttf[tn + o1], t2 = (R)ttf[tn + o2];
-                    emax = fmx(fmx(fmx(l0, t0), fmx(l1, t1)), fmx(l2, t2));
-                    emin = fmn(fmn(fmn(l0, t0), fmn(l1, t1)), fmn(l2, t2));
+                if (nz <= (sl >> 16) - 1) {
+                    const int s0 = sl & 15, s1 = (sl >> 4) & 15, s2 = (sl >> 8) & 15;
+                    emax = fmx(fmx(pick(mxn, mxs, s0), pick(mxn, mxs, s1)), pick(mxn, mxs, s2));
+                    emin = fmn(fmn(pick(mnn, mns, s0), pick(mnn, mns, s1)), pick(mnn, mns, s2));
                 }
-                tvmax[t] = (j == 0) ? emax : fmx(tvmax[t], emax);
-                tvmin[t] = (j == 0) ? emin : fmn(tvmin[t], emin);
+                vmax = (j == 0) ? emax : fmx(vmax, emax);
+                vmin = (j == 0) ? emin : fmn(vmin, emin);
             }
+            tvmax[t] = vmax;
+            tvmin[t] = vmin;
         }
     }
     const R flux_eps = (R)1e-16;
@@ -320,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
         const R up_min = shup(tvmin[t]), dn_min = shdn(tvmin[t]);
         if (!layer) continue;
         const size_t tn = t * tsn;
-        const R lo = (R)LO[tn + c0 + k];
+        const R lo = (R)LO[tn + cl];
         R qp, qm;
         if (nz == 1 || nz == nlev - 1) {
             qp = tvmax[t] - lo;
@@ -329,13 +376,13 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
             qp = fmx(fmx(up_max, tvmax[t]), dn_max) - lo;
             qm = fmn(fmn(up_min, tvmin[t]), dn_min) - lo;
         }
-        const R av = m.areasvol[c0 + k];
-        R flux = Pp_Rp[tn + c0 + k] * (R)dt / av + flux_eps;
+        const R av = m.areasvol[cl];
+        R flux = Pp_Rp[tn + cl] * (R)dt / av + flux_eps;
         const R rp = fmn((R)1, qp / flux);
-        flux = Pm_Rm[tn + c0 + k] * (R)dt / av - flux_eps;
+        flux = Pm_Rm[tn + cl] * (R)dt / av - flux_eps;
         const R rm = fmn((R)1, qm / flux);
-        Pp_Rp[tn + c0 + k] = rp;
-        Pm_Rm[tn + c0 + k] = rm;
+        Pp_Rp[tn + cl] = rp;
+        Pm_Rm[tn + cl] = rm;
     }
 }
 
@@ -344,31 +391,28 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
 // values += (dttf_h + dttf_v)/hnode_new (src/fesom_analytic.F90:419-437)
 template <class R, int NT>
 __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__restrict__ ttf,
-                                                        const double *__restrict__ LO, size_t tsn,
+                                                        const double *__restrict__ LO, size_t tsn, size_t tsi,
                                                         const R *__restrict__ adf_h, size_t tse,
                                                         const R *__restrict__ adf_v,
                                                         const R *__restrict__ Rp,
-                                                        const R *__restrict__ Rm,
-                                                        const int *__restrict__ nother, double dt)
+                                                        const R *__restrict__ Rm, double dt)
 {
     const int n = wave_column();
     if (n >= m.N) return;
     const int k = lane_id(), nz = k + 1;
-    const int nlev = m.nlev_n[n];
-    const size_t c0 = (size_t)n * m.S;
+    const NodeRec nr = m.nrec[n];
+    const int nlev = nr.nlev;
     const bool layer = nz <= nlev - 1;
+    const size_t cl = (size_t)nr.off + k, ci = (size_t)nr.offi + k;
     NodeEdgeList L;
     load_edge_list(m, n, nz, layer, L);
-    int oth[kDeg];
-#pragma unroll
-    for (int j = 0; j < kDeg; ++j) oth[j] = nother[(size_t)n * kDeg + j];
 
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const size_t tn = t * tsn, te = t * tse;
         R rp = 0, rm = 0, fv = 0;
-        if (nz <= nlev) fv = adf_v[tn + c0 + k];
-        if (layer) { rp = Rp[tn + c0 + k]; rm = Rm[tn + c0 + k]; }
+        if (nz <= nlev) fv = adf_v[t * tsi + ci];
+        if (layer) { rp = Rp[tn + cl]; rm = Rm[tn + cl]; }
         const R rp_up = shup(rp), rm_up = shup(rm);
         R fvl = fv;
         if (nz == 1) {
@@ -384,13 +428,13 @@ __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__re
         const R fvl_next = shdn(fvl);
         if (!layer) continue;
 
-        const double av = (double)m.areasvol[c0 + k];
+        const double av = (double)m.areasvol[cl];
         double sh = 0.0;
 #pragma unroll
         for (int j = 0; j < kDeg; ++j) {
             if (!L.on[j]) continue;
-            const R f = adf_h[te + (size_t)L.eid[j] * m.S + k];
-            const size_t oo = tn + (size_t)oth[j] * m.S + k;
+            const R f = adf_h[te + (size_t)L.oe[j] + k];
+            const size_t oo = tn + (size_t)L.oo[j] + k;
             const R rpo = Rp[oo], rmo = Rm[oo];
             // edge orientation: this node is edges(1) when pos
             const R rp1 = L.pos[j] ? rp : rpo, rm1 = L.pos[j] ? rm : rmo;
@@ -401,11 +445,11 @@ __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__re
             const double fl = (double)(ae * f);
             sh = L.pos[j] ? sh + fl * dt / av : sh - fl * dt / av;
         }
-        const double T = ttf[tn + c0 + k];
-        double dv = 0.0 - T * (double)m.hnode[c0 + k] + LO[tn + c0 + k] * (double)m.hnode_new[c0 + k];
+        const double T = ttf[tn + cl];
+        double dv = 0.0 - T * (double)m.hnode[cl] + LO[tn + cl] * (double)m.hnode_new[cl];
         dv = dv + (double)(fvl - fvl_next) * dt / av;
         const double del = (0.0 + sh) + dv;
-        ttf[tn + c0 + k] = T + del / (double)m.hnode_new[c0 + k];
+        ttf[tn + cl] = T + del / (double)m.hnode_new[cl];
     }
 }
 
@@ -420,30 +464,31 @@ __global__ __launch_bounds__(kBlock) void k_node_nolim(DMesh<R> m, const R *__re
     const int n = wave_column();
     if (n >= m.N) return;
     const int k = lane_id(), nz = k + 1;
-    const int nlev = m.nlev_n[n];
-    const size_t c0 = (size_t)n * m.S;
+    const NodeRec nr = m.nrec[n];
+    const int nlev = nr.nlev;
     const bool layer = nz <= nlev - 1;
+    const size_t cl = (size_t)nr.off + k;
     NodeEdgeList L;
     load_edge_list(m, n, nz, layer, L);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const size_t tn = t * tsn, te = t * tse;
-        VCol<R> hc{ttfAB + tn + c0, WEv + c0, m.area + c0, m.Zn + c0, m.zbarn + c0, m.hnode + c0, m.hnode_new + c0, nlev, nz};
+        const VCol<R> hc = vcol(m, nr, ttfAB + tn, WEv, nz);
         const R fv = ver_flux<R, VER>(hc, num_ord, (R)dt, (R)0);
         const R fv_next = shdn(fv);
         if (!layer) continue;
-        const double av = (double)m.areasvol[c0 + k];
+        const double av = (double)m.areasvol[cl];
         double sh = 0.0;
 #pragma unroll
         for (int j = 0; j < kDeg; ++j) {
             if (!L.on[j]) continue;
-            const double f = (double)flux_h[te + (size_t)L.eid[j] * m.S + k];
+            const double f = (double)flux_h[te + (size_t)L.oe[j] + k];
             sh = L.pos[j] ? sh + f * dt / av : sh - f * dt / av;
         }
         const double dv = 0.0 + (double)(fv - fv_next) * dt / av;
         const double del = (0.0 + sh) + dv;
-        const double T = ttf[tn + c0 + k];
-        ttf[tn + c0 + k] = T + del / (double)m.hnode_new[c0 + k];
+        const double T = ttf[tn + cl];
+        ttf[tn + cl] = T + del / (double)m.hnode_new[cl];
     }
 }
 
@@ -458,24 +503,21 @@ __device__ inline void diag_merge(DiagAcc &a, const DiagAcc &b)
     a.amin = fmin(a.amin, b.amin); a.amax = fmax(a.amax, b.amax); a.mass += b.mass; a.nan += b.nan;
 }
 
+// reduction over the wet cells (compact layout): min, max, sum, mass, NaN count
 template <class R>
 __global__ __launch_bounds__(256) void k_diag_partial(DMesh<R> m, const double *__restrict__ v,
                                                       DiagAcc *__restrict__ part)
 {
     __shared__ DiagAcc sh[256];
     DiagAcc a{INFINITY, -INFINITY, 0.0, INFINITY, -INFINITY, 0.0, 0.0};
-    const int L = m.nl - 1;
-    const long long total = (long long)m.N * L;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
-        const int n = (int)(i / L), k = (int)(i % L);
-        const double x = v[(size_t)n * m.S + k];
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < m.CN; i += (long long)gridDim.x * 256) {
+        const double x = v[i];
         if (x != x) { a.nan += 1.0; continue; }
         a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
-        if (k + 1 <= m.nlev_n[n] - 1) {
-            a.amin = fmin(a.amin, x); a.amax = fmax(a.amax, x);
-            a.mass += x * (double)m.hnode_new[(size_t)n * m.S + k] * (double)m.areasvol[(size_t)n * m.S + k];
-        }
+        a.mass += x * (double)m.hnode_new[i] * (double)m.areasvol[i];
     }
+    a.amin = a.vmin;
+    a.amax = a.vmax;
     sh[threadIdx.x] = a;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
@@ -520,18 +562,6 @@ T *dupload(const std::vector<T> &h)
     return d;
 }
 
-// unpadded (L, ncol) double -> padded (S, ncol) of type T (pads = 0)
-template <class T>
-std::vector<T> pad_cols(const double *src, int L, int ncol, int S, int comps = 1)
-{
-    std::vector<T> out((size_t)S * ncol * comps, (T)0);
-    for (int c = 0; c < ncol; ++c)
-        for (int k = 0; k < L; ++k)
-            for (int q = 0; q < comps; ++q)
-                out[((size_t)c * S + k) * comps + q] = (T)src[((size_t)c * L + k) * comps + q];
-    return out;
-}
-
 inline unsigned grid_for(int ncols)
 {
     unsigned nb = (unsigned)((ncols + kWavesPerBlock - 1) / kWavesPerBlock);
@@ -540,47 +570,82 @@ inline unsigned grid_for(int ncols)
 
 }  // namespace
 
+// compact layout of a set of columns: column c holds len[c] levels at off[c]
+struct Layout {
+    std::vector<long long> off;
+    std::vector<int> len;
+    long long total = 0;
+    void build(const std::vector<int> &lens)
+    {
+        len = lens;
+        off.assign(lens.size() + 1, 0);
+        for (size_t c = 0; c < lens.size(); ++c) off[c + 1] = off[c] + lens[c];
+        total = off.back();
+    }
+    // unpadded column-major (L, ncol) with comps interleaved components -> compact
+    template <class T>
+    std::vector<T> pack(const double *src, int L, int comps = 1) const
+    {
+        std::vector<T> out((size_t)total * comps);
+        for (size_t c = 0; c < len.size(); ++c)
+            for (int k = 0; k < len[c]; ++k)
+                for (int q = 0; q < comps; ++q)
+                    out[((size_t)off[c] + k) * comps + q] = (T)src[((size_t)c * L + k) * comps + q];
+        return out;
+    }
+};
+
 struct Engine::Impl {
     std::vector<void *> allocs;
     template <class T> T *track(T *p) { allocs.push_back((void *)p); return p; }
 
     DMesh<double> m64{};
     DMesh<float> m32{};
-    int *nother = nullptr;  // (kDeg,N) other endpoint of each incident edge
+    Layout ln, li, lt, le;  // node layers, node interfaces, element layers, edge levels
     // velocity
     void *vel = nullptr, *W = nullptr, *WE = nullptr;
-    // tracers [ntr][S*N]
+    // tracers [ntr][CN]
     double *values = nullptr, *valuesAB = nullptr;
     // work, per tracer
-    void *flux_lo = nullptr, *adf_h = nullptr;   // [ntr][S*E]
-    void *adf_v = nullptr, *Pp = nullptr, *Pm = nullptr;  // [ntr][S*N]
-    double *LO = nullptr;                        // [ntr][S*N]
-    void *gnod = nullptr;                        // [ntr][2*S*N]
+    void *flux_lo = nullptr, *adf_h = nullptr;             // [ntr][CE]
+    void *adf_v = nullptr;                                 // [ntr][CI]
+    void *Pp = nullptr, *Pm = nullptr;                     // [ntr][CN]
+    double *LO = nullptr;                                  // [ntr][CN]
+    void *gnod = nullptr;                                  // [ntr][2*CN]
     DiagAcc *dpart = nullptr, *dout = nullptr;
     int ndiag = 1024;
     float gscale = 1.0f, ginv = 1.0f;
-    size_t tsn = 0, tse = 0, tsg = 0;
+    size_t tsn = 0, tsi = 0, tse = 0;
+    // host copy of each tracer as last set (levels below the bottom are never
+    // touched by a step, so get_values and the all-values statistics read them here)
+    std::vector<std::vector<double>> host_vals;
 };
 
 template <class R>
-static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, int S)
+static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
 {
     const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
-    d.N = N; d.T = T; d.E = E; d.nl = nl; d.S = S;
+    d.N = N; d.T = T; d.E = E; d.nl = nl;
+    d.CN = I.ln.total; d.CI = I.li.total; d.CT = I.lt.total; d.CE = I.le.total;
     std::vector<EdgeMeta> em(E);
     std::vector<R> eg(8 * (size_t)E, (R)0);
     for (int e = 0; e < E; ++e) {
         EdgeMeta &r = em[e];
-        r.n1 = m.edges[2 * e]; r.n2 = m.edges[2 * e + 1];
-        r.el1 = m.edge_tri[2 * e]; r.el2 = m.edge_tri[2 * e + 1];
+        std::memset(&r, 0, sizeof(r));
+        const int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
+        const int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
+        r.o1 = (int)I.ln.off[n1]; r.o2 = (int)I.ln.off[n2];
+        r.t1 = (int)I.lt.off[el1]; r.t2 = el2 >= 0 ? (int)I.lt.off[el2] : -1;
         r.up = m.edge_up_dn_tri[2 * e]; r.dn = m.edge_up_dn_tri[2 * e + 1];
-        int nl1 = m.nlevels[r.el1] - 1, nl2 = r.el2 >= 0 ? m.nlevels[r.el2] - 1 : 0;
-        int nzmax_e = std::min(m.nlevels_nod2D_min[r.n1], m.nlevels_nod2D_min[r.n2]);
-        int nlmax = std::max(nl1, nl2);
+        const int nl1 = m.nlevels[el1] - 1, nl2 = el2 >= 0 ? m.nlevels[el2] - 1 : 0;
+        const int nzmax_e = std::min(m.nlevels_nod2D_min[n1], m.nlevels_nod2D_min[n2]);
+        const int nlmax = std::max(nl1, nl2);
         r.nl = nl1 | (nl2 << 8) | (nzmax_e << 16) | (nlmax << 24);
-        r.nb = (m.nboundary_lay[r.n1] & 0xFFFF) | ((m.nboundary_lay[r.n2] & 0xFFFF) << 16);
-        double a = kRE * m.elem_cos[r.el1];
-        if (r.el2 >= 0) a = 0.5 * (a + kRE * m.elem_cos[r.el2]);
+        r.nb = (m.nboundary_lay[n1] & 0xFFFF) | ((m.nboundary_lay[n2] & 0xFFFF) << 16);
+        r.oe = (int)I.le.off[e];
+        r.el2 = el2;
+        double a = kRE * m.elem_cos[el1];
+        if (el2 >= 0) a = 0.5 * (a + kRE * m.elem_cos[el2]);
         R *g = &eg[8 * (size_t)e];
         for (int q = 0; q < 4; ++q) g[q] = (R)m.edge_cross_dxdy[4 * e + q];
         g[4] = (R)m.edge_dxdy[2 * e];
@@ -592,48 +657,66 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, int S)
     std::vector<ElemMeta> tm(T);
     std::vector<R> tg(8 * (size_t)T, (R)0);
     for (int el = 0; el < T; ++el) {
-        tm[el] = ElemMeta{m.elem2D_nodes[3 * el], m.elem2D_nodes[3 * el + 1], m.elem2D_nodes[3 * el + 2], m.nlevels[el]};
+        const int *en = &m.elem2D_nodes[3 * el];
+        tm[el] = ElemMeta{(int)I.ln.off[en[0]], (int)I.ln.off[en[1]], (int)I.ln.off[en[2]], m.nlevels[el]};
         for (int q = 0; q < 6; ++q) tg[8 * (size_t)el + q] = (R)m.gradient_sca[6 * el + q];
         tg[8 * (size_t)el + 6] = (R)m.elem_area[el];
     }
     d.tmeta = I.track(dupload(tm));
     d.tgeo = I.track(dupload(tg));
-    d.nlev_n = I.track(dupload(m.nlevels_nod2D));
-    std::vector<NodeEdges> ne(N);
-    std::vector<NodeEdgeLev> nel(N);
+    std::vector<NodeRec> nr(N);
+    std::vector<NodeEdgeRec> ne(N);
     std::vector<NodeElems> nt(N);
-    std::vector<int> oth((size_t)N * kDeg, 0);
     for (int n = 0; n < N; ++n) {
+        nr[n] = NodeRec{m.nlevels_nod2D[n], (int)I.ln.off[n], (int)I.li.off[n], 0};
         const int deg = m.n2e_ptr[n + 1] - m.n2e_ptr[n];
         if (deg > kDeg || m.nod_in_elem2D_num[n] > kDeg)
             throw std::runtime_error("node degree above " + std::to_string(kDeg) + " is not supported");
+        int nbr[kDeg];
         for (int j = 0; j < kDeg; ++j) {
-            ne[n].e[j] = 0;
-            nel[n].nl[j] = 0;
+            ne[n].oe[j] = -1;
+            ne[n].info[j] = 0;
+            ne[n].oo[j] = 0;
             nt[n].t[j] = -1;
+            nt[n].sl[j] = 0;
+            nbr[j] = -1;
         }
         for (int j = 0; j < deg; ++j) {
-            int se = m.n2e[m.n2e_ptr[n] + j];
-            int e = (se > 0 ? se : -se) - 1;
-            ne[n].e[j] = se;
-            nel[n].nl[j] = (int)((unsigned)em[e].nl >> 24);
-            oth[(size_t)n * kDeg + j] = se > 0 ? em[e].n2 : em[e].n1;
+            const int se = m.n2e[m.n2e_ptr[n] + j];
+            const int e = (se > 0 ? se : -se) - 1;
+            ne[n].oe[j] = (int)I.le.off[e];
+            const int other = se > 0 ? m.edges[2 * e + 1] : m.edges[2 * e];
+            ne[n].info[j] = ((unsigned)em[e].nl >> 24) | (se > 0 ? 256 : 0) | ((m.nlevels_nod2D[other] - 1) << 16);
+            ne[n].oo[j] = (int)I.ln.off[other];
+            nbr[j] = other;
         }
-        for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) nt[n].t[j] = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
+        for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) {
+            const int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
+            nt[n].t[j] = el;
+            int slots = 0;
+            for (int q = 0; q < 3; ++q) {
+                const int node = m.elem2D_nodes[3 * el + q];
+                int slot = -1;
+                if (node == n) slot = kSelf;
+                else
+                    for (int jj = 0; jj < deg; ++jj)
+                        if (nbr[jj] == node) slot = jj;
+                if (slot < 0) throw std::runtime_error("element node is not a 1-ring neighbour");
+                slots |= slot << (4 * q);
+            }
+            nt[n].sl[j] = slots | (m.nlevels[el] << 16);
+        }
     }
+    d.nrec = I.track(dupload(nr));
     d.nedge = I.track(dupload(ne));
-    d.nedgel = I.track(dupload(nel));
     d.nelem = I.track(dupload(nt));
-    if (!I.nother) I.nother = I.track(dupload(oth));
-    auto cv = [](const std::vector<double> &v) { return std::vector<R>(v.begin(), v.end()); };
-    (void)cv;
-    d.area = I.track(dupload(pad_cols<R>(m.area.data(), nl, N, S)));
-    d.areasvol = I.track(dupload(pad_cols<R>(m.areasvol.data(), nl, N, S)));
-    d.hnode = I.track(dupload(pad_cols<R>(m.hnode.data(), nl - 1, N, S)));
-    d.hnode_new = I.track(dupload(pad_cols<R>(m.hnode_new.data(), nl - 1, N, S)));
-    d.Zn = I.track(dupload(pad_cols<R>(m.Z_3d_n.data(), nl - 1, N, S)));
-    d.zbarn = I.track(dupload(pad_cols<R>(m.zbar_3d_n.data(), nl, N, S)));
-    d.helem = I.track(dupload(pad_cols<R>(m.helem.data(), nl - 1, T, S)));
+    d.area = I.track(dupload(I.li.pack<R>(m.area.data(), nl)));
+    d.zbarn = I.track(dupload(I.li.pack<R>(m.zbar_3d_n.data(), nl)));
+    d.areasvol = I.track(dupload(I.ln.pack<R>(m.areasvol.data(), nl)));
+    d.hnode = I.track(dupload(I.ln.pack<R>(m.hnode.data(), nl - 1)));
+    d.hnode_new = I.track(dupload(I.ln.pack<R>(m.hnode_new.data(), nl - 1)));
+    d.Zn = I.track(dupload(I.ln.pack<R>(m.Z_3d_n.data(), nl - 1)));
+    d.helem = I.track(dupload(I.lt.pack<R>(m.helem.data(), nl - 1)));
 }
 
 Engine::Engine(const HostMesh &m, int ntracers, int precision)
@@ -643,36 +726,49 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     if (ntracers < 1) throw std::runtime_error("need at least one tracer");
     if (precision < PREC_FP64 || precision > PREC_FP32_H16) throw std::runtime_error("unknown precision");
     for (int u : m.ulevels) if (u != 1) throw std::runtime_error("cavities (ulevels > 1) are not supported");
-    S_ = std::max(8, (nl_ + 7) / 8 * 8);
+    S_ = 0;
     spec_.assign(ntr_, TracerSpec());
     Impl &I = *p_;
+    {
+        std::vector<int> a(N_), b(N_), c(T_), d(E_);
+        for (int n = 0; n < N_; ++n) { a[n] = m.nlevels_nod2D[n] - 1; b[n] = m.nlevels_nod2D[n]; }
+        for (int el = 0; el < T_; ++el) c[el] = m.nlevels[el] - 1;
+        for (int e = 0; e < E_; ++e) {
+            int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
+            d[e] = std::max(m.nlevels[el1] - 1, el2 >= 0 ? m.nlevels[el2] - 1 : 0);
+        }
+        I.ln.build(a); I.li.build(b); I.lt.build(c); I.le.build(d);
+        if (I.le.total >= (1ll << 31) || I.li.total >= (1ll << 31))
+            throw std::runtime_error("mesh too large for 32-bit column offsets");
+    }
     HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
     const size_t gs = prec_ == PREC_FP64 ? sizeof(double) : (prec_ == PREC_FP32 ? sizeof(float) : sizeof(__half));
-    if (prec_ == PREC_FP64) build_dmesh(I, I.m64, m, S_);
-    else build_dmesh(I, I.m32, m, S_);
-    I.tsn = (size_t)S_ * N_;
-    I.tse = (size_t)S_ * E_;
-    I.tsg = 2 * (size_t)S_ * N_;
+    if (prec_ == PREC_FP64) build_dmesh(I, I.m64, m);
+    else build_dmesh(I, I.m32, m);
+    I.tsn = (size_t)I.ln.total;
+    I.tsi = (size_t)I.li.total;
+    I.tse = (size_t)I.le.total;
     auto zalloc = [&](size_t bytes) {
         void *p = I.track((void *)dalloc<char>(bytes));
         HIP_OK(hipMemset(p, 0, bytes));
         return p;
     };
-    I.vel = zalloc(2 * (size_t)S_ * T_ * rs);
-    I.W = zalloc((size_t)S_ * N_ * rs);
-    I.WE = zalloc((size_t)S_ * N_ * rs);
+    I.vel = zalloc(2 * (size_t)I.lt.total * rs);
+    I.W = zalloc(I.tsi * rs);
+    I.WE = zalloc(I.tsi * rs);
     I.values = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
     I.valuesAB = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
     I.flux_lo = zalloc(I.tse * ntr_ * rs);
     I.adf_h = zalloc(I.tse * ntr_ * rs);
-    I.adf_v = zalloc(I.tsn * ntr_ * rs);
+    I.adf_v = zalloc(I.tsi * ntr_ * rs);
     I.Pp = zalloc(I.tsn * ntr_ * rs);
     I.Pm = zalloc(I.tsn * ntr_ * rs);
     I.LO = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
-    I.gnod = zalloc(I.tsg * ntr_ * gs);
+    I.gnod = zalloc(2 * I.tsn * ntr_ * gs);
     I.dpart = I.track(dalloc<DiagAcc>(I.ndiag));
     I.dout = I.track(dalloc<DiagAcc>(1));
+    I.host_vals.assign(ntr_, std::vector<double>((size_t)(nl_ - 1) * N_, 0.0));
     // FP16 gradient scale: mean edge length (gradient * length = tracer increment)
     double len = 0.0;
     for (int e = 0; e < E_; ++e) len += std::hypot(m.edge_dxdy[2 * e], m.edge_dxdy[2 * e + 1]) * kRE;
@@ -692,44 +788,48 @@ Engine::~Engine()
 void Engine::set_velocity(const double *uv, const double *w, const double *we)
 {
     Impl &I = *p_;
-    const int L = nl_ - 1;
-    auto put = [&](void *dst, const double *src, int Lc, int ncol, int comps) {
+    auto put = [&](void *dst, const Layout &lay, const double *src, int L, int comps) {
         if (prec_ == PREC_FP64) {
-            auto v = pad_cols<double>(src, Lc, ncol, S_, comps);
+            auto v = lay.pack<double>(src, L, comps);
             HIP_OK(hipMemcpy(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
         } else {
-            auto v = pad_cols<float>(src, Lc, ncol, S_, comps);
+            auto v = lay.pack<float>(src, L, comps);
             HIP_OK(hipMemcpy(dst, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice));
         }
     };
-    put(I.vel, uv, L, T_, 2);
-    put(I.W, w, nl_, N_, 1);
-    put(I.WE, we, nl_, N_, 1);
+    put(I.vel, I.lt, uv, nl_ - 1, 2);
+    put(I.W, I.li, w, nl_, 1);
+    put(I.WE, I.li, we, nl_, 1);
 }
 
 void Engine::set_values(int tr, const double *v)
 {
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
-    auto a = pad_cols<double>(v, nl_ - 1, N_, S_);
-    HIP_OK(hipMemcpy(p_->values + tr * p_->tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
+    Impl &I = *p_;
+    std::memcpy(I.host_vals[tr].data(), v, sizeof(double) * I.host_vals[tr].size());
+    auto a = I.ln.pack<double>(v, nl_ - 1);
+    HIP_OK(hipMemcpy(I.values + tr * I.tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
 void Engine::set_valuesAB(int tr, const double *v)
 {
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
-    auto a = pad_cols<double>(v, nl_ - 1, N_, S_);
-    HIP_OK(hipMemcpy(p_->valuesAB + tr * p_->tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
+    Impl &I = *p_;
+    auto a = I.ln.pack<double>(v, nl_ - 1);
+    HIP_OK(hipMemcpy(I.valuesAB + tr * I.tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
 void Engine::get_values(int tr, double *v)
 {
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
-    std::vector<double> a((size_t)S_ * N_);
+    Impl &I = *p_;
+    std::vector<double> a(I.tsn);
     HIP_OK(hipStreamSynchronize(stream_));
-    HIP_OK(hipMemcpy(a.data(), p_->values + tr * p_->tsn, a.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(a.data(), I.values + tr * I.tsn, a.size() * sizeof(double), hipMemcpyDeviceToHost));
     const int L = nl_ - 1;
+    std::memcpy(v, I.host_vals[tr].data(), sizeof(double) * I.host_vals[tr].size());
     for (int n = 0; n < N_; ++n)
-        for (int k = 0; k < L; ++k) v[(size_t)n * L + k] = a[(size_t)n * S_ + k];
+        for (int k = 0; k < I.ln.len[n]; ++k) v[(size_t)n * L + k] = a[(size_t)I.ln.off[n] + k];
 }
 
 struct LaunchHook {
@@ -750,21 +850,29 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
                          hipStream_t st, LaunchHook &hk)
 {
     const R *vel = (const R *)I.vel, *W = (const R *)I.W, *WE = (const R *)I.WE;
-    const size_t tsn = I.tsn, tse = I.tse, tsg = I.tsg;
+    const size_t tsn = I.tsn, tsi = I.tsi, tse = I.tse;
     double *vals = I.values + t0 * tsn;
     const double *ab = refresh ? vals : I.valuesAB + t0 * tsn;
     R *flux_lo = (R *)I.flux_lo + t0 * tse, *adf_h = (R *)I.adf_h + t0 * tse;
-    R *adf_v = (R *)I.adf_v + t0 * tsn, *Pp = (R *)I.Pp + t0 * tsn, *Pm = (R *)I.Pm + t0 * tsn;
+    R *adf_v = (R *)I.adf_v + t0 * tsi, *Pp = (R *)I.Pp + t0 * tsn, *Pm = (R *)I.Pm + t0 * tsn;
     double *LO = I.LO + t0 * tsn;
-    G *gnod = (G *)I.gnod + t0 * tsg;
+    G *gnod = (G *)I.gnod + 2 * t0 * tsn;
     const dim3 blk(kBlock), ge(grid_for(m.E)), gn(grid_for(m.N));
     const bool fct = sp.lim == LIM_FCT;
     const R ph = (R)sp.ph, pv = (R)sp.pv;
     if (sp.hor == HOR_MUSCL || sp.hor == HOR_MFCT) {
-        hipLaunchKernelGGL((k_grad<R, G, NT>), gn, blk, 0, st, m, ab, tsn, gnod, tsg, I.gscale);
+        hipLaunchKernelGGL((k_grad<R, G, NT>), gn, blk, 0, st, m, ab, tsn, gnod, I.gscale);
         hk.mark("k_grad", st);
     }
-#define EDGE(HOR, F) hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT>), ge, blk, 0, st, m, vel, vals, ab, tsn, gnod, tsg, ph, I.ginv, flux_lo, adf_h, tse)
+#define EDGE(HOR, F)                                                                                        \
+    do {                                                                                                    \
+        if (refresh)                                                                                        \
+            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, true>), ge, blk, 0, st, m, vel, vals, ab, tsn, gnod, ph, \
+                               I.ginv, flux_lo, adf_h, tse);                                                \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, false>), ge, blk, 0, st, m, vel, vals, ab, tsn, gnod, ph, \
+                               I.ginv, flux_lo, adf_h, tse);                                                \
+    } while (0)
     if (fct) {
         if (sp.hor == HOR_MUSCL) EDGE(HOR_MUSCL, true);
         else if (sp.hor == HOR_MFCT) EDGE(HOR_MFCT, true);
@@ -777,7 +885,15 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
 #undef EDGE
     hk.mark("k_edge", st);
     if (fct) {
-#define NLO(V) hipLaunchKernelGGL((k_node_lo<R, V, NT>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, flux_lo, adf_h, tse, dt, pv, LO, adf_v, Pp, Pm)
+#define NLO(V)                                                                                              \
+    do {                                                                                                    \
+        if (refresh)                                                                                        \
+            hipLaunchKernelGGL((k_node_lo<R, V, NT, true>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, tsi, flux_lo, \
+                               adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_node_lo<R, V, NT, false>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, tsi, flux_lo, \
+                               adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
+    } while (0)
         switch (sp.ver) {
         case VER_QR4C: NLO(VER_QR4C); break;
         case VER_PPM: NLO(VER_PPM); break;
@@ -788,8 +904,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
         hk.mark("k_node_lo", st);
         hipLaunchKernelGGL((k_node_limit<R, NT>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm);
         hk.mark("k_node_limit", st);
-        hipLaunchKernelGGL((k_node_update<R, NT>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v, Pp, Pm,
-                           I.nother, dt);
+        hipLaunchKernelGGL((k_node_update<R, NT>), gn, blk, 0, st, m, vals, LO, tsn, tsi, adf_h, tse, adf_v, Pp, Pm, dt);
         hk.mark("k_node_update", st);
     } else {
 #define NNL(V) hipLaunchKernelGGL((k_node_nolim<R, V, NT>), gn, blk, 0, st, m, WE, vals, ab, tsn, adf_h, tse, dt, pv)
@@ -878,7 +993,19 @@ Diag Engine::diagnostics(int tr)
     DiagAcc a;
     HIP_OK(hipMemcpyAsync(&a, I.dout, sizeof(a), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
-    return Diag{a.vmin, a.vmax, a.sum, a.amin, a.amax, a.mass, a.nan};
+    // the levels below the bottom (never stepped) from the host copy
+    Diag d{a.vmin, a.vmax, a.sum, a.amin, a.amax, a.mass, a.nan};
+    const int L = nl_ - 1;
+    const std::vector<double> &h = I.host_vals[tr];
+    for (int n = 0; n < N_; ++n)
+        for (int k = I.ln.len[n]; k < L; ++k) {
+            const double x = h[(size_t)n * L + k];
+            if (x != x) { d.nan_count += 1.0; continue; }
+            d.vmin = std::min(d.vmin, x);
+            d.vmax = std::max(d.vmax, x);
+            d.sum += x;
+        }
+    return d;
 }
 
 size_t Engine::work_count(const std::string &name) const
@@ -898,25 +1025,27 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
     if (need == 0 || count < need || tr < 0 || tr >= ntr_) return -1;
     HIP_OK(hipStreamSynchronize(stream_));
     const char *src = nullptr;
-    int L = nl_ - 1, ncol = N_, comps = 1;
+    const Layout *lay = &I.ln;
+    int L = nl_ - 1, comps = 1;
     size_t es = prec_ == PREC_FP64 ? 8 : 4;
-    size_t ts = I.tsn;
     if (name == "fct_LO") { src = (const char *)I.LO; es = 8; }
     else if (name == "fct_plus") src = (const char *)I.Pp;
     else if (name == "fct_minus") src = (const char *)I.Pm;
-    else if (name == "adv_flux_hor") { src = (const char *)I.adf_h; ncol = E_; ts = I.tse; }
-    else if (name == "adv_flux_lo") { src = (const char *)I.flux_lo; ncol = E_; ts = I.tse; }
-    else if (name == "adv_flux_ver") { src = (const char *)I.adf_v; L = nl_; }
+    else if (name == "adv_flux_hor") { src = (const char *)I.adf_h; lay = &I.le; }
+    else if (name == "adv_flux_lo") { src = (const char *)I.flux_lo; lay = &I.le; }
+    else if (name == "adv_flux_ver") { src = (const char *)I.adf_v; lay = &I.li; L = nl_; }
     else if (name == "gnod") {
-        src = (const char *)I.gnod; comps = 2; ts = I.tsg;
+        src = (const char *)I.gnod; comps = 2;
         es = prec_ == PREC_FP64 ? 8 : (prec_ == PREC_FP32 ? 4 : 2);
     }
-    std::vector<char> buf((size_t)S_ * ncol * comps * es);
-    HIP_OK(hipMemcpy(buf.data(), src + tr * ts * es, buf.size(), hipMemcpyDeviceToHost));
-    for (int c = 0; c < ncol; ++c)
-        for (int k = 0; k < L; ++k)
+    const size_t per = (size_t)lay->total * comps;
+    std::vector<char> buf(per * es);
+    HIP_OK(hipMemcpy(buf.data(), src + tr * per * es, buf.size(), hipMemcpyDeviceToHost));
+    std::fill(out, out + need, 0.0);
+    for (size_t c = 0; c < lay->len.size(); ++c)
+        for (int k = 0; k < lay->len[c]; ++k)
             for (int q = 0; q < comps; ++q) {
-                size_t i = ((size_t)c * S_ + k) * comps + q;
+                size_t i = ((size_t)lay->off[c] + k) * comps + q;
                 double v;
                 if (es == 8) v = ((const double *)buf.data())[i];
                 else if (es == 4) v = ((const float *)buf.data())[i];

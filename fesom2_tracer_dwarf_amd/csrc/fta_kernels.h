@@ -29,46 +29,72 @@ constexpr int kBlock = kWave * kWavesPerBlock;
 constexpr int kDeg = 8;  // padded node degree (incident edges / elements)
 constexpr double kRE = 6367500.0;  // r_earth, lib/oce_modules.F90:24
 
-// per-edge record (32 bytes): nodes, elements, up/down-wind triangles, and
-// packed level counts: nl1 | nl2 << 8 | nzmax_e << 16 | nlmax << 24 where
-// nzmax_e = min(nlevels_nod2D_min(n1), nlevels_nod2D_min(n2)) bounds the
-// levels shared by all elements around both nodes (fill_up_dn_grad), and
+// Compact layout: every level-indexed array stores only the wet levels of
+// each column, columns back to back: a node's layers 1..nlev-1 start at its
+// layer offset, its interfaces 1..nlev at its interface offset (= layer
+// offset + node index), an element's layers at its element offset, an edge's
+// levels 1..nlmax at its edge offset. The records below carry these offsets
+// directly, so a gather never waits on a separate offset load.
+
+// per-edge record (64 bytes): layer offsets of both nodes and both elements,
+// the up/down-wind triangles, packed level counts
+// nl = nl1 | nl2 << 8 | nzmax_e << 16 | nlmax << 24, where nzmax_e =
+// min(nlevels_nod2D_min(n1), nlevels_nod2D_min(n2)) bounds the levels shared
+// by all elements around both nodes (fill_up_dn_grad), and
 // nb = nboundary_lay(n1) | nboundary_lay(n2) << 16
-struct alignas(32) EdgeMeta {
-    int n1, n2, el1, el2, up, dn, nl, nb;
+struct alignas(64) EdgeMeta {
+    int o1, o2;   // node layer offsets
+    int t1, t2;   // element layer offsets (t2 = -1 on a boundary edge)
+    int up, dn;   // up/down-wind triangle indices (-1 = none)
+    int nl, nb;
+    int oe;       // this edge's offset in the edge arrays
+    int el2;      // second element index (-1 = boundary)
+    int pad[6];
 };
 
-// per-element record: nodes and nlevels
+// per-element record: layer offsets of its three nodes and nlevels
 struct alignas(16) ElemMeta {
-    int n0, n1, n2, nlev;
+    int o0, o1, o2, nlev;
 };
 
-// per-node record: nlevels_nod2D and incident-edge / element lists padded to
-// kDeg. edge entry = +(e+1) if the node is edges(1,e), -(e+1) if edges(2,e),
-// 0 = empty; enl = max(nlevels(el1), nlevels(el2)) - 1 of that edge.
-struct alignas(32) NodeEdges {
-    int e[kDeg];
+// per-node record: nlevels_nod2D, layer offset, interface offset
+struct alignas(16) NodeRec {
+    int nlev, off, offi, pad;
 };
-struct alignas(32) NodeEdgeLev {
-    int nl[kDeg];
+
+// incident edges of a node padded to kDeg, sorted by edge index:
+// oe = edge offset (-1 = empty); info = nlmax | 256 if the node is edges(1)
+// | (layers of the other node) << 16; oo = layer offset of the edge's other
+// node (the node's 1-ring neighbours, slot j)
+struct alignas(32) NodeEdgeRec {
+    int oe[kDeg];
+    int info[kDeg];
+    int oo[kDeg];
 };
+// elements around a node (nod_in_elem2D order), padded to kDeg: element
+// index (-1 = empty) and sl = s0 | s1 << 4 | s2 << 8 | nlevels << 16, where
+// s0..s2 locate the element's nodes 1..3 in the node's neighbourhood:
+// slot j < kDeg = the other node of incident edge j, slot 15 = the node itself
 struct alignas(32) NodeElems {
-    int t[kDeg];  // -1 = empty, nod_in_elem2D order
+    int t[kDeg];
+    int sl[kDeg];
 };
+constexpr int kSelf = 15;
 
 template <class R>
 struct DMesh {
-    int N, T, E, nl, S;
+    int N, T, E, nl;
+    long long CN, CI, CT, CE;   // cell counts: node layers, node interfaces, element layers, edge levels
+    const NodeRec *nrec;        // (N)
     const EdgeMeta *emeta;      // (E)
     const R *egeo;              // (8,E): dx1 dy1 dx2 dy2 (edge_cross_dxdy) ex ey (edge_dxdy) a
     const ElemMeta *tmeta;      // (T)
     const R *tgeo;              // (8,T): gradient_sca(1:6), elem_area, 0
-    const int *nlev_n;          // nlevels_nod2D (N)
-    const NodeEdges *nedge;     // (N)
-    const NodeEdgeLev *nedgel;  // (N)
+    const NodeEdgeRec *nedge;   // (N)
     const NodeElems *nelem;     // (N)
-    const R *area, *areasvol, *hnode, *hnode_new, *Zn, *zbarn;  // (S,N)
-    const R *helem;             // (S,T)
+    const R *area, *zbarn;      // interface arrays (CI)
+    const R *areasvol, *hnode, *hnode_new, *Zn;  // layer arrays (CN)
+    const R *helem;             // element layer array (CT)
 };
 
 __device__ __forceinline__ int wave_column()
