@@ -2,14 +2,15 @@
 //
 // One step (reference: do_oce_adv_tra, lib/oce_adv_tra_driver.F90:23-294,
 // plus the caller's update, src/fesom_analytic.F90:405-437) for a batch of
-// NT tracers that share their schemes is five launches:
-//   k_grad        nodal mean of the elemental gradients of valuesAB (MUSCL/MFCT)
+// NT tracers that share their schemes is six launches:
+//   k_trxy        per element-level: elemental gradient of valuesAB (MUSCL/MFCT)
+//   k_gnod        per node-level: area-weighted mean of those gradients
 //   k_edge        per edge-level: LO upwind flux and HO-LO antidiffusive flux
 //   k_node_lo     per node-level: LO solution, vertical HO-LO flux, P+/P- sums
 //   k_node_limit  per node-level: Zalesak bounds (a1-a3) and R+/R- (b2)
 //   k_node_update per node-level: limited fluxes (b3), divergence, T update
-// (without FCT the last three collapse into k_node_nolim). Elemental
-// gradients are recomputed from their three nodes where needed, never stored.
+// (without FCT the last three collapse into k_node_nolim). Every kernel runs
+// one wave per 128-cell chunk of its layout, two cells per lane.
 #include "fta_engine.h"
 #include "fta_kernels.h"
 #include "fta_vertical.h"
@@ -36,79 +37,122 @@ void set_error(const std::string &s) { g_last_error = s; }
                                      " at " #call);                                           \
     } while (0)
 
-// the elemental gradient of tracer A at layer k (tracer_gradient_elements:
-// gradient_sca(1:3)*T(nodes) and gradient_sca(4:6)*T(nodes), node order 1,2,3)
+// node->edge slot info: nlmax | 64 if the node is edges(1) | other node's layers << 7
+__device__ __forceinline__ int info_nlmax(unsigned i) { return (int)(i & 63u); }
+__device__ __forceinline__ bool info_pos(unsigned i) { return (i & 64u) != 0; }
+
 template <class R>
-__device__ __forceinline__ void elem_gradient(const ElemMeta &em, const R *g, const double *A, int k, R &gx, R &gy)
+__device__ __forceinline__ void load_node_edges(const DMesh<R> &m, int n, bool live, int (&oe)[kDeg],
+                                                unsigned (&info)[kDeg])
 {
-    const R A0 = (R)A[em.o0 + k], A1 = (R)A[em.o1 + k], A2 = (R)A[em.o2 + k];
-    gx = g[0] * A0 + g[1] * A1 + g[2] * A2;
-    gy = g[3] * A0 + g[4] * A1 + g[5] * A2;
+    if (live) {
+        const int4 *q = reinterpret_cast<const int4 *>(&m.nedge[n]);
+        const int4 a = q[0], b = q[1];
+        oe[0] = a.x; oe[1] = a.y; oe[2] = a.z; oe[3] = a.w;
+        oe[4] = b.x; oe[5] = b.y; oe[6] = b.z; oe[7] = b.w;
+        const uint4 u = *reinterpret_cast<const uint4 *>(&m.ninfo[n]);
+        const unsigned w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            info[2 * j] = w[j] & 0xFFFFu;
+            info[2 * j + 1] = w[j] >> 16;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kDeg; ++j) { oe[j] = -1; info[j] = 0; }
+    }
+}
+
+template <class R>
+__device__ __forceinline__ void load_other_nodes(const DMesh<R> &m, int n, bool live, int (&oo)[kDeg])
+{
+    if (live) {
+        const int4 *q = reinterpret_cast<const int4 *>(&m.nedge[n]);
+        const int4 a = q[2], b = q[3];
+        oo[0] = a.x; oo[1] = a.y; oo[2] = a.z; oo[3] = a.w;
+        oo[4] = b.x; oo[5] = b.y; oo[6] = b.z; oo[7] = b.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kDeg; ++j) oo[j] = 0;
+    }
 }
 
 // ===========================================================================
-// MUSCL prerequisite: area-weighted mean over the node's elements that have
-// layer nz of their elemental gradients (fill_up_dn_grad, lib/oce_muscl_adv.F90:397-407)
+// MUSCL prerequisites (fill_up_dn_grad, lib/oce_muscl_adv.F90:361-530):
+// elemental gradients of valuesAB (tracer_gradient_elements:
+// gradient_sca(1:3)*T(nodes), gradient_sca(4:6)*T(nodes), node order 1,2,3)
+// and their area-weighted mean over the node's elements that have the layer
+// (:397-407)
 // ===========================================================================
-// value of neighbourhood slot s (wave-uniform): incident-edge neighbour s, or the node itself
-template <class R>
-__device__ __forceinline__ R pick(const R (&nb)[kDeg], R self, int s)
+template <class R, class G, int NT>
+__global__ __launch_bounds__(kBlock) void k_trxy(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
+                                                 G *__restrict__ trxy, size_t tst, float gscale)
 {
-    R v = self;  // branch-free select chain on the uniform slot
+    const int ch = wave_chunk();
+    if (ch >= m.nch_t) return;
+    const Pair p = lane_pair(m.slot_t, ch);
+    if (!p.live) return;
+    const ElemRec er = m.trec[p.col];
+    const bool v1 = p.k0 + 1 < er.nlev - 1;
+    const R *g = m.tgeo + 8 * (size_t)p.col;
+    const R g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5];
 #pragma unroll
-    for (int j = 0; j < kDeg; ++j) v = (s == j) ? nb[j] : v;
-    return v;
-}
-
-// the 1-ring neighbours' values of array A at this level (each distinct
-// column once; a neighbour shallower than this level is not read)
-template <class R>
-__device__ __forceinline__ void load_ring(const NodeEdgeRec &ne, const double *A, int k, R (&nb)[kDeg])
-{
-    const int nz = k + 1;
-#pragma unroll
-    for (int j = 0; j < kDeg; ++j) {
-        const bool ok = ne.oe[j] >= 0 && nz <= ((ne.info[j] >> 16) & 255);
-        nb[j] = ok ? (R)A[(size_t)ne.oo[j] + k] : (R)0;
+    for (int t = 0; t < NT; ++t) {
+        const double *A = ab + t * tsn + p.k0;
+        const V2<double> a0 = ld2(A + er.o0), a1 = ld2(A + er.o1), a2 = ld2(A + er.o2);
+        const R gx0 = g0 * (R)a0.x + g1 * (R)a1.x + g2 * (R)a2.x;
+        const R gy0 = g3 * (R)a0.x + g4 * (R)a1.x + g5 * (R)a2.x;
+        R gx1 = 0, gy1 = 0;
+        if (v1) {
+            gx1 = g0 * (R)a0.y + g1 * (R)a1.y + g2 * (R)a2.y;
+            gy1 = g3 * (R)a0.y + g4 * (R)a1.y + g5 * (R)a2.y;
+        }
+        GradIO<G>::st(trxy + t * tst + 2 * p.c, gx0, gy0, gx1, gy1, gscale);
     }
 }
 
 template <class R, class G, int NT>
-__global__ __launch_bounds__(kBlock) void k_grad(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
-                                                 G *__restrict__ gnod, float gscale)
+__global__ __launch_bounds__(kBlock) void k_gnod(DMesh<R> m, const G *__restrict__ trxy, size_t tst,
+                                                 G *__restrict__ gnod, size_t tsg, float gscale, float ginv)
 {
-    const int n = wave_column();
-    if (n >= m.N) return;
-    const int k = lane_id(), nz = k + 1;
-    const NodeRec nr = m.nrec[n];
-    if (nz > nr.nlev - 1) return;
-    const NodeElems ne = m.nelem[n];
-    R tvol = 0, tx[NT], ty[NT];
+    const int ch = wave_chunk();
+    if (ch >= m.nch_n) return;
+    const Pair p = lane_pair(m.slot_n, ch);
+    if (!p.live) return;
+    const int L = m.nlev[p.col] - 1;
+    if (p.k0 >= L) return;  // the pair holds only the bottom interface
+    const bool v1 = p.k0 + 1 < L;
+    const NodeElems ne = m.nelem[p.col];
+    const R *ar = m.nelem_area + kDeg * (size_t)p.col;
+    R tvol0 = 0, tvol1 = 0;
+    R tx0[NT], ty0[NT], tx1[NT], ty1[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) { tx[t] = 0; ty[t] = 0; }
+    for (int t = 0; t < NT; ++t) { tx0[t] = 0; ty0[t] = 0; tx1[t] = 0; ty1[t] = 0; }
 #pragma unroll
     for (int j = 0; j < kDeg; ++j) {
-        const int el = ne.t[j];
-        if (el < 0) break;
-        const ElemMeta em = m.tmeta[el];
-        if (em.nlev - 1 < nz) continue;
-        const R *g = m.tgeo + 8 * (size_t)el;
-        const R a = g[6];
-        tvol = tvol + a;
+        const int ot = ne.ot[j];
+        if (ot < 0) break;
+        const int le = ne.le[j];
+        if (p.k0 >= le) continue;
+        const bool a1 = p.k0 + 1 < le;
+        const R a = ar[j];
+        tvol0 = tvol0 + a;
+        if (a1) tvol1 = tvol1 + a;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            R gx, gy;
-            elem_gradient(em, g, ab + t * tsn, k, gx, gy);
-            tx[t] = tx[t] + gx * a;
-            ty[t] = ty[t] + gy * a;
+            const auto gg = GradIO<G>::ld(trxy + t * tst + 2 * ((size_t)ot + p.k0), ginv);
+            tx0[t] = tx0[t] + (R)gg.x0 * a;
+            ty0[t] = ty0[t] + (R)gg.y0 * a;
+            if (a1) {
+                tx1[t] = tx1[t] + (R)gg.x1 * a;
+                ty1[t] = ty1[t] + (R)gg.y1 * a;
+            }
         }
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        G *o = gnod + 2 * (t * tsn + nr.off + k);
-        GradIO<G>::st(o, tx[t] / tvol, gscale);
-        GradIO<G>::st(o + 1, ty[t] / tvol, gscale);
-    }
+    for (int t = 0; t < NT; ++t)
+        GradIO<G>::st(gnod + t * tsg + 2 * p.c, tx0[t] / tvol0, ty0[t] / tvol0, v1 ? tx1[t] / tvol1 : (R)0,
+                      v1 ? ty1[t] / tvol1 : (R)0, gscale);
 }
 
 // ===========================================================================
@@ -116,127 +160,148 @@ __global__ __launch_bounds__(kBlock) void k_grad(DMesh<R> m, const double *__res
 // MUSCL (:265-550), MFCT (:554-846). Segments C (both elements), D (el1
 // only) and E (el2 only); no cavities, so segments A/B are empty.
 // FCT: writes the LO flux (from ttf) and HO-LO (HO from ttfAB) per level.
-// No limiter: writes the HO flux. Gradients follow fill_up_dn_grad
-// (lib/oce_muscl_adv.F90:361-530): up/down-wind triangle on levels shared by
-// all elements around both nodes, nodal means below.
+// No limiter: writes the HO flux. Gradients follow fill_up_dn_grad: the
+// up/down-wind triangle on levels shared by all elements around both nodes,
+// nodal means below.
 // ===========================================================================
 template <class R, class G, int HOR, bool FCT, int NT, bool SAME>
 __global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict__ vel,
                                                  const double *__restrict__ ttf,
                                                  const double *__restrict__ ttfAB, size_t tsn,
-                                                 const G *__restrict__ gnod,
+                                                 const G *__restrict__ trxy, size_t tst,
+                                                 const G *__restrict__ gnod, size_t tsg,
                                                  R num_ord, float ginv,
                                                  R *__restrict__ flux_lo, R *__restrict__ flux, size_t tse)
 {
-    const int e = wave_column();
-    if (e >= m.E) return;
-    const int k = lane_id(), nz = k + 1;
-    const EdgeMeta em = m.emeta[e];
-    const int nl1 = em.nl & 255, nl2 = (em.nl >> 8) & 255, nzmax_e = (em.nl >> 16) & 255;
-    const int nlmax = (int)((unsigned)em.nl >> 24);
+    const int ch = wave_chunk();
+    if (ch >= m.nch_e) return;
+    const Pair p = lane_pair(m.slot_e, ch);
+    if (!p.live) return;
+    const EdgeRec er = m.erec[p.col];
+    const int nl1 = er.nl & 255, nl2 = (er.nl >> 8) & 255, nzmax_e = (er.nl >> 16) & 255;
+    const int nlmax = (int)((unsigned)er.nl >> 24);
     const int nl12 = min(nl1, nl2);
-    if (nz > nlmax) return;
-    const R *geo = m.egeo + 8 * (size_t)e;
+    const int k0 = p.k0, nz0 = k0 + 1, nz1 = k0 + 2;
+    const bool valid1 = nz1 <= nlmax;
+    const R *geo = m.egeo + 8 * (size_t)p.col;
     const R dx1 = geo[0], dy1 = geo[1], dx2 = geo[2], dy2 = geo[3];
-    const bool has1 = nz <= nl1;
-    const bool has2 = em.el2 >= 0 && (nz <= nl12 || nz > nl1);
-    R vf1 = 0, vf2 = 0;
-    if (has1) {
-        const R *u = vel + 2 * ((size_t)em.t1 + k);
-        vf1 = (-u[1] * dx1 + u[0] * dy1) * m.helem[em.t1 + k];
+    const bool el2 = er.t2 >= 0;
+    const bool h10 = nz0 <= nl1, h11 = valid1 && nz1 <= nl1;
+    const bool h20 = el2 && (nz0 <= nl12 || nz0 > nl1);
+    const bool h21 = el2 && valid1 && (nz1 <= nl12 || nz1 > nl1);
+    R vf10 = 0, vf11 = 0, vf20 = 0, vf21 = 0;
+    if (h10) {
+        const V4<R> u = ld4(vel + 2 * ((size_t)er.t1 + k0));
+        const V2<R> h = ld2(m.helem + er.t1 + k0);
+        vf10 = (-u.y0 * dx1 + u.x0 * dy1) * h.x;
+        vf11 = (-u.y1 * dx1 + u.x1 * dy1) * h.y;
     }
-    if (has2) {
-        const R *u = vel + 2 * ((size_t)em.t2 + k);
-        vf2 = (u[1] * dx2 - u[0] * dy2) * m.helem[em.t2 + k];
+    if (h20) {
+        const V4<R> u = ld4(vel + 2 * ((size_t)er.t2 + k0));
+        const V2<R> h = ld2(m.helem + er.t2 + k0);
+        vf20 = (u.y0 * dx2 - u.x0 * dy2) * h.x;
+        vf21 = (u.y1 * dx2 - u.x1 * dy2) * h.y;
     }
-    const R vflux = (has1 && has2) ? vf1 + vf2 : (has1 ? vf1 : vf2);
-    const size_t o = (size_t)em.oe + k;
-    const size_t o1 = (size_t)em.o1 + k, o2 = (size_t)em.o2 + k;
+    const R vflux0 = (h10 && h20) ? vf10 + vf20 : (h10 ? vf10 : vf20);
+    const R vflux1 = (h11 && h21) ? vf11 + vf21 : (h11 ? vf11 : (h21 ? vf21 : (R)0));
 
     // MUSCL/MFCT: reconstruction geometry, gradient source per level
-    R ex = 0, ey = 0, a = 0, clo1 = 1, clo2 = 1;
-    bool shared = false;
-    ElemMeta tu{}, td{};
-    const R *gu = nullptr, *gd = nullptr;
+    R ex = 0, ey = 0, a = 0, clo10 = 1, clo20 = 1, clo11 = 1, clo21 = 1;
+    bool sh0 = false, sh1 = false;
     if constexpr (HOR != HOR_UPW1) {
         ex = geo[4]; ey = geo[5]; a = geo[6];
         if constexpr (HOR == HOR_MUSCL) {
-            clo1 = ((em.nb & 0xFFFF) - nz) >= 0 ? (R)1 : (R)0;
-            clo2 = ((em.nb >> 16) - nz) >= 0 ? (R)1 : (R)0;
+            const int nb1 = er.nb & 0xFFFF, nb2 = er.nb >> 16;
+            clo10 = (nb1 - nz0) >= 0 ? (R)1 : (R)0;
+            clo20 = (nb2 - nz0) >= 0 ? (R)1 : (R)0;
+            clo11 = (nb1 - nz1) >= 0 ? (R)1 : (R)0;
+            clo21 = (nb2 - nz1) >= 0 ? (R)1 : (R)0;
         }
-        const bool updn = em.up >= 0 && em.dn >= 0;
-        shared = updn && nz <= nzmax_e - 1;
-        const int up = updn ? em.up : 0, dn = updn ? em.dn : 0;
-        tu = m.tmeta[up];
-        td = m.tmeta[dn];
-        gu = m.tgeo + 8 * (size_t)up;
-        gd = m.tgeo + 8 * (size_t)dn;
+        const bool updn = er.tu >= 0 && er.td >= 0;
+        sh0 = updn && nz0 <= nzmax_e - 1;
+        sh1 = updn && nz1 <= nzmax_e - 1;
     }
+    const size_t o = (size_t)p.c;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        R lo = 0;
+        const size_t tn = t * tsn;
+        R lo0 = 0, lo1 = 0;
+        V2<double> T1{0.0, 0.0}, T2{0.0, 0.0};
+        if constexpr (FCT || SAME) {
+            T1 = ld2(ttf + tn + er.o1 + k0);
+            T2 = ld2(ttf + tn + er.o2 + k0);
+        }
         if constexpr (FCT) {
-            const R T1 = (R)ttf[t * tsn + o1], T2 = (R)ttf[t * tsn + o2];
-            lo = (R)(-0.5) * (T1 * (vflux + fabs(vflux)) + T2 * (vflux - fabs(vflux))) - (R)0;
-            flux_lo[t * tse + o] = lo;
+            lo0 = (R)(-0.5) * ((R)T1.x * (vflux0 + fabs(vflux0)) + (R)T2.x * (vflux0 - fabs(vflux0))) - (R)0;
+            if (valid1)
+                lo1 = (R)(-0.5) * ((R)T1.y * (vflux1 + fabs(vflux1)) + (R)T2.y * (vflux1 - fabs(vflux1))) - (R)0;
+            st2(flux_lo + t * tse + o, lo0, lo1);
         }
-        const double *AB = ttfAB + t * tsn;
-        R A1, A2;
-        if constexpr (FCT && SAME) {
-            A1 = (R)ttf[t * tsn + o1];  // the same loads as T1, T2 above
-            A2 = (R)ttf[t * tsn + o2];
-        } else {
-            A1 = (R)AB[o1];
-            A2 = (R)AB[o2];
+        V2<double> B1 = T1, B2 = T2;
+        if constexpr (!SAME) {
+            B1 = ld2(ttfAB + tn + er.o1 + k0);
+            B2 = ld2(ttfAB + tn + er.o2 + k0);
         }
-        R ho;
+        const R A10 = (R)B1.x, A20 = (R)B2.x, A11 = (R)B1.y, A21 = (R)B2.y;
+        R ho0, ho1;
         if constexpr (HOR == HOR_UPW1) {
-            ho = (R)(-0.5) * (A1 * (vflux + fabs(vflux)) + A2 * (vflux - fabs(vflux)));
+            ho0 = (R)(-0.5) * (A10 * (vflux0 + fabs(vflux0)) + A20 * (vflux0 - fabs(vflux0)));
+            ho1 = (R)(-0.5) * (A11 * (vflux1 + fabs(vflux1)) + A21 * (vflux1 - fabs(vflux1)));
         } else {
-            R g1x, g1y, g2x, g2y;
-            if (shared) {
-                elem_gradient(tu, gu, AB, k, g1x, g1y);
-                elem_gradient(td, gd, AB, k, g2x, g2y);
-            } else {
-                const G *p1 = gnod + 2 * (t * tsn + o1), *p2 = gnod + 2 * (t * tsn + o2);
-                g1x = (R)GradIO<G>::ld(p1, ginv); g1y = (R)GradIO<G>::ld(p1 + 1, ginv);
-                g2x = (R)GradIO<G>::ld(p2, ginv); g2y = (R)GradIO<G>::ld(p2 + 1, ginv);
+            V4<R> gu{0, 0, 0, 0}, gd{0, 0, 0, 0}, n1{0, 0, 0, 0}, n2{0, 0, 0, 0};
+            if (sh0) {
+                const auto u = GradIO<G>::ld(trxy + t * tst + 2 * ((size_t)er.tu + k0), ginv);
+                const auto d = GradIO<G>::ld(trxy + t * tst + 2 * ((size_t)er.td + k0), ginv);
+                gu = {(R)u.x0, (R)u.y0, (R)u.x1, (R)u.y1};
+                gd = {(R)d.x0, (R)d.y0, (R)d.x1, (R)d.y1};
             }
-            const R Tmean2 = A2 - ((R)2 * (A2 - A1) + ex * a * g2x + ey * (R)kRE * g2y) / (R)6 * clo2;
-            const R Tmean1 = A1 + ((R)2 * (A2 - A1) + ex * a * g1x + ey * (R)kRE * g1y) / (R)6 * clo1;
-            const R cHO = (vflux + fabs(vflux)) * Tmean1 + (vflux - fabs(vflux)) * Tmean2;
-            ho = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux * num_ord * (R)0.5 * (Tmean1 + Tmean2);
+            if (!sh0 || (valid1 && !sh1)) {
+                const auto u = GradIO<G>::ld(gnod + t * tsg + 2 * ((size_t)er.o1 + k0), ginv);
+                const auto d = GradIO<G>::ld(gnod + t * tsg + 2 * ((size_t)er.o2 + k0), ginv);
+                n1 = {(R)u.x0, (R)u.y0, (R)u.x1, (R)u.y1};
+                n2 = {(R)d.x0, (R)d.y0, (R)d.x1, (R)d.y1};
+            }
+            const R g1x0 = sh0 ? gu.x0 : n1.x0, g1y0 = sh0 ? gu.y0 : n1.y0;
+            const R g2x0 = sh0 ? gd.x0 : n2.x0, g2y0 = sh0 ? gd.y0 : n2.y0;
+            const R g1x1 = sh1 ? gu.x1 : n1.x1, g1y1 = sh1 ? gu.y1 : n1.y1;
+            const R g2x1 = sh1 ? gd.x1 : n2.x1, g2y1 = sh1 ? gd.y1 : n2.y1;
+            {
+                const R Tmean2 = A20 - ((R)2 * (A20 - A10) + ex * a * g2x0 + ey * (R)kRE * g2y0) / (R)6 * clo20;
+                const R Tmean1 = A10 + ((R)2 * (A20 - A10) + ex * a * g1x0 + ey * (R)kRE * g1y0) / (R)6 * clo10;
+                const R cHO = (vflux0 + fabs(vflux0)) * Tmean1 + (vflux0 - fabs(vflux0)) * Tmean2;
+                ho0 = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux0 * num_ord * (R)0.5 * (Tmean1 + Tmean2);
+            }
+            {
+                const R Tmean2 = A21 - ((R)2 * (A21 - A11) + ex * a * g2x1 + ey * (R)kRE * g2y1) / (R)6 * clo21;
+                const R Tmean1 = A11 + ((R)2 * (A21 - A11) + ex * a * g1x1 + ey * (R)kRE * g1y1) / (R)6 * clo11;
+                const R cHO = (vflux1 + fabs(vflux1)) * Tmean1 + (vflux1 - fabs(vflux1)) * Tmean2;
+                ho1 = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux1 * num_ord * (R)0.5 * (Tmean1 + Tmean2);
+            }
         }
-        flux[t * tse + o] = ho - lo;
+        st2(flux + t * tse + o, ho0 - lo0, valid1 ? ho1 - lo1 : (R)0);
     }
 }
 
-// decoded incident-edge list of a node, valid for this lane's layer
-struct NodeEdgeList {
-    int oe[kDeg];
-    int oo[kDeg];
-    bool pos[kDeg];
-    bool on[kDeg];
-};
-
-template <class R>
-__device__ __forceinline__ void load_edge_list(const DMesh<R> &m, int n, int nz, bool layer, NodeEdgeList &L)
+// the geometry a node pair's vertical scheme reads (all lanes must call)
+template <class R, int VER>
+__device__ __forceinline__ VCol2<R> node_vcol(const DMesh<R> &m, const Pair &p, int nlev, const R *__restrict__ Wv)
 {
-    const NodeEdgeRec ne = m.nedge[n];
-#pragma unroll
-    for (int j = 0; j < kDeg; ++j) {
-        L.oe[j] = ne.oe[j];
-        L.oo[j] = ne.oo[j];
-        L.pos[j] = (ne.info[j] & 256) != 0;
-        L.on[j] = ne.oe[j] >= 0 && layer && nz <= (ne.info[j] & 255);
+    VCol2<R> c{};
+    c.nlev = nlev;
+    c.k0 = p.k0;
+    const size_t i = (size_t)p.c;
+    V2<R> W{0, 0}, ar{0, 0}, zb{0, 0}, Z{0, 0}, h{0, 0}, hn{0, 0};
+    if (p.live) {
+        W = ld2(Wv + i);
+        ar = ld2(m.area + i);
+        if constexpr (VER == VER_QR4C) { zb = ld2(m.zbarn + i); Z = ld2(m.Zn + i); }
+        if constexpr (VER == VER_PPM) { h = ld2(m.hnode + i); hn = ld2(m.hnode_new + i); }
     }
-}
-
-template <class R>
-__device__ __forceinline__ VCol<R> vcol(const DMesh<R> &m, const NodeRec &nr, const double *A, const R *W, int nz)
-{
-    return VCol<R>{A + nr.off, W + nr.offi, m.area + nr.offi, m.Zn + nr.off, m.zbarn + nr.offi,
-                   m.hnode + nr.off, m.hnode_new + nr.off, nr.nlev, nz};
+    c.W = W;
+    c.area = ar;
+    if constexpr (VER == VER_QR4C) { c.zbar = zb; c.Z = window(Z); }
+    if constexpr (VER == VER_PPM) { c.h = window(h); c.hn = window(hn); }
+    return c;
 }
 
 // ===========================================================================
@@ -247,143 +312,226 @@ template <class R, int VER, int NT, bool SAME>
 __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restrict__ Wv,
                                                     const R *__restrict__ WEv,
                                                     const double *__restrict__ ttf,
-                                                    const double *__restrict__ ttfAB, size_t tsn, size_t tsi,
+                                                    const double *__restrict__ ttfAB, size_t tsn,
                                                     const R *__restrict__ flux_lo,
                                                     const R *__restrict__ adf_h, size_t tse,
                                                     double dt, R num_ord,
                                                     double *__restrict__ LO, R *__restrict__ adf_v,
                                                     R *__restrict__ Pp, R *__restrict__ Pm)
 {
-    const int n = wave_column();
-    if (n >= m.N) return;
-    const int k = lane_id(), nz = k + 1;
-    const NodeRec nr = m.nrec[n];
-    const int nlev = nr.nlev;
-    const bool layer = nz <= nlev - 1;
-    const size_t cl = (size_t)nr.off + k, ci = (size_t)nr.offi + k;
-    NodeEdgeList L;
-    load_edge_list(m, n, nz, layer, L);
+    const int ch = wave_chunk();
+    if (ch >= m.nch_n) return;
+    const Pair p = lane_pair(m.slot_n, ch);
+    const int nlev = p.live ? m.nlev[p.col] : 0;
+    const int L = nlev - 1, k0 = p.k0;
+    const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
+    const bool if0 = p.live && k0 < nlev, if1 = p.live && k0 + 1 < nlev;
+    const size_t c = (size_t)p.c;
+    int oe[kDeg];
+    unsigned info[kDeg];
+    load_node_edges(m, p.col, p.live && lay0, oe, info);
+    VCol2<R> lc{};
+    lc.nlev = nlev;
+    lc.k0 = k0;
+    V2<R> av{1, 1}, h{0, 0}, hn{1, 1};
+    if (p.live) {
+        lc.W = ld2(WEv + (size_t)c);
+        lc.area = ld2(m.area + c);
+    }
+    if (lay0) {
+        av = ld2(m.areasvol + c);
+        h = ld2(m.hnode + c);
+        hn = ld2(m.hnode_new + c);
+    }
+    VCol2<R> hc = node_vcol<R, VER>(m, p, nlev, Wv);
 
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const size_t tn = t * tsn, te = t * tse;
+        V2<double> T{0.0, 0.0};
+        if (p.live) T = ld2(ttf + tn + c);
+        lc.A = tracer_window<R>(T, k0, nlev);
+        // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
+        const R fl0 = vflux_upw1<0>(lc, (R)0), fl1 = vflux_upw1<1>(lc, (R)0);
+        const R fl2 = from_next(fl0);
+        if constexpr (SAME) {
+            hc.A = lc.A;
+        } else {
+            V2<double> B{0.0, 0.0};
+            if (p.live) B = ld2(ttfAB + tn + c);
+            hc.A = tracer_window<R>(B, k0, nlev);
+        }
+        R ad0, ad1;
+        ver_flux<R, VER>(hc, num_ord, (R)dt, fl0, fl1, ad0, ad1);
+        const R ad2 = from_next(ad0);
+        if (p.live) st2(adf_v + tn + c, if0 ? ad0 : (R)0, if1 ? ad1 : (R)0);
+        if (!lay0) continue;
+
         // LO horizontal sum over incident edges, edge-index order
-        double slo = 0.0;
-        R fadf[kDeg];
+        double slo0 = 0.0, slo1 = 0.0;
+        R pp0 = (R)0 + (fmx((R)0, ad0) + fmx((R)0, -ad1));
+        R pm0 = (R)0 + (fmn((R)0, ad0) + fmn((R)0, -ad1));
+        R pp1 = (R)0 + (fmx((R)0, ad1) + fmx((R)0, -ad2));
+        R pm1 = (R)0 + (fmn((R)0, ad1) + fmn((R)0, -ad2));
 #pragma unroll
         for (int j = 0; j < kDeg; ++j) {
-            fadf[j] = 0;
-            if (L.on[j]) {
-                const size_t oe = te + (size_t)L.oe[j] + k;
-                const double f = (double)flux_lo[oe];
-                fadf[j] = adf_h[oe];
-                slo = L.pos[j] ? slo + f : slo - f;
+            const int nlm = info_nlmax(info[j]);
+            if (oe[j] < 0 || k0 >= nlm) continue;
+            const bool on1 = lay1 && k0 + 1 < nlm;
+            const bool pos = info_pos(info[j]);
+            const V2<R> f = ld2(flux_lo + te + (size_t)oe[j] + k0);
+            const V2<R> a = ld2(adf_h + te + (size_t)oe[j] + k0);
+            slo0 = pos ? slo0 + (double)f.x : slo0 - (double)f.x;
+            if (pos) { pp0 = pp0 + fmx((R)0, a.x); pm0 = pm0 + fmn((R)0, a.x); }
+            else     { pp0 = pp0 + fmx((R)0, -a.x); pm0 = pm0 + fmn((R)0, -a.x); }
+            if (on1) {
+                slo1 = pos ? slo1 + (double)f.y : slo1 - (double)f.y;
+                if (pos) { pp1 = pp1 + fmx((R)0, a.y); pm1 = pm1 + fmn((R)0, a.y); }
+                else     { pp1 = pp1 + fmx((R)0, -a.y); pm1 = pm1 + fmn((R)0, -a.y); }
             }
         }
-        // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
-        const VCol<R> lc = vcol(m, nr, ttf + tn, WEv, nz);
-        const R fv_lo = vflux_upw1(lc, (R)0);
-        const R fv_lo_next = shdn(fv_lo);
-        const VCol<R> hc = vcol(m, nr, (SAME ? ttf : ttfAB) + tn, Wv, nz);
-        const R adfv = ver_flux<R, VER>(hc, num_ord, (R)dt, fv_lo);
-        const R adfv_next = shdn(adfv);
-        if (nz <= nlev) adf_v[t * tsi + ci] = adfv;
-        if (layer) {
-            const double T = ttf[tn + cl];
-            LO[tn + cl] = (T * (double)m.hnode[cl] +
-                           (slo + (double)(fv_lo - fv_lo_next)) * dt / (double)m.areasvol[cl]) /
-                          (double)m.hnode_new[cl];
-            R pp = (R)0 + (fmx((R)0, adfv) + fmx((R)0, -adfv_next));
-            R pm = (R)0 + (fmn((R)0, adfv) + fmn((R)0, -adfv_next));
-#pragma unroll
-            for (int j = 0; j < kDeg; ++j) {
-                if (!L.on[j]) continue;
-                const R a = fadf[j];
-                if (L.pos[j]) { pp = pp + fmx((R)0, a); pm = pm + fmn((R)0, a); }
-                else          { pp = pp + fmx((R)0, -a); pm = pm + fmn((R)0, -a); }
-            }
-            Pp[tn + cl] = pp;
-            Pm[tn + cl] = pm;
-        }
+        const double LO0 = (T.x * (double)h.x + (slo0 + (double)(fl0 - fl1)) * dt / (double)av.x) / (double)hn.x;
+        double LO1 = 0.0;
+        if (lay1) LO1 = (T.y * (double)h.y + (slo1 + (double)(fl1 - fl2)) * dt / (double)av.y) / (double)hn.y;
+        else { pp1 = 0; pm1 = 0; }
+        st2(LO + tn + c, LO0, LO1);
+        st2(Pp + tn + c, pp0, pp1);
+        st2(Pm + tn + c, pm0, pm1);
     }
 }
 
 // Zalesak bounds a1-a3 and limiting factors b2 (lib/oce_adv_tra_fct.F90:94-217,361-374);
 // R+/R- overwrite P+/P- in place. a1 (max/min of LO and T) is formed once per
-// distinct node of the 1-ring, a2 per element from its three nodes' a1.
+// 1-ring node, a2 per fan element from the node and two consecutive ring nodes.
 template <class R, int NT>
 __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double *__restrict__ ttf,
                                                        const double *__restrict__ LO, size_t tsn,
                                                        double dt, R *__restrict__ Pp_Rp,
                                                        R *__restrict__ Pm_Rm)
 {
-    const int n = wave_column();
-    if (n >= m.N) return;
-    const int k = lane_id(), nz = k + 1;
-    const NodeRec nr = m.nrec[n];
-    const int nlev = nr.nlev;
-    const bool layer = nz <= nlev - 1;
-    const size_t cl = (size_t)nr.off + k;
+    const int ch = wave_chunk();
+    if (ch >= m.nch_n) return;
+    const Pair p = lane_pair(m.slot_n, ch);
+    const int nlev = p.live ? m.nlev[p.col] : 0;
+    const int L = nlev - 1, k0 = p.k0;
+    const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
+    const size_t c = (size_t)p.c;
     const R bignumber = (R)1000;
 
-    R tvmax[NT], tvmin[NT];
+    R tvmax0[NT], tvmin0[NT], tvmax1[NT], tvmin1[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) { tvmax[t] = 0; tvmin[t] = 0; }
-    if (layer) {
-        const NodeEdgeRec ne = m.nedge[n];
-        const NodeElems nel = m.nelem[n];
+    for (int t = 0; t < NT; ++t) { tvmax0[t] = 0; tvmin0[t] = 0; tvmax1[t] = 0; tvmin1[t] = 0; }
+    if (lay0) {
+        const NodeRing rg = m.nring[p.col];
+        int nr = 0;
+#pragma unroll
+        for (int j = 0; j < kDeg; ++j) nr += rg.oo[j] >= 0 ? 1 : 0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const size_t tn = t * tsn;
-            R lnb[kDeg], tnb[kDeg], mxn[kDeg], mnn[kDeg];
-            load_ring<R>(ne, LO + tn, k, lnb);
-            load_ring<R>(ne, ttf + tn, k, tnb);
-            const R ls = (R)LO[tn + cl], ts = (R)ttf[tn + cl];
-            const R mxs = fmx(ls, ts), mns = fmn(ls, ts);
-#pragma unroll
-            for (int j = 0; j < kDeg; ++j) { mxn[j] = fmx(lnb[j], tnb[j]); mnn[j] = fmn(lnb[j], tnb[j]); }
-            R vmax = 0, vmin = 0;
+            const V2<double> ls = ld2(LO + tn + c), ts = ld2(ttf + tn + c);
+            const R mxs0 = fmx((R)ls.x, (R)ts.x), mns0 = fmn((R)ls.x, (R)ts.x);
+            const R mxs1 = fmx((R)ls.y, (R)ts.y), mns1 = fmn((R)ls.y, (R)ts.y);
+            R mx0[kDeg], mn0[kDeg], mx1[kDeg], mn1[kDeg];
 #pragma unroll
             for (int j = 0; j < kDeg; ++j) {
-                if (nel.t[j] < 0) break;
-                const int sl = nel.sl[j];
-                R emax = -bignumber, emin = bignumber;
-                if (nz <= (sl >> 16) - 1) {
-                    const int s0 = sl & 15, s1 = (sl >> 4) & 15, s2 = (sl >> 8) & 15;
-                    emax = fmx(fmx(pick(mxn, mxs, s0), pick(mxn, mxs, s1)), pick(mxn, mxs, s2));
-                    emin = fmn(fmn(pick(mnn, mns, s0), pick(mnn, mns, s1)), pick(mnn, mns, s2));
+                mx0[j] = 0; mn0[j] = 0; mx1[j] = 0; mn1[j] = 0;
+                if (rg.oo[j] >= 0 && k0 < (int)rg.lr[j]) {
+                    const V2<double> l = ld2(LO + tn + (size_t)rg.oo[j] + k0);
+                    const V2<double> v = ld2(ttf + tn + (size_t)rg.oo[j] + k0);
+                    mx0[j] = fmx((R)l.x, (R)v.x); mn0[j] = fmn((R)l.x, (R)v.x);
+                    mx1[j] = fmx((R)l.y, (R)v.y); mn1[j] = fmn((R)l.y, (R)v.y);
                 }
-                vmax = (j == 0) ? emax : fmx(vmax, emax);
-                vmin = (j == 0) ? emin : fmn(vmin, emin);
             }
-            tvmax[t] = vmax;
-            tvmin[t] = vmin;
+            R vmax0 = 0, vmin0 = 0, vmax1 = 0, vmin1 = 0;
+            bool first = true;
+#pragma unroll
+            for (int j = 0; j < kDeg; ++j) {
+                const int le = rg.le[j];
+                if (le == 0) continue;
+                const bool wrap = j + 1 >= nr;
+                const R bx0 = wrap ? mx0[0] : mx0[(j + 1) & (kDeg - 1)];
+                const R bn0 = wrap ? mn0[0] : mn0[(j + 1) & (kDeg - 1)];
+                const R bx1 = wrap ? mx1[0] : mx1[(j + 1) & (kDeg - 1)];
+                const R bn1 = wrap ? mn1[0] : mn1[(j + 1) & (kDeg - 1)];
+                const R emax0 = k0 < le ? fmx(fmx(mxs0, mx0[j]), bx0) : -bignumber;
+                const R emin0 = k0 < le ? fmn(fmn(mns0, mn0[j]), bn0) : bignumber;
+                const R emax1 = k0 + 1 < le ? fmx(fmx(mxs1, mx1[j]), bx1) : -bignumber;
+                const R emin1 = k0 + 1 < le ? fmn(fmn(mns1, mn1[j]), bn1) : bignumber;
+                vmax0 = first ? emax0 : fmx(vmax0, emax0);
+                vmin0 = first ? emin0 : fmn(vmin0, emin0);
+                vmax1 = first ? emax1 : fmx(vmax1, emax1);
+                vmin1 = first ? emin1 : fmn(vmin1, emin1);
+                first = false;
+            }
+            tvmax0[t] = vmax0; tvmin0[t] = vmin0;
+            tvmax1[t] = lay1 ? vmax1 : (R)0; tvmin1[t] = lay1 ? vmin1 : (R)0;
         }
     }
+    V2<R> av{1, 1};
+    if (lay0) av = ld2(m.areasvol + c);
     const R flux_eps = (R)1e-16;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const R up_max = shup(tvmax[t]), dn_max = shdn(tvmax[t]);
-        const R up_min = shup(tvmin[t]), dn_min = shdn(tvmin[t]);
-        if (!layer) continue;
+        const R upx0 = from_prev(tvmax1[t]), upn0 = from_prev(tvmin1[t]);
+        const R dnx1 = from_next(tvmax0[t]), dnn1 = from_next(tvmin0[t]);
+        if (!lay0) continue;
         const size_t tn = t * tsn;
-        const R lo = (R)LO[tn + cl];
-        R qp, qm;
-        if (nz == 1 || nz == nlev - 1) {
-            qp = tvmax[t] - lo;
-            qm = tvmin[t] - lo;
-        } else {
-            qp = fmx(fmx(up_max, tvmax[t]), dn_max) - lo;
-            qm = fmn(fmn(up_min, tvmin[t]), dn_min) - lo;
+        const V2<double> lo = ld2(LO + tn + c);
+        const V2<R> P = ld2(Pp_Rp + tn + c), M = ld2(Pm_Rm + tn + c);
+        R rp0, rm0, rp1 = 0, rm1 = 0;
+        {
+            const int nz = k0 + 1;
+            R qp, qm;
+            if (nz == 1 || nz == L) {
+                qp = tvmax0[t] - (R)lo.x;
+                qm = tvmin0[t] - (R)lo.x;
+            } else {
+                qp = fmx(fmx(upx0, tvmax0[t]), tvmax1[t]) - (R)lo.x;
+                qm = fmn(fmn(upn0, tvmin0[t]), tvmin1[t]) - (R)lo.x;
+            }
+            R flux = P.x * (R)dt / av.x + flux_eps;
+            rp0 = fmn((R)1, qp / flux);
+            flux = M.x * (R)dt / av.x - flux_eps;
+            rm0 = fmn((R)1, qm / flux);
         }
-        const R av = m.areasvol[cl];
-        R flux = Pp_Rp[tn + cl] * (R)dt / av + flux_eps;
-        const R rp = fmn((R)1, qp / flux);
-        flux = Pm_Rm[tn + cl] * (R)dt / av - flux_eps;
-        const R rm = fmn((R)1, qm / flux);
-        Pp_Rp[tn + cl] = rp;
-        Pm_Rm[tn + cl] = rm;
+        if (lay1) {
+            const int nz = k0 + 2;
+            R qp, qm;
+            if (nz == L) {
+                qp = tvmax1[t] - (R)lo.y;
+                qm = tvmin1[t] - (R)lo.y;
+            } else {
+                qp = fmx(fmx(tvmax0[t], tvmax1[t]), dnx1) - (R)lo.y;
+                qm = fmn(fmn(tvmin0[t], tvmin1[t]), dnn1) - (R)lo.y;
+            }
+            R flux = P.y * (R)dt / av.y + flux_eps;
+            rp1 = fmn((R)1, qp / flux);
+            flux = M.y * (R)dt / av.y - flux_eps;
+            rm1 = fmn((R)1, qm / flux);
+        }
+        st2(Pp_Rp + tn + c, rp0, rp1);
+        st2(Pm_Rm + tn + c, rm0, rm1);
     }
+}
+
+// vertical limiting of interface nz (b3, lib/oce_adv_tra_fct.F90:401-428):
+// rp/rm of the layer below the interface, rpu/rmu of the layer above
+template <class R>
+__device__ __forceinline__ R limit_vertical(int nz, bool layer, R fv, R rp, R rm, R rpu, R rmu)
+{
+    if (nz == 1) {
+        R ae = (R)1;
+        ae = fv >= (R)0 ? fmn(ae, rp) : fmn(ae, rm);
+        return ae * fv;
+    }
+    if (layer) {
+        R ae = (R)1;
+        if (fv >= (R)0) { ae = fmn(ae, rmu); ae = fmn(ae, rp); }
+        else            { ae = fmn(ae, rpu); ae = fmn(ae, rm); }
+        return ae * fv;
+    }
+    return fv;
 }
 
 // limiting b3 (lib/oce_adv_tra_fct.F90:394-471), flux divergence
@@ -391,65 +539,88 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
 // values += (dttf_h + dttf_v)/hnode_new (src/fesom_analytic.F90:419-437)
 template <class R, int NT>
 __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__restrict__ ttf,
-                                                        const double *__restrict__ LO, size_t tsn, size_t tsi,
+                                                        const double *__restrict__ LO, size_t tsn,
                                                         const R *__restrict__ adf_h, size_t tse,
                                                         const R *__restrict__ adf_v,
                                                         const R *__restrict__ Rp,
                                                         const R *__restrict__ Rm, double dt)
 {
-    const int n = wave_column();
-    if (n >= m.N) return;
-    const int k = lane_id(), nz = k + 1;
-    const NodeRec nr = m.nrec[n];
-    const int nlev = nr.nlev;
-    const bool layer = nz <= nlev - 1;
-    const size_t cl = (size_t)nr.off + k, ci = (size_t)nr.offi + k;
-    NodeEdgeList L;
-    load_edge_list(m, n, nz, layer, L);
+    const int ch = wave_chunk();
+    if (ch >= m.nch_n) return;
+    const Pair p = lane_pair(m.slot_n, ch);
+    const int nlev = p.live ? m.nlev[p.col] : 0;
+    const int L = nlev - 1, k0 = p.k0;
+    const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
+    const bool if0 = p.live && k0 < nlev, if1 = p.live && k0 + 1 < nlev;
+    const size_t c = (size_t)p.c;
+    int oe[kDeg], oo[kDeg];
+    unsigned info[kDeg];
+    load_node_edges(m, p.col, lay0, oe, info);
+    load_other_nodes(m, p.col, lay0, oo);
+    V2<R> av{1, 1}, h{0, 0}, hn{1, 1};
+    if (lay0) {
+        av = ld2(m.areasvol + c);
+        h = ld2(m.hnode + c);
+        hn = ld2(m.hnode_new + c);
+    }
 
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const size_t tn = t * tsn, te = t * tse;
-        R rp = 0, rm = 0, fv = 0;
-        if (nz <= nlev) fv = adf_v[t * tsi + ci];
-        if (layer) { rp = Rp[tn + cl]; rm = Rm[tn + cl]; }
-        const R rp_up = shup(rp), rm_up = shup(rm);
-        R fvl = fv;
-        if (nz == 1) {
-            R ae = (R)1;
-            ae = fv >= (R)0 ? fmn(ae, rp) : fmn(ae, rm);
-            fvl = ae * fv;
-        } else if (layer) {
-            R ae = (R)1;
-            if (fv >= (R)0) { ae = fmn(ae, rm_up); ae = fmn(ae, rp); }
-            else            { ae = fmn(ae, rp_up); ae = fmn(ae, rm); }
-            fvl = ae * fv;
-        }
-        const R fvl_next = shdn(fvl);
-        if (!layer) continue;
+        V2<R> fv{0, 0}, rp{0, 0}, rm{0, 0};
+        if (if0) fv = ld2(adf_v + tn + c);
+        if (!if1) fv.y = 0;
+        if (lay0) { rp = ld2(Rp + tn + c); rm = ld2(Rm + tn + c); }
+        if (!lay1) { rp.y = 0; rm.y = 0; }
+        const R rpu0 = from_prev(rp.y), rmu0 = from_prev(rm.y);
+        const R fvl0 = limit_vertical(k0 + 1, lay0, fv.x, rp.x, rm.x, rpu0, rmu0);
+        const R fvl1 = limit_vertical(k0 + 2, lay1, fv.y, rp.y, rm.y, rp.x, rm.x);
+        const R fvl2 = from_next(fvl0);
+        if (!lay0) continue;
 
-        const double av = (double)m.areasvol[cl];
-        double sh = 0.0;
+        double sh0 = 0.0, sh1 = 0.0;
 #pragma unroll
         for (int j = 0; j < kDeg; ++j) {
-            if (!L.on[j]) continue;
-            const R f = adf_h[te + (size_t)L.oe[j] + k];
-            const size_t oo = tn + (size_t)L.oo[j] + k;
-            const R rpo = Rp[oo], rmo = Rm[oo];
-            // edge orientation: this node is edges(1) when pos
-            const R rp1 = L.pos[j] ? rp : rpo, rm1 = L.pos[j] ? rm : rmo;
-            const R rp2 = L.pos[j] ? rpo : rp, rm2 = L.pos[j] ? rmo : rm;
-            R ae = (R)1;
-            if (f >= (R)0) { ae = fmn(ae, rp1); ae = fmn(ae, rm2); }
-            else           { ae = fmn(ae, rm1); ae = fmn(ae, rp2); }
-            const double fl = (double)(ae * f);
-            sh = L.pos[j] ? sh + fl * dt / av : sh - fl * dt / av;
+            const int nlm = info_nlmax(info[j]);
+            if (oe[j] < 0 || k0 >= nlm) continue;
+            const bool on1 = lay1 && k0 + 1 < nlm;
+            const bool pos = info_pos(info[j]);  // this node is edges(1)
+            const V2<R> f = ld2(adf_h + te + (size_t)oe[j] + k0);
+            const V2<R> rpo = ld2(Rp + tn + (size_t)oo[j] + k0), rmo = ld2(Rm + tn + (size_t)oo[j] + k0);
+            {
+                const R rp1 = pos ? rp.x : rpo.x, rm1 = pos ? rm.x : rmo.x;
+                const R rp2 = pos ? rpo.x : rp.x, rm2 = pos ? rmo.x : rm.x;
+                R ae = (R)1;
+                if (f.x >= (R)0) { ae = fmn(ae, rp1); ae = fmn(ae, rm2); }
+                else             { ae = fmn(ae, rm1); ae = fmn(ae, rp2); }
+                const double fl = (double)(ae * f.x);
+                sh0 = pos ? sh0 + fl * dt / (double)av.x : sh0 - fl * dt / (double)av.x;
+            }
+            if (on1) {
+                const R rp1 = pos ? rp.y : rpo.y, rm1 = pos ? rm.y : rmo.y;
+                const R rp2 = pos ? rpo.y : rp.y, rm2 = pos ? rmo.y : rm.y;
+                R ae = (R)1;
+                if (f.y >= (R)0) { ae = fmn(ae, rp1); ae = fmn(ae, rm2); }
+                else             { ae = fmn(ae, rm1); ae = fmn(ae, rp2); }
+                const double fl = (double)(ae * f.y);
+                sh1 = pos ? sh1 + fl * dt / (double)av.y : sh1 - fl * dt / (double)av.y;
+            }
         }
-        const double T = ttf[tn + cl];
-        double dv = 0.0 - T * (double)m.hnode[cl] + LO[tn + cl] * (double)m.hnode_new[cl];
-        dv = dv + (double)(fvl - fvl_next) * dt / av;
-        const double del = (0.0 + sh) + dv;
-        ttf[tn + cl] = T + del / (double)m.hnode_new[cl];
+        const V2<double> T = ld2(ttf + tn + c), lo = ld2(LO + tn + c);
+        double n0, n1 = T.y;
+        {
+            double dv = 0.0 - T.x * (double)h.x + lo.x * (double)hn.x;
+            dv = dv + (double)(fvl0 - fvl1) * dt / (double)av.x;
+            const double del = (0.0 + sh0) + dv;
+            n0 = T.x + del / (double)hn.x;
+        }
+        if (lay1) {
+            double dv = 0.0 - T.y * (double)h.y + lo.y * (double)hn.y;
+            dv = dv + (double)(fvl1 - fvl2) * dt / (double)av.y;
+            const double del = (0.0 + sh1) + dv;
+            n1 = T.y + del / (double)hn.y;
+        }
+        st2(ttf + tn + c, n0, n1);
     }
 }
 
@@ -461,34 +632,57 @@ __global__ __launch_bounds__(kBlock) void k_node_nolim(DMesh<R> m, const R *__re
                                                        const R *__restrict__ flux_h, size_t tse,
                                                        double dt, R num_ord)
 {
-    const int n = wave_column();
-    if (n >= m.N) return;
-    const int k = lane_id(), nz = k + 1;
-    const NodeRec nr = m.nrec[n];
-    const int nlev = nr.nlev;
-    const bool layer = nz <= nlev - 1;
-    const size_t cl = (size_t)nr.off + k;
-    NodeEdgeList L;
-    load_edge_list(m, n, nz, layer, L);
+    const int ch = wave_chunk();
+    if (ch >= m.nch_n) return;
+    const Pair p = lane_pair(m.slot_n, ch);
+    const int nlev = p.live ? m.nlev[p.col] : 0;
+    const int L = nlev - 1, k0 = p.k0;
+    const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
+    const size_t c = (size_t)p.c;
+    int oe[kDeg];
+    unsigned info[kDeg];
+    load_node_edges(m, p.col, lay0, oe, info);
+    V2<R> av{1, 1}, hn{1, 1};
+    if (lay0) {
+        av = ld2(m.areasvol + c);
+        hn = ld2(m.hnode_new + c);
+    }
+    VCol2<R> hc = node_vcol<R, VER>(m, p, nlev, WEv);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const size_t tn = t * tsn, te = t * tse;
-        const VCol<R> hc = vcol(m, nr, ttfAB + tn, WEv, nz);
-        const R fv = ver_flux<R, VER>(hc, num_ord, (R)dt, (R)0);
-        const R fv_next = shdn(fv);
-        if (!layer) continue;
-        const double av = (double)m.areasvol[cl];
-        double sh = 0.0;
+        V2<double> B{0.0, 0.0};
+        if (p.live) B = ld2(ttfAB + tn + c);
+        hc.A = tracer_window<R>(B, k0, nlev);
+        R fv0, fv1;
+        ver_flux<R, VER>(hc, num_ord, (R)dt, (R)0, (R)0, fv0, fv1);
+        const R fv2 = from_next(fv0);
+        if (!lay0) continue;
+        double sh0 = 0.0, sh1 = 0.0;
 #pragma unroll
         for (int j = 0; j < kDeg; ++j) {
-            if (!L.on[j]) continue;
-            const double f = (double)flux_h[te + (size_t)L.oe[j] + k];
-            sh = L.pos[j] ? sh + f * dt / av : sh - f * dt / av;
+            const int nlm = info_nlmax(info[j]);
+            if (oe[j] < 0 || k0 >= nlm) continue;
+            const bool on1 = lay1 && k0 + 1 < nlm;
+            const bool pos = info_pos(info[j]);
+            const V2<R> f = ld2(flux_h + te + (size_t)oe[j] + k0);
+            const double f0 = (double)f.x, f1 = (double)f.y;
+            sh0 = pos ? sh0 + f0 * dt / (double)av.x : sh0 - f0 * dt / (double)av.x;
+            if (on1) sh1 = pos ? sh1 + f1 * dt / (double)av.y : sh1 - f1 * dt / (double)av.y;
         }
-        const double dv = 0.0 + (double)(fv - fv_next) * dt / av;
-        const double del = (0.0 + sh) + dv;
-        const double T = ttf[tn + cl];
-        ttf[tn + cl] = T + del / (double)m.hnode_new[cl];
+        const V2<double> T = ld2(ttf + tn + c);
+        double n0, n1 = T.y;
+        {
+            const double dv = 0.0 + (double)(fv0 - fv1) * dt / (double)av.x;
+            const double del = (0.0 + sh0) + dv;
+            n0 = T.x + del / (double)hn.x;
+        }
+        if (lay1) {
+            const double dv = 0.0 + (double)(fv1 - fv2) * dt / (double)av.y;
+            const double del = (0.0 + sh1) + dv;
+            n1 = T.y + del / (double)hn.y;
+        }
+        st2(ttf + tn + c, n0, n1);
     }
 }
 
@@ -503,18 +697,27 @@ __device__ inline void diag_merge(DiagAcc &a, const DiagAcc &b)
     a.amin = fmin(a.amin, b.amin); a.amax = fmax(a.amax, b.amax); a.mass += b.mass; a.nan += b.nan;
 }
 
-// reduction over the wet cells (compact layout): min, max, sum, mass, NaN count
+// reduction over the wet layers (node layout): min, max, sum, mass, NaN count
 template <class R>
 __global__ __launch_bounds__(256) void k_diag_partial(DMesh<R> m, const double *__restrict__ v,
                                                       DiagAcc *__restrict__ part)
 {
     __shared__ DiagAcc sh[256];
     DiagAcc a{INFINITY, -INFINITY, 0.0, INFINITY, -INFINITY, 0.0, 0.0};
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < m.CN; i += (long long)gridDim.x * 256) {
-        const double x = v[i];
-        if (x != x) { a.nan += 1.0; continue; }
-        a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
-        a.mass += x * (double)m.hnode_new[i] * (double)m.areasvol[i];
+    const long long npair = m.CN / 2;
+    for (long long s = (long long)blockIdx.x * 256 + threadIdx.x; s < npair; s += (long long)gridDim.x * 256) {
+        const int w = m.slot_n[s];
+        if (w < 0) continue;
+        const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
+        const int L = m.nlev[n] - 1;
+        for (int q = 0; q < 2; ++q) {
+            if (k0 + q >= L) break;
+            const long long i = 2 * s + q;
+            const double x = v[i];
+            if (x != x) { a.nan += 1.0; continue; }
+            a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
+            a.mass += x * (double)m.hnode_new[i] * (double)m.areasvol[i];
+        }
     }
     a.amin = a.vmin;
     a.amax = a.vmax;
@@ -562,35 +765,56 @@ T *dupload(const std::vector<T> &h)
     return d;
 }
 
-inline unsigned grid_for(int ncols)
+inline unsigned grid_for(int nchunks)
 {
-    unsigned nb = (unsigned)((ncols + kWavesPerBlock - 1) / kWavesPerBlock);
+    unsigned nb = (unsigned)((nchunks + kWavesPerBlock - 1) / kWavesPerBlock);
+    nb = nb ? nb : 1u;
     return (nb + 7u) & ~7u;  // multiple of 8 for the XCD remap
 }
 
 }  // namespace
 
-// compact layout of a set of columns: column c holds len[c] levels at off[c]
+// chunked layout of a set of columns: column c uses len[c] cells from off[c]
+// (padded to an even count); columns are packed whole into kChunk-cell chunks
 struct Layout {
     std::vector<long long> off;
     std::vector<int> len;
-    long long total = 0;
+    std::vector<int> slot;  // per cell pair: column | pair << kColBits, -1 = padding
+    long long total = 0;    // cells, nch * kChunk
+    int nch = 0;
     void build(const std::vector<int> &lens)
     {
         len = lens;
-        off.assign(lens.size() + 1, 0);
-        for (size_t c = 0; c < lens.size(); ++c) off[c + 1] = off[c] + lens[c];
-        total = off.back();
+        off.assign(lens.size(), 0);
+        if (lens.size() >= (size_t)1 << kColBits) throw std::runtime_error("too many columns for the slot word");
+        long long chunk = 0, pos = 0;
+        for (size_t c = 0; c < lens.size(); ++c) {
+            const int plen = lens[c] + (lens[c] & 1);
+            if (plen > kChunk) throw std::runtime_error("column longer than a wave chunk");
+            if (pos + plen > kChunk) { ++chunk; pos = 0; }
+            off[c] = chunk * kChunk + pos;
+            pos += plen;
+        }
+        nch = lens.empty() ? 0 : (int)(chunk + 1);
+        total = (long long)nch * kChunk;
+        slot.assign((size_t)total / 2, -1);
+        for (size_t c = 0; c < lens.size(); ++c) {
+            const int pairs = (lens[c] + 1) / 2;
+            for (int q = 0; q < pairs; ++q) slot[(size_t)(off[c] / 2 + q)] = (int)c | (q << kColBits);
+        }
     }
-    // unpadded column-major (L, ncol) with comps interleaved components -> compact
+    // unpadded column-major (ld, ncol) with comps interleaved components ->
+    // layout; the first len[c]-sub levels of each column are copied
     template <class T>
-    std::vector<T> pack(const double *src, int L, int comps = 1) const
+    std::vector<T> pack(const double *src, int ld, int sub, int comps = 1) const
     {
-        std::vector<T> out((size_t)total * comps);
-        for (size_t c = 0; c < len.size(); ++c)
-            for (int k = 0; k < len[c]; ++k)
+        std::vector<T> out((size_t)total * comps, (T)0);
+        for (size_t c = 0; c < len.size(); ++c) {
+            const int cnt = std::min(len[c] - sub, ld);
+            for (int k = 0; k < cnt; ++k)
                 for (int q = 0; q < comps; ++q)
-                    out[((size_t)off[c] + k) * comps + q] = (T)src[((size_t)c * L + k) * comps + q];
+                    out[((size_t)off[c] + k) * comps + q] = (T)src[((size_t)c * ld + k) * comps + q];
+        }
         return out;
     }
 };
@@ -601,49 +825,104 @@ struct Engine::Impl {
 
     DMesh<double> m64{};
     DMesh<float> m32{};
-    Layout ln, li, lt, le;  // node layers, node interfaces, element layers, edge levels
+    Layout ln, lt, le;  // node interfaces/layers (len nlev), element layers, edge levels
     // velocity
     void *vel = nullptr, *W = nullptr, *WE = nullptr;
     // tracers [ntr][CN]
     double *values = nullptr, *valuesAB = nullptr;
     // work, per tracer
     void *flux_lo = nullptr, *adf_h = nullptr;             // [ntr][CE]
-    void *adf_v = nullptr;                                 // [ntr][CI]
+    void *adf_v = nullptr;                                 // [ntr][CN] (interfaces)
     void *Pp = nullptr, *Pm = nullptr;                     // [ntr][CN]
     double *LO = nullptr;                                  // [ntr][CN]
+    void *trxy = nullptr;                                  // [ntr][2*CT]
     void *gnod = nullptr;                                  // [ntr][2*CN]
     DiagAcc *dpart = nullptr, *dout = nullptr;
     int ndiag = 1024;
     float gscale = 1.0f, ginv = 1.0f;
-    size_t tsn = 0, tsi = 0, tse = 0;
+    size_t tsn = 0, tst = 0, tse = 0;
     // host copy of each tracer as last set (levels below the bottom are never
     // touched by a step, so get_values and the all-values statistics read them here)
     std::vector<std::vector<double>> host_vals;
 };
+
+// the elements around node n in fan order: ring[j], ring[(j+1) mod R] and
+// the node form element fan[j] (fan[j] = -1: open end of a boundary fan)
+static void node_fan(const HostMesh &m, int n, std::vector<int> &ring, std::vector<int> &fan)
+{
+    const int ne = m.nod_in_elem2D_num[n];
+    std::vector<int> ea(ne), eb(ne), el(ne);
+    for (int j = 0; j < ne; ++j) {
+        el[j] = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
+        int o[2], q = 0;
+        for (int v = 0; v < 3; ++v) {
+            const int node = m.elem2D_nodes[3 * el[j] + v];
+            if (node != n && q < 2) o[q++] = node;
+        }
+        if (q != 2) throw std::runtime_error("degenerate element around a node");
+        ea[j] = o[0];
+        eb[j] = o[1];
+    }
+    // start at a ring node met by one element only (open fan), else anywhere
+    int start = ne ? ea[0] : -1;
+    for (int j = 0; j < ne; ++j)
+        for (int s = 0; s < 2; ++s) {
+            const int v = s ? eb[j] : ea[j];
+            int cnt = 0;
+            for (int i = 0; i < ne; ++i) cnt += (ea[i] == v) + (eb[i] == v);
+            if (cnt == 1) start = v;
+        }
+    ring.assign(1, start);
+    fan.clear();
+    std::vector<char> used(ne, 0);
+    int cur = start;
+    for (int it = 0; it < ne; ++it) {
+        int pick = -1, nxt = -1;
+        for (int i = 0; i < ne && pick < 0; ++i) {
+            if (used[i]) continue;
+            if (ea[i] == cur) { pick = i; nxt = eb[i]; }
+            else if (eb[i] == cur) { pick = i; nxt = ea[i]; }
+        }
+        if (pick < 0) throw std::runtime_error("the elements around a node do not form one fan");
+        used[pick] = 1;
+        fan.push_back(el[pick]);
+        cur = nxt;
+        if (nxt == start && it == ne - 1) break;  // closed fan
+        ring.push_back(nxt);
+    }
+    if ((int)fan.size() != ne) throw std::runtime_error("the elements around a node do not form one fan");
+    if ((int)ring.size() > (int)fan.size()) fan.push_back(-1);  // open fan: no element after the last ring node
+}
 
 template <class R>
 static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
 {
     const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
     d.N = N; d.T = T; d.E = E; d.nl = nl;
-    d.CN = I.ln.total; d.CI = I.li.total; d.CT = I.lt.total; d.CE = I.le.total;
-    std::vector<EdgeMeta> em(E);
+    d.nch_n = I.ln.nch; d.nch_t = I.lt.nch; d.nch_e = I.le.nch;
+    d.CN = I.ln.total; d.CT = I.lt.total; d.CE = I.le.total;
+    d.slot_n = I.track(dupload(I.ln.slot));
+    d.slot_t = I.track(dupload(I.lt.slot));
+    d.slot_e = I.track(dupload(I.le.slot));
+    d.nlev = I.track(dupload(m.nlevels_nod2D));
+    std::vector<EdgeRec> er(E);
+    std::vector<int> enlmax(E);
     std::vector<R> eg(8 * (size_t)E, (R)0);
     for (int e = 0; e < E; ++e) {
-        EdgeMeta &r = em[e];
-        std::memset(&r, 0, sizeof(r));
+        EdgeRec &r = er[e];
         const int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
         const int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
         r.o1 = (int)I.ln.off[n1]; r.o2 = (int)I.ln.off[n2];
         r.t1 = (int)I.lt.off[el1]; r.t2 = el2 >= 0 ? (int)I.lt.off[el2] : -1;
-        r.up = m.edge_up_dn_tri[2 * e]; r.dn = m.edge_up_dn_tri[2 * e + 1];
+        const int up = m.edge_up_dn_tri[2 * e], dn = m.edge_up_dn_tri[2 * e + 1];
+        r.tu = up >= 0 ? (int)I.lt.off[up] : -1;
+        r.td = dn >= 0 ? (int)I.lt.off[dn] : -1;
         const int nl1 = m.nlevels[el1] - 1, nl2 = el2 >= 0 ? m.nlevels[el2] - 1 : 0;
         const int nzmax_e = std::min(m.nlevels_nod2D_min[n1], m.nlevels_nod2D_min[n2]);
         const int nlmax = std::max(nl1, nl2);
+        enlmax[e] = nlmax;
         r.nl = nl1 | (nl2 << 8) | (nzmax_e << 16) | (nlmax << 24);
         r.nb = (m.nboundary_lay[n1] & 0xFFFF) | ((m.nboundary_lay[n2] & 0xFFFF) << 16);
-        r.oe = (int)I.le.off[e];
-        r.el2 = el2;
         double a = kRE * m.elem_cos[el1];
         if (el2 >= 0) a = 0.5 * (a + kRE * m.elem_cos[el2]);
         R *g = &eg[8 * (size_t)e];
@@ -652,71 +931,66 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
         g[5] = (R)m.edge_dxdy[2 * e + 1];
         g[6] = (R)a;
     }
-    d.emeta = I.track(dupload(em));
+    d.erec = I.track(dupload(er));
     d.egeo = I.track(dupload(eg));
-    std::vector<ElemMeta> tm(T);
+    std::vector<ElemRec> tr(T);
     std::vector<R> tg(8 * (size_t)T, (R)0);
     for (int el = 0; el < T; ++el) {
         const int *en = &m.elem2D_nodes[3 * el];
-        tm[el] = ElemMeta{(int)I.ln.off[en[0]], (int)I.ln.off[en[1]], (int)I.ln.off[en[2]], m.nlevels[el]};
+        tr[el] = ElemRec{(int)I.ln.off[en[0]], (int)I.ln.off[en[1]], (int)I.ln.off[en[2]], m.nlevels[el]};
         for (int q = 0; q < 6; ++q) tg[8 * (size_t)el + q] = (R)m.gradient_sca[6 * el + q];
-        tg[8 * (size_t)el + 6] = (R)m.elem_area[el];
     }
-    d.tmeta = I.track(dupload(tm));
+    d.trec = I.track(dupload(tr));
     d.tgeo = I.track(dupload(tg));
-    std::vector<NodeRec> nr(N);
-    std::vector<NodeEdgeRec> ne(N);
+    std::vector<NodeEdges> ne(N);
+    std::vector<NodeEdgeInfo> ni(N);
+    std::vector<NodeRing> rg(N);
     std::vector<NodeElems> nt(N);
+    std::vector<R> na(kDeg * (size_t)N, (R)0);
+    std::vector<int> ring, fan;
     for (int n = 0; n < N; ++n) {
-        nr[n] = NodeRec{m.nlevels_nod2D[n], (int)I.ln.off[n], (int)I.li.off[n], 0};
         const int deg = m.n2e_ptr[n + 1] - m.n2e_ptr[n];
         if (deg > kDeg || m.nod_in_elem2D_num[n] > kDeg)
             throw std::runtime_error("node degree above " + std::to_string(kDeg) + " is not supported");
-        int nbr[kDeg];
         for (int j = 0; j < kDeg; ++j) {
-            ne[n].oe[j] = -1;
-            ne[n].info[j] = 0;
-            ne[n].oo[j] = 0;
-            nt[n].t[j] = -1;
-            nt[n].sl[j] = 0;
-            nbr[j] = -1;
+            ne[n].oe[j] = -1; ne[n].oo[j] = 0; ni[n].info[j] = 0;
+            rg[n].oo[j] = -1; rg[n].lr[j] = 0; rg[n].le[j] = 0;
+            nt[n].ot[j] = -1; nt[n].le[j] = 0;
         }
         for (int j = 0; j < deg; ++j) {
             const int se = m.n2e[m.n2e_ptr[n] + j];
             const int e = (se > 0 ? se : -se) - 1;
-            ne[n].oe[j] = (int)I.le.off[e];
             const int other = se > 0 ? m.edges[2 * e + 1] : m.edges[2 * e];
-            ne[n].info[j] = ((unsigned)em[e].nl >> 24) | (se > 0 ? 256 : 0) | ((m.nlevels_nod2D[other] - 1) << 16);
+            ne[n].oe[j] = (int)I.le.off[e];
             ne[n].oo[j] = (int)I.ln.off[other];
-            nbr[j] = other;
+            ni[n].info[j] = (unsigned short)(enlmax[e] | (se > 0 ? 64 : 0) | ((m.nlevels_nod2D[other] - 1) << 7));
         }
         for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) {
             const int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
-            nt[n].t[j] = el;
-            int slots = 0;
-            for (int q = 0; q < 3; ++q) {
-                const int node = m.elem2D_nodes[3 * el + q];
-                int slot = -1;
-                if (node == n) slot = kSelf;
-                else
-                    for (int jj = 0; jj < deg; ++jj)
-                        if (nbr[jj] == node) slot = jj;
-                if (slot < 0) throw std::runtime_error("element node is not a 1-ring neighbour");
-                slots |= slot << (4 * q);
-            }
-            nt[n].sl[j] = slots | (m.nlevels[el] << 16);
+            nt[n].ot[j] = (int)I.lt.off[el];
+            nt[n].le[j] = (unsigned char)(m.nlevels[el] - 1);
+            na[kDeg * (size_t)n + j] = (R)m.elem_area[el];
+        }
+        node_fan(m, n, ring, fan);
+        if ((int)ring.size() > kDeg) throw std::runtime_error("1-ring above " + std::to_string(kDeg) + " nodes");
+        for (size_t j = 0; j < ring.size(); ++j) {
+            rg[n].oo[j] = (int)I.ln.off[ring[j]];
+            rg[n].lr[j] = (unsigned char)(m.nlevels_nod2D[ring[j]] - 1);
+            rg[n].le[j] = fan[j] >= 0 ? (unsigned char)(m.nlevels[fan[j]] - 1) : 0;
         }
     }
-    d.nrec = I.track(dupload(nr));
     d.nedge = I.track(dupload(ne));
+    d.ninfo = I.track(dupload(ni));
+    d.nring = I.track(dupload(rg));
     d.nelem = I.track(dupload(nt));
-    d.area = I.track(dupload(I.li.pack<R>(m.area.data(), nl)));
-    d.zbarn = I.track(dupload(I.li.pack<R>(m.zbar_3d_n.data(), nl)));
-    d.areasvol = I.track(dupload(I.ln.pack<R>(m.areasvol.data(), nl)));
-    d.hnode = I.track(dupload(I.ln.pack<R>(m.hnode.data(), nl - 1)));
-    d.hnode_new = I.track(dupload(I.ln.pack<R>(m.hnode_new.data(), nl - 1)));
-    d.Zn = I.track(dupload(I.ln.pack<R>(m.Z_3d_n.data(), nl - 1)));
-    d.helem = I.track(dupload(I.lt.pack<R>(m.helem.data(), nl - 1)));
+    d.nelem_area = I.track(dupload(na));
+    d.area = I.track(dupload(I.ln.pack<R>(m.area.data(), nl, 0)));
+    d.zbarn = I.track(dupload(I.ln.pack<R>(m.zbar_3d_n.data(), nl, 0)));
+    d.areasvol = I.track(dupload(I.ln.pack<R>(m.areasvol.data(), nl, 1)));
+    d.hnode = I.track(dupload(I.ln.pack<R>(m.hnode.data(), nl - 1, 1)));
+    d.hnode_new = I.track(dupload(I.ln.pack<R>(m.hnode_new.data(), nl - 1, 1)));
+    d.Zn = I.track(dupload(I.ln.pack<R>(m.Z_3d_n.data(), nl - 1, 1)));
+    d.helem = I.track(dupload(I.lt.pack<R>(m.helem.data(), nl - 1, 0)));
 }
 
 Engine::Engine(const HostMesh &m, int ntracers, int precision)
@@ -730,15 +1004,15 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     spec_.assign(ntr_, TracerSpec());
     Impl &I = *p_;
     {
-        std::vector<int> a(N_), b(N_), c(T_), d(E_);
-        for (int n = 0; n < N_; ++n) { a[n] = m.nlevels_nod2D[n] - 1; b[n] = m.nlevels_nod2D[n]; }
+        std::vector<int> a(N_), c(T_), d(E_);
+        for (int n = 0; n < N_; ++n) a[n] = m.nlevels_nod2D[n];
         for (int el = 0; el < T_; ++el) c[el] = m.nlevels[el] - 1;
         for (int e = 0; e < E_; ++e) {
             int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
             d[e] = std::max(m.nlevels[el1] - 1, el2 >= 0 ? m.nlevels[el2] - 1 : 0);
         }
-        I.ln.build(a); I.li.build(b); I.lt.build(c); I.le.build(d);
-        if (I.le.total >= (1ll << 31) || I.li.total >= (1ll << 31))
+        I.ln.build(a); I.lt.build(c); I.le.build(d);
+        if (I.le.total >= (1ll << 31) || I.ln.total >= (1ll << 31) || I.lt.total >= (1ll << 31))
             throw std::runtime_error("mesh too large for 32-bit column offsets");
     }
     HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -747,7 +1021,7 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     if (prec_ == PREC_FP64) build_dmesh(I, I.m64, m);
     else build_dmesh(I, I.m32, m);
     I.tsn = (size_t)I.ln.total;
-    I.tsi = (size_t)I.li.total;
+    I.tst = 2 * (size_t)I.lt.total;
     I.tse = (size_t)I.le.total;
     auto zalloc = [&](size_t bytes) {
         void *p = I.track((void *)dalloc<char>(bytes));
@@ -755,16 +1029,17 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
         return p;
     };
     I.vel = zalloc(2 * (size_t)I.lt.total * rs);
-    I.W = zalloc(I.tsi * rs);
-    I.WE = zalloc(I.tsi * rs);
+    I.W = zalloc(I.tsn * rs);
+    I.WE = zalloc(I.tsn * rs);
     I.values = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
     I.valuesAB = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
     I.flux_lo = zalloc(I.tse * ntr_ * rs);
     I.adf_h = zalloc(I.tse * ntr_ * rs);
-    I.adf_v = zalloc(I.tsi * ntr_ * rs);
+    I.adf_v = zalloc(I.tsn * ntr_ * rs);
     I.Pp = zalloc(I.tsn * ntr_ * rs);
     I.Pm = zalloc(I.tsn * ntr_ * rs);
     I.LO = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
+    I.trxy = zalloc(I.tst * ntr_ * gs);
     I.gnod = zalloc(2 * I.tsn * ntr_ * gs);
     I.dpart = I.track(dalloc<DiagAcc>(I.ndiag));
     I.dout = I.track(dalloc<DiagAcc>(1));
@@ -790,16 +1065,16 @@ void Engine::set_velocity(const double *uv, const double *w, const double *we)
     Impl &I = *p_;
     auto put = [&](void *dst, const Layout &lay, const double *src, int L, int comps) {
         if (prec_ == PREC_FP64) {
-            auto v = lay.pack<double>(src, L, comps);
+            auto v = lay.pack<double>(src, L, 0, comps);
             HIP_OK(hipMemcpy(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
         } else {
-            auto v = lay.pack<float>(src, L, comps);
+            auto v = lay.pack<float>(src, L, 0, comps);
             HIP_OK(hipMemcpy(dst, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice));
         }
     };
     put(I.vel, I.lt, uv, nl_ - 1, 2);
-    put(I.W, I.li, w, nl_, 1);
-    put(I.WE, I.li, we, nl_, 1);
+    put(I.W, I.ln, w, nl_, 1);
+    put(I.WE, I.ln, we, nl_, 1);
 }
 
 void Engine::set_values(int tr, const double *v)
@@ -807,7 +1082,7 @@ void Engine::set_values(int tr, const double *v)
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     Impl &I = *p_;
     std::memcpy(I.host_vals[tr].data(), v, sizeof(double) * I.host_vals[tr].size());
-    auto a = I.ln.pack<double>(v, nl_ - 1);
+    auto a = I.ln.pack<double>(v, nl_ - 1, 1);
     HIP_OK(hipMemcpy(I.values + tr * I.tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
@@ -815,7 +1090,7 @@ void Engine::set_valuesAB(int tr, const double *v)
 {
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     Impl &I = *p_;
-    auto a = I.ln.pack<double>(v, nl_ - 1);
+    auto a = I.ln.pack<double>(v, nl_ - 1, 1);
     HIP_OK(hipMemcpy(I.valuesAB + tr * I.tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
@@ -829,7 +1104,7 @@ void Engine::get_values(int tr, double *v)
     const int L = nl_ - 1;
     std::memcpy(v, I.host_vals[tr].data(), sizeof(double) * I.host_vals[tr].size());
     for (int n = 0; n < N_; ++n)
-        for (int k = 0; k < I.ln.len[n]; ++k) v[(size_t)n * L + k] = a[(size_t)I.ln.off[n] + k];
+        for (int k = 0; k < I.ln.len[n] - 1; ++k) v[(size_t)n * L + k] = a[(size_t)I.ln.off[n] + k];
 }
 
 struct LaunchHook {
@@ -850,28 +1125,31 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
                          hipStream_t st, LaunchHook &hk)
 {
     const R *vel = (const R *)I.vel, *W = (const R *)I.W, *WE = (const R *)I.WE;
-    const size_t tsn = I.tsn, tsi = I.tsi, tse = I.tse;
+    const size_t tsn = I.tsn, tst = I.tst, tse = I.tse, tsg = 2 * I.tsn;
     double *vals = I.values + t0 * tsn;
     const double *ab = refresh ? vals : I.valuesAB + t0 * tsn;
     R *flux_lo = (R *)I.flux_lo + t0 * tse, *adf_h = (R *)I.adf_h + t0 * tse;
-    R *adf_v = (R *)I.adf_v + t0 * tsi, *Pp = (R *)I.Pp + t0 * tsn, *Pm = (R *)I.Pm + t0 * tsn;
+    R *adf_v = (R *)I.adf_v + t0 * tsn, *Pp = (R *)I.Pp + t0 * tsn, *Pm = (R *)I.Pm + t0 * tsn;
     double *LO = I.LO + t0 * tsn;
-    G *gnod = (G *)I.gnod + 2 * t0 * tsn;
-    const dim3 blk(kBlock), ge(grid_for(m.E)), gn(grid_for(m.N));
+    G *trxy = (G *)I.trxy + t0 * tst;
+    G *gnod = (G *)I.gnod + t0 * tsg;
+    const dim3 blk(kBlock), ge(grid_for(m.nch_e)), gn(grid_for(m.nch_n)), gt(grid_for(m.nch_t));
     const bool fct = sp.lim == LIM_FCT;
     const R ph = (R)sp.ph, pv = (R)sp.pv;
     if (sp.hor == HOR_MUSCL || sp.hor == HOR_MFCT) {
-        hipLaunchKernelGGL((k_grad<R, G, NT>), gn, blk, 0, st, m, ab, tsn, gnod, I.gscale);
-        hk.mark("k_grad", st);
+        hipLaunchKernelGGL((k_trxy<R, G, NT>), gt, blk, 0, st, m, ab, tsn, trxy, tst, I.gscale);
+        hk.mark("k_trxy", st);
+        hipLaunchKernelGGL((k_gnod<R, G, NT>), gn, blk, 0, st, m, trxy, tst, gnod, tsg, I.gscale, I.ginv);
+        hk.mark("k_gnod", st);
     }
 #define EDGE(HOR, F)                                                                                        \
     do {                                                                                                    \
         if (refresh)                                                                                        \
-            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, true>), ge, blk, 0, st, m, vel, vals, ab, tsn, gnod, ph, \
-                               I.ginv, flux_lo, adf_h, tse);                                                \
+            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, true>), ge, blk, 0, st, m, vel, vals, ab, tsn, trxy, tst, \
+                               gnod, tsg, ph, I.ginv, flux_lo, adf_h, tse);                                 \
         else                                                                                                \
-            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, false>), ge, blk, 0, st, m, vel, vals, ab, tsn, gnod, ph, \
-                               I.ginv, flux_lo, adf_h, tse);                                                \
+            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, false>), ge, blk, 0, st, m, vel, vals, ab, tsn, trxy, tst, \
+                               gnod, tsg, ph, I.ginv, flux_lo, adf_h, tse);                                 \
     } while (0)
     if (fct) {
         if (sp.hor == HOR_MUSCL) EDGE(HOR_MUSCL, true);
@@ -888,10 +1166,10 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
 #define NLO(V)                                                                                              \
     do {                                                                                                    \
         if (refresh)                                                                                        \
-            hipLaunchKernelGGL((k_node_lo<R, V, NT, true>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, tsi, flux_lo, \
+            hipLaunchKernelGGL((k_node_lo<R, V, NT, true>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, flux_lo, \
                                adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
         else                                                                                                \
-            hipLaunchKernelGGL((k_node_lo<R, V, NT, false>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, tsi, flux_lo, \
+            hipLaunchKernelGGL((k_node_lo<R, V, NT, false>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, flux_lo, \
                                adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
     } while (0)
         switch (sp.ver) {
@@ -904,7 +1182,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
         hk.mark("k_node_lo", st);
         hipLaunchKernelGGL((k_node_limit<R, NT>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm);
         hk.mark("k_node_limit", st);
-        hipLaunchKernelGGL((k_node_update<R, NT>), gn, blk, 0, st, m, vals, LO, tsn, tsi, adf_h, tse, adf_v, Pp, Pm, dt);
+        hipLaunchKernelGGL((k_node_update<R, NT>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v, Pp, Pm, dt);
         hk.mark("k_node_update", st);
     } else {
 #define NNL(V) hipLaunchKernelGGL((k_node_nolim<R, V, NT>), gn, blk, 0, st, m, WE, vals, ab, tsn, adf_h, tse, dt, pv)
@@ -998,7 +1276,7 @@ Diag Engine::diagnostics(int tr)
     const int L = nl_ - 1;
     const std::vector<double> &h = I.host_vals[tr];
     for (int n = 0; n < N_; ++n)
-        for (int k = I.ln.len[n]; k < L; ++k) {
+        for (int k = I.ln.len[n] - 1; k < L; ++k) {
             const double x = h[(size_t)n * L + k];
             if (x != x) { d.nan_count += 1.0; continue; }
             d.vmin = std::min(d.vmin, x);
@@ -1026,14 +1304,14 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
     HIP_OK(hipStreamSynchronize(stream_));
     const char *src = nullptr;
     const Layout *lay = &I.ln;
-    int L = nl_ - 1, comps = 1;
+    int L = nl_ - 1, comps = 1, sub = 1;
     size_t es = prec_ == PREC_FP64 ? 8 : 4;
     if (name == "fct_LO") { src = (const char *)I.LO; es = 8; }
     else if (name == "fct_plus") src = (const char *)I.Pp;
     else if (name == "fct_minus") src = (const char *)I.Pm;
-    else if (name == "adv_flux_hor") { src = (const char *)I.adf_h; lay = &I.le; }
-    else if (name == "adv_flux_lo") { src = (const char *)I.flux_lo; lay = &I.le; }
-    else if (name == "adv_flux_ver") { src = (const char *)I.adf_v; lay = &I.li; L = nl_; }
+    else if (name == "adv_flux_hor") { src = (const char *)I.adf_h; lay = &I.le; sub = 0; }
+    else if (name == "adv_flux_lo") { src = (const char *)I.flux_lo; lay = &I.le; sub = 0; }
+    else if (name == "adv_flux_ver") { src = (const char *)I.adf_v; L = nl_; sub = 0; }
     else if (name == "gnod") {
         src = (const char *)I.gnod; comps = 2;
         es = prec_ == PREC_FP64 ? 8 : (prec_ == PREC_FP32 ? 4 : 2);
@@ -1043,7 +1321,7 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
     HIP_OK(hipMemcpy(buf.data(), src + tr * per * es, buf.size(), hipMemcpyDeviceToHost));
     std::fill(out, out + need, 0.0);
     for (size_t c = 0; c < lay->len.size(); ++c)
-        for (int k = 0; k < lay->len[c]; ++k)
+        for (int k = 0; k < std::min(lay->len[c] - sub, L); ++k)
             for (int q = 0; q < comps; ++q) {
                 size_t i = ((size_t)lay->off[c] + k) * comps + q;
                 double v;

@@ -1,17 +1,25 @@
 // fta_kernels.h -- gfx950 device data structures of the tracer-advection step.
 //
-// Mapping: one 64-lane wavefront per column (node or edge), lane k handles
-// level k+1 (layers 1..nl-1, interfaces 1..nl). Column indices are made
-// wave-uniform with readfirstlane so that all per-column metadata comes in
-// through scalar loads of small fixed-width records (one s_load per record,
-// no chains of dependent index loads), and blocks are remapped so that each
-// of the 8 XCDs sweeps one contiguous range of columns (its own L2 then holds
-// the neighbourhood that the gathers revisit).
+// Cell-packed mapping. Every level-indexed array (node layers and
+// interfaces, element layers, edge levels) stores the wet levels of each
+// column back to back, each column padded to an even length and columns
+// packed whole, in mesh order, into chunks of kChunk = 128 cells; a column
+// never straddles two chunks. One 64-lane wavefront processes one chunk and
+// each lane two consecutive cells of one column (levels k0 = 2p, k0+1), so
+// every tracer/flux/geometry access of a lane is one 16-byte load for FP64
+// data (8 bytes for FP32). On gfx950 a vector-memory instruction of 64- or
+// 128-bit lanes occupies the texture path for the same time whatever its
+// active-lane count, so filling all lanes with wet cells and moving two
+// cells per lane is what sets the throughput of these gather-heavy kernels.
+//
+// A per-slot word (slot = cell pair) names the lane's column and pair index;
+// the column's records (incident edges, 1-ring, elements) are fixed-width
+// and read by vector loads. Vertical neighbours of a cell are in the same
+// lane or in lane +-1 (the whole column is in the wave): cross-lane moves.
 //
 // Tracers that share their advection schemes go through the kernels
 // together (NT per launch): the mesh records and velocity are fetched once
-// per column for all of them, and each wave keeps NT times more independent
-// loads in flight.
+// per cell pair for all of them.
 //
 // Scatter-free: every node-side sum over incident edges is a gather over a
 // node->edge list sorted by edge index, which is the order the reference's
@@ -26,110 +34,196 @@ namespace fta {
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
-constexpr int kDeg = 8;  // padded node degree (incident edges / elements)
-constexpr double kRE = 6367500.0;  // r_earth, lib/oce_modules.F90:24
+constexpr int kChunk = 2 * kWave;   // cells per wave chunk
+constexpr int kDeg = 8;             // padded node degree (incident edges / elements / 1-ring)
+constexpr int kColBits = 26;        // slot word: column | pair index << kColBits
+constexpr double kRE = 6367500.0;   // r_earth, lib/oce_modules.F90:24
 
-// Compact layout: every level-indexed array stores only the wet levels of
-// each column, columns back to back: a node's layers 1..nlev-1 start at its
-// layer offset, its interfaces 1..nlev at its interface offset (= layer
-// offset + node index), an element's layers at its element offset, an edge's
-// levels 1..nlmax at its edge offset. The records below carry these offsets
-// directly, so a gather never waits on a separate offset load.
-
-// per-edge record (64 bytes): layer offsets of both nodes and both elements,
-// the up/down-wind triangles, packed level counts
-// nl = nl1 | nl2 << 8 | nzmax_e << 16 | nlmax << 24, where nzmax_e =
-// min(nlevels_nod2D_min(n1), nlevels_nod2D_min(n2)) bounds the levels shared
-// by all elements around both nodes (fill_up_dn_grad), and
+// per-edge record: layer offsets of both nodes, of both elements (t2 = -1 on
+// a boundary edge) and of the up/down-wind triangles (-1 = none);
+// nl = nl1 | nl2 << 8 | nzmax_e << 16 | nlmax << 24 where nl1/nl2 are the
+// elements' layer counts, nzmax_e = min(nlevels_nod2D_min(n1),
+// nlevels_nod2D_min(n2)) bounds the levels shared by all elements around
+// both nodes (fill_up_dn_grad) and nlmax = max(nl1, nl2);
 // nb = nboundary_lay(n1) | nboundary_lay(n2) << 16
-struct alignas(64) EdgeMeta {
-    int o1, o2;   // node layer offsets
-    int t1, t2;   // element layer offsets (t2 = -1 on a boundary edge)
-    int up, dn;   // up/down-wind triangle indices (-1 = none)
-    int nl, nb;
-    int oe;       // this edge's offset in the edge arrays
-    int el2;      // second element index (-1 = boundary)
-    int pad[6];
+struct alignas(32) EdgeRec {
+    int o1, o2, t1, t2, tu, td, nl, nb;
 };
 
 // per-element record: layer offsets of its three nodes and nlevels
-struct alignas(16) ElemMeta {
+struct alignas(16) ElemRec {
     int o0, o1, o2, nlev;
 };
 
-// per-node record: nlevels_nod2D, layer offset, interface offset
-struct alignas(16) NodeRec {
-    int nlev, off, offi, pad;
-};
-
-// incident edges of a node padded to kDeg, sorted by edge index:
-// oe = edge offset (-1 = empty); info = nlmax | 256 if the node is edges(1)
-// | (layers of the other node) << 16; oo = layer offset of the edge's other
-// node (the node's 1-ring neighbours, slot j)
-struct alignas(32) NodeEdgeRec {
+// node -> incident edges (sorted by edge index), kDeg slots:
+// oe = edge offset (-1 = empty), oo = layer offset of the edge's other node,
+// info = nlmax | 64 if the node is edges(1) | (layers of the other node) << 7
+struct alignas(16) NodeEdges {
     int oe[kDeg];
-    int info[kDeg];
     int oo[kDeg];
 };
-// elements around a node (nod_in_elem2D order), padded to kDeg: element
-// index (-1 = empty) and sl = s0 | s1 << 4 | s2 << 8 | nlevels << 16, where
-// s0..s2 locate the element's nodes 1..3 in the node's neighbourhood:
-// slot j < kDeg = the other node of incident edge j, slot 15 = the node itself
-struct alignas(32) NodeElems {
-    int t[kDeg];
-    int sl[kDeg];
+struct alignas(16) NodeEdgeInfo {
+    unsigned short info[kDeg];
 };
-constexpr int kSelf = 15;
+
+// node -> 1-ring in cyclic (fan) order, for the Zalesak bounds: element j of
+// the fan is (node, ring j, ring (j+1) mod R); oo = ring node layer offsets
+// (-1 = empty), lr = ring node layer counts, le = layer count of fan element
+// j (0 = no element: the open end of a boundary fan)
+struct alignas(16) NodeRing {
+    int oo[kDeg];
+    unsigned char lr[kDeg];
+    unsigned char le[kDeg];
+};
+
+// node -> elements (nod_in_elem2D order) for the nodal mean gradient:
+// ot = element layer offset (-1 = empty), le = element layer count
+struct alignas(16) NodeElems {
+    int ot[kDeg];
+    unsigned char le[kDeg];
+};
 
 template <class R>
 struct DMesh {
     int N, T, E, nl;
-    long long CN, CI, CT, CE;   // cell counts: node layers, node interfaces, element layers, edge levels
-    const NodeRec *nrec;        // (N)
-    const EdgeMeta *emeta;      // (E)
-    const R *egeo;              // (8,E): dx1 dy1 dx2 dy2 (edge_cross_dxdy) ex ey (edge_dxdy) a
-    const ElemMeta *tmeta;      // (T)
-    const R *tgeo;              // (8,T): gradient_sca(1:6), elem_area, 0
-    const NodeEdgeRec *nedge;   // (N)
-    const NodeElems *nelem;     // (N)
-    const R *area, *zbarn;      // interface arrays (CI)
-    const R *areasvol, *hnode, *hnode_new, *Zn;  // layer arrays (CN)
-    const R *helem;             // element layer array (CT)
+    int nch_n, nch_t, nch_e;      // wave chunks of the node / element / edge layouts
+    long long CN, CT, CE;         // padded cell counts (nch * kChunk)
+    const int *slot_n, *slot_t, *slot_e;   // per cell pair: column | pair << kColBits, -1 = pad
+    const int *nlev;              // (N) nlevels_nod2D
+    const EdgeRec *erec;          // (E)
+    const R *egeo;                // (8,E): dx1 dy1 dx2 dy2 (edge_cross_dxdy) ex ey (edge_dxdy) a
+    const ElemRec *trec;          // (T)
+    const R *tgeo;                // (8,T): gradient_sca(1:6), 0, 0
+    const NodeEdges *nedge;       // (N)
+    const NodeEdgeInfo *ninfo;    // (N)
+    const NodeRing *nring;        // (N)
+    const NodeElems *nelem;       // (N)
+    const R *nelem_area;          // (8,N): elem_area of the node's elements
+    // node layout (interfaces 1..nlev and layers 1..nlev-1 share a column)
+    const R *area, *zbarn;        // interface arrays
+    const R *areasvol, *hnode, *hnode_new, *Zn;  // layer arrays
+    const R *helem;               // element layout
 };
 
-__device__ __forceinline__ int wave_column()
+__device__ __forceinline__ int wave_chunk()
 {
     // hardware dispatches workgroup b to XCD b % 8; give XCD x the
     // contiguous block range [x*per, (x+1)*per) (gridDim.x is a multiple of 8)
     const unsigned nb = gridDim.x, b = blockIdx.x, per = nb >> 3;
     const unsigned lb = (b & 7u) * per + (b >> 3);
-    int col = (int)(lb * kWavesPerBlock + (threadIdx.x >> 6));
-    return __builtin_amdgcn_readfirstlane(col);
+    int ch = (int)(lb * kWavesPerBlock + (threadIdx.x >> 6));
+    return __builtin_amdgcn_readfirstlane(ch);
 }
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+// this lane's cell pair
+struct Pair {
+    int col;      // column index (valid when live)
+    int k0;       // 0-based level of the first cell
+    long long c;  // layout index of the first cell
+    bool live;
+};
+
+__device__ __forceinline__ Pair lane_pair(const int *slot, int chunk)
+{
+    const long long s = (long long)chunk * kWave + lane_id();
+    const int w = slot[s];
+    Pair p;
+    p.live = w >= 0;
+    p.col = w & ((1 << kColBits) - 1);
+    p.k0 = p.live ? 2 * (int)((unsigned)w >> kColBits) : 0;
+    p.c = 2 * s;
+    return p;
+}
 
 // Fortran max/min as the reference compiles them (a > b ? a : b)
 template <class T> __device__ __forceinline__ T fmx(T a, T b) { return a > b ? a : b; }
 template <class T> __device__ __forceinline__ T fmn(T a, T b) { return a < b ? a : b; }
 
-template <class T> __device__ __forceinline__ T shdn(T v) { return __shfl_down(v, 1, kWave); }
-template <class T> __device__ __forceinline__ T shup(T v) { return __shfl_up(v, 1, kWave); }
+// lane l receives lane l-1's / l+1's value (own value at the wave ends)
+template <class T> __device__ __forceinline__ T from_prev(T v) { return __shfl_up(v, 1, kWave); }
+template <class T> __device__ __forceinline__ T from_next(T v) { return __shfl_down(v, 1, kWave); }
 
+// two consecutive cells
+template <class T>
+struct V2 {
+    T x, y;
+};
+
+__device__ __forceinline__ V2<double> ld2(const double *p)
+{
+    const double2 v = *reinterpret_cast<const double2 *>(p);
+    return {v.x, v.y};
+}
+__device__ __forceinline__ V2<float> ld2(const float *p)
+{
+    const float2 v = *reinterpret_cast<const float2 *>(p);
+    return {v.x, v.y};
+}
+__device__ __forceinline__ void st2(double *p, double a, double b)
+{
+    *reinterpret_cast<double2 *>(p) = make_double2(a, b);
+}
+__device__ __forceinline__ void st2(float *p, float a, float b)
+{
+    *reinterpret_cast<float2 *>(p) = make_float2(a, b);
+}
+
+// two cells x two components (x0 y0 x1 y1), gradients and velocity
+template <class T>
+struct V4 {
+    T x0, y0, x1, y1;
+};
+__device__ __forceinline__ V4<double> ld4(const double *p)
+{
+    const double2 a = reinterpret_cast<const double2 *>(p)[0];
+    const double2 b = reinterpret_cast<const double2 *>(p)[1];
+    return {a.x, a.y, b.x, b.y};
+}
+__device__ __forceinline__ V4<float> ld4(const float *p)
+{
+    const float4 a = *reinterpret_cast<const float4 *>(p);
+    return {a.x, a.y, a.z, a.w};
+}
+
+// gradient storage: double, float, or half holding gradient * gscale (a
+// mesh length scale, so the stored values are O(tracer increments per edge)
+// instead of subnormal)
 template <class G> struct GradIO;
 template <> struct GradIO<double> {
-    __device__ static double ld(const double *p, double) { return *p; }
-    __device__ static void st(double *p, double v, double) { *p = v; }
+    __device__ static V4<double> ld(const double *p, double) { return ld4(p); }
+    __device__ static void st(double *p, double a, double b, double c, double d, double)
+    {
+        reinterpret_cast<double2 *>(p)[0] = make_double2(a, b);
+        reinterpret_cast<double2 *>(p)[1] = make_double2(c, d);
+    }
 };
 template <> struct GradIO<float> {
-    __device__ static float ld(const float *p, float) { return *p; }
-    __device__ static void st(float *p, float v, float) { *p = v; }
+    __device__ static V4<float> ld(const float *p, float) { return ld4(p); }
+    __device__ static void st(float *p, float a, float b, float c, float d, float)
+    {
+        *reinterpret_cast<float4 *>(p) = make_float4(a, b, c, d);
+    }
 };
-// half storage holds gradient * gscale (a mesh length scale), so the stored
-// values are O(tracer increments per edge) instead of subnormal
 template <> struct GradIO<__half> {
-    __device__ static float ld(const __half *p, float inv_scale) { return __half2float(*p) * inv_scale; }
-    __device__ static void st(__half *p, float v, float scale) { *p = __float2half(v * scale); }
+    __device__ static V4<float> ld(const __half *p, float inv)
+    {
+        const uint2 u = *reinterpret_cast<const uint2 *>(p);
+        const __half2 a = *reinterpret_cast<const __half2 *>(&u.x);
+        const __half2 b = *reinterpret_cast<const __half2 *>(&u.y);
+        const float2 fa = __half22float2(a), fb = __half22float2(b);
+        return {fa.x * inv, fa.y * inv, fb.x * inv, fb.y * inv};
+    }
+    __device__ static void st(__half *p, float a, float b, float c, float d, float scale)
+    {
+        const __half2 h0 = __floats2half2_rn(a * scale, b * scale);
+        const __half2 h1 = __floats2half2_rn(c * scale, d * scale);
+        uint2 u;
+        u.x = *reinterpret_cast<const unsigned *>(&h0);
+        u.y = *reinterpret_cast<const unsigned *>(&h1);
+        *reinterpret_cast<uint2 *>(p) = u;
+    }
 };
 
 }  // namespace fta
