@@ -31,6 +31,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "node-level tracer updates/s at 1M nodes×64 lvls, FP64 vs FP32; % HBM peak"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# this workload (copy-calibrated), for the roofline's "traffic" field
+TRAFFIC_TABLE = ROOT / "profiles" / "hbm_traffic_periodic_1000x1000x64.md"
 
 
 def counts(api):
@@ -55,27 +58,29 @@ def step_alg_bytes(c, ntr, s_t, s_v, s_m):
 
 
 def kernel_alg_bytes(name, c, ntr, s_t, s_r, s_g):
-    """Compulsory bytes of one step's launches of a kernel (all ntr tracers
+    """Compulsory bytes of one step's launch of a kernel (all ntr tracers
     batched into it): every array it reads or writes counted once over its
-    active cells; mesh/velocity terms once, tracer terms ntr times."""
+    active cells (C node layers, C+N node interfaces, Ce element layers, Ced
+    edge levels); mesh and velocity terms once, tracer terms ntr times.
+    Gradients count the nodal means only (the up/down-wind triangle values
+    replace them on shared levels), so the figure is a lower bound."""
     C, Ce, Ced, N, T, E = c["C"], c["Ce"], c["Ced"], c["N"], c["T"], c["E"]
-    D = 8  # padded node degree of the packed node records
-    node_rec = N * (4 + 3 * 4 * D)          # nlevels + edge list + edge levels + element list
-    if name == "k_grad":          # valuesAB, element records -> nodal mean gradient (2 comps)
-        return ntr * (s_t * C + 2 * s_g * C) + T * (16 + 8 * s_r) + N * (4 + 4 * D)
-    if name == "k_edge":          # tracer (T and AB share storage), gradients, vel, helem, edge records -> LO, HO-LO
-        return (ntr * (s_t * C + 2 * s_g * C + 2 * s_r * Ced)
-                + 3 * s_r * Ce + E * (32 + 8 * s_r) + T * (16 + 8 * s_r))
-    if name == "k_node_lo":       # LO and HO-LO edge fluxes, tracer -> LO, adf_v, P+, P-; w, w_e, 7 node geometry arrays
-        return (ntr * (2 * s_r * Ced + s_t * C + 8 * C + s_r * (C + N) + 2 * s_r * C)
-                + 2 * s_r * (C + N) + s_r * (C + N) * 2 + 5 * s_r * C + node_rec)
-    if name == "k_node_limit":    # LO, tracer, P+/P- -> R+/R-; areasvol, element records
-        return ntr * (8 * C + s_t * C + 2 * s_r * C + 2 * s_r * C) + s_r * C + node_rec + T * 16
-    if name == "k_node_update":   # HO-LO edge flux, adf_v, R+/R-, LO, tracer -> tracer; hnode(2), areasvol
-        return (ntr * (s_r * Ced + s_r * (C + N) + 2 * s_r * C + 8 * C + 2 * s_t * C)
-                + 3 * s_r * C + node_rec + 4 * D * N)
+    CI = C + N
+    if name == "k_trxy":          # valuesAB at the element nodes -> elemental gradients (2 comps)
+        return ntr * (s_t * C + 2 * s_g * Ce) + T * (16 + 8 * s_r)
+    if name == "k_gnod":          # elemental gradients -> nodal means; node->element records
+        return ntr * (2 * s_g * Ce + 2 * s_g * C) + N * (48 + 8 * s_r)
+    if name == "k_edge":          # tracer (T and AB share storage), gradients, vel, helem, edge records -> transport, HO-LO
+        return ntr * (s_t * C + 2 * s_g * C + s_r * Ced) + s_r * Ced + 3 * s_r * Ce + E * (32 + 8 * s_r)
+    if name == "k_node_lo":       # edge transport, HO-LO edge fluxes, tracer, w, w_e, 6 geometry arrays -> LO, adf_v, P+, P-
+        return (ntr * (s_r * Ced + s_t * C + 8 * C + s_r * CI + 2 * s_r * C)
+                + s_r * Ced + 4 * s_r * CI + 4 * s_r * C + 80 * N)
+    if name == "k_node_limit":    # LO, tracer, P+/P- -> R+/R-; areasvol, 1-ring records
+        return ntr * (8 * C + s_t * C + 4 * s_r * C) + s_r * C + 48 * N
+    if name == "k_node_update":   # HO-LO edge flux, adf_v, R+/R- (own and 1-ring), LO, tracer -> tracer; 3 geometry arrays
+        return ntr * (s_r * Ced + s_r * CI + 2 * s_r * C + 8 * C + 2 * s_t * C) + 3 * s_r * C + 80 * N
     if name == "k_node_nolim":
-        return ntr * (s_r * Ced + 2 * s_t * C) + s_r * (C + N) * 2 + 3 * s_r * C + node_rec
+        return ntr * (s_r * Ced + 2 * s_t * C) + 2 * s_r * CI + 2 * s_r * C + 48 * N
     return 0
 
 
@@ -189,6 +194,10 @@ def main():
     f64 = res["fp64"]
     dom = f64["dominant"]
     achieved = f64["kernels_gbs"][dom]
+    traffic = None
+    if (cfg["nx"], cfg["ny"], cfg["nl"]) == (1000, 1000, 64):
+        from fesom2_tracer_dwarf_amd.rocprof_summary import read_traffic
+        traffic = read_traffic(TRAFFIC_TABLE).get(dom, {}).get("fp64")
     line = {
         "metric": METRIC,
         "value": f64["updates_per_s"],
@@ -208,7 +217,9 @@ def main():
                    "active_node_cells": c["C"], "dense_node_cells": c["N"] * (cfg["nl"] - 1),
                    "dt_s": dt, "cfl": 0.3},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes": kernel_alg_bytes(dom, c, ntr, 8, 8, 8),
+                     "traffic_source": str(TRAFFIC_TABLE.relative_to(ROOT)) if traffic else None},
         "step_roofline": {"alg_bytes": f64["step_alg_bytes"], "achieved_gbs": f64["step_gbs"],
                           "frac": f64["step_hbm_frac"]},
         "variants": {k: {kk: vv for kk, vv in v.items() if kk not in ("wall_s",)} for k, v in res.items()},

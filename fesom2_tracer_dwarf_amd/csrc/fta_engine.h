@@ -3,10 +3,11 @@
 // the tracers and the work arrays, and launches the step on its own stream.
 //
 // Device layout: every level-indexed array stores only the wet levels of
-// each column, columns back to back in mesh order (nodes, elements and edges
-// follow a space-filling curve when the mesh was generated with one), so a
-// sweep over consecutive columns streams through contiguous memory. A
-// wavefront handles one column with lane k = level k+1 (requires nl <= 64).
+// each column, columns packed whole, in mesh order (nodes, elements and
+// edges follow a space-filling curve when the mesh was generated with one),
+// into 128-cell chunks; a wavefront handles one chunk, two cells per lane, so
+// a sweep over consecutive columns streams through contiguous memory
+// (requires nl <= 64).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -68,7 +69,7 @@ public:
 
     Diag diagnostics(int tr);
     // copy a device work array out in the unpadded reference layout
-    // names: fct_LO, adv_flux_lo, adv_flux_hor, adv_flux_ver, fct_plus, fct_minus, gnod
+    // names: fct_LO, adv_flux_hor, adv_flux_ver, fct_plus, fct_minus, gnod
     int get_work(const std::string &name, int tr, double *out, size_t count);
     size_t work_count(const std::string &name) const;
 

@@ -100,14 +100,13 @@ __global__ __launch_bounds__(kBlock) void k_trxy(DMesh<R> m, const double *__res
     for (int t = 0; t < NT; ++t) {
         const double *A = ab + t * tsn + p.k0;
         const V2<double> a0 = ld2(A + er.o0), a1 = ld2(A + er.o1), a2 = ld2(A + er.o2);
+        // both cells computed unconditionally (a branch on the second cell
+        // would let the compiler split each 16-byte pair load in two)
         const R gx0 = g0 * (R)a0.x + g1 * (R)a1.x + g2 * (R)a2.x;
         const R gy0 = g3 * (R)a0.x + g4 * (R)a1.x + g5 * (R)a2.x;
-        R gx1 = 0, gy1 = 0;
-        if (v1) {
-            gx1 = g0 * (R)a0.y + g1 * (R)a1.y + g2 * (R)a2.y;
-            gy1 = g3 * (R)a0.y + g4 * (R)a1.y + g5 * (R)a2.y;
-        }
-        GradIO<G>::st(trxy + t * tst + 2 * p.c, gx0, gy0, gx1, gy1, gscale);
+        const R gx1 = g0 * (R)a0.y + g1 * (R)a1.y + g2 * (R)a2.y;
+        const R gy1 = g3 * (R)a0.y + g4 * (R)a1.y + g5 * (R)a2.y;
+        GradIO<G>::st(trxy + t * tst + 2 * p.c, gx0, gy0, v1 ? gx1 : (R)0, v1 ? gy1 : (R)0, gscale);
     }
 }
 
@@ -122,31 +121,47 @@ __global__ __launch_bounds__(kBlock) void k_gnod(DMesh<R> m, const G *__restrict
     const int L = m.nlev[p.col] - 1;
     if (p.k0 >= L) return;  // the pair holds only the bottom interface
     const bool v1 = p.k0 + 1 < L;
-    const NodeElems ne = m.nelem[p.col];
-    const R *ar = m.nelem_area + kDeg * (size_t)p.col;
+    // node->element record and areas as whole 16-byte loads
+    int ot[kDeg], le[kDeg];
+    R ar[kDeg];
+    {
+        const int4 *q = reinterpret_cast<const int4 *>(&m.nelem[p.col]);
+        const int4 a = q[0], b = q[1];
+        const uint2 l = *reinterpret_cast<const uint2 *>(&m.nelem[p.col].le[0]);
+        ot[0] = a.x; ot[1] = a.y; ot[2] = a.z; ot[3] = a.w; ot[4] = b.x; ot[5] = b.y; ot[6] = b.z; ot[7] = b.w;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            le[j] = (int)((l.x >> (8 * j)) & 255u);
+            le[j + 4] = (int)((l.y >> (8 * j)) & 255u);
+        }
+        const R *ap = m.nelem_area + kDeg * (size_t)p.col;
+#pragma unroll
+        for (int j = 0; j < kDeg; j += 2) {
+            const V2<R> v = ld2(ap + j);
+            ar[j] = v.x;
+            ar[j + 1] = v.y;
+        }
+    }
     R tvol0 = 0, tvol1 = 0;
     R tx0[NT], ty0[NT], tx1[NT], ty1[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) { tx0[t] = 0; ty0[t] = 0; tx1[t] = 0; ty1[t] = 0; }
 #pragma unroll
     for (int j = 0; j < kDeg; ++j) {
-        const int ot = ne.ot[j];
-        if (ot < 0) break;
-        const int le = ne.le[j];
-        if (p.k0 >= le) continue;
-        const bool a1 = p.k0 + 1 < le;
+        if (ot[j] < 0) break;
+        if (p.k0 >= le[j]) continue;
+        const bool a1 = p.k0 + 1 < le[j];
         const R a = ar[j];
         tvol0 = tvol0 + a;
-        if (a1) tvol1 = tvol1 + a;
+        tvol1 = a1 ? tvol1 + a : tvol1;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const auto gg = GradIO<G>::ld(trxy + t * tst + 2 * ((size_t)ot + p.k0), ginv);
+            const auto gg = GradIO<G>::ld(trxy + t * tst + 2 * ((size_t)ot[j] + p.k0), ginv);
             tx0[t] = tx0[t] + (R)gg.x0 * a;
             ty0[t] = ty0[t] + (R)gg.y0 * a;
-            if (a1) {
-                tx1[t] = tx1[t] + (R)gg.x1 * a;
-                ty1[t] = ty1[t] + (R)gg.y1 * a;
-            }
+            const R x1 = tx1[t] + (R)gg.x1 * a, y1 = ty1[t] + (R)gg.y1 * a;
+            tx1[t] = a1 ? x1 : tx1[t];
+            ty1[t] = a1 ? y1 : ty1[t];
         }
     }
 #pragma unroll
@@ -159,8 +174,9 @@ __global__ __launch_bounds__(kBlock) void k_gnod(DMesh<R> m, const G *__restrict
 // Horizontal edge fluxes (lib/oce_adv_tra_hor.F90): UPW1 (:64-261),
 // MUSCL (:265-550), MFCT (:554-846). Segments C (both elements), D (el1
 // only) and E (el2 only); no cavities, so segments A/B are empty.
-// FCT: writes the LO flux (from ttf) and HO-LO (HO from ttfAB) per level.
-// No limiter: writes the HO flux. Gradients follow fill_up_dn_grad: the
+// FCT: writes HO-LO (HO from ttfAB, LO upwind from ttf) per level and tracer,
+// and the tracer-independent transport vflux per level, from which the node
+// side re-forms the LO fluxes. No limiter: writes the HO flux. Gradients follow fill_up_dn_grad: the
 // up/down-wind triangle on levels shared by all elements around both nodes,
 // nodal means below.
 // ===========================================================================
@@ -171,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict
                                                  const G *__restrict__ trxy, size_t tst,
                                                  const G *__restrict__ gnod, size_t tsg,
                                                  R num_ord, float ginv,
-                                                 R *__restrict__ flux_lo, R *__restrict__ flux, size_t tse)
+                                                 R *__restrict__ vfl, R *__restrict__ flux, size_t tse)
 {
     const int ch = wave_chunk();
     if (ch >= m.nch_e) return;
@@ -235,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict
             lo0 = (R)(-0.5) * ((R)T1.x * (vflux0 + fabs(vflux0)) + (R)T2.x * (vflux0 - fabs(vflux0))) - (R)0;
             if (valid1)
                 lo1 = (R)(-0.5) * ((R)T1.y * (vflux1 + fabs(vflux1)) + (R)T2.y * (vflux1 - fabs(vflux1))) - (R)0;
-            st2(flux_lo + t * tse + o, lo0, lo1);
+            if (t == 0) st2(vfl + o, vflux0, valid1 ? vflux1 : (R)0);
         }
         V2<double> B1 = T1, B2 = T2;
         if constexpr (!SAME) {
@@ -313,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
                                                     const R *__restrict__ WEv,
                                                     const double *__restrict__ ttf,
                                                     const double *__restrict__ ttfAB, size_t tsn,
-                                                    const R *__restrict__ flux_lo,
+                                                    const R *__restrict__ vfl,
                                                     const R *__restrict__ adf_h, size_t tse,
                                                     double dt, R num_ord,
                                                     double *__restrict__ LO, R *__restrict__ adf_v,
@@ -327,9 +343,10 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
     const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
     const bool if0 = p.live && k0 < nlev, if1 = p.live && k0 + 1 < nlev;
     const size_t c = (size_t)p.c;
-    int oe[kDeg];
+    int oe[kDeg], oo[kDeg];
     unsigned info[kDeg];
     load_node_edges(m, p.col, p.live && lay0, oe, info);
+    load_other_nodes(m, p.col, p.live && lay0, oo);
     VCol2<R> lc{};
     lc.nlev = nlev;
     lc.k0 = k0;
@@ -379,7 +396,14 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
             if (oe[j] < 0 || k0 >= nlm) continue;
             const bool on1 = lay1 && k0 + 1 < nlm;
             const bool pos = info_pos(info[j]);
-            const V2<R> f = ld2(flux_lo + te + (size_t)oe[j] + k0);
+            // LO upwind flux of the edge (adv_tra_hor_upw1, lib/oce_adv_tra_hor.F90:64-261)
+            // from its transport and the tracer at edges(1), edges(2)
+            const V2<R> v = ld2(vfl + (size_t)oe[j] + k0);
+            const V2<double> To = ld2(ttf + tn + (size_t)oo[j] + k0);
+            const R T10 = pos ? (R)T.x : (R)To.x, T20 = pos ? (R)To.x : (R)T.x;
+            const R T11 = pos ? (R)T.y : (R)To.y, T21 = pos ? (R)To.y : (R)T.y;
+            const V2<R> f{(R)(-0.5) * (T10 * (v.x + fabs(v.x)) + T20 * (v.x - fabs(v.x))) - (R)0,
+                          (R)(-0.5) * (T11 * (v.y + fabs(v.y)) + T21 * (v.y - fabs(v.y))) - (R)0};
             const V2<R> a = ld2(adf_h + te + (size_t)oe[j] + k0);
             slo0 = pos ? slo0 + (double)f.x : slo0 - (double)f.x;
             if (pos) { pp0 = pp0 + fmx((R)0, a.x); pm0 = pm0 + fmn((R)0, a.x); }
@@ -534,6 +558,19 @@ __device__ __forceinline__ R limit_vertical(int nz, bool layer, R fv, R rp, R rm
     return fv;
 }
 
+// limiter of one edge flux f seen from a node (b3, lib/oce_adv_tra_fct.F90:431-445):
+// pos = the node is edges(1); rp/rm its R+/R-, rpo/rmo those of the other node
+template <class R>
+__device__ __forceinline__ R edge_alpha(bool pos, R f, R rp, R rm, R rpo, R rmo)
+{
+    const R rp1 = pos ? rp : rpo, rm1 = pos ? rm : rmo;
+    const R rp2 = pos ? rpo : rp, rm2 = pos ? rmo : rm;
+    R ae = (R)1;
+    if (f >= (R)0) { ae = fmn(ae, rp1); ae = fmn(ae, rm2); }
+    else           { ae = fmn(ae, rm1); ae = fmn(ae, rp2); }
+    return ae;
+}
+
 // limiting b3 (lib/oce_adv_tra_fct.F90:394-471), flux divergence
 // (lib/oce_adv_tra_driver.F90:298-450) and the update
 // values += (dttf_h + dttf_v)/hnode_new (src/fesom_analytic.F90:419-437)
@@ -587,40 +624,29 @@ __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__re
             const bool pos = info_pos(info[j]);  // this node is edges(1)
             const V2<R> f = ld2(adf_h + te + (size_t)oe[j] + k0);
             const V2<R> rpo = ld2(Rp + tn + (size_t)oo[j] + k0), rmo = ld2(Rm + tn + (size_t)oo[j] + k0);
-            {
-                const R rp1 = pos ? rp.x : rpo.x, rm1 = pos ? rm.x : rmo.x;
-                const R rp2 = pos ? rpo.x : rp.x, rm2 = pos ? rmo.x : rm.x;
-                R ae = (R)1;
-                if (f.x >= (R)0) { ae = fmn(ae, rp1); ae = fmn(ae, rm2); }
-                else             { ae = fmn(ae, rm1); ae = fmn(ae, rp2); }
-                const double fl = (double)(ae * f.x);
-                sh0 = pos ? sh0 + fl * dt / (double)av.x : sh0 - fl * dt / (double)av.x;
-            }
-            if (on1) {
-                const R rp1 = pos ? rp.y : rpo.y, rm1 = pos ? rm.y : rmo.y;
-                const R rp2 = pos ? rpo.y : rp.y, rm2 = pos ? rmo.y : rm.y;
-                R ae = (R)1;
-                if (f.y >= (R)0) { ae = fmn(ae, rp1); ae = fmn(ae, rm2); }
-                else             { ae = fmn(ae, rm1); ae = fmn(ae, rp2); }
-                const double fl = (double)(ae * f.y);
-                sh1 = pos ? sh1 + fl * dt / (double)av.y : sh1 - fl * dt / (double)av.y;
-            }
+            // both cells evaluated unconditionally, the second one masked
+            // (keeps every pair load one 16-byte access)
+            const double fl0 = (double)(edge_alpha(pos, f.x, rp.x, rm.x, rpo.x, rmo.x) * f.x);
+            const double fl1 = (double)(edge_alpha(pos, f.y, rp.y, rm.y, rpo.y, rmo.y) * f.y);
+            sh0 = pos ? sh0 + fl0 * dt / (double)av.x : sh0 - fl0 * dt / (double)av.x;
+            const double s1 = pos ? sh1 + fl1 * dt / (double)av.y : sh1 - fl1 * dt / (double)av.y;
+            sh1 = on1 ? s1 : sh1;
         }
         const V2<double> T = ld2(ttf + tn + c), lo = ld2(LO + tn + c);
-        double n0, n1 = T.y;
+        double n0, n1;
         {
             double dv = 0.0 - T.x * (double)h.x + lo.x * (double)hn.x;
             dv = dv + (double)(fvl0 - fvl1) * dt / (double)av.x;
             const double del = (0.0 + sh0) + dv;
             n0 = T.x + del / (double)hn.x;
         }
-        if (lay1) {
+        {
             double dv = 0.0 - T.y * (double)h.y + lo.y * (double)hn.y;
             dv = dv + (double)(fvl1 - fvl2) * dt / (double)av.y;
             const double del = (0.0 + sh1) + dv;
             n1 = T.y + del / (double)hn.y;
         }
-        st2(ttf + tn + c, n0, n1);
+        st2(ttf + tn + c, n0, lay1 ? n1 : T.y);
     }
 }
 
@@ -668,21 +694,22 @@ __global__ __launch_bounds__(kBlock) void k_node_nolim(DMesh<R> m, const R *__re
             const V2<R> f = ld2(flux_h + te + (size_t)oe[j] + k0);
             const double f0 = (double)f.x, f1 = (double)f.y;
             sh0 = pos ? sh0 + f0 * dt / (double)av.x : sh0 - f0 * dt / (double)av.x;
-            if (on1) sh1 = pos ? sh1 + f1 * dt / (double)av.y : sh1 - f1 * dt / (double)av.y;
+            const double s1 = pos ? sh1 + f1 * dt / (double)av.y : sh1 - f1 * dt / (double)av.y;
+            sh1 = on1 ? s1 : sh1;
         }
         const V2<double> T = ld2(ttf + tn + c);
-        double n0, n1 = T.y;
+        double n0, n1;
         {
             const double dv = 0.0 + (double)(fv0 - fv1) * dt / (double)av.x;
             const double del = (0.0 + sh0) + dv;
             n0 = T.x + del / (double)hn.x;
         }
-        if (lay1) {
+        {
             const double dv = 0.0 + (double)(fv1 - fv2) * dt / (double)av.y;
             const double del = (0.0 + sh1) + dv;
             n1 = T.y + del / (double)hn.y;
         }
-        st2(ttf + tn + c, n0, n1);
+        st2(ttf + tn + c, n0, lay1 ? n1 : T.y);
     }
 }
 
@@ -831,7 +858,8 @@ struct Engine::Impl {
     // tracers [ntr][CN]
     double *values = nullptr, *valuesAB = nullptr;
     // work, per tracer
-    void *flux_lo = nullptr, *adf_h = nullptr;             // [ntr][CE]
+    void *vfl = nullptr;                                   // [CE] edge transport (FCT)
+    void *adf_h = nullptr;                                 // [ntr][CE]
     void *adf_v = nullptr;                                 // [ntr][CN] (interfaces)
     void *Pp = nullptr, *Pm = nullptr;                     // [ntr][CN]
     double *LO = nullptr;                                  // [ntr][CN]
@@ -1033,7 +1061,7 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     I.WE = zalloc(I.tsn * rs);
     I.values = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
     I.valuesAB = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
-    I.flux_lo = zalloc(I.tse * ntr_ * rs);
+    I.vfl = zalloc(I.tse * rs);
     I.adf_h = zalloc(I.tse * ntr_ * rs);
     I.adf_v = zalloc(I.tsn * ntr_ * rs);
     I.Pp = zalloc(I.tsn * ntr_ * rs);
@@ -1128,7 +1156,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
     const size_t tsn = I.tsn, tst = I.tst, tse = I.tse, tsg = 2 * I.tsn;
     double *vals = I.values + t0 * tsn;
     const double *ab = refresh ? vals : I.valuesAB + t0 * tsn;
-    R *flux_lo = (R *)I.flux_lo + t0 * tse, *adf_h = (R *)I.adf_h + t0 * tse;
+    R *vfl = (R *)I.vfl, *adf_h = (R *)I.adf_h + t0 * tse;
     R *adf_v = (R *)I.adf_v + t0 * tsn, *Pp = (R *)I.Pp + t0 * tsn, *Pm = (R *)I.Pm + t0 * tsn;
     double *LO = I.LO + t0 * tsn;
     G *trxy = (G *)I.trxy + t0 * tst;
@@ -1146,10 +1174,10 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
     do {                                                                                                    \
         if (refresh)                                                                                        \
             hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, true>), ge, blk, 0, st, m, vel, vals, ab, tsn, trxy, tst, \
-                               gnod, tsg, ph, I.ginv, flux_lo, adf_h, tse);                                 \
+                               gnod, tsg, ph, I.ginv, vfl, adf_h, tse);                                 \
         else                                                                                                \
             hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, false>), ge, blk, 0, st, m, vel, vals, ab, tsn, trxy, tst, \
-                               gnod, tsg, ph, I.ginv, flux_lo, adf_h, tse);                                 \
+                               gnod, tsg, ph, I.ginv, vfl, adf_h, tse);                                 \
     } while (0)
     if (fct) {
         if (sp.hor == HOR_MUSCL) EDGE(HOR_MUSCL, true);
@@ -1166,10 +1194,10 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
 #define NLO(V)                                                                                              \
     do {                                                                                                    \
         if (refresh)                                                                                        \
-            hipLaunchKernelGGL((k_node_lo<R, V, NT, true>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, flux_lo, \
+            hipLaunchKernelGGL((k_node_lo<R, V, NT, true>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, vfl, \
                                adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
         else                                                                                                \
-            hipLaunchKernelGGL((k_node_lo<R, V, NT, false>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, flux_lo, \
+            hipLaunchKernelGGL((k_node_lo<R, V, NT, false>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, vfl, \
                                adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
     } while (0)
         switch (sp.ver) {
@@ -1290,7 +1318,7 @@ size_t Engine::work_count(const std::string &name) const
 {
     const size_t L = nl_ - 1;
     if (name == "fct_LO" || name == "fct_plus" || name == "fct_minus") return L * N_;
-    if (name == "adv_flux_hor" || name == "adv_flux_lo") return L * E_;
+    if (name == "adv_flux_hor") return L * E_;
     if (name == "adv_flux_ver") return (size_t)nl_ * N_;
     if (name == "gnod") return 2 * L * N_;
     return 0;
@@ -1310,7 +1338,6 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
     else if (name == "fct_plus") src = (const char *)I.Pp;
     else if (name == "fct_minus") src = (const char *)I.Pm;
     else if (name == "adv_flux_hor") { src = (const char *)I.adf_h; lay = &I.le; sub = 0; }
-    else if (name == "adv_flux_lo") { src = (const char *)I.flux_lo; lay = &I.le; sub = 0; }
     else if (name == "adv_flux_ver") { src = (const char *)I.adf_v; L = nl_; sub = 0; }
     else if (name == "gnod") {
         src = (const char *)I.gnod; comps = 2;

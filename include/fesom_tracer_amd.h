@@ -172,8 +172,8 @@ int tracer_mesh_array_info(const char *name, long long *count, int *kind);
 int tracer_get_mesh_array(const char *name, void *out, long long count);
 
 /* No reference counterpart: a named GPU work array of the last step of one
- * tracer (1-based id) in the reference layout: fct_LO, adv_flux_lo (LO edge
- * flux), adv_flux_hor (HO-LO edge flux before limiting), adv_flux_ver
+ * tracer (1-based id) in the reference layout: fct_LO, adv_flux_hor (HO-LO
+ * edge flux before limiting; the HO flux without a limiter), adv_flux_ver
  * (HO-LO vertical flux before limiting), fct_plus / fct_minus (R+/R-), gnod
  * (nodal mean gradient, (2,nl-1,nod2D)) */
 int tracer_get_work_array(int tracer_id, const char *name, double *out, long long count);
