@@ -61,6 +61,7 @@ SIGNATURES = {
     "tracer_get_work_array": (c_int, [c_int, c_char_p, F64, c_ll]),
     "tracer_time_steps": (c_int, [c_int, c_double, P_double]),
     "tracer_time_kernels": (c_int, [c_double, c_int, c_char_p, c_int, F64, c_int]),
+    "tracer_precision_error": (c_int, [c_int, c_int, c_double, F64]),
     "tracer_last_error": (c_char_p, []),
 }
 REFERENCE_SYMBOLS = list(SIGNATURES)[:23]

@@ -73,6 +73,11 @@ public:
     int get_work(const std::string &name, int tr, double *out, size_t count);
     size_t work_count(const std::string &name) const;
 
+    // tracer tr of this engine against the same tracer of ref (same mesh,
+    // any precision), over the wet layers: out = {max|a-b|, ||a-b||_2,
+    // max|a-b|/max|b|, ||a-b||_2/||b||_2}
+    void difference(const Engine &ref, int tr, double out[4]);
+
     // per-kernel timing of one step with hip events on the engine stream
     // (names and milliseconds, averaged over reps)
     void time_kernels(double dt, int reps, std::vector<std::string> &names, std::vector<double> &ms);

@@ -22,6 +22,7 @@
 #include <limits>
 #include <memory>
 #include <stdexcept>
+#include <type_traits>
 
 namespace fta {
 
@@ -180,6 +181,13 @@ __global__ __launch_bounds__(kBlock) void k_gnod(DMesh<R> m, const G *__restrict
 // up/down-wind triangle on levels shared by all elements around both nodes,
 // nodal means below.
 // ===========================================================================
+// LO upwind edge flux (adv_tra_hor_upw1, lib/oce_adv_tra_hor.F90:64-261) from
+// the edge transport v and the tracer at edges(1), edges(2)
+__device__ __forceinline__ double lo_flux(double v, double T1, double T2)
+{
+    return -0.5 * (T1 * (v + fabs(v)) + T2 * (v - fabs(v))) - 0.0;
+}
+
 template <class R, class G, int HOR, bool FCT, int NT, bool SAME>
 __global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict__ vel,
                                                  const double *__restrict__ ttf,
@@ -241,16 +249,17 @@ __global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const size_t tn = t * tsn;
-        R lo0 = 0, lo1 = 0;
+        // the LO flux in FP64 whatever R is (it must see the FP64 tracer
+        // unrounded for the LO solution to stay monotone); HO in R
+        double lo0 = 0.0, lo1 = 0.0;
         V2<double> T1{0.0, 0.0}, T2{0.0, 0.0};
         if constexpr (FCT || SAME) {
             T1 = ld2(ttf + tn + er.o1 + k0);
             T2 = ld2(ttf + tn + er.o2 + k0);
         }
         if constexpr (FCT) {
-            lo0 = (R)(-0.5) * ((R)T1.x * (vflux0 + fabs(vflux0)) + (R)T2.x * (vflux0 - fabs(vflux0))) - (R)0;
-            if (valid1)
-                lo1 = (R)(-0.5) * ((R)T1.y * (vflux1 + fabs(vflux1)) + (R)T2.y * (vflux1 - fabs(vflux1))) - (R)0;
+            lo0 = lo_flux((double)vflux0, T1.x, T2.x);
+            if (valid1) lo1 = lo_flux((double)vflux1, T1.y, T2.y);
             if (t == 0) st2(vfl + o, vflux0, valid1 ? vflux1 : (R)0);
         }
         V2<double> B1 = T1, B2 = T2;
@@ -294,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict
                 ho1 = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux1 * num_ord * (R)0.5 * (Tmean1 + Tmean2);
             }
         }
-        st2(flux + t * tse + o, ho0 - lo0, valid1 ? ho1 - lo1 : (R)0);
+        st2(flux + t * tse + o, (R)((double)ho0 - lo0), valid1 ? (R)((double)ho1 - lo1) : (R)0);
     }
 }
 
@@ -347,18 +356,20 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
     unsigned info[kDeg];
     load_node_edges(m, p.col, p.live && lay0, oe, info);
     load_other_nodes(m, p.col, p.live && lay0, oo);
-    VCol2<R> lc{};
+    // the LO vertical flux in FP64 (see lo_flux)
+    VCol2<double> lc{};
     lc.nlev = nlev;
     lc.k0 = k0;
-    V2<R> av{1, 1}, h{0, 0}, hn{1, 1};
+    V2<double> av{1, 1}, h{0, 0}, hn{1, 1};  // FP64 geometry: the LO solution feeds the conservative update
     if (p.live) {
-        lc.W = ld2(WEv + (size_t)c);
-        lc.area = ld2(m.area + c);
+        const V2<R> we = ld2(WEv + (size_t)c), ar = ld2(m.area + c);
+        lc.W = {(double)we.x, (double)we.y};
+        lc.area = {(double)ar.x, (double)ar.y};
     }
     if (lay0) {
-        av = ld2(m.areasvol + c);
-        h = ld2(m.hnode + c);
-        hn = ld2(m.hnode_new + c);
+        av = ld2(m.av_d + c);
+        h = ld2(m.h_d + c);
+        hn = ld2(m.hn_d + c);
     }
     VCol2<R> hc = node_vcol<R, VER>(m, p, nlev, Wv);
 
@@ -367,19 +378,19 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
         const size_t tn = t * tsn, te = t * tse;
         V2<double> T{0.0, 0.0};
         if (p.live) T = ld2(ttf + tn + c);
-        lc.A = tracer_window<R>(T, k0, nlev);
+        lc.A = tracer_window<double>(T, k0, nlev);
         // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
-        const R fl0 = vflux_upw1<0>(lc, (R)0), fl1 = vflux_upw1<1>(lc, (R)0);
-        const R fl2 = from_next(fl0);
+        const double fl0 = vflux_upw1<0>(lc, 0.0), fl1 = vflux_upw1<1>(lc, 0.0);
+        const double fl2 = from_next(fl0);
         if constexpr (SAME) {
-            hc.A = lc.A;
+            hc.A = tracer_window<R>(T, k0, nlev);
         } else {
             V2<double> B{0.0, 0.0};
             if (p.live) B = ld2(ttfAB + tn + c);
             hc.A = tracer_window<R>(B, k0, nlev);
         }
         R ad0, ad1;
-        ver_flux<R, VER>(hc, num_ord, (R)dt, fl0, fl1, ad0, ad1);
+        ver_flux<R, VER>(hc, num_ord, (R)dt, (R)fl0, (R)fl1, ad0, ad1);  // HO - LO
         const R ad2 = from_next(ad0);
         if (p.live) st2(adf_v + tn + c, if0 ? ad0 : (R)0, if1 ? ad1 : (R)0);
         if (!lay0) continue;
@@ -400,23 +411,23 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
             // from its transport and the tracer at edges(1), edges(2)
             const V2<R> v = ld2(vfl + (size_t)oe[j] + k0);
             const V2<double> To = ld2(ttf + tn + (size_t)oo[j] + k0);
-            const R T10 = pos ? (R)T.x : (R)To.x, T20 = pos ? (R)To.x : (R)T.x;
-            const R T11 = pos ? (R)T.y : (R)To.y, T21 = pos ? (R)To.y : (R)T.y;
-            const V2<R> f{(R)(-0.5) * (T10 * (v.x + fabs(v.x)) + T20 * (v.x - fabs(v.x))) - (R)0,
-                          (R)(-0.5) * (T11 * (v.y + fabs(v.y)) + T21 * (v.y - fabs(v.y))) - (R)0};
+            const V2<double> f{lo_flux((double)v.x, pos ? T.x : To.x, pos ? To.x : T.x),
+                               lo_flux((double)v.y, pos ? T.y : To.y, pos ? To.y : T.y)};
             const V2<R> a = ld2(adf_h + te + (size_t)oe[j] + k0);
-            slo0 = pos ? slo0 + (double)f.x : slo0 - (double)f.x;
+            slo0 = pos ? slo0 + f.x : slo0 - f.x;
             if (pos) { pp0 = pp0 + fmx((R)0, a.x); pm0 = pm0 + fmn((R)0, a.x); }
             else     { pp0 = pp0 + fmx((R)0, -a.x); pm0 = pm0 + fmn((R)0, -a.x); }
             if (on1) {
-                slo1 = pos ? slo1 + (double)f.y : slo1 - (double)f.y;
+                slo1 = pos ? slo1 + f.y : slo1 - f.y;
                 if (pos) { pp1 = pp1 + fmx((R)0, a.y); pm1 = pm1 + fmn((R)0, a.y); }
                 else     { pp1 = pp1 + fmx((R)0, -a.y); pm1 = pm1 + fmn((R)0, -a.y); }
             }
         }
-        const double LO0 = (T.x * (double)h.x + (slo0 + (double)(fl0 - fl1)) * dt / (double)av.x) / (double)hn.x;
+        // flux differences formed in FP64: each interface flux then leaves one
+        // layer and enters the next with exactly opposite sign in every precision
+        const double LO0 = (T.x * (double)h.x + (slo0 + (fl0 - fl1)) * dt / (double)av.x) / (double)hn.x;
         double LO1 = 0.0;
-        if (lay1) LO1 = (T.y * (double)h.y + (slo1 + (double)(fl1 - fl2)) * dt / (double)av.y) / (double)hn.y;
+        if (lay1) LO1 = (T.y * (double)h.y + (slo1 + (fl1 - fl2)) * dt / (double)av.y) / (double)hn.y;
         else { pp1 = 0; pm1 = 0; }
         st2(LO + tn + c, LO0, LO1);
         st2(Pp + tn + c, pp0, pp1);
@@ -440,9 +451,11 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
     const int L = nlev - 1, k0 = p.k0;
     const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
     const size_t c = (size_t)p.c;
-    const R bignumber = (R)1000;
+    // bounds in FP64 whatever R is: they are selections among FP64 tracer
+    // values, so no new extrema appear beyond the flux arithmetic's rounding
+    const double bignumber = 1000.0;
 
-    R tvmax0[NT], tvmin0[NT], tvmax1[NT], tvmin1[NT];
+    double tvmax0[NT], tvmin0[NT], tvmax1[NT], tvmin1[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) { tvmax0[t] = 0; tvmin0[t] = 0; tvmax1[t] = 0; tvmin1[t] = 0; }
     if (lay0) {
@@ -454,34 +467,34 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
         for (int t = 0; t < NT; ++t) {
             const size_t tn = t * tsn;
             const V2<double> ls = ld2(LO + tn + c), ts = ld2(ttf + tn + c);
-            const R mxs0 = fmx((R)ls.x, (R)ts.x), mns0 = fmn((R)ls.x, (R)ts.x);
-            const R mxs1 = fmx((R)ls.y, (R)ts.y), mns1 = fmn((R)ls.y, (R)ts.y);
-            R mx0[kDeg], mn0[kDeg], mx1[kDeg], mn1[kDeg];
+            const double mxs0 = fmx(ls.x, ts.x), mns0 = fmn(ls.x, ts.x);
+            const double mxs1 = fmx(ls.y, ts.y), mns1 = fmn(ls.y, ts.y);
+            double mx0[kDeg], mn0[kDeg], mx1[kDeg], mn1[kDeg];
 #pragma unroll
             for (int j = 0; j < kDeg; ++j) {
                 mx0[j] = 0; mn0[j] = 0; mx1[j] = 0; mn1[j] = 0;
                 if (rg.oo[j] >= 0 && k0 < (int)rg.lr[j]) {
                     const V2<double> l = ld2(LO + tn + (size_t)rg.oo[j] + k0);
                     const V2<double> v = ld2(ttf + tn + (size_t)rg.oo[j] + k0);
-                    mx0[j] = fmx((R)l.x, (R)v.x); mn0[j] = fmn((R)l.x, (R)v.x);
-                    mx1[j] = fmx((R)l.y, (R)v.y); mn1[j] = fmn((R)l.y, (R)v.y);
+                    mx0[j] = fmx(l.x, v.x); mn0[j] = fmn(l.x, v.x);
+                    mx1[j] = fmx(l.y, v.y); mn1[j] = fmn(l.y, v.y);
                 }
             }
-            R vmax0 = 0, vmin0 = 0, vmax1 = 0, vmin1 = 0;
+            double vmax0 = 0, vmin0 = 0, vmax1 = 0, vmin1 = 0;
             bool first = true;
 #pragma unroll
             for (int j = 0; j < kDeg; ++j) {
                 const int le = rg.le[j];
                 if (le == 0) continue;
                 const bool wrap = j + 1 >= nr;
-                const R bx0 = wrap ? mx0[0] : mx0[(j + 1) & (kDeg - 1)];
-                const R bn0 = wrap ? mn0[0] : mn0[(j + 1) & (kDeg - 1)];
-                const R bx1 = wrap ? mx1[0] : mx1[(j + 1) & (kDeg - 1)];
-                const R bn1 = wrap ? mn1[0] : mn1[(j + 1) & (kDeg - 1)];
-                const R emax0 = k0 < le ? fmx(fmx(mxs0, mx0[j]), bx0) : -bignumber;
-                const R emin0 = k0 < le ? fmn(fmn(mns0, mn0[j]), bn0) : bignumber;
-                const R emax1 = k0 + 1 < le ? fmx(fmx(mxs1, mx1[j]), bx1) : -bignumber;
-                const R emin1 = k0 + 1 < le ? fmn(fmn(mns1, mn1[j]), bn1) : bignumber;
+                const double bx0 = wrap ? mx0[0] : mx0[(j + 1) & (kDeg - 1)];
+                const double bn0 = wrap ? mn0[0] : mn0[(j + 1) & (kDeg - 1)];
+                const double bx1 = wrap ? mx1[0] : mx1[(j + 1) & (kDeg - 1)];
+                const double bn1 = wrap ? mn1[0] : mn1[(j + 1) & (kDeg - 1)];
+                const double emax0 = k0 < le ? fmx(fmx(mxs0, mx0[j]), bx0) : -bignumber;
+                const double emin0 = k0 < le ? fmn(fmn(mns0, mn0[j]), bn0) : bignumber;
+                const double emax1 = k0 + 1 < le ? fmx(fmx(mxs1, mx1[j]), bx1) : -bignumber;
+                const double emin1 = k0 + 1 < le ? fmn(fmn(mns1, mn1[j]), bn1) : bignumber;
                 vmax0 = first ? emax0 : fmx(vmax0, emax0);
                 vmin0 = first ? emin0 : fmn(vmin0, emin0);
                 vmax1 = first ? emax1 : fmx(vmax1, emax1);
@@ -489,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
                 first = false;
             }
             tvmax0[t] = vmax0; tvmin0[t] = vmin0;
-            tvmax1[t] = lay1 ? vmax1 : (R)0; tvmin1[t] = lay1 ? vmin1 : (R)0;
+            tvmax1[t] = lay1 ? vmax1 : 0.0; tvmin1[t] = lay1 ? vmin1 : 0.0;
         }
     }
     V2<R> av{1, 1};
@@ -497,8 +510,8 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
     const R flux_eps = (R)1e-16;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const R upx0 = from_prev(tvmax1[t]), upn0 = from_prev(tvmin1[t]);
-        const R dnx1 = from_next(tvmax0[t]), dnn1 = from_next(tvmin0[t]);
+        const double upx0 = from_prev(tvmax1[t]), upn0 = from_prev(tvmin1[t]);
+        const double dnx1 = from_next(tvmax0[t]), dnn1 = from_next(tvmin0[t]);
         if (!lay0) continue;
         const size_t tn = t * tsn;
         const V2<double> lo = ld2(LO + tn + c);
@@ -506,33 +519,33 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
         R rp0, rm0, rp1 = 0, rm1 = 0;
         {
             const int nz = k0 + 1;
-            R qp, qm;
+            double qp, qm;
             if (nz == 1 || nz == L) {
-                qp = tvmax0[t] - (R)lo.x;
-                qm = tvmin0[t] - (R)lo.x;
+                qp = tvmax0[t] - lo.x;
+                qm = tvmin0[t] - lo.x;
             } else {
-                qp = fmx(fmx(upx0, tvmax0[t]), tvmax1[t]) - (R)lo.x;
-                qm = fmn(fmn(upn0, tvmin0[t]), tvmin1[t]) - (R)lo.x;
+                qp = fmx(fmx(upx0, tvmax0[t]), tvmax1[t]) - lo.x;
+                qm = fmn(fmn(upn0, tvmin0[t]), tvmin1[t]) - lo.x;
             }
             R flux = P.x * (R)dt / av.x + flux_eps;
-            rp0 = fmn((R)1, qp / flux);
+            rp0 = fmn((R)1, (R)qp / flux);
             flux = M.x * (R)dt / av.x - flux_eps;
-            rm0 = fmn((R)1, qm / flux);
+            rm0 = fmn((R)1, (R)qm / flux);
         }
         if (lay1) {
             const int nz = k0 + 2;
-            R qp, qm;
+            double qp, qm;
             if (nz == L) {
-                qp = tvmax1[t] - (R)lo.y;
-                qm = tvmin1[t] - (R)lo.y;
+                qp = tvmax1[t] - lo.y;
+                qm = tvmin1[t] - lo.y;
             } else {
-                qp = fmx(fmx(tvmax0[t], tvmax1[t]), dnx1) - (R)lo.y;
-                qm = fmn(fmn(tvmin0[t], tvmin1[t]), dnn1) - (R)lo.y;
+                qp = fmx(fmx(tvmax0[t], tvmax1[t]), dnx1) - lo.y;
+                qm = fmn(fmn(tvmin0[t], tvmin1[t]), dnn1) - lo.y;
             }
             R flux = P.y * (R)dt / av.y + flux_eps;
-            rp1 = fmn((R)1, qp / flux);
+            rp1 = fmn((R)1, (R)qp / flux);
             flux = M.y * (R)dt / av.y - flux_eps;
-            rm1 = fmn((R)1, qm / flux);
+            rm1 = fmn((R)1, (R)qm / flux);
         }
         st2(Pp_Rp + tn + c, rp0, rp1);
         st2(Pm_Rm + tn + c, rm0, rm1);
@@ -594,11 +607,11 @@ __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__re
     unsigned info[kDeg];
     load_node_edges(m, p.col, lay0, oe, info);
     load_other_nodes(m, p.col, lay0, oo);
-    V2<R> av{1, 1}, h{0, 0}, hn{1, 1};
+    V2<double> av{1, 1}, h{0, 0}, hn{1, 1};  // FP64 geometry: conservative accumulation
     if (lay0) {
-        av = ld2(m.areasvol + c);
-        h = ld2(m.hnode + c);
-        hn = ld2(m.hnode_new + c);
+        av = ld2(m.av_d + c);
+        h = ld2(m.h_d + c);
+        hn = ld2(m.hn_d + c);
     }
 
 #pragma unroll
@@ -628,21 +641,25 @@ __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__re
             // (keeps every pair load one 16-byte access)
             const double fl0 = (double)(edge_alpha(pos, f.x, rp.x, rm.x, rpo.x, rmo.x) * f.x);
             const double fl1 = (double)(edge_alpha(pos, f.y, rp.y, rm.y, rpo.y, rmo.y) * f.y);
-            sh0 = pos ? sh0 + fl0 * dt / (double)av.x : sh0 - fl0 * dt / (double)av.x;
-            const double s1 = pos ? sh1 + fl1 * dt / (double)av.y : sh1 - fl1 * dt / (double)av.y;
+            sh0 = pos ? sh0 + fl0 : sh0 - fl0;
+            const double s1 = pos ? sh1 + fl1 : sh1 - fl1;
             sh1 = on1 ? s1 : sh1;
         }
+        // the edge sum times dt/areasvol once (the reference scales each edge's
+        // flux before summing; the difference is rounding)
+        sh0 = sh0 * dt / (double)av.x;
+        sh1 = sh1 * dt / (double)av.y;
         const V2<double> T = ld2(ttf + tn + c), lo = ld2(LO + tn + c);
         double n0, n1;
         {
             double dv = 0.0 - T.x * (double)h.x + lo.x * (double)hn.x;
-            dv = dv + (double)(fvl0 - fvl1) * dt / (double)av.x;
+            dv = dv + ((double)fvl0 - (double)fvl1) * dt / (double)av.x;
             const double del = (0.0 + sh0) + dv;
             n0 = T.x + del / (double)hn.x;
         }
         {
             double dv = 0.0 - T.y * (double)h.y + lo.y * (double)hn.y;
-            dv = dv + (double)(fvl1 - fvl2) * dt / (double)av.y;
+            dv = dv + ((double)fvl1 - (double)fvl2) * dt / (double)av.y;
             const double del = (0.0 + sh1) + dv;
             n1 = T.y + del / (double)hn.y;
         }
@@ -668,10 +685,10 @@ __global__ __launch_bounds__(kBlock) void k_node_nolim(DMesh<R> m, const R *__re
     int oe[kDeg];
     unsigned info[kDeg];
     load_node_edges(m, p.col, lay0, oe, info);
-    V2<R> av{1, 1}, hn{1, 1};
+    V2<double> av{1, 1}, hn{1, 1};  // FP64 geometry: conservative accumulation
     if (lay0) {
-        av = ld2(m.areasvol + c);
-        hn = ld2(m.hnode_new + c);
+        av = ld2(m.av_d + c);
+        hn = ld2(m.hn_d + c);
     }
     VCol2<R> hc = node_vcol<R, VER>(m, p, nlev, WEv);
 #pragma unroll
@@ -693,19 +710,21 @@ __global__ __launch_bounds__(kBlock) void k_node_nolim(DMesh<R> m, const R *__re
             const bool pos = info_pos(info[j]);
             const V2<R> f = ld2(flux_h + te + (size_t)oe[j] + k0);
             const double f0 = (double)f.x, f1 = (double)f.y;
-            sh0 = pos ? sh0 + f0 * dt / (double)av.x : sh0 - f0 * dt / (double)av.x;
-            const double s1 = pos ? sh1 + f1 * dt / (double)av.y : sh1 - f1 * dt / (double)av.y;
+            sh0 = pos ? sh0 + f0 : sh0 - f0;
+            const double s1 = pos ? sh1 + f1 : sh1 - f1;
             sh1 = on1 ? s1 : sh1;
         }
+        sh0 = sh0 * dt / (double)av.x;
+        sh1 = sh1 * dt / (double)av.y;
         const V2<double> T = ld2(ttf + tn + c);
         double n0, n1;
         {
-            const double dv = 0.0 + (double)(fv0 - fv1) * dt / (double)av.x;
+            const double dv = 0.0 + ((double)fv0 - (double)fv1) * dt / (double)av.x;
             const double del = (0.0 + sh0) + dv;
             n0 = T.x + del / (double)hn.x;
         }
         {
-            const double dv = 0.0 + (double)(fv1 - fv2) * dt / (double)av.y;
+            const double dv = 0.0 + ((double)fv1 - (double)fv2) * dt / (double)av.y;
             const double del = (0.0 + sh1) + dv;
             n1 = T.y + del / (double)hn.y;
         }
@@ -743,7 +762,7 @@ __global__ __launch_bounds__(256) void k_diag_partial(DMesh<R> m, const double *
             const double x = v[i];
             if (x != x) { a.nan += 1.0; continue; }
             a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
-            a.mass += x * (double)m.hnode_new[i] * (double)m.areasvol[i];
+            a.mass += x * m.hn_d[i] * m.av_d[i];
         }
     }
     a.amin = a.vmin;
@@ -769,6 +788,42 @@ __global__ __launch_bounds__(256) void k_diag_final(const DiagAcc *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = sh[0];
+}
+
+// difference of two engines' node tracer over the wet layers (same layout):
+// max |a-b|, sum (a-b)^2, max |b|, sum b^2
+struct ErrAcc { double dinf, d2, rinf, r2; };
+
+__global__ __launch_bounds__(256) void k_err_partial(const int *__restrict__ slot, const int *__restrict__ nlev,
+                                                     long long npair, const double *__restrict__ a,
+                                                     const double *__restrict__ b, ErrAcc *__restrict__ part)
+{
+    __shared__ ErrAcc sh[256];
+    ErrAcc e{0.0, 0.0, 0.0, 0.0};
+    for (long long s = (long long)blockIdx.x * 256 + threadIdx.x; s < npair; s += (long long)gridDim.x * 256) {
+        const int w = slot[s];
+        if (w < 0) continue;
+        const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
+        const int L = nlev[n] - 1;
+        for (int q = 0; q < 2 && k0 + q < L; ++q) {
+            const double d = a[2 * s + q] - b[2 * s + q], r = b[2 * s + q];
+            e.dinf = fmax(e.dinf, fabs(d));
+            e.d2 += d * d;
+            e.rinf = fmax(e.rinf, fabs(r));
+            e.r2 += r * r;
+        }
+    }
+    sh[threadIdx.x] = e;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            ErrAcc &x = sh[threadIdx.x];
+            const ErrAcc &y = sh[threadIdx.x + st];
+            x.dinf = fmax(x.dinf, y.dinf); x.d2 += y.d2; x.rinf = fmax(x.rinf, y.rinf); x.r2 += y.r2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 // ===========================================================================
@@ -866,6 +921,7 @@ struct Engine::Impl {
     void *trxy = nullptr;                                  // [ntr][2*CT]
     void *gnod = nullptr;                                  // [ntr][2*CN]
     DiagAcc *dpart = nullptr, *dout = nullptr;
+    ErrAcc *epart = nullptr;
     int ndiag = 1024;
     float gscale = 1.0f, ginv = 1.0f;
     size_t tsn = 0, tst = 0, tse = 0;
@@ -1019,6 +1075,13 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
     d.hnode_new = I.track(dupload(I.ln.pack<R>(m.hnode_new.data(), nl - 1, 1)));
     d.Zn = I.track(dupload(I.ln.pack<R>(m.Z_3d_n.data(), nl - 1, 1)));
     d.helem = I.track(dupload(I.lt.pack<R>(m.helem.data(), nl - 1, 0)));
+    if constexpr (std::is_same<R, double>::value) {
+        d.av_d = d.areasvol; d.h_d = d.hnode; d.hn_d = d.hnode_new;
+    } else {
+        d.av_d = I.track(dupload(I.ln.pack<double>(m.areasvol.data(), nl, 1)));
+        d.h_d = I.track(dupload(I.ln.pack<double>(m.hnode.data(), nl - 1, 1)));
+        d.hn_d = I.track(dupload(I.ln.pack<double>(m.hnode_new.data(), nl - 1, 1)));
+    }
 }
 
 Engine::Engine(const HostMesh &m, int ntracers, int precision)
@@ -1071,6 +1134,7 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     I.gnod = zalloc(2 * I.tsn * ntr_ * gs);
     I.dpart = I.track(dalloc<DiagAcc>(I.ndiag));
     I.dout = I.track(dalloc<DiagAcc>(1));
+    I.epart = I.track(dalloc<ErrAcc>(I.ndiag));
     I.host_vals.assign(ntr_, std::vector<double>((size_t)(nl_ - 1) * N_, 0.0));
     // FP16 gradient scale: mean edge length (gradient * length = tracer increment)
     double len = 0.0;
@@ -1312,6 +1376,31 @@ Diag Engine::diagnostics(int tr)
             d.sum += x;
         }
     return d;
+}
+
+void Engine::difference(const Engine &ref, int tr, double out[4])
+{
+    Impl &I = *p_;
+    const Impl &J = *ref.p_;
+    if (tr < 0 || tr >= ntr_ || tr >= ref.ntr_) throw std::runtime_error("tracer index out of range");
+    if (I.ln.total != J.ln.total || I.ln.off != J.ln.off) throw std::runtime_error("engines on different meshes");
+    HIP_OK(hipStreamSynchronize(ref.stream_));
+    const int *slot = prec_ == PREC_FP64 ? I.m64.slot_n : I.m32.slot_n;
+    const int *nlev = prec_ == PREC_FP64 ? I.m64.nlev : I.m32.nlev;
+    hipLaunchKernelGGL(k_err_partial, dim3(I.ndiag), dim3(256), 0, stream_, slot, nlev, (long long)(I.ln.total / 2),
+                       I.values + tr * I.tsn, J.values + tr * J.tsn, I.epart);
+    HIP_OK(hipGetLastError());
+    std::vector<ErrAcc> h(I.ndiag);
+    HIP_OK(hipMemcpyAsync(h.data(), I.epart, h.size() * sizeof(ErrAcc), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    ErrAcc e{0.0, 0.0, 0.0, 0.0};
+    for (const ErrAcc &x : h) {
+        e.dinf = std::max(e.dinf, x.dinf); e.d2 += x.d2; e.rinf = std::max(e.rinf, x.rinf); e.r2 += x.r2;
+    }
+    out[0] = e.dinf;
+    out[1] = std::sqrt(e.d2);
+    out[2] = e.rinf > 0.0 ? e.dinf / e.rinf : 0.0;
+    out[3] = e.r2 > 0.0 ? std::sqrt(e.d2 / e.r2) : 0.0;
 }
 
 size_t Engine::work_count(const std::string &name) const

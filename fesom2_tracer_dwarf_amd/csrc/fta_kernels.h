@@ -103,6 +103,10 @@ struct DMesh {
     const R *area, *zbarn;        // interface arrays
     const R *areasvol, *hnode, *hnode_new, *Zn;  // layer arrays
     const R *helem;               // element layout
+    // areasvol, hnode, hnode_new in FP64 whatever R is: the LO solution and the
+    // flux-divergence accumulation into the node tracer use them, so the
+    // tracer mass is conserved to FP64 rounding in every precision form
+    const double *av_d, *h_d, *hn_d;
 };
 
 __device__ __forceinline__ int wave_chunk()

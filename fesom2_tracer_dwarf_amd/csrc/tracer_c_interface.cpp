@@ -647,6 +647,38 @@ int tracer_time_kernels(double dt, int reps, char *names, int names_len, double 
     }
 }
 
+int tracer_precision_error(int ref_precision, int nsteps, double dt, double *out)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (ref_precision != 64 && ref_precision != 32 && ref_precision != 16)
+        return fail("precision must be 64, 32 or 16");
+    if (nsteps < 0) return fail("nsteps must be >= 0");
+    if (ensure_engine()) return -1;
+    try {
+        // a second engine at the reference precision, started from the current state
+        Engine ref(s.mesh, s.ntr, prec_code(ref_precision));
+        ref.set_velocity(s.uv.data(), s.w.data(), s.we.data());
+        ref.set_refresh_ab(s.refresh != 0);
+        std::vector<double> v((size_t)(s.mesh.nl - 1) * s.mesh.nod2D);
+        for (int t = 0; t < s.ntr; ++t) {
+            ref.set_spec(t, s.specs[t]);
+            s.eng->get_values(t, v.data());
+            ref.set_values(t, v.data());
+            ref.set_valuesAB(t, s.valuesAB[t].data());
+        }
+        for (int k = 0; k < nsteps; ++k) {
+            s.eng->run_steps(1, dt, s.use_graph != 0);
+            ref.run_steps(1, dt, s.use_graph != 0);
+            for (int t = 0; t < s.ntr; ++t) s.eng->difference(ref, t, out + 4 * ((size_t)k * s.ntr + t));
+        }
+        s.eng->synchronize();
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
 const char *tracer_last_error(void) { return last_error(); }
 
 }  // extern "C"

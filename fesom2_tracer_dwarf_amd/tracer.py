@@ -77,6 +77,7 @@ class FESOMTracerAdvection:
         return out
 
     def allocate_tracers(self, num_tracers, AB_order=1):
+        self.num_tracers = int(num_tracers)
         d = self.dims()
         _lib.check(self.lib.tracer_allocate_tracers(num_tracers, d["nl"], d["nod2D"], AB_order),
                    "tracer_allocate_tracers", self.lib)
@@ -139,6 +140,16 @@ class FESOMTracerAdvection:
         if n < 0:
             raise RuntimeError(f"tracer_time_kernels failed: {_lib.last_error(self.lib)}")
         return dict(zip(names.value.decode().split(";"), ms[:n].tolist()))
+
+    def precision_error(self, ref_precision, nsteps, dt):
+        """Advance all tracers nsteps in lockstep with a second device state at
+        ref_precision; returns (nsteps, ntracers, 4): max|a-b|, ||a-b||_2 and
+        their relative forms against the reference state b."""
+        ntr = self.num_tracers
+        out = np.zeros(4 * nsteps * ntr)
+        _lib.check(self.lib.tracer_precision_error(int(ref_precision), int(nsteps), float(dt), out),
+                   "tracer_precision_error", self.lib)
+        return out.reshape(nsteps, ntr, 4)
 
     def cleanup(self):
         return self.lib.tracer_cleanup()

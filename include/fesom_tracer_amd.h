@@ -178,6 +178,14 @@ int tracer_get_mesh_array(const char *name, void *out, long long count);
  * (nodal mean gradient, (2,nl-1,nod2D)) */
 int tracer_get_work_array(int tracer_id, const char *name, double *out, long long count);
 
+/* No reference counterpart: error of the current precision against another
+ * one (64, 32 or 16). A second device state at ref_precision starts from the
+ * current tracers; both advance nsteps steps of all tracers in lockstep and
+ * after every step out[4*(step*ntracers + t)..+3] receives, over the wet
+ * layers of tracer t: max|a-b|, ||a-b||_2, max|a-b|/max|b|, ||a-b||_2/||b||_2
+ * (a = this precision, b = the reference). The current state advances too. */
+int tracer_precision_error(int ref_precision, int nsteps, double dt, double *out);
+
 /* No reference counterpart: wall time of nsteps steps of all tracers on the
  * library's stream (hipEvent), milliseconds */
 int tracer_time_steps(int nsteps, double dt, double *ms_out);
