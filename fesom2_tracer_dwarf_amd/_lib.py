@@ -19,6 +19,8 @@ P_int, P_double, P_ll = ctypes.POINTER(c_int), ctypes.POINTER(c_double), ctypes.
 F64 = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 F64F = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
 I32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+F32 = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+c_float, c_uint, c_longlong, POINTER = ctypes.c_float, ctypes.c_uint, ctypes.c_longlong, ctypes.POINTER
 
 # name -> (restype, argtypes); the 23 reference symbols first
 SIGNATURES = {
@@ -62,6 +64,15 @@ SIGNATURES = {
     "tracer_time_steps": (c_int, [c_int, c_double, P_double]),
     "tracer_time_kernels": (c_int, [c_double, c_int, c_char_p, c_int, F64, c_int]),
     "tracer_precision_error": (c_int, [c_int, c_int, c_double, F64]),
+    "tracer_mlp_param_count": (c_int, []),
+    "tracer_mlp_set_params": (c_int, [F32, c_int]),
+    "tracer_mlp_get_params": (c_int, [F32, c_int]),
+    "tracer_learn_collect": (c_int, [c_int, c_double, c_longlong, POINTER(c_longlong)]),
+    "tracer_learn_clear": (c_int, []),
+    "tracer_learn_train": (c_int, [c_int, c_longlong, c_double, c_uint, F64]),
+    "tracer_mlp_forward": (c_int, [F32, c_longlong, F32]),
+    "tracer_mlp_gradient": (c_int, [F32, F32, c_longlong, c_float, F32, POINTER(c_double)]),
+    "tracer_learn_fallback": (c_int, [c_int, POINTER(c_double)]),
     "tracer_last_error": (c_char_p, []),
 }
 REFERENCE_SYMBOLS = list(SIGNATURES)[:23]

@@ -21,7 +21,10 @@ namespace fta {
 
 enum Hor { HOR_UPW1 = 0, HOR_MUSCL = 1, HOR_MFCT = 2 };
 enum Ver { VER_UPW1 = 0, VER_QR4C = 1, VER_PPM = 2, VER_CDIFF = 3 };
-enum Lim { LIM_NONE = 0, LIM_FCT = 1 };
+// LIM_LEARNED: a small network predicts each horizontal edge-level's limiter
+// coefficient; it is applied in flux form and Zalesak's coefficient replaces
+// it on every edge of a node whose bounds the prediction would break
+enum Lim { LIM_NONE = 0, LIM_FCT = 1, LIM_LEARNED = 2 };
 // FP64: everything in double (parity path). FP32: fluxes, gradients and the
 // limiter in float, mesh and velocity stored as float, node tracer, LO
 // solution and divergence accumulation kept in double. FP32_H16: FP32 with
@@ -54,7 +57,7 @@ public:
     void set_values(int tr, const double *v);    // (nl-1,nod2D)
     void set_valuesAB(int tr, const double *v);  // (nl-1,nod2D)
     void get_values(int tr, double *v);
-    void set_spec(int tr, const TracerSpec &s) { spec_.at(tr) = s; graph_valid_ = false; }
+    void set_spec(int tr, const TracerSpec &s);
     const TracerSpec &spec(int tr) const { return spec_.at(tr); }
 
     // refresh_ab: valuesAB := values at the start of each step (AB1)
@@ -77,6 +80,25 @@ public:
     // any precision), over the wet layers: out = {max|a-b|, ||a-b||_2,
     // max|a-b|/max|b|, ||a-b||_2/||b||_2}
     void difference(const Engine &ref, int tr, double out[4]);
+
+    // ---- learned limiter (network 10-64-64-1, parameters flattened as
+    // W1[f][h], b1, W2[h1][h2], b2, w3, b3: mlp_param_count() floats)
+    static int mlp_param_count();
+    void mlp_set(const float *params);
+    void mlp_get(float *params);
+    // nsteps FCT steps of all FCT/LEARNED tracers, appending (features,
+    // Zalesak coefficient) rows of every edge level to the training set
+    // (capacity max_rows); returns the rows held
+    long long learn_collect(int nsteps, double dt, long long max_rows);
+    void learn_clear();
+    // Adam on the training set, minibatches of batch rows; mean loss per epoch
+    std::vector<double> learn_train(int epochs, long long batch, double lr, unsigned seed);
+    // the network on given rows of features (rows x 10 floats)
+    void mlp_forward(const float *X, long long rows, float *out);
+    // gradient of sum((sigmoid(z) - y)^2) * scale over the rows (no update)
+    void mlp_gradient(const float *X, const float *y, long long rows, float scale, float *grad, double *loss);
+    // fraction of wet layers of tracer tr whose Zalesak fallback was taken in the last step
+    double learn_fallback(int tr);
 
     // per-kernel timing of one step with hip events on the engine stream
     // (names and milliseconds, averaged over reps)

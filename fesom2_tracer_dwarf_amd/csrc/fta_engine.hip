@@ -13,6 +13,7 @@
 // one wave per 128-cell chunk of its layout, two cells per lane.
 #include "fta_engine.h"
 #include "fta_kernels.h"
+#include "fta_mlp.h"
 #include "fta_vertical.h"
 
 #include <algorithm>
@@ -438,11 +439,13 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
 // Zalesak bounds a1-a3 and limiting factors b2 (lib/oce_adv_tra_fct.F90:94-217,361-374);
 // R+/R- overwrite P+/P- in place. a1 (max/min of LO and T) is formed once per
 // 1-ring node, a2 per fan element from the node and two consecutive ring nodes.
-template <class R, int NT>
+template <class R, int NT, bool STOREQ>
 __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double *__restrict__ ttf,
                                                        const double *__restrict__ LO, size_t tsn,
                                                        double dt, R *__restrict__ Pp_Rp,
-                                                       R *__restrict__ Pm_Rm)
+                                                       R *__restrict__ Pm_Rm, R *__restrict__ Qp_out,
+                                                       R *__restrict__ Qm_out, R *__restrict__ Pp_out,
+                                                       R *__restrict__ Pm_out)
 {
     const int ch = wave_chunk();
     if (ch >= m.nch_n) return;
@@ -517,6 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
         const V2<double> lo = ld2(LO + tn + c);
         const V2<R> P = ld2(Pp_Rp + tn + c), M = ld2(Pm_Rm + tn + c);
         R rp0, rm0, rp1 = 0, rm1 = 0;
+        double qp0 = 0.0, qm0 = 0.0, qp1 = 0.0, qm1 = 0.0;
         {
             const int nz = k0 + 1;
             double qp, qm;
@@ -531,6 +535,8 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
             rp0 = fmn((R)1, (R)qp / flux);
             flux = M.x * (R)dt / av.x - flux_eps;
             rm0 = fmn((R)1, (R)qm / flux);
+            qp0 = qp;
+            qm0 = qm;
         }
         if (lay1) {
             const int nz = k0 + 2;
@@ -546,9 +552,17 @@ __global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double 
             rp1 = fmn((R)1, (R)qp / flux);
             flux = M.y * (R)dt / av.y - flux_eps;
             rm1 = fmn((R)1, (R)qm / flux);
+            qp1 = qp;
+            qm1 = qm;
         }
         st2(Pp_Rp + tn + c, rp0, rp1);
         st2(Pm_Rm + tn + c, rm0, rm1);
+        if constexpr (STOREQ) {  // Q+/Q- and P+/P- for the learned limiter's features and bound check
+            st2(Qp_out + tn + c, (R)qp0, (R)qp1);
+            st2(Qm_out + tn + c, (R)qm0, (R)qm1);
+            st2(Pp_out + tn + c, P.x, lay1 ? P.y : (R)0);
+            st2(Pm_out + tn + c, M.x, lay1 ? M.y : (R)0);
+        }
     }
 }
 
@@ -587,13 +601,15 @@ __device__ __forceinline__ R edge_alpha(bool pos, R f, R rp, R rm, R rpo, R rmo)
 // limiting b3 (lib/oce_adv_tra_fct.F90:394-471), flux divergence
 // (lib/oce_adv_tra_driver.F90:298-450) and the update
 // values += (dttf_h + dttf_v)/hnode_new (src/fesom_analytic.F90:419-437)
-template <class R, int NT>
+template <class R, int NT, bool LEARN>
 __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__restrict__ ttf,
                                                         const double *__restrict__ LO, size_t tsn,
                                                         const R *__restrict__ adf_h, size_t tse,
                                                         const R *__restrict__ adf_v,
                                                         const R *__restrict__ Rp,
-                                                        const R *__restrict__ Rm, double dt)
+                                                        const R *__restrict__ Rm, double dt,
+                                                        const R *__restrict__ alpha,
+                                                        const R *__restrict__ unsafe)
 {
     const int ch = wave_chunk();
     if (ch >= m.nch_n) return;
@@ -627,6 +643,8 @@ __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__re
         const R fvl1 = limit_vertical(k0 + 2, lay1, fv.y, rp.y, rm.y, rp.x, rm.x);
         const R fvl2 = from_next(fvl0);
         if (!lay0) continue;
+        V2<R> us{0, 0};
+        if constexpr (LEARN) us = ld2(unsafe + tn + c);
 
         double sh0 = 0.0, sh1 = 0.0;
 #pragma unroll
@@ -639,8 +657,17 @@ __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__re
             const V2<R> rpo = ld2(Rp + tn + (size_t)oo[j] + k0), rmo = ld2(Rm + tn + (size_t)oo[j] + k0);
             // both cells evaluated unconditionally, the second one masked
             // (keeps every pair load one 16-byte access)
-            const double fl0 = (double)(edge_alpha(pos, f.x, rp.x, rm.x, rpo.x, rmo.x) * f.x);
-            const double fl1 = (double)(edge_alpha(pos, f.y, rp.y, rm.y, rpo.y, rmo.y) * f.y);
+            R a0 = edge_alpha(pos, f.x, rp.x, rm.x, rpo.x, rmo.x);
+            R a1 = edge_alpha(pos, f.y, rp.y, rm.y, rpo.y, rmo.y);
+            if constexpr (LEARN) {
+                // the network's coefficient, Zalesak's where either node failed its bound check
+                const V2<R> al = ld2(alpha + te + (size_t)oe[j] + k0);
+                const V2<R> uo = ld2(unsafe + tn + (size_t)oo[j] + k0);
+                a0 = (us.x != (R)0 || uo.x != (R)0) ? fmn(al.x, a0) : al.x;
+                a1 = (us.y != (R)0 || uo.y != (R)0) ? fmn(al.y, a1) : al.y;
+            }
+            const double fl0 = (double)(a0 * f.x);
+            const double fl1 = (double)(a1 * f.y);
             sh0 = pos ? sh0 + fl0 : sh0 - fl0;
             const double s1 = pos ? sh1 + fl1 : sh1 - fl1;
             sh1 = on1 ? s1 : sh1;
@@ -730,6 +757,265 @@ __global__ __launch_bounds__(kBlock) void k_node_nolim(DMesh<R> m, const R *__re
         }
         st2(ttf + tn + c, n0, lay1 ? n1 : T.y);
     }
+}
+
+// ===========================================================================
+// Learned limiter. Features of a horizontal edge level (edges(1) = node 1):
+//   x0, x1  LO and HO edge flux as a tracer change of node 1 per step,
+//           flux*dt/(areasvol*hnode_new), over the local range s
+//   x2, x3  tracer at node 1 / node 2 above the local minimum, over s
+//   x4..x7  Zalesak headroom Q+ and -Q- of node 1 and node 2, over s
+//   x8, x9  the antidiffusive sums P (Zalesak b1) on the receiving side of
+//           node 1 and the giving side of node 2 for the flux's sign,
+//           P*dt/areasvol over s (the reference's R = Q / (P*dt/areasvol))
+// with the local range s = max(bounds) - min(bounds) of the two nodes; the
+// target is Zalesak's coefficient of the edge level (b3). Values are clipped
+// to [-8, 8] before the network sees them.
+// ===========================================================================
+template <class R>
+struct LearnSrc {
+    const double *T, *LO;  // node tracer and LO solution (this tracer)
+    const R *adf, *vfl;    // edge HO-LO flux (this tracer) and edge transport
+    const R *Qp, *Qm;      // node headroom (this tracer)
+    const R *Rp, *Rm;      // node R+/R- (this tracer)
+    const R *Pp, *Pm;      // node P+/P- (this tracer)
+};
+
+__device__ __forceinline__ float clip8(double v) { return (float)fmin(8.0, fmax(-8.0, v)); }
+
+// features (and Zalesak target) of the lane's two edge levels
+template <class R>
+__device__ __forceinline__ void edge_features(const DMesh<R> &m, const EdgeRec &er, int k0, size_t c, double dt,
+                                              const LearnSrc<R> &S, float (&x)[2][kMlpIn], float (&y)[2])
+{
+    const V2<double> T1 = ld2(S.T + er.o1 + k0), T2 = ld2(S.T + er.o2 + k0);
+    const V2<double> L1 = ld2(S.LO + er.o1 + k0), L2 = ld2(S.LO + er.o2 + k0);
+    const V2<R> qp1 = ld2(S.Qp + er.o1 + k0), qm1 = ld2(S.Qm + er.o1 + k0);
+    const V2<R> qp2 = ld2(S.Qp + er.o2 + k0), qm2 = ld2(S.Qm + er.o2 + k0);
+    const V2<R> rp1 = ld2(S.Rp + er.o1 + k0), rm1 = ld2(S.Rm + er.o1 + k0);
+    const V2<R> rp2 = ld2(S.Rp + er.o2 + k0), rm2 = ld2(S.Rm + er.o2 + k0);
+    const V2<R> a = ld2(S.adf + c), v = ld2(S.vfl + c);
+    const V2<double> av1 = ld2(m.av_d + er.o1 + k0), hn1 = ld2(m.hn_d + er.o1 + k0);
+    const V2<double> av2 = ld2(m.av_d + er.o2 + k0);
+    const V2<R> pp1 = ld2(S.Pp + er.o1 + k0), pm1 = ld2(S.Pm + er.o1 + k0);
+    const V2<R> pp2 = ld2(S.Pp + er.o2 + k0), pm2 = ld2(S.Pm + er.o2 + k0);
+    const double Pp1[2] = {(double)pp1.x, (double)pp1.y}, Pm1[2] = {(double)pm1.x, (double)pm1.y};
+    const double Pp2[2] = {(double)pp2.x, (double)pp2.y}, Pm2[2] = {(double)pm2.x, (double)pm2.y};
+    const double AV1[2] = {av1.x, av1.y}, AV2[2] = {av2.x, av2.y};
+    const double t1[2] = {T1.x, T1.y}, t2[2] = {T2.x, T2.y}, l1[2] = {L1.x, L1.y}, l2[2] = {L2.x, L2.y};
+    const double Qp1[2] = {(double)qp1.x, (double)qp1.y}, Qm1[2] = {(double)qm1.x, (double)qm1.y};
+    const double Qp2[2] = {(double)qp2.x, (double)qp2.y}, Qm2[2] = {(double)qm2.x, (double)qm2.y};
+    const R Rp1[2] = {rp1.x, rp1.y}, Rm1[2] = {rm1.x, rm1.y}, Rp2[2] = {rp2.x, rp2.y}, Rm2[2] = {rm2.x, rm2.y};
+    const double A[2] = {(double)a.x, (double)a.y}, Vf[2] = {(double)v.x, (double)v.y};
+    const double V1[2] = {av1.x * hn1.x, av1.y * hn1.y};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const double mx = fmax(l1[q] + Qp1[q], l2[q] + Qp2[q]);
+        const double mn = fmin(l1[q] + Qm1[q], l2[q] + Qm2[q]);
+        const double sc = mx - mn > 1e-300 ? mx - mn : 1.0;
+        const double lo = lo_flux(Vf[q], t1[q], t2[q]);
+        const double toT = dt / (V1[q] * sc);
+        x[q][0] = clip8(lo * toT);
+        x[q][1] = clip8((lo + A[q]) * toT);
+        x[q][2] = clip8((t1[q] - mn) / sc);
+        x[q][3] = clip8((t2[q] - mn) / sc);
+        x[q][4] = clip8(Qp1[q] / sc);
+        x[q][5] = clip8(-Qm1[q] / sc);
+        x[q][6] = clip8(Qp2[q] / sc);
+        x[q][7] = clip8(-Qm2[q] / sc);
+        const bool in1 = A[q] >= 0.0;  // flux into node 1
+        x[q][8] = clip8((in1 ? Pp1[q] : -Pm1[q]) * dt / (AV1[q] * sc));
+        x[q][9] = clip8((in1 ? -Pm2[q] : Pp2[q]) * dt / (AV2[q] * sc));
+        y[q] = (float)edge_alpha(true, (R)A[q], Rp1[q], Rm1[q], Rp2[q], Rm2[q]);
+    }
+}
+
+template <class R, int NT>
+struct LearnArgs {
+    const double *ttf, *LO;
+    const R *adf, *vfl, *Qp, *Qm, *Rp, *Rm, *Pp, *Pm;
+    size_t tsn, tse;
+};
+
+template <class R, int NT>
+__device__ __forceinline__ LearnSrc<R> learn_src(const LearnArgs<R, NT> &a, int t)
+{
+    return LearnSrc<R>{a.ttf + t * a.tsn, a.LO + t * a.tsn, a.adf + t * a.tse, a.vfl,
+                       a.Qp + t * a.tsn, a.Qm + t * a.tsn, a.Rp + t * a.tsn, a.Rm + t * a.tsn,
+                       a.Pp + t * a.tsn, a.Pm + t * a.tsn};
+}
+
+// training rows: features (bf16, 16 per row) and Zalesak target of every
+// edge-level cell of every tracer; padding cells get target -1
+template <class R, int NT>
+__global__ __launch_bounds__(kBlock) void k_learn_dump(DMesh<R> m, LearnArgs<R, NT> a, double dt,
+                                                       unsigned short *__restrict__ X, float *__restrict__ Y,
+                                                       long long row0)
+{
+    const int ch = wave_chunk();
+    if (ch >= m.nch_e) return;
+    const Pair p = lane_pair(m.slot_e, ch);
+    EdgeRec er{};
+    int nlmax = 0;
+    if (p.live) {
+        er = m.erec[p.col];
+        nlmax = (int)((unsigned)er.nl >> 24);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        float x[2][kMlpIn], y[2] = {-1.0f, -1.0f};
+        if (p.live) edge_features(m, er, p.k0, (size_t)p.c, dt, learn_src(a, t), x, y);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const bool ok = p.live && p.k0 + q < nlmax;
+            const long long row = row0 + (long long)t * m.CE + p.c + q;
+            unsigned short b[kMlpInPad];
+#pragma unroll
+            for (int f = 0; f < kMlpInPad; ++f) b[f] = (ok && f < kMlpIn) ? f2bf(x[q][f]) : (unsigned short)0;
+            uint4 *dst = reinterpret_cast<uint4 *>(X + row * kMlpInPad);
+            unsigned u[8];
+#pragma unroll
+            for (int h = 0; h < 8; ++h) u[h] = (unsigned)b[2 * h] | ((unsigned)b[2 * h + 1] << 16);
+            dst[0] = make_uint4(u[0], u[1], u[2], u[3]);
+            dst[1] = make_uint4(u[4], u[5], u[6], u[7]);
+            Y[row] = ok ? y[q] : -1.0f;
+        }
+    }
+}
+
+// the network's coefficient of every horizontal edge level: each wave packs
+// its chunk's 128 feature rows into LDS and runs them through the matrix
+// cores as eight 16-row tiles. Persistent waves (weights loaded once);
+// block-uniform trip counts.
+template <class R, int NT>
+__global__ __launch_bounds__(kBlock) void k_learn_alpha(DMesh<R> m, MlpDev w, LearnArgs<R, NT> a, double dt,
+                                                        R *__restrict__ alpha, int iters)
+{
+    __shared__ unsigned short sX[kWavesPerBlock][kChunk][kMlpInPad];
+    __shared__ float sY[kWavesPerBlock][kChunk];
+    const int wv = (int)(threadIdx.x >> 6), l = lane_id(), i = l & 15, g = l >> 4;
+    MlpRegs r;
+    mlp_load(w, r);
+    for (int it = 0; it < iters; ++it) {
+        const int ch = (it * (int)gridDim.x + (int)blockIdx.x) * kWavesPerBlock + wv;
+        Pair p{0, 0, 0, false};
+        if (ch < m.nch_e) p = lane_pair(m.slot_e, ch);
+        EdgeRec er{};
+        int nlmax = 0;
+        if (p.live) {
+            er = m.erec[p.col];
+            nlmax = (int)((unsigned)er.nl >> 24);
+        }
+#pragma unroll 1
+        for (int t = 0; t < NT; ++t) {
+            float x[2][kMlpIn], y[2];
+            if (p.live) edge_features(m, er, p.k0, (size_t)p.c, dt, learn_src(a, t), x, y);
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int f = 0; f < kMlpInPad; ++f)
+                    sX[wv][2 * l + q][f] = (p.live && f < kMlpIn) ? f2bf(x[q][f]) : (unsigned short)0;
+            __syncthreads();
+#pragma unroll 1
+            for (int tile = 0; tile < kChunk / 16; ++tile) {
+                const mlp_v4s xb = ld_bf4(&sX[wv][16 * tile + i][4 * g]);
+                MlpTile T;
+                mlp_forward(r, xb, T);
+                if (g == 0) sY[wv][16 * tile + i] = mlp_sigmoid(T.z3);
+            }
+            __syncthreads();
+            if (p.live)
+                st2(alpha + t * a.tse + p.c, (R)sY[wv][2 * l], p.k0 + 1 < nlmax ? (R)sY[wv][2 * l + 1] : (R)0);
+            __syncthreads();
+        }
+    }
+}
+
+// per node level: would the network's horizontal coefficients (vertical
+// ones as Zalesak's) keep the node inside its bounds? Checked as Zalesak's
+// own sufficient condition, positive and negative contributions apart:
+// sum(max(0, c)) * dt / areasvol <= Q+ and sum(min(0, c)) * dt / areasvol >= Q-
+template <class R, int NT>
+__global__ __launch_bounds__(kBlock) void k_learn_check(DMesh<R> m, const R *__restrict__ adf_h,
+                                                        const R *__restrict__ alpha, size_t tse,
+                                                        const R *__restrict__ adf_v, const R *__restrict__ Rp,
+                                                        const R *__restrict__ Rm, const R *__restrict__ Qp,
+                                                        const R *__restrict__ Qm, size_t tsn, double dt,
+                                                        R *__restrict__ unsafe)
+{
+    const int ch = wave_chunk();
+    if (ch >= m.nch_n) return;
+    const Pair p = lane_pair(m.slot_n, ch);
+    const int nlev = p.live ? m.nlev[p.col] : 0;
+    const int L = nlev - 1, k0 = p.k0;
+    const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
+    const bool if0 = p.live && k0 < nlev, if1 = p.live && k0 + 1 < nlev;
+    const size_t c = (size_t)p.c;
+    int oe[kDeg];
+    unsigned info[kDeg];
+    load_node_edges(m, p.col, lay0, oe, info);
+    V2<R> av{1, 1};
+    if (lay0) av = ld2(m.areasvol + c);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const size_t tn = t * tsn, te = t * tse;
+        V2<R> fv{0, 0}, rp{0, 0}, rm{0, 0};
+        if (if0) fv = ld2(adf_v + tn + c);
+        if (!if1) fv.y = 0;
+        if (lay0) { rp = ld2(Rp + tn + c); rm = ld2(Rm + tn + c); }
+        if (!lay1) { rp.y = 0; rm.y = 0; }
+        const R rpu0 = from_prev(rp.y), rmu0 = from_prev(rm.y);
+        const R fvl0 = limit_vertical(k0 + 1, lay0, fv.x, rp.x, rm.x, rpu0, rmu0);
+        const R fvl1 = limit_vertical(k0 + 2, lay1, fv.y, rp.y, rm.y, rp.x, rm.x);
+        const R fvl2 = from_next(fvl0);
+        if (!lay0) continue;
+        double sp0 = fmax(0.0, (double)fvl0) + fmax(0.0, -(double)fvl1);
+        double sn0 = fmin(0.0, (double)fvl0) + fmin(0.0, -(double)fvl1);
+        double sp1 = fmax(0.0, (double)fvl1) + fmax(0.0, -(double)fvl2);
+        double sn1 = fmin(0.0, (double)fvl1) + fmin(0.0, -(double)fvl2);
+#pragma unroll
+        for (int j = 0; j < kDeg; ++j) {
+            const int nlm = info_nlmax(info[j]);
+            if (oe[j] < 0 || k0 >= nlm) continue;
+            const bool on1 = lay1 && k0 + 1 < nlm;
+            const bool pos = info_pos(info[j]);
+            const V2<R> f = ld2(adf_h + te + (size_t)oe[j] + k0), al = ld2(alpha + te + (size_t)oe[j] + k0);
+            const double c0 = (double)(al.x * f.x) * (pos ? 1.0 : -1.0);
+            const double c1 = (double)(al.y * f.y) * (pos ? 1.0 : -1.0);
+            sp0 += fmax(0.0, c0);
+            sn0 += fmin(0.0, c0);
+            sp1 = on1 ? sp1 + fmax(0.0, c1) : sp1;
+            sn1 = on1 ? sn1 + fmin(0.0, c1) : sn1;
+        }
+        const V2<R> qp = ld2(Qp + tn + c), qm = ld2(Qm + tn + c);
+        const double s0 = dt / (double)av.x, s1 = dt / (double)av.y;
+        const bool bad0 = sp0 * s0 > (double)qp.x + 1e-7 * fabs((double)qp.x) ||
+                          sn0 * s0 < (double)qm.x - 1e-7 * fabs((double)qm.x);
+        const bool bad1 = lay1 && (sp1 * s1 > (double)qp.y + 1e-7 * fabs((double)qp.y) ||
+                                   sn1 * s1 < (double)qm.y - 1e-7 * fabs((double)qm.y));
+        st2(unsafe + tn + c, bad0 ? (R)1 : (R)0, bad1 ? (R)1 : (R)0);
+    }
+}
+
+// wet layers of tracer 0 flagged unsafe (count) and wet layers (count)
+template <class R>
+__global__ __launch_bounds__(256) void k_count_flags(DMesh<R> m, const R *__restrict__ unsafe,
+                                                     unsigned long long *__restrict__ out)
+{
+    unsigned long long nf = 0, nc = 0;
+    const long long npair = m.CN / 2;
+    for (long long s = (long long)blockIdx.x * 256 + threadIdx.x; s < npair; s += (long long)gridDim.x * 256) {
+        const int w = m.slot_n[s];
+        if (w < 0) continue;
+        const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
+        const int L = m.nlev[n] - 1;
+        for (int q = 0; q < 2 && k0 + q < L; ++q) {
+            nc += 1;
+            nf += unsafe[2 * s + q] != (R)0 ? 1 : 0;
+        }
+    }
+    atomicAdd(out, nf);
+    atomicAdd(out + 1, nc);
 }
 
 // ===========================================================================
@@ -922,6 +1208,20 @@ struct Engine::Impl {
     void *gnod = nullptr;                                  // [ntr][2*CN]
     DiagAcc *dpart = nullptr, *dout = nullptr;
     ErrAcc *epart = nullptr;
+    // learned limiter: per-tracer Q+/Q-, coefficients and fallback flags
+    // (allocated when a tracer first selects it), network parameters (FP32
+    // master + Adam moments + bf16 operand copies), training rows
+    void *Qp = nullptr, *Qm = nullptr, *Sp = nullptr, *Sm = nullptr, *alpha = nullptr, *unsafe = nullptr;
+    float *mlp_p = nullptr, *mlp_m1 = nullptr, *mlp_m2 = nullptr, *mlp_g = nullptr, *mlp_loss = nullptr;
+    unsigned short *w1t = nullptr, *w2t = nullptr, *w2 = nullptr, *w3 = nullptr;
+    float *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
+    unsigned short *dsX = nullptr;
+    float *dsY = nullptr;
+    long long ds_cap = 0, ds_rows = 0;
+    int adam_t = 0;
+    unsigned long long *cnt = nullptr;
+    // set while collecting training rows: launch_batch dumps after the limiter
+    struct Dump { unsigned short *X; float *Y; long long row; long long cap; } *dump = nullptr;
     int ndiag = 1024;
     float gscale = 1.0f, ginv = 1.0f;
     size_t tsn = 0, tst = 0, tse = 0;
@@ -1135,6 +1435,38 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     I.dpart = I.track(dalloc<DiagAcc>(I.ndiag));
     I.dout = I.track(dalloc<DiagAcc>(1));
     I.epart = I.track(dalloc<ErrAcc>(I.ndiag));
+    // network parameters: FP32 master, Adam moments, gradient, bf16 operand copies
+    I.mlp_p = I.track(dalloc<float>(kMlpParams));
+    I.mlp_m1 = (float *)zalloc(kMlpParams * sizeof(float));
+    I.mlp_m2 = (float *)zalloc(kMlpParams * sizeof(float));
+    I.mlp_g = (float *)zalloc(kMlpParams * sizeof(float));
+    I.mlp_loss = (float *)zalloc(sizeof(float));
+    I.w1t = (unsigned short *)zalloc(kMlpHid * kMlpInPad * 2);
+    I.w2t = (unsigned short *)zalloc(kMlpHid * kMlpHid * 2);
+    I.w2 = (unsigned short *)zalloc(kMlpHid * kMlpHid * 2);
+    I.w3 = (unsigned short *)zalloc(kMlpHid * 2);
+    I.b1 = (float *)zalloc(kMlpHid * sizeof(float));
+    I.b2 = (float *)zalloc(kMlpHid * sizeof(float));
+    I.b3 = (float *)zalloc(sizeof(float));
+    I.cnt = (unsigned long long *)zalloc(2 * sizeof(unsigned long long));
+    {
+        // untrained network: uniform Glorot weights from a fixed seed, zero
+        // biases, output bias 2 (coefficient ~0.88)
+        std::vector<float> p(kMlpParams, 0.0f);
+        uint64_t st = 0x9E3779B97F4A7C15ull;
+        auto uni = [&]() {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+        };
+        auto fill = [&](int off, int n, double lim) {
+            for (int k = 0; k < n; ++k) p[off + k] = (float)((2.0 * uni() - 1.0) * lim);
+        };
+        fill(kMlpOffW1, kMlpIn * kMlpHid, std::sqrt(6.0 / (kMlpIn + kMlpHid)));
+        fill(kMlpOffW2, kMlpHid * kMlpHid, std::sqrt(6.0 / (2 * kMlpHid)));
+        fill(kMlpOffW3, kMlpHid, std::sqrt(6.0 / (kMlpHid + 1)));
+        p[kMlpOffB3] = 2.0f;
+        mlp_set(p.data());
+    }
     I.host_vals.assign(ntr_, std::vector<double>((size_t)(nl_ - 1) * N_, 0.0));
     // FP16 gradient scale: mean edge length (gradient * length = tracer increment)
     double len = 0.0;
@@ -1148,8 +1480,32 @@ Engine::~Engine()
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void *p : p_->allocs) (void)hipFree(p);
+    if (p_->dsX) { (void)hipFree(p_->dsX); (void)hipFree(p_->dsY); }
     if (stream_) (void)hipStreamDestroy(stream_);
     delete p_;
+}
+
+void Engine::set_spec(int tr, const TracerSpec &sp)
+{
+    spec_.at(tr) = sp;
+    graph_valid_ = false;
+    Impl &I = *p_;
+    if (sp.lim == LIM_LEARNED && !I.alpha) {
+        const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
+        auto zalloc = [&](size_t bytes) {
+            void *p = I.track((void *)dalloc<char>(bytes));
+            HIP_OK(hipMemset(p, 0, bytes));
+            return p;
+        };
+        I.alpha = zalloc(I.tse * ntr_ * rs);
+        I.unsafe = zalloc(I.tsn * ntr_ * rs);
+        if (!I.Qp) {
+            I.Qp = zalloc(I.tsn * ntr_ * rs);
+            I.Qm = zalloc(I.tsn * ntr_ * rs);
+            I.Sp = zalloc(I.tsn * ntr_ * rs);
+            I.Sm = zalloc(I.tsn * ntr_ * rs);
+        }
+    }
 }
 
 void Engine::set_velocity(const double *uv, const double *w, const double *we)
@@ -1211,6 +1567,11 @@ struct LaunchHook {
     }
 };
 
+static MlpDev mlp_dev(const Engine::Impl &I)
+{
+    return MlpDev{I.w1t, I.w2t, I.w2, I.w3, I.b1, I.b2, I.b3};
+}
+
 // one step of the NT tracers [t0, t0+NT), all with schemes sp
 template <class R, class G, int NT>
 static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &sp, double dt, int t0, bool refresh,
@@ -1226,7 +1587,9 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
     G *trxy = (G *)I.trxy + t0 * tst;
     G *gnod = (G *)I.gnod + t0 * tsg;
     const dim3 blk(kBlock), ge(grid_for(m.nch_e)), gn(grid_for(m.nch_n)), gt(grid_for(m.nch_t));
-    const bool fct = sp.lim == LIM_FCT;
+    const bool learned = sp.lim == LIM_LEARNED;
+    const bool fct = sp.lim == LIM_FCT || learned;
+    Engine::Impl::Dump *dump = I.dump;
     const R ph = (R)sp.ph, pv = (R)sp.pv;
     if (sp.hor == HOR_MUSCL || sp.hor == HOR_MFCT) {
         hipLaunchKernelGGL((k_trxy<R, G, NT>), gt, blk, 0, st, m, ab, tsn, trxy, tst, I.gscale);
@@ -1272,9 +1635,41 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
         }
 #undef NLO
         hk.mark("k_node_lo", st);
-        hipLaunchKernelGGL((k_node_limit<R, NT>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm);
+        R *Qp = learned || dump ? (R *)I.Qp + t0 * tsn : nullptr;
+        R *Qm = learned || dump ? (R *)I.Qm + t0 * tsn : nullptr;
+        R *Sp = learned || dump ? (R *)I.Sp + t0 * tsn : nullptr;
+        R *Sm = learned || dump ? (R *)I.Sm + t0 * tsn : nullptr;
+        if (learned || dump)
+            hipLaunchKernelGGL((k_node_limit<R, NT, true>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm, Qp, Qm,
+                               Sp, Sm);
+        else
+            hipLaunchKernelGGL((k_node_limit<R, NT, false>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm, Qp, Qm,
+                               Sp, Sm);
         hk.mark("k_node_limit", st);
-        hipLaunchKernelGGL((k_node_update<R, NT>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v, Pp, Pm, dt);
+        const LearnArgs<R, NT> la{vals, LO, adf_h, vfl, Qp, Qm, Pp, Pm, Sp, Sm, tsn, tse};
+        R *alpha = learned ? (R *)I.alpha + t0 * tse : nullptr, *unsafe = learned ? (R *)I.unsafe + t0 * tsn : nullptr;
+        if (dump) {
+            if (dump->row + (long long)NT * m.CE <= dump->cap) {
+                hipLaunchKernelGGL((k_learn_dump<R, NT>), ge, blk, 0, st, m, la, dt, dump->X, dump->Y, dump->row);
+                dump->row += (long long)NT * m.CE;
+            }
+            hipLaunchKernelGGL((k_node_update<R, NT, false>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
+                               Pp, Pm, dt, alpha, unsafe);
+        } else if (learned) {
+            const MlpDev w = mlp_dev(I);
+            const int gb = std::min((m.nch_e + kWavesPerBlock - 1) / kWavesPerBlock, 2048);
+            const int iters = (m.nch_e + gb * kWavesPerBlock - 1) / (gb * kWavesPerBlock);
+            hipLaunchKernelGGL((k_learn_alpha<R, NT>), dim3(gb), blk, 0, st, m, w, la, dt, alpha, iters);
+            hk.mark("k_learn_alpha", st);
+            hipLaunchKernelGGL((k_learn_check<R, NT>), gn, blk, 0, st, m, adf_h, alpha, tse, adf_v, Pp, Pm, Qp, Qm,
+                               tsn, dt, unsafe);
+            hk.mark("k_learn_check", st);
+            hipLaunchKernelGGL((k_node_update<R, NT, true>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
+                               Pp, Pm, dt, alpha, unsafe);
+        } else {
+            hipLaunchKernelGGL((k_node_update<R, NT, false>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
+                               Pp, Pm, dt, alpha, unsafe);
+        }
         hk.mark("k_node_update", st);
     } else {
 #define NNL(V) hipLaunchKernelGGL((k_node_nolim<R, V, NT>), gn, blk, 0, st, m, WE, vals, ab, tsn, adf_h, tse, dt, pv)
@@ -1345,6 +1740,182 @@ void Engine::run_steps(int nsteps, double dt, bool use_graph)
         graph_dt_ = dt;
     }
     for (int s = 0; s < nsteps; ++s) HIP_OK(hipGraphLaunch(graph_exec_, stream_));
+}
+
+int Engine::mlp_param_count() { return kMlpParams; }
+
+static void mlp_pack(Engine::Impl &I, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_pack, dim3((kMlpHid * kMlpHid + 255) / 256), dim3(256), 0, st, I.mlp_p,
+                       MlpMut{I.w1t, I.w2t, I.w2, I.w3, I.b1, I.b2, I.b3});
+    HIP_OK(hipGetLastError());
+}
+
+void Engine::mlp_set(const float *params)
+{
+    Impl &I = *p_;
+    HIP_OK(hipMemcpy(I.mlp_p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(I.mlp_m1, 0, kMlpParams * sizeof(float)));
+    HIP_OK(hipMemset(I.mlp_m2, 0, kMlpParams * sizeof(float)));
+    I.adam_t = 0;
+    mlp_pack(I, stream_);
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void Engine::mlp_get(float *params)
+{
+    Impl &I = *p_;
+    HIP_OK(hipStreamSynchronize(stream_));
+    HIP_OK(hipMemcpy(params, I.mlp_p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+void Engine::learn_clear() { p_->ds_rows = 0; }
+
+long long Engine::learn_collect(int nsteps, double dt, long long max_rows)
+{
+    Impl &I = *p_;
+    if (max_rows > I.ds_cap) {
+        unsigned short *X = nullptr;
+        float *Y = nullptr;
+        HIP_OK(hipMalloc(&X, (size_t)max_rows * kMlpInPad * sizeof(unsigned short)));
+        HIP_OK(hipMalloc(&Y, (size_t)max_rows * sizeof(float)));
+        if (I.ds_rows) {
+            HIP_OK(hipMemcpy(X, I.dsX, (size_t)I.ds_rows * kMlpInPad * 2, hipMemcpyDeviceToDevice));
+            HIP_OK(hipMemcpy(Y, I.dsY, (size_t)I.ds_rows * sizeof(float), hipMemcpyDeviceToDevice));
+        }
+        if (I.dsX) { HIP_OK(hipFree(I.dsX)); HIP_OK(hipFree(I.dsY)); }
+        I.dsX = X;
+        I.dsY = Y;
+        I.ds_cap = max_rows;
+    }
+    if (!I.Qp) {
+        const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
+        for (void **q : {&I.Qp, &I.Qm, &I.Sp, &I.Sm}) {
+            HIP_OK(hipMalloc(q, I.tsn * ntr_ * rs));
+            HIP_OK(hipMemset(*q, 0, I.tsn * ntr_ * rs));
+            I.allocs.push_back(*q);
+        }
+    }
+    Impl::Dump d{I.dsX, I.dsY, I.ds_rows, I.ds_cap};
+    I.dump = &d;
+    try {
+        for (int s = 0; s < nsteps; ++s) enqueue(I, prec_, spec_, dt, 0, ntr_, refresh_ab_, stream_);
+        HIP_OK(hipStreamSynchronize(stream_));
+    } catch (...) {
+        I.dump = nullptr;
+        throw;
+    }
+    I.dump = nullptr;
+    I.ds_rows = d.row;
+    graph_valid_ = false;
+    return I.ds_rows;
+}
+
+static void mlp_train_pass(Engine::Impl &I, const unsigned short *X, const float *Y, long long rows, float scale,
+                           hipStream_t st)
+{
+    const long long tiles = (rows + 15) / 16;
+    const int grid = (int)std::max(1LL, std::min((tiles + 3) / 4, 1024LL));
+    const int iters = (int)((tiles + 4LL * grid - 1) / (4LL * grid));
+    hipLaunchKernelGGL(k_mlp_train, dim3(grid), dim3(256), 0, st, mlp_dev(I), X, Y, rows, iters, scale, I.mlp_g,
+                       I.mlp_loss);
+    HIP_OK(hipGetLastError());
+}
+
+std::vector<double> Engine::learn_train(int epochs, long long batch, double lr, unsigned seed)
+{
+    Impl &I = *p_;
+    std::vector<double> out;
+    if (I.ds_rows == 0 || batch < 16) throw std::runtime_error("no training rows collected (or batch < 16)");
+    const long long nb = (I.ds_rows + batch - 1) / batch;
+    std::vector<long long> order(nb);
+    uint64_t st = 0x2545F4914F6CDD1Dull ^ seed;
+    for (int e = 0; e < epochs; ++e) {
+        for (long long b = 0; b < nb; ++b) order[b] = b;
+        for (long long b = nb - 1; b > 0; --b) {  // minibatches in a seeded random order
+            st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+            std::swap(order[b], order[st % (uint64_t)(b + 1)]);
+        }
+        hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(256), 0, stream_, I.mlp_loss, 1LL, 0.0f);
+        for (long long b : order) {
+            const long long r0 = b * batch, rows = std::min(batch, I.ds_rows - r0);
+            hipLaunchKernelGGL(k_fill_f32, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream_, I.mlp_g,
+                               (long long)kMlpParams, 0.0f);
+            mlp_train_pass(I, I.dsX + r0 * kMlpInPad, I.dsY + r0, rows, 1.0f / (float)rows, stream_);
+            I.adam_t += 1;
+            const float c1 = 1.0f - std::pow(0.9f, (float)I.adam_t), c2 = 1.0f - std::pow(0.999f, (float)I.adam_t);
+            hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream_, I.mlp_p, I.mlp_m1,
+                               I.mlp_m2, I.mlp_g, (float)lr, c1, c2);
+            mlp_pack(I, stream_);
+        }
+        float loss = 0.0f;
+        HIP_OK(hipMemcpyAsync(&loss, I.mlp_loss, sizeof(float), hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        out.push_back((double)loss / (double)I.ds_rows);
+    }
+    graph_valid_ = false;
+    return out;
+}
+
+static std::vector<unsigned short> rows_to_bf16(const float *X, long long rows)
+{
+    std::vector<unsigned short> b((size_t)rows * kMlpInPad, 0);
+    for (long long r = 0; r < rows; ++r)
+        for (int f = 0; f < kMlpIn; ++f) b[(size_t)r * kMlpInPad + f] = f2bf(X[(size_t)r * kMlpIn + f]);
+    return b;
+}
+
+void Engine::mlp_forward(const float *X, long long rows, float *out)
+{
+    Impl &I = *p_;
+    if (rows <= 0) return;
+    auto b = rows_to_bf16(X, rows);
+    unsigned short *dX = dupload(b);
+    float *dO = dalloc<float>(rows);
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((rows + 63) / 64)), dim3(256), 0, stream_, mlp_dev(I), dX, rows,
+                       dO);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(stream_));
+    HIP_OK(hipMemcpy(out, dO, rows * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_OK(hipFree(dX));
+    HIP_OK(hipFree(dO));
+}
+
+void Engine::mlp_gradient(const float *X, const float *y, long long rows, float scale, float *grad, double *loss)
+{
+    Impl &I = *p_;
+    auto b = rows_to_bf16(X, rows);
+    unsigned short *dX = dupload(b);
+    float *dY = dupload(std::vector<float>(y, y + rows));
+    hipLaunchKernelGGL(k_fill_f32, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream_, I.mlp_g,
+                       (long long)kMlpParams, 0.0f);
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(256), 0, stream_, I.mlp_loss, 1LL, 0.0f);
+    mlp_train_pass(I, dX, dY, rows, scale, stream_);
+    HIP_OK(hipStreamSynchronize(stream_));
+    HIP_OK(hipMemcpy(grad, I.mlp_g, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    float l = 0.0f;
+    HIP_OK(hipMemcpy(&l, I.mlp_loss, sizeof(float), hipMemcpyDeviceToHost));
+    *loss = l;
+    HIP_OK(hipFree(dX));
+    HIP_OK(hipFree(dY));
+}
+
+double Engine::learn_fallback(int tr)
+{
+    Impl &I = *p_;
+    if (!I.unsafe || tr < 0 || tr >= ntr_) return 0.0;
+    HIP_OK(hipMemsetAsync(I.cnt, 0, 2 * sizeof(unsigned long long), stream_));
+    const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
+    const char *u = (const char *)I.unsafe + tr * I.tsn * rs;
+    if (prec_ == PREC_FP64)
+        hipLaunchKernelGGL((k_count_flags<double>), dim3(256), dim3(256), 0, stream_, I.m64, (const double *)u, I.cnt);
+    else
+        hipLaunchKernelGGL((k_count_flags<float>), dim3(256), dim3(256), 0, stream_, I.m32, (const float *)u, I.cnt);
+    HIP_OK(hipGetLastError());
+    unsigned long long h[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(h, I.cnt, sizeof(h), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    return h[1] ? (double)h[0] / (double)h[1] : 0.0;
 }
 
 void Engine::synchronize() { HIP_OK(hipStreamSynchronize(stream_)); }

@@ -32,6 +32,7 @@ struct State {
     std::vector<double> uv, w, we;
     std::unique_ptr<Engine> eng;
     int precision = 64, refresh = 1, use_graph = 1;
+    std::vector<float> mlp;  // learned-limiter parameters carried across engine rebuilds
 };
 
 State &S()
@@ -79,6 +80,7 @@ int ensure_engine()
         s.eng.reset(new Engine(s.mesh, s.ntr, prec_code(s.precision)));
         s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
         s.eng->set_refresh_ab(s.refresh != 0);
+        if (!s.mlp.empty()) s.eng->mlp_set(s.mlp.data());
         for (int t = 0; t < s.ntr; ++t) {
             s.eng->set_spec(t, s.specs[t]);
             s.eng->set_values(t, s.values[t].data());
@@ -477,6 +479,7 @@ int tracer_set_schemes(int tracer_id, const char *hor, const char *ver, const ch
     else return fail("Unknown vertical advection type " + v);
     if (l == "NONE") sp.lim = LIM_NONE;
     else if (l == "FCT") sp.lim = LIM_FCT;
+    else if (l == "LEARNED") sp.lim = LIM_LEARNED;
     else return fail("Unknown limiter type " + l);
     if (check_vertical_scheme(sp.ver)) return -1;
     sp.ph = ph;
@@ -676,6 +679,92 @@ int tracer_precision_error(int ref_precision, int nsteps, double dt, double *out
     } catch (const std::exception &e) {
         return fail(e.what());
     }
+    return 0;
+}
+
+int tracer_mlp_param_count(void) { return Engine::mlp_param_count(); }
+
+int tracer_mlp_set_params(const float *params, int n)
+{
+    State &s = S();
+    if (n != Engine::mlp_param_count()) return fail("learned limiter needs " +
+                                                    std::to_string(Engine::mlp_param_count()) + " parameters");
+    s.mlp.assign(params, params + n);
+    if (s.eng) {
+        try { s.eng->mlp_set(params); } catch (const std::exception &e) { return fail(e.what()); }
+    }
+    return 0;
+}
+
+int tracer_mlp_get_params(float *params, int n)
+{
+    State &s = S();
+    if (n != Engine::mlp_param_count()) return fail("learned limiter has " +
+                                                    std::to_string(Engine::mlp_param_count()) + " parameters");
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (ensure_engine()) return -1;
+    try { s.eng->mlp_get(params); } catch (const std::exception &e) { return fail(e.what()); }
+    return 0;
+}
+
+int tracer_learn_collect(int nsteps, double dt, long long max_rows, long long *rows_out)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (nsteps < 0 || max_rows < 0) return fail("nsteps and max_rows must be >= 0");
+    if (ensure_engine()) return -1;
+    try { *rows_out = s.eng->learn_collect(nsteps, dt, max_rows); } catch (const std::exception &e) { return fail(e.what()); }
+    return 0;
+}
+
+int tracer_learn_clear(void)
+{
+    State &s = S();
+    if (s.eng) s.eng->learn_clear();
+    return 0;
+}
+
+int tracer_learn_train(int epochs, long long batch, double lr, unsigned seed, double *losses)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (ensure_engine()) return -1;
+    try {
+        std::vector<double> l = s.eng->learn_train(epochs, batch, lr, seed);
+        for (size_t i = 0; i < l.size(); ++i) losses[i] = l[i];
+        s.mlp.assign(Engine::mlp_param_count(), 0.0f);
+        s.eng->mlp_get(s.mlp.data());
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int tracer_mlp_forward(const float *X, long long rows, float *out)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (ensure_engine()) return -1;
+    try { s.eng->mlp_forward(X, rows, out); } catch (const std::exception &e) { return fail(e.what()); }
+    return 0;
+}
+
+int tracer_mlp_gradient(const float *X, const float *y, long long rows, float scale, float *grad, double *loss)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (ensure_engine()) return -1;
+    try { s.eng->mlp_gradient(X, y, rows, scale, grad, loss); } catch (const std::exception &e) { return fail(e.what()); }
+    return 0;
+}
+
+int tracer_learn_fallback(int tracer_id, double *fraction)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id");
+    if (ensure_engine()) return -1;
+    try { *fraction = s.eng->learn_fallback(tracer_id - 1); } catch (const std::exception &e) { return fail(e.what()); }
     return 0;
 }
 

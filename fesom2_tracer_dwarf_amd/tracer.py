@@ -151,5 +151,53 @@ class FESOMTracerAdvection:
                    "tracer_precision_error", self.lib)
         return out.reshape(nsteps, ntr, 4)
 
+    # ---- learned limiter
+    def mlp_param_count(self):
+        return int(self.lib.tracer_mlp_param_count())
+
+    def mlp_params(self):
+        p = np.zeros(self.mlp_param_count(), dtype=np.float32)
+        _lib.check(self.lib.tracer_mlp_get_params(p, p.size), "tracer_mlp_get_params", self.lib)
+        return p
+
+    def set_mlp_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _lib.check(self.lib.tracer_mlp_set_params(p, p.size), "tracer_mlp_set_params", self.lib)
+
+    def learn_collect(self, nsteps, dt, max_rows):
+        rows = ctypes.c_longlong()
+        _lib.check(self.lib.tracer_learn_collect(int(nsteps), float(dt), int(max_rows), ctypes.byref(rows)),
+                   "tracer_learn_collect", self.lib)
+        return rows.value
+
+    def learn_clear(self):
+        self.lib.tracer_learn_clear()
+
+    def learn_train(self, epochs, batch=65536, lr=1e-3, seed=0):
+        out = np.zeros(int(epochs))
+        _lib.check(self.lib.tracer_learn_train(int(epochs), int(batch), float(lr), int(seed), out),
+                   "tracer_learn_train", self.lib)
+        return out
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(X.shape[0], dtype=np.float32)
+        _lib.check(self.lib.tracer_mlp_forward(X, X.shape[0], out), "tracer_mlp_forward", self.lib)
+        return out
+
+    def mlp_gradient(self, X, y, scale=1.0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(self.mlp_param_count(), dtype=np.float32)
+        loss = ctypes.c_double()
+        _lib.check(self.lib.tracer_mlp_gradient(X, y, X.shape[0], float(scale), g, ctypes.byref(loss)),
+                   "tracer_mlp_gradient", self.lib)
+        return g, loss.value
+
+    def learn_fallback(self, tracer_id=1):
+        f = ctypes.c_double()
+        _lib.check(self.lib.tracer_learn_fallback(tracer_id, ctypes.byref(f)), "tracer_learn_fallback", self.lib)
+        return f.value
+
     def cleanup(self):
         return self.lib.tracer_cleanup()

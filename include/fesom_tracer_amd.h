@@ -186,6 +186,35 @@ int tracer_get_work_array(int tracer_id, const char *name, double *out, long lon
  * (a = this precision, b = the reference). The current state advances too. */
 int tracer_precision_error(int ref_precision, int nsteps, double dt, double *out);
 
+/* ---- Learned limiter (no reference counterpart). tracer_set_schemes(id,
+ * hor, ver, "LEARNED", ...) selects it: a network 10-64-64-1 (ReLU, ReLU,
+ * sigmoid; bf16 weights on the matrix cores, FP32 accumulation) predicts the
+ * limiter coefficient of every horizontal edge level from its LO/HO fluxes,
+ * node values, Zalesak headroom and cell geometry; the coefficient is applied
+ * in flux form (mass exact) and every edge of a node level whose bounds the
+ * prediction would break takes Zalesak's coefficient instead.
+ * Parameters are FP32, flattened as W1[10][64], b1[64], W2[64][64], b2[64],
+ * w3[64], b3 (tracer_mlp_param_count() values). */
+int tracer_mlp_param_count(void);
+int tracer_mlp_set_params(const float *params, int n);
+int tracer_mlp_get_params(float *params, int n);
+/* nsteps FCT steps of all tracers, appending one training row (features and
+ * Zalesak's coefficient) per edge level of every FCT/LEARNED tracer to the
+ * on-device training set of capacity max_rows; *rows_out = rows held */
+int tracer_learn_collect(int nsteps, double dt, long long max_rows, long long *rows_out);
+int tracer_learn_clear(void);
+/* Adam on the training set (minibatches of batch rows, seeded order);
+ * losses[epochs] = mean squared error per epoch */
+int tracer_learn_train(int epochs, long long batch, double lr, unsigned seed, double *losses);
+/* the network on rows x 10 features (row-major floats) -> out[rows] */
+int tracer_mlp_forward(const float *X, long long rows, float *out);
+/* gradient of scale * sum((sigmoid(z) - y)^2) over the rows with respect to
+ * the parameters (no update); rows with y < 0 are ignored */
+int tracer_mlp_gradient(const float *X, const float *y, long long rows, float scale, float *grad, double *loss);
+/* fraction of wet layers of a tracer where the last LEARNED step fell back
+ * to Zalesak */
+int tracer_learn_fallback(int tracer_id, double *fraction);
+
 /* No reference counterpart: wall time of nsteps steps of all tracers on the
  * library's stream (hipEvent), milliseconds */
 int tracer_time_steps(int nsteps, double dt, double *ms_out);
