@@ -1815,7 +1815,8 @@ static void mlp_train_pass(Engine::Impl &I, const unsigned short *X, const float
                            hipStream_t st)
 {
     const long long tiles = (rows + 15) / 16;
-    const int grid = (int)std::max(1LL, std::min((tiles + 3) / 4, 1024LL));
+    // few blocks, many tiles each: every block adds its gradient with atomics once
+    const int grid = (int)std::max(1LL, std::min((tiles + 3) / 4, 256LL));
     const int iters = (int)((tiles + 4LL * grid - 1) / (4LL * grid));
     hipLaunchKernelGGL(k_mlp_train, dim3(grid), dim3(256), 0, st, mlp_dev(I), X, Y, rows, iters, scale, I.mlp_g,
                        I.mlp_loss);
