@@ -73,6 +73,16 @@ SIGNATURES = {
     "tracer_mlp_forward": (c_int, [F32, c_longlong, F32]),
     "tracer_mlp_gradient": (c_int, [F32, F32, c_longlong, c_float, F32, POINTER(c_double)]),
     "tracer_learn_fallback": (c_int, [c_int, POINTER(c_double)]),
+    "tracer_batch_create": (c_int, []),
+    "tracer_batch_destroy": (c_int, []),
+    "tracer_batch_add_periodic_mesh": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_int, c_int,
+                                               c_ull, c_int, c_int, c_char_p, c_char_p, c_char_p, c_double,
+                                               POINTER(c_double)]),
+    "tracer_batch_run": (c_int, [c_int, c_int]),
+    "tracer_batch_time": (c_int, [c_int, POINTER(c_double)]),
+    "tracer_batch_mesh_info": (c_int, [c_int, P_int, P_int, P_int, P_ll, P_double]),
+    "tracer_batch_get_values": (c_int, [c_int, c_int, F64F]),
+    "tracer_batch_diagnostics": (c_int, [c_int, c_int, F64]),
     "tracer_last_error": (c_char_p, []),
 }
 REFERENCE_SYMBOLS = list(SIGNATURES)[:23]

@@ -62,6 +62,7 @@ public:
 
     // refresh_ab: valuesAB := values at the start of each step (AB1)
     void set_refresh_ab(bool r) { refresh_ab_ = r; graph_valid_ = false; }
+    static constexpr int kMaxBatch = 4;  // tracers per launch (and per work-array set)
 
     // enqueue one step for tracers [first, first+count) on the stream
     void enqueue_step(double dt, int first, int count);
@@ -71,7 +72,9 @@ public:
     hipStream_t stream() const { return stream_; }
 
     Diag diagnostics(int tr);
-    // copy a device work array out in the unpadded reference layout
+    // copy a device work array out in the unpadded reference layout (the
+    // work arrays are shared by the tracer batches of a step: a tracer's
+    // values are those of the last step of its batch)
     // names: fct_LO, adv_flux_hor, adv_flux_ver, fct_plus, fct_minus, gnod
     int get_work(const std::string &name, int tr, double *out, size_t count);
     size_t work_count(const std::string &name) const;

@@ -1225,6 +1225,7 @@ struct Engine::Impl {
     int ndiag = 1024;
     float gscale = 1.0f, ginv = 1.0f;
     size_t tsn = 0, tst = 0, tse = 0;
+    int wslots = 1;  // tracers the work arrays hold
     // host copy of each tracer as last set (levels below the bottom are never
     // touched by a step, so get_values and the all-values statistics read them here)
     std::vector<std::vector<double>> host_vals;
@@ -1414,6 +1415,9 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     I.tsn = (size_t)I.ln.total;
     I.tst = 2 * (size_t)I.lt.total;
     I.tse = (size_t)I.le.total;
+    // work arrays hold one tracer batch (at most 4 tracers); batches run in
+    // stream order, so every batch reuses them
+    I.wslots = std::min(ntr_, kMaxBatch);
     auto zalloc = [&](size_t bytes) {
         void *p = I.track((void *)dalloc<char>(bytes));
         HIP_OK(hipMemset(p, 0, bytes));
@@ -1423,15 +1427,15 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     I.W = zalloc(I.tsn * rs);
     I.WE = zalloc(I.tsn * rs);
     I.values = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
-    I.valuesAB = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
+    // valuesAB only exists once a caller sets it (the default refresh mode reads values)
     I.vfl = zalloc(I.tse * rs);
-    I.adf_h = zalloc(I.tse * ntr_ * rs);
-    I.adf_v = zalloc(I.tsn * ntr_ * rs);
-    I.Pp = zalloc(I.tsn * ntr_ * rs);
-    I.Pm = zalloc(I.tsn * ntr_ * rs);
-    I.LO = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
-    I.trxy = zalloc(I.tst * ntr_ * gs);
-    I.gnod = zalloc(2 * I.tsn * ntr_ * gs);
+    I.adf_h = zalloc(I.tse * I.wslots * rs);
+    I.adf_v = zalloc(I.tsn * I.wslots * rs);
+    I.Pp = zalloc(I.tsn * I.wslots * rs);
+    I.Pm = zalloc(I.tsn * I.wslots * rs);
+    I.LO = (double *)zalloc(I.tsn * I.wslots * sizeof(double));
+    I.trxy = zalloc(I.tst * I.wslots * gs);
+    I.gnod = zalloc(2 * I.tsn * I.wslots * gs);
     I.dpart = I.track(dalloc<DiagAcc>(I.ndiag));
     I.dout = I.track(dalloc<DiagAcc>(1));
     I.epart = I.track(dalloc<ErrAcc>(I.ndiag));
@@ -1497,13 +1501,13 @@ void Engine::set_spec(int tr, const TracerSpec &sp)
             HIP_OK(hipMemset(p, 0, bytes));
             return p;
         };
-        I.alpha = zalloc(I.tse * ntr_ * rs);
-        I.unsafe = zalloc(I.tsn * ntr_ * rs);
+        I.alpha = zalloc(I.tse * I.wslots * rs);
+        I.unsafe = zalloc(I.tsn * I.wslots * rs);
         if (!I.Qp) {
-            I.Qp = zalloc(I.tsn * ntr_ * rs);
-            I.Qm = zalloc(I.tsn * ntr_ * rs);
-            I.Sp = zalloc(I.tsn * ntr_ * rs);
-            I.Sm = zalloc(I.tsn * ntr_ * rs);
+            I.Qp = zalloc(I.tsn * I.wslots * rs);
+            I.Qm = zalloc(I.tsn * I.wslots * rs);
+            I.Sp = zalloc(I.tsn * I.wslots * rs);
+            I.Sm = zalloc(I.tsn * I.wslots * rs);
         }
     }
 }
@@ -1538,6 +1542,10 @@ void Engine::set_valuesAB(int tr, const double *v)
 {
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     Impl &I = *p_;
+    if (!I.valuesAB) {
+        I.valuesAB = I.track(dalloc<double>(I.tsn * ntr_));
+        HIP_OK(hipMemcpy(I.valuesAB, I.values, I.tsn * ntr_ * sizeof(double), hipMemcpyDeviceToDevice));
+    }
     auto a = I.ln.pack<double>(v, nl_ - 1, 1);
     HIP_OK(hipMemcpy(I.valuesAB + tr * I.tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
 }
@@ -1580,12 +1588,13 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
     const R *vel = (const R *)I.vel, *W = (const R *)I.W, *WE = (const R *)I.WE;
     const size_t tsn = I.tsn, tst = I.tst, tse = I.tse, tsg = 2 * I.tsn;
     double *vals = I.values + t0 * tsn;
+    if (!refresh && !I.valuesAB) throw std::runtime_error("valuesAB was never set");
     const double *ab = refresh ? vals : I.valuesAB + t0 * tsn;
-    R *vfl = (R *)I.vfl, *adf_h = (R *)I.adf_h + t0 * tse;
-    R *adf_v = (R *)I.adf_v + t0 * tsn, *Pp = (R *)I.Pp + t0 * tsn, *Pm = (R *)I.Pm + t0 * tsn;
-    double *LO = I.LO + t0 * tsn;
-    G *trxy = (G *)I.trxy + t0 * tst;
-    G *gnod = (G *)I.gnod + t0 * tsg;
+    R *vfl = (R *)I.vfl, *adf_h = (R *)I.adf_h;
+    R *adf_v = (R *)I.adf_v, *Pp = (R *)I.Pp, *Pm = (R *)I.Pm;
+    double *LO = I.LO;
+    G *trxy = (G *)I.trxy;
+    G *gnod = (G *)I.gnod;
     const dim3 blk(kBlock), ge(grid_for(m.nch_e)), gn(grid_for(m.nch_n)), gt(grid_for(m.nch_t));
     const bool learned = sp.lim == LIM_LEARNED;
     const bool fct = sp.lim == LIM_FCT || learned;
@@ -1635,10 +1644,10 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
         }
 #undef NLO
         hk.mark("k_node_lo", st);
-        R *Qp = learned || dump ? (R *)I.Qp + t0 * tsn : nullptr;
-        R *Qm = learned || dump ? (R *)I.Qm + t0 * tsn : nullptr;
-        R *Sp = learned || dump ? (R *)I.Sp + t0 * tsn : nullptr;
-        R *Sm = learned || dump ? (R *)I.Sm + t0 * tsn : nullptr;
+        R *Qp = learned || dump ? (R *)I.Qp : nullptr;
+        R *Qm = learned || dump ? (R *)I.Qm : nullptr;
+        R *Sp = learned || dump ? (R *)I.Sp : nullptr;
+        R *Sm = learned || dump ? (R *)I.Sm : nullptr;
         if (learned || dump)
             hipLaunchKernelGGL((k_node_limit<R, NT, true>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm, Qp, Qm,
                                Sp, Sm);
@@ -1647,7 +1656,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
                                Sp, Sm);
         hk.mark("k_node_limit", st);
         const LearnArgs<R, NT> la{vals, LO, adf_h, vfl, Qp, Qm, Pp, Pm, Sp, Sm, tsn, tse};
-        R *alpha = learned ? (R *)I.alpha + t0 * tse : nullptr, *unsafe = learned ? (R *)I.unsafe + t0 * tsn : nullptr;
+        R *alpha = learned ? (R *)I.alpha : nullptr, *unsafe = learned ? (R *)I.unsafe : nullptr;
         if (dump) {
             if (dump->row + (long long)NT * m.CE <= dump->cap) {
                 hipLaunchKernelGGL((k_learn_dump<R, NT>), ge, blk, 0, st, m, la, dt, dump->X, dump->Y, dump->row);
@@ -1688,6 +1697,24 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
 static bool same_spec(const TracerSpec &a, const TracerSpec &b)
 {
     return a.hor == b.hor && a.ver == b.ver && a.lim == b.lim && a.ph == b.ph && a.pv == b.pv;
+}
+
+// position of tracer tr in the work arrays: its index within the batch that
+// launch_range puts it in (runs of identical schemes split into 4/2/1)
+static int work_slot(const std::vector<TracerSpec> &spec, int tr)
+{
+    const int n = (int)spec.size();
+    for (int t = 0; t < n;) {
+        int run = 1;
+        while (t + run < n && same_spec(spec[t], spec[t + run])) ++run;
+        while (run > 0) {
+            const int nb = run >= 4 ? 4 : (run >= 2 ? 2 : 1);
+            if (tr >= t && tr < t + nb) return tr - t;
+            t += nb;
+            run -= nb;
+        }
+    }
+    return 0;
 }
 
 template <class R, class G>
@@ -1791,8 +1818,8 @@ long long Engine::learn_collect(int nsteps, double dt, long long max_rows)
     if (!I.Qp) {
         const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
         for (void **q : {&I.Qp, &I.Qm, &I.Sp, &I.Sm}) {
-            HIP_OK(hipMalloc(q, I.tsn * ntr_ * rs));
-            HIP_OK(hipMemset(*q, 0, I.tsn * ntr_ * rs));
+            HIP_OK(hipMalloc(q, I.tsn * I.wslots * rs));
+            HIP_OK(hipMemset(*q, 0, I.tsn * I.wslots * rs));
             I.allocs.push_back(*q);
         }
     }
@@ -1907,7 +1934,7 @@ double Engine::learn_fallback(int tr)
     if (!I.unsafe || tr < 0 || tr >= ntr_) return 0.0;
     HIP_OK(hipMemsetAsync(I.cnt, 0, 2 * sizeof(unsigned long long), stream_));
     const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
-    const char *u = (const char *)I.unsafe + tr * I.tsn * rs;
+    const char *u = (const char *)I.unsafe + work_slot(spec_, tr) * I.tsn * rs;
     if (prec_ == PREC_FP64)
         hipLaunchKernelGGL((k_count_flags<double>), dim3(256), dim3(256), 0, stream_, I.m64, (const double *)u, I.cnt);
     else
@@ -2004,9 +2031,10 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
         src = (const char *)I.gnod; comps = 2;
         es = prec_ == PREC_FP64 ? 8 : (prec_ == PREC_FP32 ? 4 : 2);
     }
+    const int slot = work_slot(spec_, tr);
     const size_t per = (size_t)lay->total * comps;
     std::vector<char> buf(per * es);
-    HIP_OK(hipMemcpy(buf.data(), src + tr * per * es, buf.size(), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(buf.data(), src + slot * per * es, buf.size(), hipMemcpyDeviceToHost));
     std::fill(out, out + need, 0.0);
     for (size_t c = 0; c < lay->len.size(); ++c)
         for (int k = 0; k < std::min(lay->len[c] - sub, L); ++k)

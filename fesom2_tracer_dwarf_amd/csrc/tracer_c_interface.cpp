@@ -84,7 +84,7 @@ int ensure_engine()
         for (int t = 0; t < s.ntr; ++t) {
             s.eng->set_spec(t, s.specs[t]);
             s.eng->set_values(t, s.values[t].data());
-            s.eng->set_valuesAB(t, s.valuesAB[t].data());
+            if (!s.refresh) s.eng->set_valuesAB(t, s.valuesAB[t].data());
         }
     } catch (const std::exception &e) {
         s.eng.reset();
@@ -313,7 +313,7 @@ int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D)
     if (s.eng) {
         try {
             s.eng->set_values(tracer_id - 1, values);
-            s.eng->set_valuesAB(tracer_id - 1, values);
+            if (!s.refresh) s.eng->set_valuesAB(tracer_id - 1, values);
         } catch (const std::exception &e) {
             return fail(e.what());
         }
@@ -505,7 +505,15 @@ int tracer_set_option(const char *name, int value)
     }
     if (n == "refresh_valuesAB") {
         s.refresh = value ? 1 : 0;
-        if (s.eng) s.eng->set_refresh_ab(value != 0);
+        if (s.eng) {
+            try {
+                if (!value)
+                    for (int t = 0; t < s.ntr; ++t) s.eng->set_valuesAB(t, s.valuesAB[t].data());
+                s.eng->set_refresh_ab(value != 0);
+            } catch (const std::exception &e) {
+                return fail(e.what());
+            }
+        }
         return 0;
     }
     if (n == "use_graph") {
@@ -668,7 +676,7 @@ int tracer_precision_error(int ref_precision, int nsteps, double dt, double *out
             ref.set_spec(t, s.specs[t]);
             s.eng->get_values(t, v.data());
             ref.set_values(t, v.data());
-            ref.set_valuesAB(t, s.valuesAB[t].data());
+            if (!s.refresh) ref.set_valuesAB(t, s.valuesAB[t].data());
         }
         for (int k = 0; k < nsteps; ++k) {
             s.eng->run_steps(1, dt, s.use_graph != 0);

@@ -215,6 +215,26 @@ int tracer_mlp_gradient(const float *X, const float *y, long long rows, float sc
  * to Zalesak */
 int tracer_learn_fallback(int tracer_id, double *fraction);
 
+/* ---- Batched form (no reference counterpart): several independent periodic
+ * meshes, each with its own tracers, advanced together by one hipGraph
+ * launch per step (the meshes run as concurrent branches of the graph).
+ * tracer_batch_add_periodic_mesh builds the mesh (bottom levels U{lev_lo..
+ * lev_hi}, seeded), the analytic velocity at the given CFL (its dt in
+ * *dt_out) and ntracers tracers: T, S, then passive Gaussian blobs, 0/1 steps
+ * and white noise in turn, seeded; it returns the mesh index (>= 0) or -1. */
+int tracer_batch_create(void);
+int tracer_batch_destroy(void);
+int tracer_batch_add_periodic_mesh(int nx, int ny, int nl, double Lx, double Ly, double dz, int lev_lo, int lev_hi,
+                                   unsigned long long seed, int ntracers, int precision, const char *hor,
+                                   const char *ver, const char *lim, double cfl, double *dt_out);
+/* nsteps steps of every mesh with its own dt (use_graph: replay the captured step) */
+int tracer_batch_run(int nsteps, int use_graph);
+/* hipEvent time of nsteps graph launches, milliseconds */
+int tracer_batch_time(int nsteps, double *ms_out);
+int tracer_batch_mesh_info(int mesh, int *nl, int *nod2D, int *ntracers, long long *active_cells, double *dt);
+int tracer_batch_get_values(int mesh, int tracer_id, double *values);
+int tracer_batch_diagnostics(int mesh, int tracer_id, double *out);
+
 /* No reference counterpart: wall time of nsteps steps of all tracers on the
  * library's stream (hipEvent), milliseconds */
 int tracer_time_steps(int nsteps, double dt, double *ms_out);
