@@ -57,7 +57,29 @@ def build(verbose: bool = False) -> Path:
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs,
           f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, out)
+    build_fortran(verbose)
     return out
+
+
+FORTRAN_DIR = PKG / "fortran"
+DRIVER = LIBDIR / "fesom_tracer_amd_driver"
+
+
+def build_fortran(verbose: bool = False) -> Path | None:
+    """The ISO_C_BINDING module and the Fortran analytic driver (amdflang), if
+    the compiler is present; the driver links the library with an $ORIGIN rpath."""
+    fc = shutil.which("amdflang") or (str(ROCM / "bin" / "amdflang") if (ROCM / "bin" / "amdflang").exists() else None)
+    if not fc:
+        return None
+    mod = BUILD / "fesom_tracer_amd_f.o"
+    cmds = [[fc, "-O2", "-c", str(FORTRAN_DIR / "fesom_tracer_amd.F90"), "-J", str(BUILD), "-o", str(mod)],
+            [fc, "-O2", "-I", str(BUILD), str(FORTRAN_DIR / "analytic_driver.F90"), str(mod), f"-L{LIBDIR}",
+             "-lfesom_tracer_amd", "-Wl,-rpath,$ORIGIN", "-o", str(DRIVER)]]
+    for c in cmds:
+        if verbose:
+            print(" ".join(c))
+        _run(c)
+    return DRIVER
 
 
 if __name__ == "__main__":
