@@ -84,14 +84,15 @@ def kernel_alg_bytes(name, c, ntr, s_t, s_r, s_g):
     return 0
 
 
-def run_precision(api, prec, dt, steps, warmup, ntr):
+def run_precision(api, prec, dt, steps, warmup, ntr, tune=True):
     api.set_option("precision", prec)
+    table = api.autotune(dt, reps=5) if tune else []
     api.advect_steps(warmup, dt)
     t0 = time.perf_counter()
     ms = api.time_steps(steps, dt)
     wall = time.perf_counter() - t0
     kern = api.time_kernels(dt, reps=3)
-    return ms, wall, kern
+    return ms, wall, kern, table
 
 
 def cpu_baseline(api, dt, ntr, c, max_seconds=20.0):
@@ -136,6 +137,7 @@ def main():
     ap.add_argument("--nl", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedule (no autotuning)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -174,8 +176,9 @@ def main():
     res = {}
     for prec, tag in ((64, "fp64"), (32, "fp32")):
         log(f"timing {tag}")
-        ms, wall, kern = run_precision(api, prec, dt, args.steps, args.warmup, ntr)
-        log(f"{tag}: {ms / args.steps:.3f} ms/step, kernels {kern}")
+        ms, wall, kern, table = run_precision(api, prec, dt, args.steps, args.warmup, ntr, not args.no_tune)
+        best = min(table, key=lambda r: r["ms_per_step"]) if table else None
+        log(f"{tag}: {ms / args.steps:.3f} ms/step, schedule {best}, kernels {kern}")
         s_r = 8 if prec == 64 else 4
         upd = ntr * c["C"] * args.steps / (ms * 1e-3)
         balg = step_alg_bytes(c, ntr, 8, s_r, s_r)
@@ -186,7 +189,8 @@ def main():
                         step_hbm_frac=balg / (ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
                         kernels_ms=kern,
                         kernels_gbs={k: kb[k] / (kern[k] * 1e-3) / 1e9 for k in kern if kern[k] > 0},
-                        dominant=dom)
+                        dominant=dom,
+                        schedule=best, autotune=table)
         diag = api.diagnostics(1)
         res[tag]["T_mass"] = diag["mass"]
         res[tag]["nan"] = diag["nan_count"]

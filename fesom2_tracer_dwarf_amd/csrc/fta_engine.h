@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -102,6 +103,14 @@ public:
     void mlp_gradient(const float *X, const float *y, long long rows, float scale, float *grad, double *loss);
     // fraction of wet layers of tracer tr whose Zalesak fallback was taken in the last step
     double learn_fallback(int tr);
+
+    // launch schedule of the chunk kernels: waves per workgroup (1, 2, 4, 8)
+    // and most tracers per launch (1, 2, 4); results do not depend on it
+    void set_schedule(int waves, int max_nt);
+    void schedule(int &waves, int &max_nt) const;
+    // time every schedule on reps graph-replayed steps of the current state,
+    // keep the fastest, restore the tracers; rows {waves, max_nt, ms/step}
+    std::vector<std::array<double, 3>> autotune(double dt, int reps);
 
     // per-kernel timing of one step with hip events on the engine stream
     // (names and milliseconds, averaged over reps)

@@ -17,6 +17,7 @@
 #include "fta_vertical.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -87,7 +88,7 @@ __device__ __forceinline__ void load_other_nodes(const DMesh<R> &m, int n, bool 
 // (:397-407)
 // ===========================================================================
 template <class R, class G, int NT>
-__global__ __launch_bounds__(kBlock) void k_trxy(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
+__global__ __launch_bounds__(kMaxBlock) void k_trxy(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
                                                  G *__restrict__ trxy, size_t tst, float gscale)
 {
     const int ch = wave_chunk();
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_trxy(DMesh<R> m, const double *__res
 }
 
 template <class R, class G, int NT>
-__global__ __launch_bounds__(kBlock) void k_gnod(DMesh<R> m, const G *__restrict__ trxy, size_t tst,
+__global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const G *__restrict__ trxy, size_t tst,
                                                  G *__restrict__ gnod, size_t tsg, float gscale, float ginv)
 {
     const int ch = wave_chunk();
@@ -190,7 +191,7 @@ __device__ __forceinline__ double lo_flux(double v, double T1, double T2)
 }
 
 template <class R, class G, int HOR, bool FCT, int NT, bool SAME>
-__global__ __launch_bounds__(kBlock) void k_edge(DMesh<R> m, const R *__restrict__ vel,
+__global__ __launch_bounds__(kMaxBlock) void k_edge(DMesh<R> m, const R *__restrict__ vel,
                                                  const double *__restrict__ ttf,
                                                  const double *__restrict__ ttfAB, size_t tsn,
                                                  const G *__restrict__ trxy, size_t tst,
@@ -335,7 +336,7 @@ __device__ __forceinline__ VCol2<R> node_vcol(const DMesh<R> &m, const Pair &p, 
 // (:257-283), and the raw antidiffusive sums P+/P- (lib/oce_adv_tra_fct.F90:232-346)
 // ===========================================================================
 template <class R, int VER, int NT, bool SAME>
-__global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restrict__ Wv,
+__global__ __launch_bounds__(kMaxBlock) void k_node_lo(DMesh<R> m, const R *__restrict__ Wv,
                                                     const R *__restrict__ WEv,
                                                     const double *__restrict__ ttf,
                                                     const double *__restrict__ ttfAB, size_t tsn,
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_node_lo(DMesh<R> m, const R *__restr
 // R+/R- overwrite P+/P- in place. a1 (max/min of LO and T) is formed once per
 // 1-ring node, a2 per fan element from the node and two consecutive ring nodes.
 template <class R, int NT, bool STOREQ>
-__global__ __launch_bounds__(kBlock) void k_node_limit(DMesh<R> m, const double *__restrict__ ttf,
+__global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const double *__restrict__ ttf,
                                                        const double *__restrict__ LO, size_t tsn,
                                                        double dt, R *__restrict__ Pp_Rp,
                                                        R *__restrict__ Pm_Rm, R *__restrict__ Qp_out,
@@ -602,7 +603,7 @@ __device__ __forceinline__ R edge_alpha(bool pos, R f, R rp, R rm, R rpo, R rmo)
 // (lib/oce_adv_tra_driver.F90:298-450) and the update
 // values += (dttf_h + dttf_v)/hnode_new (src/fesom_analytic.F90:419-437)
 template <class R, int NT, bool LEARN>
-__global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__restrict__ ttf,
+__global__ __launch_bounds__(kMaxBlock) void k_node_update(DMesh<R> m, double *__restrict__ ttf,
                                                         const double *__restrict__ LO, size_t tsn,
                                                         const R *__restrict__ adf_h, size_t tse,
                                                         const R *__restrict__ adf_v,
@@ -696,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(DMesh<R> m, double *__re
 
 // lim = 'NONE': HO fluxes straight into the divergence (driver :291)
 template <class R, int VER, int NT>
-__global__ __launch_bounds__(kBlock) void k_node_nolim(DMesh<R> m, const R *__restrict__ WEv,
+__global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *__restrict__ WEv,
                                                        double *__restrict__ ttf,
                                                        const double *__restrict__ ttfAB, size_t tsn,
                                                        const R *__restrict__ flux_h, size_t tse,
@@ -848,7 +849,7 @@ __device__ __forceinline__ LearnSrc<R> learn_src(const LearnArgs<R, NT> &a, int 
 // training rows: features (bf16, 16 per row) and Zalesak target of every
 // edge-level cell of every tracer; padding cells get target -1
 template <class R, int NT>
-__global__ __launch_bounds__(kBlock) void k_learn_dump(DMesh<R> m, LearnArgs<R, NT> a, double dt,
+__global__ __launch_bounds__(kMaxBlock) void k_learn_dump(DMesh<R> m, LearnArgs<R, NT> a, double dt,
                                                        unsigned short *__restrict__ X, float *__restrict__ Y,
                                                        long long row0)
 {
@@ -936,7 +937,7 @@ __global__ __launch_bounds__(kBlock) void k_learn_alpha(DMesh<R> m, MlpDev w, Le
 // own sufficient condition, positive and negative contributions apart:
 // sum(max(0, c)) * dt / areasvol <= Q+ and sum(min(0, c)) * dt / areasvol >= Q-
 template <class R, int NT>
-__global__ __launch_bounds__(kBlock) void k_learn_check(DMesh<R> m, const R *__restrict__ adf_h,
+__global__ __launch_bounds__(kMaxBlock) void k_learn_check(DMesh<R> m, const R *__restrict__ adf_h,
                                                         const R *__restrict__ alpha, size_t tse,
                                                         const R *__restrict__ adf_v, const R *__restrict__ Rp,
                                                         const R *__restrict__ Rm, const R *__restrict__ Qp,
@@ -1133,9 +1134,9 @@ T *dupload(const std::vector<T> &h)
     return d;
 }
 
-inline unsigned grid_for(int nchunks)
+inline unsigned grid_for(int nchunks, int waves = kWavesPerBlock)
 {
-    unsigned nb = (unsigned)((nchunks + kWavesPerBlock - 1) / kWavesPerBlock);
+    unsigned nb = (unsigned)((nchunks + waves - 1) / waves);
     nb = nb ? nb : 1u;
     return (nb + 7u) & ~7u;  // multiple of 8 for the XCD remap
 }
@@ -1226,6 +1227,8 @@ struct Engine::Impl {
     float gscale = 1.0f, ginv = 1.0f;
     size_t tsn = 0, tst = 0, tse = 0;
     int wslots = 1;  // tracers the work arrays hold
+    int tune_waves = kWavesPerBlock;   // waves per block of the chunk kernels
+    int tune_nt = 4;                   // most tracers per launch
     // host copy of each tracer as last set (levels below the bottom are never
     // touched by a step, so get_values and the all-values statistics read them here)
     std::vector<std::vector<double>> host_vals;
@@ -1595,7 +1598,8 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
     double *LO = I.LO;
     G *trxy = (G *)I.trxy;
     G *gnod = (G *)I.gnod;
-    const dim3 blk(kBlock), ge(grid_for(m.nch_e)), gn(grid_for(m.nch_n)), gt(grid_for(m.nch_t));
+    const int wv = I.tune_waves;
+    const dim3 blk(kWave * wv), ge(grid_for(m.nch_e, wv)), gn(grid_for(m.nch_n, wv)), gt(grid_for(m.nch_t, wv));
     const bool learned = sp.lim == LIM_LEARNED;
     const bool fct = sp.lim == LIM_FCT || learned;
     Engine::Impl::Dump *dump = I.dump;
@@ -1668,7 +1672,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
             const MlpDev w = mlp_dev(I);
             const int gb = std::min((m.nch_e + kWavesPerBlock - 1) / kWavesPerBlock, 2048);
             const int iters = (m.nch_e + gb * kWavesPerBlock - 1) / (gb * kWavesPerBlock);
-            hipLaunchKernelGGL((k_learn_alpha<R, NT>), dim3(gb), blk, 0, st, m, w, la, dt, alpha, iters);
+            hipLaunchKernelGGL((k_learn_alpha<R, NT>), dim3(gb), dim3(kBlock), 0, st, m, w, la, dt, alpha, iters);
             hk.mark("k_learn_alpha", st);
             hipLaunchKernelGGL((k_learn_check<R, NT>), gn, blk, 0, st, m, adf_h, alpha, tse, adf_v, Pp, Pm, Qp, Qm,
                                tsn, dt, unsafe);
@@ -1699,16 +1703,24 @@ static bool same_spec(const TracerSpec &a, const TracerSpec &b)
     return a.hor == b.hor && a.ver == b.ver && a.lim == b.lim && a.ph == b.ph && a.pv == b.pv;
 }
 
+// tracers in the next launch of a run of identical schemes (4, 2 or 1, at most max_nt)
+static int batch_size(int run, int max_nt)
+{
+    if (run >= 4 && max_nt >= 4) return 4;
+    if (run >= 2 && max_nt >= 2) return 2;
+    return 1;
+}
+
 // position of tracer tr in the work arrays: its index within the batch that
-// launch_range puts it in (runs of identical schemes split into 4/2/1)
-static int work_slot(const std::vector<TracerSpec> &spec, int tr)
+// launch_range puts it in
+static int work_slot(const std::vector<TracerSpec> &spec, int tr, int max_nt)
 {
     const int n = (int)spec.size();
     for (int t = 0; t < n;) {
         int run = 1;
         while (t + run < n && same_spec(spec[t], spec[t + run])) ++run;
         while (run > 0) {
-            const int nb = run >= 4 ? 4 : (run >= 2 ? 2 : 1);
+            const int nb = batch_size(run, max_nt);
             if (tr >= t && tr < t + nb) return tr - t;
             t += nb;
             run -= nb;
@@ -1729,9 +1741,12 @@ static void launch_range(Engine::Impl &I, const DMesh<R> &m, const std::vector<T
         while (t + run < end && same_spec(spec[t], spec[t + run])) ++run;
         while (run > 0) {
             hk.mark("start", st);
-            if (run >= 4) { launch_batch<R, G, 4>(I, m, spec[t], dt, t, refresh, st, hk); t += 4; run -= 4; }
-            else if (run >= 2) { launch_batch<R, G, 2>(I, m, spec[t], dt, t, refresh, st, hk); t += 2; run -= 2; }
-            else { launch_batch<R, G, 1>(I, m, spec[t], dt, t, refresh, st, hk); t += 1; run -= 1; }
+            const int nb = batch_size(run, I.tune_nt);
+            if (nb == 4) launch_batch<R, G, 4>(I, m, spec[t], dt, t, refresh, st, hk);
+            else if (nb == 2) launch_batch<R, G, 2>(I, m, spec[t], dt, t, refresh, st, hk);
+            else launch_batch<R, G, 1>(I, m, spec[t], dt, t, refresh, st, hk);
+            t += nb;
+            run -= nb;
         }
     }
 }
@@ -1934,7 +1949,7 @@ double Engine::learn_fallback(int tr)
     if (!I.unsafe || tr < 0 || tr >= ntr_) return 0.0;
     HIP_OK(hipMemsetAsync(I.cnt, 0, 2 * sizeof(unsigned long long), stream_));
     const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
-    const char *u = (const char *)I.unsafe + work_slot(spec_, tr) * I.tsn * rs;
+    const char *u = (const char *)I.unsafe + work_slot(spec_, tr, p_->tune_nt) * I.tsn * rs;
     if (prec_ == PREC_FP64)
         hipLaunchKernelGGL((k_count_flags<double>), dim3(256), dim3(256), 0, stream_, I.m64, (const double *)u, I.cnt);
     else
@@ -1944,6 +1959,71 @@ double Engine::learn_fallback(int tr)
     HIP_OK(hipMemcpyAsync(h, I.cnt, sizeof(h), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
     return h[1] ? (double)h[0] / (double)h[1] : 0.0;
+}
+
+void Engine::set_schedule(int waves, int max_nt)
+{
+    if (waves != 1 && waves != 2 && waves != 4 && waves != 8) throw std::runtime_error("waves per block: 1, 2, 4 or 8");
+    if (max_nt != 1 && max_nt != 2 && max_nt != 4) throw std::runtime_error("tracers per launch: 1, 2 or 4");
+    p_->tune_waves = waves;
+    p_->tune_nt = max_nt;
+    graph_valid_ = false;
+}
+
+void Engine::schedule(int &waves, int &max_nt) const
+{
+    waves = p_->tune_waves;
+    max_nt = p_->tune_nt;
+}
+
+std::vector<std::array<double, 3>> Engine::autotune(double dt, int reps)
+{
+    Impl &I = *p_;
+    // the tracers are restored afterwards: the trial steps leave no trace
+    const size_t nv = I.tsn * ntr_;
+    double *save = dalloc<double>(nv), *save_ab = I.valuesAB ? dalloc<double>(nv) : nullptr;
+    HIP_OK(hipStreamSynchronize(stream_));
+    HIP_OK(hipMemcpy(save, I.values, nv * sizeof(double), hipMemcpyDeviceToDevice));
+    if (save_ab) HIP_OK(hipMemcpy(save_ab, I.valuesAB, nv * sizeof(double), hipMemcpyDeviceToDevice));
+    const int w0 = I.tune_waves, n0 = I.tune_nt;
+    std::vector<std::array<double, 3>> table;
+    double best = 1e300;
+    int bw = w0, bn = n0;
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    try {
+        for (int nt : {4, 2, 1}) {
+            if (nt > std::max(1, std::min(ntr_, kMaxBatch))) continue;
+            for (int w : {2, 4, 8}) {
+                set_schedule(w, nt);
+                run_steps(1, dt, true);  // capture + warm-up
+                HIP_OK(hipEventRecord(a, stream_));
+                run_steps(reps, dt, true);
+                HIP_OK(hipEventRecord(b, stream_));
+                HIP_OK(hipEventSynchronize(b));
+                float ms = 0.0f;
+                HIP_OK(hipEventElapsedTime(&ms, a, b));
+                const double per = (double)ms / std::max(reps, 1);
+                table.push_back({(double)w, (double)nt, per});
+                if (per < best) { best = per; bw = w; bn = nt; }
+            }
+        }
+    } catch (...) {
+        set_schedule(w0, n0);
+        (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+        (void)hipFree(save); if (save_ab) (void)hipFree(save_ab);
+        throw;
+    }
+    HIP_OK(hipEventDestroy(a));
+    HIP_OK(hipEventDestroy(b));
+    set_schedule(bw, bn);
+    HIP_OK(hipStreamSynchronize(stream_));
+    HIP_OK(hipMemcpy(I.values, save, nv * sizeof(double), hipMemcpyDeviceToDevice));
+    if (save_ab) HIP_OK(hipMemcpy(I.valuesAB, save_ab, nv * sizeof(double), hipMemcpyDeviceToDevice));
+    HIP_OK(hipFree(save));
+    if (save_ab) HIP_OK(hipFree(save_ab));
+    return table;
 }
 
 void Engine::synchronize() { HIP_OK(hipStreamSynchronize(stream_)); }
@@ -2031,7 +2111,7 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
         src = (const char *)I.gnod; comps = 2;
         es = prec_ == PREC_FP64 ? 8 : (prec_ == PREC_FP32 ? 4 : 2);
     }
-    const int slot = work_slot(spec_, tr);
+    const int slot = work_slot(spec_, tr, p_->tune_nt);
     const size_t per = (size_t)lay->total * comps;
     std::vector<char> buf(per * es);
     HIP_OK(hipMemcpy(buf.data(), src + slot * per * es, buf.size(), hipMemcpyDeviceToHost));

@@ -32,8 +32,9 @@
 namespace fta {
 
 constexpr int kWave = 64;
-constexpr int kWavesPerBlock = 4;
+constexpr int kWavesPerBlock = 4;                  // default launch shape (and the fixed one of LDS kernels)
 constexpr int kBlock = kWave * kWavesPerBlock;
+constexpr int kMaxBlock = 512;                     // chunk kernels take 64..512 threads (autotuned)
 constexpr int kChunk = 2 * kWave;   // cells per wave chunk
 constexpr int kDeg = 8;             // padded node degree (incident edges / elements / 1-ring)
 constexpr int kColBits = 26;        // slot word: column | pair index << kColBits
@@ -115,7 +116,7 @@ __device__ __forceinline__ int wave_chunk()
     // contiguous block range [x*per, (x+1)*per) (gridDim.x is a multiple of 8)
     const unsigned nb = gridDim.x, b = blockIdx.x, per = nb >> 3;
     const unsigned lb = (b & 7u) * per + (b >> 3);
-    int ch = (int)(lb * kWavesPerBlock + (threadIdx.x >> 6));
+    int ch = (int)(lb * (blockDim.x >> 6) + (threadIdx.x >> 6));
     return __builtin_amdgcn_readfirstlane(ch);
 }
 

@@ -33,6 +33,7 @@ struct State {
     std::unique_ptr<Engine> eng;
     int precision = 64, refresh = 1, use_graph = 1;
     std::vector<float> mlp;  // learned-limiter parameters carried across engine rebuilds
+    int tune_waves = 4, tune_nt = 4;  // launch schedule carried across engine rebuilds
 };
 
 State &S()
@@ -81,6 +82,7 @@ int ensure_engine()
         s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
         s.eng->set_refresh_ab(s.refresh != 0);
         if (!s.mlp.empty()) s.eng->mlp_set(s.mlp.data());
+        s.eng->set_schedule(s.tune_waves, s.tune_nt);
         for (int t = 0; t < s.ntr; ++t) {
             s.eng->set_spec(t, s.specs[t]);
             s.eng->set_values(t, s.values[t].data());
@@ -516,6 +518,15 @@ int tracer_set_option(const char *name, int value)
         }
         return 0;
     }
+    if (n == "block_waves" || n == "batch_tracers") {
+        const int w = n == "block_waves" ? value : s.tune_waves, b = n == "batch_tracers" ? value : s.tune_nt;
+        if (w != 1 && w != 2 && w != 4 && w != 8) return fail("block_waves must be 1, 2, 4 or 8");
+        if (b != 1 && b != 2 && b != 4) return fail("batch_tracers must be 1, 2 or 4");
+        s.tune_waves = w;
+        s.tune_nt = b;
+        if (s.eng) s.eng->set_schedule(w, b);
+        return 0;
+    }
     if (n == "use_graph") {
         s.use_graph = value ? 1 : 0;
         return 0;
@@ -774,6 +785,22 @@ int tracer_learn_fallback(int tracer_id, double *fraction)
     if (ensure_engine()) return -1;
     try { *fraction = s.eng->learn_fallback(tracer_id - 1); } catch (const std::exception &e) { return fail(e.what()); }
     return 0;
+}
+
+int tracer_autotune(double dt, int reps, double *table, int max_rows)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (ensure_engine()) return -1;
+    try {
+        auto t = s.eng->autotune(dt, reps);
+        for (size_t i = 0; i < t.size() && (int)i < max_rows; ++i)
+            for (int j = 0; j < 3; ++j) table[3 * i + j] = t[i][j];
+        s.eng->schedule(s.tune_waves, s.tune_nt);
+        return (int)t.size();
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
 }
 
 const char *tracer_last_error(void) { return last_error(); }

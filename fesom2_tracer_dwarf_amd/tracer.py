@@ -151,6 +151,15 @@ class FESOMTracerAdvection:
                    "tracer_precision_error", self.lib)
         return out.reshape(nsteps, ntr, 4)
 
+    def autotune(self, dt, reps=10):
+        """Pick the fastest launch schedule on the current state (tracers are restored)."""
+        table = np.zeros(3 * 32)
+        n = self.lib.tracer_autotune(float(dt), int(reps), table, 32)
+        if n < 0:
+            raise RuntimeError(f"tracer_autotune failed: {_lib.last_error(self.lib)}")
+        rows = table[:3 * n].reshape(n, 3)
+        return [dict(block_waves=int(w), batch_tracers=int(b), ms_per_step=float(ms)) for w, b, ms in rows]
+
     # ---- learned limiter
     def mlp_param_count(self):
         return int(self.lib.tracer_mlp_param_count())

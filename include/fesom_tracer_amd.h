@@ -153,8 +153,10 @@ int tracer_get_values_id(int tracer_id, double *values, int nz, int nn);
 /* No reference counterpart. Options:
  *   "precision"          64 (default), 32, 16 (FP32 with FP16 gradients)
  *   "refresh_valuesAB"   1 (default): valuesAB := values each step; 0: as set
- *   "accumulate_del_ttf" 0 (default); 1: never zero del_ttf (reference C ABI)
- *   "use_graph"          1 (default): replay multi-step runs from a hipGraph */
+ *   "accumulate_del_ttf" 0 (default); 1 is refused (the device path zeroes del_ttf each step)
+ *   "use_graph"          1 (default): replay multi-step runs from a hipGraph
+ *   "block_waves"        4 (default): waves per workgroup of the chunk kernels (1, 2, 4, 8)
+ *   "batch_tracers"      4 (default): most tracers per launch (1, 2, 4) */
 int tracer_set_option(const char *name, int value);
 
 /* No reference counterpart: nsteps steps of ALL tracers */
@@ -234,6 +236,14 @@ int tracer_batch_time(int nsteps, double *ms_out);
 int tracer_batch_mesh_info(int mesh, int *nl, int *nod2D, int *ntracers, long long *active_cells, double *dt);
 int tracer_batch_get_values(int mesh, int tracer_id, double *values);
 int tracer_batch_diagnostics(int mesh, int tracer_id, double *out);
+
+/* No reference counterpart: the autotuner. Times every launch schedule of
+ * the chunk kernels (waves per workgroup 2, 4, 8 x tracers per launch 4, 2,
+ * 1) on reps graph-replayed steps of the current state, keeps the fastest
+ * and restores the tracers. table[3*i..] = {waves, tracers per launch,
+ * ms per step}; returns the number of rows (or -1). The schedule can also be
+ * set with tracer_set_option("block_waves" | "batch_tracers", value). */
+int tracer_autotune(double dt, int reps, double *table, int max_rows);
 
 /* No reference counterpart: wall time of nsteps steps of all tracers on the
  * library's stream (hipEvent), milliseconds */

@@ -142,3 +142,28 @@ def test_diagnostics_match_host(api):
     st = api.get_statistics()
     g1 = api.get_tracer_values(1)
     assert st["min"] == g1.min() and st["max"] == g1.max()
+
+
+def test_schedule_does_not_change_results_and_autotune_restores(api):
+    """Every launch schedule gives bitwise the same tracers; the autotuner picks one and restores the state."""
+    print("[gpu] schedules and autotuner", flush=True)
+    d, dt, trs = setup_case(api, MESHES["periodic_24x20x12_random_bottom"], ("MUSCL", "QR4C", "FCT"), ntr=5)
+    api.set_option("block_waves", 4)
+    api.set_option("batch_tracers", 4)
+    api.advect_steps(2, dt)
+    ref = [api.get_tracer_values(i + 1) for i in range(5)]
+    for w, b in ((2, 1), (8, 2), (1, 4)):
+        for i, t in enumerate(trs):
+            api.set_values(i + 1, t)
+        api.set_option("block_waves", w)
+        api.set_option("batch_tracers", b)
+        api.advect_steps(2, dt)
+        for i in range(5):
+            assert np.array_equal(api.get_tracer_values(i + 1), ref[i]), f"schedule {w},{b} tracer {i + 1}"
+    before = [api.get_tracer_values(i + 1) for i in range(5)]
+    table = api.autotune(dt, reps=3)
+    assert len(table) == 9
+    for i in range(5):
+        assert np.array_equal(api.get_tracer_values(i + 1), before[i])
+    api.set_option("block_waves", 4)
+    api.set_option("batch_tracers", 4)
