@@ -193,8 +193,9 @@ int tracer_precision_error(int ref_precision, int nsteps, double dt, double *out
  * sigmoid; bf16 weights on the matrix cores, FP32 accumulation) predicts the
  * limiter coefficient of every horizontal edge level from its LO/HO fluxes,
  * node values, Zalesak headroom and cell geometry; the coefficient is applied
- * in flux form (mass exact) and every edge of a node level whose bounds the
- * prediction would break takes Zalesak's coefficient instead.
+ * in flux form (mass exact); every edge of a node level whose bounds the
+ * prediction would break takes the smaller of the network's and Zalesak's
+ * coefficients, which keeps both of its nodes inside their bounds.
  * Parameters are FP32, flattened as W1[10][64], b1[64], W2[64][64], b2[64],
  * w3[64], b3 (tracer_mlp_param_count() values). */
 int tracer_mlp_param_count(void);

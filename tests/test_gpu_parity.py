@@ -167,3 +167,27 @@ def test_schedule_does_not_change_results_and_autotune_restores(api):
         assert np.array_equal(api.get_tracer_values(i + 1), before[i])
     api.set_option("block_waves", 4)
     api.set_option("batch_tracers", 4)
+
+
+def test_stale_valuesAB_matches_oracle(api):
+    """The reference's stale-AB mode (valuesAB supplied once, never refreshed):
+    the HO fluxes read the caller's valuesAB on every step, as the oracle does."""
+    import oracle as orc
+    print("[gpu] stale valuesAB vs oracle", flush=True)
+    d, dt, trs = setup_case(api, MESHES["periodic_24x20x12_random_bottom"], ("MUSCL", "QR4C", "FCT"), ntr=2)
+    m = mesh_dict(api)
+    o = orc.Oracle(m)
+    uv, w, we = api.mesh_array("uv"), api.mesh_array("w"), api.mesh_array("w_e")
+    rng = np.random.default_rng(11)
+    ab = [np.asfortranarray(t + 0.05 * (np.max(t) - np.min(t)) * rng.standard_normal(t.shape)) for t in trs]
+    api.set_option("refresh_valuesAB", 0)
+    for i in range(2):
+        api.set_values_ab(i + 1, ab[i])
+    ref = [np.array(t, order="F", copy=True) for t in trs]
+    for step in range(3):
+        api.advect_steps(1, dt)
+        for i in range(2):
+            o.step(dt, uv, w, we, ref[i], valuesAB=ab[i], hor="MUSCL", ver="QR4C", lim="FCT")
+            err = rel_err(api.get_tracer_values(i + 1), ref[i])
+            assert err <= RTOL, f"tracer {i + 1} step {step + 1}: rel err {err:.3e}"
+    api.set_option("refresh_valuesAB", 1)

@@ -28,6 +28,39 @@ def test_bf16_rounding_bits():
     assert abs(r[5]) < 1e-39
 
 
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    x = np.random.default_rng(5).standard_normal(4096).astype(np.float32) * np.float32(1e3)
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(mlp.bf16(x), ref)
+
+
+def test_forward_closed_form():
+    """A network wired by hand: hidden unit 0 carries relu(2*x0 - 1), unit 1 carries relu(-x3);
+    layer 2 passes them through with gains 1 and 3; the output is sigmoid(w.a2 + b3).
+    All weights are bf16-exact, so the rounded and the exact forward agree with the formula."""
+    p = np.zeros(mlp.N_PARAMS, dtype=np.float32)
+    W1 = np.zeros((10, 64), np.float32)
+    W1[0, 0], W1[3, 1] = 2.0, -1.0
+    b1 = np.zeros(64, np.float32)
+    b1[0] = -1.0
+    W2 = np.zeros((64, 64), np.float32)
+    W2[0, 0], W2[1, 1] = 1.0, 3.0
+    w3 = np.zeros(64, np.float32)
+    w3[0], w3[1] = 0.5, -0.25
+    for k, v in (("W1", W1), ("b1", b1), ("W2", W2), ("w3", w3)):
+        p[mlp.OFF[k][0]:mlp.OFF[k][1]] = v.ravel()
+    p[mlp.OFF["b3"][0]] = 0.125
+    X = np.zeros((5, 10))
+    X[:, 0] = [0.0, 0.5, 1.0, 2.0, -1.0]
+    X[:, 3] = [1.0, -2.0, -0.5, 0.0, -4.0]
+    z = 0.5 * np.maximum(2 * X[:, 0] - 1, 0) - 0.25 * 3 * np.maximum(-X[:, 3], 0) + 0.125
+    want = 1.0 / (1.0 + np.exp(-z))
+    for rnd in (mlp.bf16, mlp.exact):
+        got, _ = mlp.forward(p, X, rnd)
+        assert np.allclose(got, want, rtol=0, atol=1e-15)
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_gradient_matches_finite_differences(seed):
     rng = np.random.default_rng(seed)
