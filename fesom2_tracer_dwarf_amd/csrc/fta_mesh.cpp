@@ -4,7 +4,10 @@
 #include "fta_mesh.h"
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 #include <stdexcept>
 #include <unordered_map>
@@ -241,26 +244,218 @@ void set_levels_from_elements(HostMesh &m, const int *nlevels_elem)
     node_levels_from_elements(m);
 }
 
-void derive_all(HostMesh &m)
+// ---------------------------------------------------------------------------
+// FESOM mesh files
+// ---------------------------------------------------------------------------
+void rotation_matrix(double al, double be, double ga, double M[9])
 {
-    const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
-    if (nl < 3) throw std::runtime_error("vertical grid not set");
-    if ((int)m.nlevels.size() != T) throw std::runtime_error("levels not set");
+    // set_mesh_transform_matrix, lib/gen_modules_rotate_grid.F90:33-56
+    M[0] = std::cos(ga) * std::cos(al) - std::sin(ga) * std::cos(be) * std::sin(al);
+    M[1] = std::cos(ga) * std::sin(al) + std::sin(ga) * std::cos(be) * std::cos(al);
+    M[2] = std::sin(ga) * std::sin(be);
+    M[3] = -std::sin(ga) * std::cos(al) - std::cos(ga) * std::cos(be) * std::sin(al);
+    M[4] = -std::sin(ga) * std::sin(al) + std::cos(ga) * std::cos(be) * std::cos(al);
+    M[5] = std::cos(ga) * std::sin(be);
+    M[6] = std::sin(be) * std::sin(al);
+    M[7] = -std::sin(be) * std::cos(al);
+    M[8] = std::cos(be);
+}
 
-    // find_levels_min_e2n (lib/oce_mesh.F90:1522-1560)
-    m.nlevels_nod2D_min.assign(N, 0);
-    m.ulevels_nod2D_max.assign(N, 0);
-    for (int n = 0; n < N; ++n) {
-        int mn = 1 << 30, mx = 0;
-        for (int k = 0; k < m.nod_in_elem2D_num[n]; ++k) {
-            int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + k];
-            mn = std::min(mn, m.nlevels[el]);
-            mx = std::max(mx, m.ulevels[el]);
+void geo_to_rot(const double M[9], double glon, double glat, double &rlon, double &rlat)
+{
+    // g2r, lib/gen_modules_rotate_grid.F90:92-119: rotated = M * geographic
+    const double xg = std::cos(glat) * std::cos(glon), yg = std::cos(glat) * std::sin(glon), zg = std::sin(glat);
+    const double xr = M[0] * xg + M[1] * yg + M[2] * zg;
+    const double yr = M[3] * xg + M[4] * yg + M[5] * zg;
+    const double zr = M[6] * xg + M[7] * yg + M[8] * zg;
+    rlat = std::asin(zr);
+    rlon = (yr == 0.0 && xr == 0.0) ? 0.0 : std::atan2(yr, xr);
+}
+
+void rot_to_geo(const double M[9], double rlon, double rlat, double &glon, double &glat)
+{
+    // r2g, lib/gen_modules_rotate_grid.F90:60-88: geographic = M^T * rotated
+    const double xr = std::cos(rlat) * std::cos(rlon), yr = std::cos(rlat) * std::sin(rlon), zr = std::sin(rlat);
+    const double xg = M[0] * xr + M[3] * yr + M[6] * zr;
+    const double yg = M[1] * xr + M[4] * yr + M[7] * zr;
+    const double zg = M[2] * xr + M[5] * yr + M[8] * zr;
+    glat = std::asin(zg);
+    glon = (yg == 0.0 && xg == 0.0) ? 0.0 : std::atan2(yg, xg);
+}
+
+namespace {
+
+// whitespace-separated numbers of a text file, read as Fortran list-directed
+// input reads them
+class Tokens {
+  public:
+    Tokens(const std::string &path, bool required) : path_(path)
+    {
+        FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f) {
+            if (required) throw std::runtime_error("mesh file not found: " + path);
+            return;
         }
-        m.nlevels_nod2D_min[n] = mn;
-        m.ulevels_nod2D_max[n] = mx;
+        std::fseek(f, 0, SEEK_END);
+        const long n = std::ftell(f);
+        std::fseek(f, 0, SEEK_SET);
+        buf_.resize(n > 0 ? (size_t)n : 0);
+        const size_t got = n > 0 ? std::fread(&buf_[0], 1, (size_t)n, f) : 0;
+        std::fclose(f);
+        buf_.resize(got);
+        ok_ = true;
+    }
+    bool ok() const { return ok_; }
+    double real()
+    {
+        skip();
+        if (pos_ >= buf_.size()) throw std::runtime_error("unexpected end of " + path_);
+        const char *b = buf_.c_str() + pos_;
+        char *e = nullptr;
+        errno = 0;
+        const double v = std::strtod(b, &e);
+        if (e == b || errno == ERANGE) throw std::runtime_error("bad number in " + path_ + " near byte " + std::to_string(pos_));
+        pos_ += (size_t)(e - b);
+        return v;
+    }
+    int integer()
+    {
+        const double v = real();
+        if (v != std::floor(v) || std::fabs(v) > 2.0e9) throw std::runtime_error("expected an integer in " + path_);
+        return (int)v;
     }
 
+  private:
+    void skip()
+    {
+        while (pos_ < buf_.size() && (buf_[pos_] == ' ' || buf_[pos_] == '\t' || buf_[pos_] == '\n' ||
+                                      buf_[pos_] == '\r' || buf_[pos_] == ','))
+            ++pos_;
+    }
+    std::string path_, buf_;
+    size_t pos_ = 0;
+    bool ok_ = false;
+};
+
+}  // namespace
+
+void read_fesom_mesh(HostMesh &m, const std::string &dir_in, const FesomFileOptions &opt)
+{
+    const double PI = 3.14159265358979323846, rad = PI / 180.0;
+    const std::string dir = (!dir_in.empty() && dir_in.back() != '/') ? dir_in + "/" : dir_in;
+    m = HostMesh();
+    m.cartesian = false;
+    m.cyc_x = opt.cyclic_length_deg * rad * kREarth;
+    m.cyc_y = 0.0;
+
+    // nodes (read_mesh :220-313): index, lon, lat, boundary flag; degrees,
+    // rotated to the model grid
+    {
+        Tokens t(dir + "nod2d.out", true);
+        m.nod2D = t.integer();
+        if (m.nod2D < 3) throw std::runtime_error("nod2d.out: bad node count");
+        double M[9];
+        rotation_matrix(opt.alpha_deg * rad, opt.beta_deg * rad, opt.gamma_deg * rad, M);
+        m.coord_nod2D.assign(2 * (size_t)m.nod2D, 0.0);
+        for (int n = 0; n < m.nod2D; ++n) {
+            const int id = t.integer();
+            if (id != n + 1) throw std::runtime_error("nod2d.out: node " + std::to_string(n + 1) + " out of order");
+            double x = t.real() * rad, y = t.real() * rad;
+            (void)t.integer();
+            if (opt.force_rotation) {
+                const double gx = x, gy = y;
+                geo_to_rot(M, gx, gy, x, y);
+            }
+            m.coord_nod2D[2 * n] = x;
+            m.coord_nod2D[2 * n + 1] = y;
+        }
+        m.coord.resize(m.coord_nod2D.size());
+        for (size_t i = 0; i < m.coord.size(); ++i) m.coord[i] = m.coord_nod2D[i] * kREarth;
+    }
+
+    // elements (read_mesh :366-434), made clockwise (test_tri)
+    {
+        Tokens t(dir + "elem2d.out", true);
+        m.elem2D = t.integer();
+        if (m.elem2D < 1) throw std::runtime_error("elem2d.out: bad element count");
+        m.elem2D_nodes.assign(3 * (size_t)m.elem2D, 0);
+        for (size_t i = 0; i < m.elem2D_nodes.size(); ++i) {
+            const int v = t.integer();
+            if (v < 1 || v > m.nod2D) throw std::runtime_error("elem2d.out: node index out of range");
+            m.elem2D_nodes[i] = v - 1;
+        }
+        const double C = m.cyc_x / kREarth;
+        for (int el = 0; el < m.elem2D; ++el) {
+            int *en = &m.elem2D_nodes[3 * el];
+            const double bx = trim(m.coord_nod2D[2 * en[1]] - m.coord_nod2D[2 * en[0]], C);
+            const double by = m.coord_nod2D[2 * en[1] + 1] - m.coord_nod2D[2 * en[0] + 1];
+            const double cx = trim(m.coord_nod2D[2 * en[2]] - m.coord_nod2D[2 * en[0]], C);
+            const double cy = m.coord_nod2D[2 * en[2] + 1] - m.coord_nod2D[2 * en[0] + 1];
+            if (bx * cy - by * cx > 0.0) std::swap(en[1], en[2]);
+        }
+    }
+
+    // vertical grid (read_mesh :440-468): nl, zbar (made negative), Z
+    {
+        Tokens t(dir + "aux3d.out", true);
+        m.nl = t.integer();
+        if (m.nl < 3) throw std::runtime_error("aux3d.out: number of levels is less than 3");
+        m.zbar.assign(m.nl, 0.0);
+        for (int k = 0; k < m.nl; ++k) m.zbar[k] = t.real();
+        if (m.zbar[1] > 0.0)
+            for (double &z : m.zbar) z = -z;
+        m.Z.assign(m.nl - 1, 0.0);
+        for (int k = 0; k < m.nl - 1; ++k) m.Z[k] = 0.5 * (m.zbar[k] + m.zbar[k + 1]);
+    }
+
+    // edges (load_edges :1617-1790): edges.out, edge_tri.out (<= 0: none)
+    {
+        Tokens tn(dir + "edgenum.out", false), te(dir + "edges.out", false), tt(dir + "edge_tri.out", false);
+        if (tn.ok() && te.ok() && tt.ok()) {
+            m.edge2D = tn.integer();
+            if (m.edge2D < 1) throw std::runtime_error("edgenum.out: bad edge count");
+            m.edges.assign(2 * (size_t)m.edge2D, 0);
+            m.edge_tri.assign(2 * (size_t)m.edge2D, -1);
+            for (size_t i = 0; i < m.edges.size(); ++i) {
+                const int v = te.integer();
+                if (v < 1 || v > m.nod2D) throw std::runtime_error("edges.out: node index out of range");
+                m.edges[i] = v - 1;
+                const int w = tt.integer();
+                if (w > m.elem2D) throw std::runtime_error("edge_tri.out: element index out of range");
+                m.edge_tri[i] = w > 0 ? w - 1 : -1;
+            }
+            for (int e = 0; e < m.edge2D; ++e)
+                if (m.edge_tri[2 * e] < 0) throw std::runtime_error("edge_tri.out: edge without a first element");
+        } else {
+            build_edges(m);
+        }
+    }
+    build_nod_in_elem(m);
+
+    // levels (find_levels :807-1001): elvls.out per element, nlvls.out per node
+    {
+        Tokens te(dir + "elvls.out", true);
+        m.nlevels.assign(m.elem2D, 0);
+        for (int el = 0; el < m.elem2D; ++el) {
+            m.nlevels[el] = te.integer();
+            if (m.nlevels[el] < 2 || m.nlevels[el] > m.nl) throw std::runtime_error("elvls.out: level out of range");
+        }
+        node_levels_from_elements(m);
+        Tokens tn(dir + "nlvls.out", false);
+        if (tn.ok())
+            for (int n = 0; n < m.nod2D; ++n)
+                if (tn.integer() != m.nlevels_nod2D[n])
+                    throw std::runtime_error("nlvls.out: node " + std::to_string(n + 1) +
+                                             " is not the deepest of its elements");
+    }
+    derive_all(m);
+}
+
+// Geometry of a planar mesh in metres (the generator's space): orientation
+// check, mesh_areas and mesh_auxiliary_arrays with elem_cos = 1
+static void cartesian_geometry(HostMesh &m)
+{
+    const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
     // edge orientation check: edge_tri(1) must lie left of edges(1)->edges(2)
     auto centroid = [&](int el, double &cx, double &cy) {
         const int *en = &m.elem2D_nodes[3 * el];
@@ -336,6 +531,144 @@ void derive_all(HostMesh &m)
         g[4] = -dX31 * dfactor;
         g[5] = dX21 * dfactor;
     }
+
+}
+
+// Geometry of a mesh on the sphere, coord_nod2D in radians, x cyclic with
+// C = cyc_x / r_earth (0: not cyclic):
+//   elem_center / edge_center       lib/oce_mesh.F90:2001-2041
+//   mesh_areas                      lib/oce_mesh.F90:2043-2309 (no cavity)
+//   mesh_auxiliary_arrays           lib/oce_mesh.F90:2313-2534
+// elem_area and area accumulate in radians^2 and are scaled by r_earth^2
+// afterwards; elem_cos is the cosine of the element centre's latitude and
+// scales x differences inside an element; gradient_sca carries r_earth.
+static void spherical_geometry(HostMesh &m)
+{
+    const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
+    const double C = m.cyc_x / kREarth;
+    const double *X = m.coord_nod2D.data();
+    auto elem_center = [&](int el, double &cx, double &cy) {
+        const int *en = &m.elem2D_nodes[3 * el];
+        double ax[3] = {X[2 * en[0]], X[2 * en[1]], X[2 * en[2]]};
+        const double amin = std::min(ax[0], std::min(ax[1], ax[2]));
+        if (C > 0.0)
+            for (double &a : ax) {
+                if (a - amin >= C / 2.0) a -= C;
+                if (a - amin < -C / 2.0) a += C;
+            }
+        cx = (ax[0] + ax[1] + ax[2]) / 3.0;
+        cy = (X[2 * en[0] + 1] + X[2 * en[1] + 1] + X[2 * en[2] + 1]) / 3.0;
+    };
+    auto edge_center = [&](int n1, int n2, double &x, double &y) {
+        double a = X[2 * n1], b = X[2 * n2];
+        if (C > 0.0) {
+            if (a - b > C / 2.0) a -= C;
+            if (a - b < -C / 2.0) b -= C;
+        }
+        x = 0.5 * (a + b);
+        y = 0.5 * (X[2 * n1 + 1] + X[2 * n2 + 1]);
+    };
+
+    // edge orientation check: edge_tri(1) must lie left of edges(1)->edges(2)
+    for (int e = 0; e < E; ++e) {
+        const int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
+        const double ex = trim(X[2 * n2] - X[2 * n1], C), ey = X[2 * n2 + 1] - X[2 * n1 + 1];
+        double cx, cy;
+        elem_center(m.edge_tri[2 * e], cx, cy);
+        const double rx = trim(cx - X[2 * n1], C), ry = cy - X[2 * n1 + 1];
+        if (ex * ry - ey * rx <= 0.0) throw std::runtime_error("mesh: edge_tri(1) is not left of its edge");
+    }
+
+    // mesh_areas
+    m.elem_area.assign(T, 0.0);
+    for (int el = 0; el < T; ++el) {
+        const int *en = &m.elem2D_nodes[3 * el];
+        const double ay = std::cos((X[2 * en[0] + 1] + X[2 * en[1] + 1] + X[2 * en[2] + 1]) / 3.0);
+        double a0 = trim(X[2 * en[1]] - X[2 * en[0]], C), a1 = X[2 * en[1] + 1] - X[2 * en[0] + 1];
+        double b0 = trim(X[2 * en[2]] - X[2 * en[0]], C), b1 = X[2 * en[2] + 1] - X[2 * en[0] + 1];
+        a0 *= ay;
+        b0 *= ay;
+        m.elem_area[el] = 0.5 * std::fabs(a0 * b1 - b0 * a1);
+    }
+    m.area.assign((size_t)nl * N, 0.0);
+    m.areasvol.assign((size_t)nl * N, 0.0);
+    for (int n = 0; n < N; ++n)
+        for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) {
+            const int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
+            for (int nz = m.ulevels[el]; nz <= m.nlevels[el] - 1; ++nz)
+                m.area[(size_t)n * nl + nz - 1] += m.elem_area[el] / 3.0;
+        }
+    for (int n = 0; n < N; ++n)
+        for (int nz = m.ulevels_nod2D[n]; nz <= m.nlevels_nod2D[n] - 1; ++nz)
+            m.areasvol[(size_t)n * nl + nz - 1] = m.area[(size_t)n * nl + nz - 1];
+    for (double &a : m.elem_area) a = a * kREarth * kREarth;
+    for (double &a : m.area) a = a * kREarth * kREarth;
+    for (double &a : m.areasvol) a = a * kREarth * kREarth;
+
+    // mesh_auxiliary_arrays
+    std::vector<double> center_x(T), center_y(T);
+    m.elem_cos.assign(T, 1.0);
+    for (int el = 0; el < T; ++el) {
+        elem_center(el, center_x[el], center_y[el]);
+        m.elem_cos[el] = std::cos(center_y[el]);
+    }
+    m.edge_dxdy.assign(2 * (size_t)E, 0.0);
+    m.edge_cross_dxdy.assign(4 * (size_t)E, 0.0);
+    for (int e = 0; e < E; ++e) {
+        const int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
+        m.edge_dxdy[2 * e] = trim(X[2 * n2] - X[2 * n1], C);
+        m.edge_dxdy[2 * e + 1] = X[2 * n2 + 1] - X[2 * n1 + 1];
+        double ax, ay;
+        edge_center(n1, n2, ax, ay);
+        for (int side = 0; side < 2; ++side) {
+            const int el = m.edge_tri[2 * e + side];
+            if (el < 0) continue;
+            double b0 = trim(center_x[el] - ax, C), b1 = center_y[el] - ay;
+            b0 = b0 * m.elem_cos[el];
+            m.edge_cross_dxdy[4 * e + 2 * side] = b0 * kREarth;
+            m.edge_cross_dxdy[4 * e + 2 * side + 1] = b1 * kREarth;
+        }
+    }
+    m.gradient_sca.assign(6 * (size_t)T, 0.0);
+    for (int el = 0; el < T; ++el) {
+        const int *en = &m.elem2D_nodes[3 * el];
+        const double dX31 = m.elem_cos[el] * trim(X[2 * en[2]] - X[2 * en[0]], C);
+        const double dX21 = m.elem_cos[el] * trim(X[2 * en[1]] - X[2 * en[0]], C);
+        const double dY31 = X[2 * en[2] + 1] - X[2 * en[0] + 1];
+        const double dY21 = X[2 * en[1] + 1] - X[2 * en[0] + 1];
+        const double dfactor = -0.5 * kREarth / m.elem_area[el];
+        double *g = &m.gradient_sca[6 * el];
+        g[0] = (-dY31 + dY21) * dfactor;
+        g[1] = dY31 * dfactor;
+        g[2] = -dY21 * dfactor;
+        g[3] = (dX31 - dX21) * dfactor;
+        g[4] = -dX31 * dfactor;
+        g[5] = dX21 * dfactor;
+    }
+}
+
+void derive_all(HostMesh &m)
+{
+    const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
+    if (nl < 3) throw std::runtime_error("vertical grid not set");
+    if ((int)m.nlevels.size() != T) throw std::runtime_error("levels not set");
+
+    // find_levels_min_e2n (lib/oce_mesh.F90:1522-1560)
+    m.nlevels_nod2D_min.assign(N, 0);
+    m.ulevels_nod2D_max.assign(N, 0);
+    for (int n = 0; n < N; ++n) {
+        int mn = 1 << 30, mx = 0;
+        for (int k = 0; k < m.nod_in_elem2D_num[n]; ++k) {
+            int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + k];
+            mn = std::min(mn, m.nlevels[el]);
+            mx = std::max(mx, m.ulevels[el]);
+        }
+        m.nlevels_nod2D_min[n] = mn;
+        m.ulevels_nod2D_max[n] = mx;
+    }
+
+    if (m.cartesian) cartesian_geometry(m);
+    else spherical_geometry(m);
 
     // ALE thicknesses and 3-D depth arrays (src/fesom_analytic.F90:226-244)
     m.hnode.assign((size_t)(nl - 1) * N, 0.0);
@@ -438,14 +771,45 @@ double make_velocity(const HostMesh &m, double u_bg, double v_bg, double cell_am
     Href /= T;
     // transport streamfunction: rotation cells; closed meshes get psi = 0 on the rim
     std::vector<double> psi(N, 0.0);
-    double xm = 0, ym = 0;
-    for (int q = 0; q < N; ++q) { xm = std::max(xm, m.coord[2 * q]); ym = std::max(ym, m.coord[2 * q + 1]); }
-    for (int n = 0; n < N; ++n) {
-        double x = m.coord[2 * n], y = m.coord[2 * n + 1];
-        double sx, sy;
-        if (m.cyc_x > 0) { sx = std::sin(2.0 * PI * x / Lx); sy = std::sin(2.0 * PI * y / Ly); }
-        else { sx = std::sin(PI * x / xm); sy = std::sin(PI * y / ym); }
-        psi[n] = cell_amp * Href * (Lx / (2.0 * PI)) * sx * sy;
+    if (m.cartesian) {
+        double xm = 0, ym = 0;
+        for (int q = 0; q < N; ++q) { xm = std::max(xm, m.coord[2 * q]); ym = std::max(ym, m.coord[2 * q + 1]); }
+        for (int n = 0; n < N; ++n) {
+            double x = m.coord[2 * n], y = m.coord[2 * n + 1];
+            double sx, sy;
+            if (m.cyc_x > 0) { sx = std::sin(2.0 * PI * x / Lx); sy = std::sin(2.0 * PI * y / Ly); }
+            else { sx = std::sin(PI * x / xm); sy = std::sin(PI * y / ym); }
+            psi[n] = cell_amp * Href * (Lx / (2.0 * PI)) * sx * sy;
+        }
+    } else {
+        // sphere: a zonal current u_bg*cos(lat) plus wave-2 cells, tapered to
+        // psi = 0 on every coast (graph distance 0..3 from a boundary node) so
+        // that no transport crosses a coast and the column divergence vanishes
+        std::vector<int> hop(N, 1 << 20), queue;
+        for (int e = 0; e < E; ++e)
+            if (m.edge_tri[2 * e + 1] < 0)
+                for (int q = 0; q < 2; ++q)
+                    if (hop[m.edges[2 * e + q]] != 0) {
+                        hop[m.edges[2 * e + q]] = 0;
+                        queue.push_back(m.edges[2 * e + q]);
+                    }
+        for (size_t h = 0; h < queue.size(); ++h) {
+            const int n = queue[h];
+            for (int i = m.n2e_ptr[n]; i < m.n2e_ptr[n + 1]; ++i) {
+                const int e = std::abs(m.n2e[i]) - 1;
+                const int o = m.edges[2 * e] == n ? m.edges[2 * e + 1] : m.edges[2 * e];
+                if (hop[o] > hop[n] + 1) {
+                    hop[o] = hop[n] + 1;
+                    queue.push_back(o);
+                }
+            }
+        }
+        for (int n = 0; n < N; ++n) {
+            const double lon = m.coord_nod2D[2 * n], lat = m.coord_nod2D[2 * n + 1];
+            const double taper = std::min(1.0, hop[n] / 3.0);
+            psi[n] = taper * Href * kREarth *
+                     (-u_bg * std::sin(lat) + cell_amp * std::cos(lat) * std::cos(lat) * std::sin(2.0 * lon));
+        }
     }
     uv.assign(2 * (size_t)L * T, 0.0);
     double umax = 0.0;
@@ -454,7 +818,7 @@ double make_velocity(const HostMesh &m, double u_bg, double v_bg, double cell_am
         const double *g = &m.gradient_sca[6 * el];
         double px = g[0] * psi[en[0]] + g[1] * psi[en[1]] + g[2] * psi[en[2]];
         double py = g[3] * psi[en[0]] + g[4] * psi[en[1]] + g[5] * psi[en[2]];
-        double bg = (m.cyc_x > 0) ? Href : 0.0;  // a uniform current only on periodic meshes
+        double bg = (m.cartesian && m.cyc_x > 0) ? Href : 0.0;  // a uniform current only on periodic planes
         double tx = u_bg * bg - py, ty = v_bg * bg + px;
         int nk = m.nlevels[el] - 1;
         // vertical shear profile with zero thickness-weighted mean
@@ -507,8 +871,10 @@ double make_velocity(const HostMesh &m, double u_bg, double v_bg, double cell_am
         wc[0] = 0.0;  // rigid lid: the column-integrated divergence vanishes
     }
     double dxmin = 1e300;
-    for (int e = 0; e < E; ++e)
-        dxmin = std::min(dxmin, std::hypot(m.edge_dxdy[2 * e], m.edge_dxdy[2 * e + 1]) * kREarth);
+    for (int e = 0; e < E; ++e) {
+        const double c = m.elem_cos[m.edge_tri[2 * e]];  // 1 on planes
+        dxmin = std::min(dxmin, std::hypot(m.edge_dxdy[2 * e] * c, m.edge_dxdy[2 * e + 1]) * kREarth);
+    }
     return umax > 0.0 ? cfl * dxmin / umax : 0.0;
 }
 

@@ -20,9 +20,9 @@ struct HostMesh {
     int nl = 0;        // number of level interfaces (layers = nl-1)
     int nod2D = 0, elem2D = 0, edge2D = 0, maxAdj = 0;
     bool cartesian = true;
-    double cyc_x = 0.0, cyc_y = 0.0;  // periodic lengths in metres (0 = closed)
+    double cyc_x = 0.0, cyc_y = 0.0;  // periodic lengths in metres (0 = closed); spherical: cyclic_length * r_earth
 
-    std::vector<double> coord;        // (2,N) metres (cartesian) -- generator space
+    std::vector<double> coord;        // (2,N) metres: generator space; spherical meshes: coord_nod2D * r_earth
     std::vector<double> coord_nod2D;  // (2,N) radians, as FESOM stores it
     std::vector<int> elem2D_nodes;    // (3,T), clockwise
     std::vector<int> edges;           // (2,E)
@@ -59,6 +59,29 @@ void generate_periodic(HostMesh &m, int nx, int ny, double Lx, double Ly, int or
 
 // Closed (non-periodic) rectangle: boundary edges have edge_tri(2) = -1.
 void generate_closed(HostMesh &m, int nx, int ny, double Lx, double Ly, int order);
+
+// FESOM mesh files in directory dir (read_mesh, find_levels, test_tri,
+// load_edges: lib/oce_mesh.F90:98-806,807-1001,1565-1615,1617-1849) on one
+// rank: nod2d.out (lon/lat in degrees), elem2d.out, aux3d.out (nl, zbar;
+// the depth column is not needed once the levels are read), elvls.out and
+// nlvls.out, and edges.out / edge_tri.out / edgenum.out when present (built
+// from the elements otherwise). Node coordinates are rotated to the model
+// grid when force_rotation is set (Euler angles in degrees, converted to
+// radians as FESOM's namelist reader does); the x axis is cyclic with
+// cyclic_length_deg. The geometry is then spherical (cartesian = false).
+struct FesomFileOptions {
+    bool force_rotation = true;
+    double alpha_deg = 50.0, beta_deg = 15.0, gamma_deg = -90.0;  // lib/gen_modules_config.F90:85-88
+    double cyclic_length_deg = 360.0;                             // lib/gen_modules_config.F90:83
+};
+void read_fesom_mesh(HostMesh &m, const std::string &dir, const FesomFileOptions &opt);
+
+// Euler rotation of lib/gen_modules_rotate_grid.F90 (set_mesh_transform_matrix,
+// g2r, r2g), angles in radians; M is the 3x3 r2g matrix, row-major M[i][j] =
+// r2g_matrix(i+1, j+1).
+void rotation_matrix(double alpha, double beta, double gamma, double M[9]);
+void geo_to_rot(const double M[9], double glon, double glat, double &rlon, double &rlat);
+void rot_to_geo(const double M[9], double rlon, double rlat, double &glon, double &glat);
 
 // Vertical grid: zbar(1)=0, layer k thickness dz*(1+stretch*(k-1)/(nl-2)).
 void set_vertical_grid(HostMesh &m, int nl, double dz, double stretch);

@@ -34,6 +34,7 @@ struct State {
     int precision = 64, refresh = 1, use_graph = 1;
     std::vector<float> mlp;  // learned-limiter parameters carried across engine rebuilds
     int tune_waves = 4, tune_nt = 4;  // launch schedule carried across engine rebuilds
+    FesomFileOptions file_opt;         // mesh-file reader settings (force_rotation)
 };
 
 State &S()
@@ -403,9 +404,31 @@ int tracer_load_mesh_partition(const char *restart_dir)
 
 int tracer_load_mesh_from_files(const char *mesh_path)
 {
+    State &s = S();
     std::string p = mesh_path ? mesh_path : "";
     if (!dir_exists(p)) return fail("could not find mesh directory: " + p);
-    return fail("FESOM mesh-file reader is not part of this build: " + p);
+    try {
+        drop_engine();
+        HostMesh m;
+        read_fesom_mesh(m, p, s.file_opt);
+        s.mesh = std::move(m);
+    } catch (const std::exception &e) {
+        s.mesh_ready = false;
+        return fail(std::string("tracer_load_mesh_from_files: ") + e.what());
+    }
+    s.mesh_ready = true;
+    s.initialized = true;
+    adopt_mesh_dims();
+    s.ntr = 0;
+    s.values.clear();
+    s.valuesAB.clear();
+    s.specs.clear();
+    s.uv.clear();
+    s.w.clear();
+    s.we.clear();
+    ensure_host_arrays();
+    std::printf("Mesh loaded from files: nl=%8d, nod2D=%8d, edge2D=%8d\n", s.mesh.nl, s.mesh.nod2D, s.mesh.edge2D);
+    return 0;
 }
 
 int tracer_init_ale_arrays(void)
@@ -495,6 +518,10 @@ int tracer_set_option(const char *name, int value)
 {
     State &s = S();
     std::string n = name ? name : "";
+    if (n == "force_rotation") {  // mesh-file reader: rotate nod2d.out coordinates (default 1)
+        s.file_opt.force_rotation = value != 0;
+        return 0;
+    }
     if (n == "precision") {
         if (value != 64 && value != 32 && value != 16) return fail("precision must be 64, 32 or 16");
         if (s.eng) {

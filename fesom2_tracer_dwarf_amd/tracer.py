@@ -63,6 +63,13 @@ class FESOMTracerAdvection:
                                                  seed, int(sfc)), "tracer_generate_mesh", self.lib)
         return self.dims()
 
+    def load_mesh_from_files(self, mesh_path):
+        """FESOM mesh files (nod2d.out, elem2d.out, aux3d.out, elvls.out,
+        nlvls.out, edges.out, edge_tri.out, edgenum.out) from a directory."""
+        _lib.check(self.lib.tracer_load_mesh_from_files(str(mesh_path).encode()), "tracer_load_mesh_from_files",
+                   self.lib)
+        return self.dims()
+
     def dims(self):
         d = self.mesh_array("dims")
         return dict(nl=int(d[0]), nod2D=int(d[1]), elem2D=int(d[2]), edge2D=int(d[3]), maxAdj=int(d[4]))

@@ -99,9 +99,17 @@ int tracer_init_complete(void);
 /* lib/tracer_c_interface.F90:769-821: -1 (restart-based, see tracer_init) */
 int tracer_load_mesh_partition(const char *restart_dir);
 
-/* lib/tracer_c_interface.F90:830-868: FESOM mesh files (nod2d.out,
- * elem2d.out, aux3d.out, edges.out, edge_tri.out, nlvls.out, elvls.out);
- * -1 when the directory or a file is missing */
+/* lib/tracer_c_interface.F90:830-868 (mesh_setup, lib/oce_mesh.F90:49-94 on
+ * one rank): FESOM mesh files in mesh_path -- nod2d.out (lon, lat in degrees),
+ * elem2d.out, aux3d.out (nl, zbar), elvls.out, nlvls.out, and edges.out /
+ * edge_tri.out / edgenum.out (built from the elements when absent). Node
+ * coordinates are rotated by FESOM's default Euler angles (50, 15, -90
+ * degrees; tracer_set_option("force_rotation", 0) keeps them as read), the
+ * longitude is cyclic (360 degrees), and the geometry is spherical
+ * (mesh_areas / mesh_auxiliary_arrays). nlvls.out must equal the deepest
+ * level of each node's elements. -1 when the directory or a required file
+ * is missing or inconsistent. Follow with tracer_allocate_tracers,
+ * tracer_set_values and a velocity, as after tracer_generate_mesh. */
 int tracer_load_mesh_from_files(const char *mesh_path);
 
 /* lib/tracer_c_interface.F90:875-921: hnode = hnode_new = helem = zbar(k)-zbar(k+1) */
@@ -156,7 +164,9 @@ int tracer_get_values_id(int tracer_id, double *values, int nz, int nn);
  *   "accumulate_del_ttf" 0 (default); 1 is refused (the device path zeroes del_ttf each step)
  *   "use_graph"          1 (default): replay multi-step runs from a hipGraph
  *   "block_waves"        4 (default): waves per workgroup of the chunk kernels (1, 2, 4, 8)
- *   "batch_tracers"      4 (default): most tracers per launch (1, 2, 4) */
+ *   "batch_tracers"      4 (default): most tracers per launch (1, 2, 4)
+ *   "force_rotation"     1 (default): tracer_load_mesh_from_files rotates the
+ *                        node coordinates to the model grid; 0: as read */
 int tracer_set_option(const char *name, int value);
 
 /* No reference counterpart: nsteps steps of ALL tracers */

@@ -30,3 +30,16 @@ def mesh_dict(api):
 def api():
     from fesom2_tracer_dwarf_amd import FESOMTracerAdvection
     return FESOMTracerAdvection()
+
+
+PI_MESH = ROOT / "tests" / "golden" / "pi_mesh"
+
+
+@pytest.fixture(scope="session")
+def pi_mesh_dir(tmp_path_factory):
+    """The reference's pi mesh (3140 nodes, 5839 triangles, 48 levels), unpacked."""
+    import gzip
+    d = tmp_path_factory.mktemp("pi_mesh")
+    for f in PI_MESH.glob("*.out.gz"):
+        (d / f.name[:-3]).write_bytes(gzip.decompress(f.read_bytes()))
+    return d
