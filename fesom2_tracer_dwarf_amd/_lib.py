@@ -45,6 +45,10 @@ SIGNATURES = {
     "tracer_init_complete": (c_int, []),
     "tracer_load_mesh_partition": (c_int, [c_char_p]),
     "tracer_load_mesh_from_files": (c_int, [c_char_p]),
+    "tracer_write_mesh": (c_int, [c_char_p]),
+    "tracer_output_open": (c_int, [c_char_p, c_int]),
+    "tracer_output_step": (c_int, []),
+    "tracer_output_close": (c_int, []),
     "tracer_init_ale_arrays": (c_int, []),
     "tracer_init_for_advection": (c_int, []),
     # extensions

@@ -406,6 +406,16 @@ void read_fesom_mesh(HostMesh &m, const std::string &dir_in, const FesomFileOpti
             for (double &z : m.zbar) z = -z;
         m.Z.assign(m.nl - 1, 0.0);
         for (int k = 0; k < m.nl - 1; ++k) m.Z[k] = 0.5 * (m.zbar[k] + m.zbar[k + 1]);
+        // node depths (read_mesh :606-646): made negative, no shallower than
+        // zbar(thers_zbar_lev = 5) (lib/gen_modules_config.F90:90)
+        m.depth.assign(m.nod2D, 0.0);
+        const double shallow = m.zbar[std::min(4, m.nl - 1)];
+        for (int n = 0; n < m.nod2D; ++n) {
+            double x = t.real();
+            if (x > 0.0) x = -x;
+            if (x > shallow) x = shallow;
+            m.depth[n] = x;
+        }
     }
 
     // edges (load_edges :1617-1790): edges.out, edge_tri.out (<= 0: none)
@@ -669,6 +679,10 @@ void derive_all(HostMesh &m)
 
     if (m.cartesian) cartesian_geometry(m);
     else spherical_geometry(m);
+    if ((int)m.depth.size() != N) {  // generated meshes: the bottom of each node's deepest layer
+        m.depth.assign(N, 0.0);
+        for (int n = 0; n < N; ++n) m.depth[n] = m.zbar[m.nlevels_nod2D[n] - 1];
+    }
 
     // ALE thicknesses and 3-D depth arrays (src/fesom_analytic.F90:226-244)
     m.hnode.assign((size_t)(nl - 1) * N, 0.0);

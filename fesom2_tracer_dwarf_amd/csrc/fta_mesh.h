@@ -35,6 +35,7 @@ struct HostMesh {
     std::vector<int> nod_in_elem2D;                    // (maxAdj,N)
 
     std::vector<double> zbar, Z;                       // (nl), (nl-1)
+    std::vector<double> depth;                         // (N) bottom depth, negative (mesh files: aux3d.out)
     std::vector<double> elem_cos, elem_area;           // (T)
     std::vector<double> gradient_sca;                  // (6,T)
     std::vector<double> edge_dxdy;                     // (2,E) radians

@@ -35,6 +35,7 @@ struct State {
     std::vector<float> mlp;  // learned-limiter parameters carried across engine rebuilds
     int tune_waves = 4, tune_nt = 4;  // launch schedule carried across engine rebuilds
     FesomFileOptions file_opt;         // mesh-file reader settings (force_rotation)
+    std::vector<FILE *> frames;        // open scalar-output files, one per tracer
 };
 
 State &S()
@@ -219,6 +220,97 @@ int tracer_get_values_id(int tracer_id, double *values, int nz, int nn)
 }
 
 int tracer_get_values(double *values, int nz, int nn) { return tracer_get_values_id(1, values, nz, nn); }
+
+// ---- mesh and scalar output for the reference's plotting scripts
+// (lib/mesh_output.F90; read by visualization/plot_scalars.py:21-80)
+namespace {
+
+bool make_dir(const std::string &p) { return ::mkdir(p.c_str(), 0755) == 0 || dir_exists(p); }
+
+bool write_raw(const std::string &path, const void *data, size_t bytes)
+{
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(data, 1, bytes, f) == bytes;
+    return std::fclose(f) == 0 && ok;
+}
+
+std::string frame_name(int t)
+{
+    // the reference names tracer 1 and 2 (mesh_output.F90:109-110); further
+    // tracers get their number
+    return t == 0 ? "temperature" : (t == 1 ? "salinity" : "tracer_" + std::to_string(t + 1));
+}
+
+}  // namespace
+
+int tracer_write_mesh(const char *output_dir)
+{
+    State &s = S();
+    if (!s.mesh_ready) return fail("tracer_write_mesh: no mesh");
+    const std::string d = output_dir ? output_dir : "";
+    if (d.empty() || !make_dir(d)) return fail("tracer_write_mesh: cannot create " + d);
+    const HostMesh &m = s.mesh;
+    {
+        std::ofstream f(d + "/mesh_info.txt");
+        f << "nod2D=" << m.nod2D << "\nelem2D=" << m.elem2D << "\nnl=" << m.nl << "\n";
+        if (!f) return fail("tracer_write_mesh: cannot write mesh_info.txt");
+    }
+    std::vector<double> xy(m.coord_nod2D.size());
+    for (size_t i = 0; i < xy.size(); ++i) xy[i] = m.coord_nod2D[i] * kREarth;  // metres, (2, nod2D)
+    std::vector<int32_t> el(m.elem2D_nodes.begin(), m.elem2D_nodes.end());     // 0-based, (3, elem2D)
+    if (!write_raw(d + "/coord_nod2D.bin", xy.data(), xy.size() * sizeof(double)) ||
+        !write_raw(d + "/elem2D_nodes.bin", el.data(), el.size() * sizeof(int32_t)) ||
+        !write_raw(d + "/zbar.bin", m.zbar.data(), m.zbar.size() * sizeof(double)) ||
+        !write_raw(d + "/depth.bin", m.depth.data(), m.depth.size() * sizeof(double)))
+        return fail("tracer_write_mesh: cannot write the binary files in " + d);
+    return 0;
+}
+
+int tracer_output_close(void)
+{
+    State &s = S();
+    int bad = 0;
+    for (FILE *f : s.frames) bad |= std::fclose(f) != 0;
+    s.frames.clear();
+    return bad ? fail("tracer_output_close: write error") : 0;
+}
+
+int tracer_output_open(const char *output_dir, int nsteps)
+{
+    State &s = S();
+    if (!s.initialized || s.ntr < 1) return fail("tracer_output_open: no tracers");
+    tracer_output_close();
+    const std::string d = output_dir ? output_dir : "";
+    if (d.empty() || !make_dir(d)) return fail("tracer_output_open: cannot create " + d);
+    std::ofstream f(d + "/scalar_info.txt");
+    f << "nod2D=" << s.nod2D << "\nnz=" << s.nl - 1 << "\nnsteps=" << nsteps << "\nnum_tracers=" << s.ntr
+      << "\nwp=8\n";
+    for (int t = 0; t < s.ntr; ++t) f << "tracer_" << t + 1 << "=" << frame_name(t) << "\n";
+    if (!f) return fail("tracer_output_open: cannot write scalar_info.txt");
+    for (int t = 0; t < s.ntr; ++t) {
+        FILE *o = std::fopen((d + "/" + frame_name(t) + ".bin").c_str(), "wb");
+        if (!o) {
+            tracer_output_close();
+            return fail("tracer_output_open: cannot create the frame files in " + d);
+        }
+        s.frames.push_back(o);
+    }
+    return 0;
+}
+
+int tracer_output_step(void)
+{
+    State &s = S();
+    if (s.frames.size() != (size_t)s.ntr) return fail("tracer_output_step: output not open");
+    const size_t n = (size_t)(s.nl - 1) * s.nod2D;
+    std::vector<double> v(n);
+    for (int t = 0; t < s.ntr; ++t) {
+        if (tracer_get_values_id(t + 1, v.data(), s.nl - 1, s.nod2D) != 0) return -1;
+        if (std::fwrite(v.data(), sizeof(double), n, s.frames[t]) != n) return fail("tracer_output_step: write error");
+    }
+    return 0;
+}
 
 int tracer_finalize(const char *output_dir)
 {
@@ -603,7 +695,7 @@ bool mesh_array(const std::string &name, ArrRef &r)
     IA("ulevels_nod2D_max", m.ulevels_nod2D_max) IA("nod_in_elem2D_num", m.nod_in_elem2D_num)
     IA("nod_in_elem2D", m.nod_in_elem2D) IA("edge_up_dn_tri", m.edge_up_dn_tri)
     IA("nboundary_lay", m.nboundary_lay)
-    DA("coord", m.coord) DA("coord_nod2D", m.coord_nod2D) DA("zbar", m.zbar) DA("Z", m.Z)
+    DA("coord", m.coord) DA("coord_nod2D", m.coord_nod2D) DA("zbar", m.zbar) DA("Z", m.Z) DA("depth", m.depth)
     DA("elem_cos", m.elem_cos) DA("elem_area", m.elem_area) DA("gradient_sca", m.gradient_sca)
     DA("edge_dxdy", m.edge_dxdy) DA("edge_cross_dxdy", m.edge_cross_dxdy) DA("area", m.area)
     DA("areasvol", m.areasvol) DA("hnode", m.hnode) DA("hnode_new", m.hnode_new) DA("helem", m.helem)

@@ -70,6 +70,20 @@ class FESOMTracerAdvection:
                    self.lib)
         return self.dims()
 
+    def write_mesh(self, output_dir):
+        """mesh_info.txt and the binary mesh files the reference's plotting scripts read."""
+        _lib.check(self.lib.tracer_write_mesh(str(output_dir).encode()), "tracer_write_mesh", self.lib)
+
+    def output_open(self, output_dir, nsteps):
+        _lib.check(self.lib.tracer_output_open(str(output_dir).encode(), int(nsteps)), "tracer_output_open",
+                   self.lib)
+
+    def output_step(self):
+        _lib.check(self.lib.tracer_output_step(), "tracer_output_step", self.lib)
+
+    def output_close(self):
+        _lib.check(self.lib.tracer_output_close(), "tracer_output_close", self.lib)
+
     def dims(self):
         d = self.mesh_array("dims")
         return dict(nl=int(d[0]), nod2D=int(d[1]), elem2D=int(d[2]), edge2D=int(d[3]), maxAdj=int(d[4]))

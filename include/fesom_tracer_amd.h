@@ -264,6 +264,21 @@ int tracer_time_steps(int nsteps, double dt, double *ms_out);
  * with hipEvents on the library's stream; names written ';'-separated */
 int tracer_time_kernels(double dt, int reps, char *names, int names_len, double *ms, int max_kernels);
 
+/* ---- Output for the reference's plotting scripts (lib/mesh_output.F90:19-161,
+ * read by visualization/plot_scalars.py). tracer_write_mesh writes
+ * mesh_info.txt, coord_nod2D.bin (float64 (2,nod2D), metres), elem2D_nodes.bin
+ * (int32 (3,elem2D), 0-based), zbar.bin and depth.bin (float64) into
+ * output_dir (write_mesh_for_python :19-88). tracer_output_open writes
+ * scalar_info.txt (nod2D, nz, nsteps, num_tracers, wp=8, tracer names:
+ * temperature, salinity, tracer_3, ...) and opens one <name>.bin per tracer
+ * (open_scalar_output :92-128); every tracer_output_step appends one float64
+ * (nz,nod2D) frame per tracer (write_scalar_step :131-146); the reader expects
+ * nsteps+1 frames (the initial state and one per step). */
+int tracer_write_mesh(const char *output_dir);
+int tracer_output_open(const char *output_dir, int nsteps);
+int tracer_output_step(void);
+int tracer_output_close(void);
+
 /* No reference counterpart: the last error message */
 const char *tracer_last_error(void);
 

@@ -96,3 +96,21 @@ def test_pi_mesh_reduced_precision(api, pi_mesh_dir, prec):
         tol = 2.0 ** -23 * 10 * max(abs(lo), abs(hi))
         assert g[act].min() >= lo - tol and g[act].max() <= hi + tol
     api.set_option("precision", 64)
+
+
+def test_pi_mesh_scalar_frames_follow_the_device_state(api, pi_mesh_dir, tmp_path):
+    print("[gpu] pi mesh scalar output frames", flush=True)
+    d, m, dt, trs, act = _setup(api, pi_mesh_dir, ("MUSCL", "QR4C", "FCT"), ntr=2)
+    api.output_open(tmp_path, 2)
+    api.output_step()
+    want = [[t.T.copy() for t in trs]]
+    for _ in range(2):
+        api.advect_steps(1, dt)
+        api.output_step()
+        want.append([api.get_tracer_values(i + 1).T.copy() for i in range(2)])
+    api.output_close()
+    L = d["nl"] - 1
+    for i, name in enumerate(["temperature", "salinity"]):
+        frames = np.fromfile(tmp_path / f"{name}.bin", dtype=np.float64).reshape((3, d["nod2D"], L))
+        for f in range(3):
+            assert np.array_equal(frames[f], want[f][i])

@@ -201,3 +201,48 @@ def test_without_rotation_and_errors(api, pi_mesh_dir, tmp_path):
     assert lib.tracer_load_mesh_from_files(str(tmp_path / "missing").encode()) == -1
     assert lib.tracer_load_mesh_from_files(str(tmp_path).encode()) == -1  # no nod2d.out
     api.load_mesh_from_files(pi_mesh_dir)
+
+
+def _info(path):
+    out = {}
+    for line in path.read_text().splitlines():
+        k, v = line.split("=", 1)
+        out[k] = v
+    return out
+
+
+def test_mesh_and_scalar_output_format(api, pi, pi_mesh_dir, tmp_path):
+    """The files of lib/mesh_output.F90, read the way visualization/plot_scalars.py
+    reads them (np.fromfile, Fortran order)."""
+    d, m = pi
+    api.load_mesh_from_files(pi_mesh_dir)
+    out = tmp_path / "out"
+    api.write_mesh(out)
+    info = _info(out / "mesh_info.txt")
+    assert (int(info["nod2D"]), int(info["elem2D"]), int(info["nl"])) == (3140, 5839, 48)
+    xy = np.fromfile(out / "coord_nod2D.bin", dtype=np.float64).reshape((2, 3140), order="F")
+    assert np.array_equal(xy, (api.mesh_array("coord_nod2D") * R).reshape(-1, 2).T)
+    tri = np.fromfile(out / "elem2D_nodes.bin", dtype=np.int32).reshape((3, 5839), order="F").T
+    assert np.array_equal(tri, api.mesh_array("elem2D_nodes").reshape(-1, 3))
+    assert np.array_equal(np.fromfile(out / "zbar.bin"), api.mesh_array("zbar"))
+    depth = np.fromfile(out / "depth.bin")
+    raw = _read(pi_mesh_dir, "aux3d.out")[49:]
+    assert np.array_equal(depth, np.minimum(-np.abs(raw), api.mesh_array("zbar")[4]))
+    api.allocate_tracers(3)
+    L = d["nl"] - 1
+    vals = [np.asfortranarray(np.random.default_rng(t).uniform(0, 1, (L, 3140))) for t in range(3)]
+    for t, v in enumerate(vals):
+        api.set_values(t + 1, v)
+    api.output_open(out, 1)
+    api.output_step()
+    api.output_step()
+    api.output_close()
+    sinfo = _info(out / "scalar_info.txt")
+    assert sinfo["nz"] == str(L) and sinfo["nsteps"] == "1" and sinfo["wp"] == "8"
+    names = [sinfo[f"tracer_{t + 1}"] for t in range(3)]
+    assert names == ["temperature", "salinity", "tracer_3"]
+    for t, name in enumerate(names):
+        raw = np.fromfile(out / f"{name}.bin", dtype=np.float64)
+        frames = raw.reshape((2, 3140, L))  # plot_scalars.py: (nframes, nod2D, nz)
+        for f in range(2):
+            assert np.array_equal(frames[f], vals[t].T)
