@@ -140,7 +140,25 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedule (no autotuning)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
+    ap.add_argument("--copy-bandwidth", action="store_true",
+                    help="only time a 4 GiB device-to-device copy (the practical HBM ceiling)")
     args = ap.parse_args()
+    if args.copy_bandwidth:
+        import torch
+        a = torch.ones(1 << 29, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(20):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 20
+        print(json.dumps({"copy_bytes_each_way": a.numel() * 8, "ms_per_copy": ms,
+                          "GBps_read_plus_write": 2 * a.numel() * 8 / ms / 1e6}))
+        return
     if args.calibrate_copy:
         import torch
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")

@@ -782,9 +782,12 @@ struct LearnSrc {
     const R *Pp, *Pm;      // node P+/P- (this tracer)
 };
 
-__device__ __forceinline__ float clip8(double v) { return (float)fmin(8.0, fmax(-8.0, v)); }
+__device__ __forceinline__ float clip8(float v) { return fminf(8.0f, fmaxf(-8.0f, v)); }
 
-// features (and Zalesak target) of the lane's two edge levels
+// features (and Zalesak target) of the lane's two edge levels. Differences of
+// tracer values are formed in double (they may be small against the values);
+// the scalings by the local range and the cell volume are float reciprocals:
+// the features are clipped to [-8, 8] and rounded to bf16 afterwards.
 template <class R>
 __device__ __forceinline__ void edge_features(const DMesh<R> &m, const EdgeRec &er, int k0, size_t c, double dt,
                                               const LearnSrc<R> &S, float (&x)[2][kMlpIn], float (&y)[2])
@@ -793,41 +796,43 @@ __device__ __forceinline__ void edge_features(const DMesh<R> &m, const EdgeRec &
     const V2<double> L1 = ld2(S.LO + er.o1 + k0), L2 = ld2(S.LO + er.o2 + k0);
     const V2<R> qp1 = ld2(S.Qp + er.o1 + k0), qm1 = ld2(S.Qm + er.o1 + k0);
     const V2<R> qp2 = ld2(S.Qp + er.o2 + k0), qm2 = ld2(S.Qm + er.o2 + k0);
-    const V2<R> rp1 = ld2(S.Rp + er.o1 + k0), rm1 = ld2(S.Rm + er.o1 + k0);
-    const V2<R> rp2 = ld2(S.Rp + er.o2 + k0), rm2 = ld2(S.Rm + er.o2 + k0);
     const V2<R> a = ld2(S.adf + c), v = ld2(S.vfl + c);
     const V2<double> av1 = ld2(m.av_d + er.o1 + k0), hn1 = ld2(m.hn_d + er.o1 + k0);
     const V2<double> av2 = ld2(m.av_d + er.o2 + k0);
     const V2<R> pp1 = ld2(S.Pp + er.o1 + k0), pm1 = ld2(S.Pm + er.o1 + k0);
     const V2<R> pp2 = ld2(S.Pp + er.o2 + k0), pm2 = ld2(S.Pm + er.o2 + k0);
-    const double Pp1[2] = {(double)pp1.x, (double)pp1.y}, Pm1[2] = {(double)pm1.x, (double)pm1.y};
-    const double Pp2[2] = {(double)pp2.x, (double)pp2.y}, Pm2[2] = {(double)pm2.x, (double)pm2.y};
-    const double AV1[2] = {av1.x, av1.y}, AV2[2] = {av2.x, av2.y};
     const double t1[2] = {T1.x, T1.y}, t2[2] = {T2.x, T2.y}, l1[2] = {L1.x, L1.y}, l2[2] = {L2.x, L2.y};
-    const double Qp1[2] = {(double)qp1.x, (double)qp1.y}, Qm1[2] = {(double)qm1.x, (double)qm1.y};
-    const double Qp2[2] = {(double)qp2.x, (double)qp2.y}, Qm2[2] = {(double)qm2.x, (double)qm2.y};
-    const R Rp1[2] = {rp1.x, rp1.y}, Rm1[2] = {rm1.x, rm1.y}, Rp2[2] = {rp2.x, rp2.y}, Rm2[2] = {rm2.x, rm2.y};
-    const double A[2] = {(double)a.x, (double)a.y}, Vf[2] = {(double)v.x, (double)v.y};
-    const double V1[2] = {av1.x * hn1.x, av1.y * hn1.y};
+    const R Qp1[2] = {qp1.x, qp1.y}, Qm1[2] = {qm1.x, qm1.y}, Qp2[2] = {qp2.x, qp2.y}, Qm2[2] = {qm2.x, qm2.y};
+    const R Pp1[2] = {pp1.x, pp1.y}, Pm1[2] = {pm1.x, pm1.y}, Pp2[2] = {pp2.x, pp2.y}, Pm2[2] = {pm2.x, pm2.y};
+    const double AV1[2] = {av1.x, av1.y}, AV2[2] = {av2.x, av2.y}, HN1[2] = {hn1.x, hn1.y};
+    const R A[2] = {a.x, a.y}, Vf[2] = {v.x, v.y};
+    const float fdt = (float)dt;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const double mx = fmax(l1[q] + Qp1[q], l2[q] + Qp2[q]);
-        const double mn = fmin(l1[q] + Qm1[q], l2[q] + Qm2[q]);
-        const double sc = mx - mn > 1e-300 ? mx - mn : 1.0;
-        const double lo = lo_flux(Vf[q], t1[q], t2[q]);
-        const double toT = dt / (V1[q] * sc);
-        x[q][0] = clip8(lo * toT);
-        x[q][1] = clip8((lo + A[q]) * toT);
-        x[q][2] = clip8((t1[q] - mn) / sc);
-        x[q][3] = clip8((t2[q] - mn) / sc);
-        x[q][4] = clip8(Qp1[q] / sc);
-        x[q][5] = clip8(-Qm1[q] / sc);
-        x[q][6] = clip8(Qp2[q] / sc);
-        x[q][7] = clip8(-Qm2[q] / sc);
-        const bool in1 = A[q] >= 0.0;  // flux into node 1
-        x[q][8] = clip8((in1 ? Pp1[q] : -Pm1[q]) * dt / (AV1[q] * sc));
-        x[q][9] = clip8((in1 ? -Pm2[q] : Pp2[q]) * dt / (AV2[q] * sc));
-        y[q] = (float)edge_alpha(true, (R)A[q], Rp1[q], Rm1[q], Rp2[q], Rm2[q]);
+        const double mx = fmax(l1[q] + (double)Qp1[q], l2[q] + (double)Qp2[q]);
+        const double mn = fmin(l1[q] + (double)Qm1[q], l2[q] + (double)Qm2[q]);
+        const float sc = mx - mn > 1e-30 ? (float)(mx - mn) : 1.0f;
+        const float isc = __builtin_amdgcn_rcpf(sc);
+        const double lo = lo_flux((double)Vf[q], t1[q], t2[q]);
+        const float toT = fdt * __builtin_amdgcn_rcpf((float)(AV1[q] * HN1[q]) * sc);
+        x[q][0] = clip8((float)lo * toT);
+        x[q][1] = clip8((float)(lo + (double)A[q]) * toT);
+        x[q][2] = clip8((float)(t1[q] - mn) * isc);
+        x[q][3] = clip8((float)(t2[q] - mn) * isc);
+        x[q][4] = clip8((float)Qp1[q] * isc);
+        x[q][5] = clip8(-(float)Qm1[q] * isc);
+        x[q][6] = clip8((float)Qp2[q] * isc);
+        x[q][7] = clip8(-(float)Qm2[q] * isc);
+        const bool in1 = A[q] >= (R)0;  // flux into node 1
+        x[q][8] = clip8((float)(in1 ? Pp1[q] : -Pm1[q]) * (fdt * __builtin_amdgcn_rcpf((float)AV1[q] * sc)));
+        x[q][9] = clip8((float)(in1 ? -Pm2[q] : Pp2[q]) * (fdt * __builtin_amdgcn_rcpf((float)AV2[q] * sc)));
+        y[q] = 0.0f;
+    }
+    if (S.Rp) {  // training rows: Zalesak's coefficient (b3) as the target
+        const V2<R> rp1 = ld2(S.Rp + er.o1 + k0), rm1 = ld2(S.Rm + er.o1 + k0);
+        const V2<R> rp2 = ld2(S.Rp + er.o2 + k0), rm2 = ld2(S.Rm + er.o2 + k0);
+        y[0] = (float)edge_alpha(true, a.x, rp1.x, rm1.x, rp2.x, rm2.x);
+        y[1] = (float)edge_alpha(true, a.y, rp1.y, rm1.y, rp2.y, rm2.y);
     }
 }
 
@@ -888,19 +893,34 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_dump(DMesh<R> m, LearnArgs<
 // its chunk's 128 feature rows into LDS and runs them through the matrix
 // cores as eight 16-row tiles. Persistent waves (weights loaded once);
 // block-uniform trip counts.
+// the network's coefficient of every edge level: a persistent grid; each
+// wave forms its chunk's 128 feature rows (bf16) in its own LDS slice, runs
+// eight 16-row tiles through the matrix cores and reads its two outputs back.
+// The lanes of one wave exchange data only with each other, so a wave-scope
+// fence replaces block barriers and waves leave the loop independently.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class R, int NT>
-__global__ __launch_bounds__(kBlock) void k_learn_alpha(DMesh<R> m, MlpDev w, LearnArgs<R, NT> a, double dt,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_learn_alpha(DMesh<R> m, MlpDev w, LearnArgs<R, NT> a, double dt,
                                                         R *__restrict__ alpha, int iters)
 {
-    __shared__ unsigned short sX[kWavesPerBlock][kChunk][kMlpInPad];
-    __shared__ float sY[kWavesPerBlock][kChunk];
+    // tracers are staged two at a time: their gathers overlap, and the LDS
+    // of a block (36 KB) leaves room for more resident waves than four would
+    constexpr int G = NT < 2 ? NT : 2;
+    __shared__ uint4 sX[kWavesPerBlock][G][kChunk][2];  // 16 bf16 features per row
+    __shared__ float sY[kWavesPerBlock][G][kChunk];
     const int wv = (int)(threadIdx.x >> 6), l = lane_id(), i = l & 15, g = l >> 4;
-    MlpRegs r;
-    mlp_load(w, r);
+    MlpInfer r;
+    mlp_infer_load(w, r);
     for (int it = 0; it < iters; ++it) {
         const int ch = (it * (int)gridDim.x + (int)blockIdx.x) * kWavesPerBlock + wv;
-        Pair p{0, 0, 0, false};
-        if (ch < m.nch_e) p = lane_pair(m.slot_e, ch);
+        if (ch >= m.nch_e) break;  // wave-uniform
+        const Pair p = lane_pair(m.slot_e, ch);
         EdgeRec er{};
         int nlmax = 0;
         if (p.live) {
@@ -908,26 +928,46 @@ __global__ __launch_bounds__(kBlock) void k_learn_alpha(DMesh<R> m, MlpDev w, Le
             nlmax = (int)((unsigned)er.nl >> 24);
         }
 #pragma unroll 1
-        for (int t = 0; t < NT; ++t) {
-            float x[2][kMlpIn], y[2];
-            if (p.live) edge_features(m, er, p.k0, (size_t)p.c, dt, learn_src(a, t), x, y);
+        for (int t0 = 0; t0 < NT; t0 += G) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
+            for (int s = 0; s < G; ++s) {
+                float x[2][kMlpIn], y[2];
+                LearnSrc<R> src = learn_src(a, t0 + s);
+                src.Rp = nullptr;  // no target needed
+                if (p.live) edge_features(m, er, p.k0, (size_t)p.c, dt, src, x, y);
 #pragma unroll
-                for (int f = 0; f < kMlpInPad; ++f)
-                    sX[wv][2 * l + q][f] = (p.live && f < kMlpIn) ? f2bf(x[q][f]) : (unsigned short)0;
-            __syncthreads();
-#pragma unroll 1
-            for (int tile = 0; tile < kChunk / 16; ++tile) {
-                const mlp_v4s xb = ld_bf4(&sX[wv][16 * tile + i][4 * g]);
-                MlpTile T;
-                mlp_forward(r, xb, T);
-                if (g == 0) sY[wv][16 * tile + i] = mlp_sigmoid(T.z3);
+                for (int q = 0; q < 2; ++q) {
+                    unsigned u[8];
+#pragma unroll
+                    for (int f = 0; f < 8; ++f) {
+                        // v_cvt_pk_bf16_f32: two features per instruction, nearest even
+                        const mlp_v2f v2 = {2 * f < kMlpIn ? x[q][2 * f] : 0.0f,
+                                            2 * f + 1 < kMlpIn ? x[q][2 * f + 1] : 0.0f};
+                        const unsigned b = __builtin_bit_cast(unsigned, __builtin_convertvector(v2, mlp_v2b));
+                        u[f] = (p.live && 2 * f < kMlpIn) ? b : 0u;
+                    }
+                    sX[wv][s][2 * l + q][0] = make_uint4(u[0], u[1], u[2], u[3]);
+                    sX[wv][s][2 * l + q][1] = make_uint4(u[4], u[5], u[6], u[7]);
+                }
             }
-            __syncthreads();
-            if (p.live)
-                st2(alpha + t * a.tse + p.c, (R)sY[wv][2 * l], p.k0 + 1 < nlmax ? (R)sY[wv][2 * l + 1] : (R)0);
-            __syncthreads();
+            wave_lds_sync();
+#pragma unroll 1
+            for (int s = 0; s < G; ++s) {
+#pragma unroll 2
+                for (int tile = 0; tile < kChunk / 16; ++tile) {
+                    const mlp_v4s xb =
+                        ld_bf4(reinterpret_cast<const unsigned short *>(&sX[wv][s][16 * tile + i][0]) + 4 * g);
+                    const float z = mlp_infer(r, xb);
+                    if (g == 0) sY[wv][s][16 * tile + i] = mlp_sigmoid_fast(z);
+                }
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int s = 0; s < G; ++s) {
+                const float y0 = sY[wv][s][2 * l], y1 = sY[wv][s][2 * l + 1];
+                if (p.live) st2(alpha + (t0 + s) * a.tse + p.c, (R)y0, p.k0 + 1 < nlmax ? (R)y1 : (R)0);
+            }
+            wave_lds_sync();
         }
     }
 }

@@ -371,20 +371,120 @@ __global__ __launch_bounds__(256) void k_fill_f32(float *__restrict__ a, long lo
 
 // inference of arbitrary rows (bf16 [rows][16]) -> sigmoid outputs; one
 // 16-row tile per wave
+// ---------------------------------------------------------------------------
+// Inference-only forward pass (the step's limiter and k_mlp_infer).
+// Layer 1 as above (K = 16 features). Layer 2 on v_mfma_f32_16x16x32_bf16:
+// the order of K inside one instruction is free as long as A and B agree, so
+// the B operand of K block c (hidden 32c..32c+31) is simply the two layer-1
+// accumulator tiles 2c and 2c+1 of the lane (lane l, element e: hidden
+// 32c + 16*(e/4) + 4*(l/16) + e%4), and the weights are loaded in that same
+// order. Layer 3 (64 -> 1) is a VALU dot product over the lane's 16 hidden
+// values and a sum over the four lane groups -- one useful row out of 16 did
+// not pay for 4 more matrix instructions. Roundings are those of the
+// training pass: bf16 inputs, weights and activations, FP32 sums.
+// ---------------------------------------------------------------------------
+typedef __bf16 mlp_v8b __attribute__((ext_vector_type(8)));
+typedef short mlp_v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 mlp_v2b __attribute__((ext_vector_type(2)));
+typedef float mlp_v2f __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ mlp_v8b cat8(mlp_v4s a, mlp_v4s b)
+{
+    const mlp_v8s s = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+    return __builtin_bit_cast(mlp_v8b, s);
+}
+
+// round to bf16 (v_cvt_pk_bf16_f32: nearest even) and apply the ReLU on the
+// bf16 bits (v_pk_max_i16 with 0: a set sign bit is a negative int16), two
+// values per instruction each; equal to bf16(max(x, 0)) for finite x
+typedef short mlp_v2s __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ mlp_v2b relu_bf2(float a, float b)
+{
+    const mlp_v2f v = {a, b};
+    const mlp_v2s h = __builtin_bit_cast(mlp_v2s, __builtin_convertvector(v, mlp_v2b));
+    const mlp_v2s z = {0, 0};
+    return __builtin_bit_cast(mlp_v2b, __builtin_elementwise_max(h, z));
+}
+__device__ __forceinline__ mlp_v4s relu_bf4(mlp_v4f acc)
+{
+    const mlp_v2b lo = relu_bf2(acc[0], acc[1]), hi = relu_bf2(acc[2], acc[3]);
+    const unsigned ul = __builtin_bit_cast(unsigned, lo), uh = __builtin_bit_cast(unsigned, hi);
+    const mlp_v4s v = {(short)(ul & 0xffffu), (short)(ul >> 16), (short)(uh & 0xffffu), (short)(uh >> 16)};
+    return v;
+}
+
+struct MlpInfer {
+    mlp_v4s w1[4];        // W1^T tile t (A operand, K = features)
+    mlp_v8b w2[4][2];     // W2^T row tile t2, K block c (A operand, permuted K)
+    float b1[4][4], b2[4][4];
+    mlp_v2b w3[4][2];     // w3[16t + 4*(l/16) + 2p .. +1] as bf16 pairs
+    float b3;
+};
+
+__device__ __forceinline__ void mlp_infer_load(const MlpDev &w, MlpInfer &r)
+{
+    const int l = (int)(threadIdx.x & 63), i = l & 15, g = l >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        r.w1[t] = ld_bf4(w.w1t + (16 * t + i) * kMlpInPad + 4 * g);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const unsigned short *row = w.w2t + (16 * t + i) * kMlpHid + 32 * c + 4 * g;
+            r.w2[t][c] = cat8(ld_bf4(row), ld_bf4(row + 16));
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            r.b1[t][e] = w.b1[16 * t + 4 * g + e];
+            r.b2[t][e] = w.b2[16 * t + 4 * g + e];
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const unsigned u = (unsigned)w.w3[16 * t + 4 * g + 2 * q] | ((unsigned)w.w3[16 * t + 4 * g + 2 * q + 1] << 16);
+            r.w3[t][q] = __builtin_bit_cast(mlp_v2b, u);
+        }
+    }
+    r.b3 = w.b3[0];
+}
+
+// z3 (pre-sigmoid) of tile row l%16, in every lane
+__device__ __forceinline__ float mlp_infer(const MlpInfer &r, mlp_v4s x)
+{
+    mlp_v4s a1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const mlp_v4f acc = mfma16(r.w1[t], x, mlp_v4f{r.b1[t][0], r.b1[t][1], r.b1[t][2], r.b1[t][3]});
+        a1[t] = relu_bf4(acc);
+    }
+    const mlp_v8b b0 = cat8(a1[0], a1[1]), b1 = cat8(a1[2], a1[3]);
+    float z = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        mlp_v4f acc = {r.b2[t][0], r.b2[t][1], r.b2[t][2], r.b2[t][3]};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w2[t][0], b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w2[t][1], b1, acc, 0, 0, 0);
+        z = __builtin_amdgcn_fdot2_f32_bf16(relu_bf2(acc[0], acc[1]), r.w3[t][0], z, false);
+        z = __builtin_amdgcn_fdot2_f32_bf16(relu_bf2(acc[2], acc[3]), r.w3[t][1], z, false);
+    }
+    z += __shfl_xor(z, 16, 64);
+    z += __shfl_xor(z, 32, 64);
+    return z + r.b3;
+}
+
+__device__ __forceinline__ float mlp_sigmoid_fast(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
 __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev w, const unsigned short *__restrict__ X, long long rows,
                                                    float *__restrict__ out)
 {
     const int l = (int)(threadIdx.x & 63), i = l & 15, g = l >> 4;
     const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (tile * 16 >= rows) return;
-    MlpRegs r;
-    mlp_load(w, r);
+    MlpInfer r;
+    mlp_infer_load(w, r);
     const long long row = tile * 16 + i;
     const mlp_v4s zero4 = {0, 0, 0, 0};
     const mlp_v4s x = row < rows ? ld_bf4(X + row * kMlpInPad + 4 * g) : zero4;
-    MlpTile T;
-    mlp_forward(r, x, T);
-    if (g == 0 && row < rows) out[row] = mlp_sigmoid(T.z3);
+    const float z = mlp_infer(r, x);
+    if (g == 0 && row < rows) out[row] = mlp_sigmoid_fast(z);
 }
 
 }  // namespace fta
