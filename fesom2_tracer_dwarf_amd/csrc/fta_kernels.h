@@ -143,8 +143,13 @@ __device__ __forceinline__ Pair lane_pair(const int *slot, int chunk)
 }
 
 // Fortran max/min as the reference compiles them (a > b ? a : b)
-template <class T> __device__ __forceinline__ T fmx(T a, T b) { return a > b ? a : b; }
-template <class T> __device__ __forceinline__ T fmn(T a, T b) { return a < b ? a : b; }
+// max / min on the hardware instructions (v_max_f64, v_min_f32, ...); they
+// differ from a compare-and-select only in the sign of an exact zero and for
+// NaN, which no limiter sum or bound of this path can tell apart
+__device__ __forceinline__ double fmx(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double fmn(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float fmx(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float fmn(float a, float b) { return __builtin_fminf(a, b); }
 
 // lane l receives lane l-1's / l+1's value (own value at the wave ends)
 template <class T> __device__ __forceinline__ T from_prev(T v) { return __shfl_up(v, 1, kWave); }
