@@ -123,6 +123,12 @@ class FESOMTracerAdvection:
                    "tracer_make_velocity", self.lib)
         return dt.value
 
+    def set_thickness(self, hnode_new):
+        """hnode_new, (nl-1, nod2D) Fortran order (tracer_set_thickness: hnode stays)."""
+        d = self.dims()
+        _lib.check(self.lib.tracer_set_thickness(np.asfortranarray(hnode_new, dtype=np.float64), d["nl"], d["nod2D"]),
+                   "tracer_set_thickness", self.lib)
+
     def set_velocity_elem(self, uv, w, w_e):
         d = self.dims()
         _lib.check(self.lib.tracer_set_velocity_elem(np.asfortranarray(uv, dtype=np.float64),

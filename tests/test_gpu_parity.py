@@ -191,3 +191,28 @@ def test_stale_valuesAB_matches_oracle(api):
             err = rel_err(api.get_tracer_values(i + 1), ref[i])
             assert err <= RTOL, f"tracer {i + 1} step {step + 1}: rel err {err:.3e}"
     api.set_option("refresh_valuesAB", 1)
+
+
+@pytest.mark.parametrize("schemes", [("MUSCL", "QR4C", "FCT"), ("MFCT", "PPM", "FCT")], ids=lambda s: "-".join(s))
+def test_new_layer_thickness_matches_oracle(api, schemes):
+    """tracer_set_thickness (lib/tracer_c_interface.F90:477-503) sets hnode_new only: the LO solution, the
+    PPM reconstruction and the update then see layers that change thickness over the step (the ALE form of
+    the formulas), per node."""
+    import oracle as orc
+    print(f"[gpu] hnode_new != hnode, {schemes}", flush=True)
+    d, dt, trs = setup_case(api, MESHES["periodic_24x20x12_random_bottom"], schemes, ntr=2)
+    L, N = d["nl"] - 1, d["nod2D"]
+    hn = api.mesh_array("hnode").reshape(N, L).T
+    rng = np.random.default_rng(5)
+    api.set_thickness(hn * (1.0 + 0.02 * rng.uniform(-1, 1, hn.shape)))
+    m = mesh_dict(api)
+    assert not np.array_equal(m["hnode_new"], m["hnode"])
+    o = orc.Oracle(m)
+    uv, w, we = api.mesh_array("uv"), api.mesh_array("w"), api.mesh_array("w_e")
+    ref = [np.array(t, order="F", copy=True) for t in trs]
+    for step in range(2):
+        api.advect_steps(1, dt)
+        for i in range(2):
+            o.step(dt, uv, w, we, ref[i], hor=schemes[0], ver=schemes[1], lim=schemes[2])
+            err = rel_err(api.get_tracer_values(i + 1), ref[i])
+            assert err <= RTOL, f"tracer {i + 1} step {step + 1}: rel err {err:.3e}"
