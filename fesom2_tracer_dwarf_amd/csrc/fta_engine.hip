@@ -411,19 +411,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_lo(DMesh<R> m, const R *__re
             const bool pos = info_pos(info[j]);
             // LO upwind flux of the edge (adv_tra_hor_upw1, lib/oce_adv_tra_hor.F90:64-261)
             // from its transport and the tracer at edges(1), edges(2)
+            // seen from edges(2) the flux enters with the opposite sign:
+            // -lo_flux(v, To, T) == lo_flux(-v, T, To) exactly (sign flips and a
+            // commuted two-term sum), so one sign on the transport orients it
+            const double sg = pos ? 1.0 : -1.0;
             const V2<R> v = ld2(vfl + (size_t)oe[j] + k0);
             const V2<double> To = ld2(ttf + tn + (size_t)oo[j] + k0);
-            const V2<double> f{lo_flux((double)v.x, pos ? T.x : To.x, pos ? To.x : T.x),
-                               lo_flux((double)v.y, pos ? T.y : To.y, pos ? To.y : T.y)};
             const V2<R> a = ld2(adf_h + te + (size_t)oe[j] + k0);
-            slo0 = pos ? slo0 + f.x : slo0 - f.x;
-            if (pos) { pp0 = pp0 + fmx((R)0, a.x); pm0 = pm0 + fmn((R)0, a.x); }
-            else     { pp0 = pp0 + fmx((R)0, -a.x); pm0 = pm0 + fmn((R)0, -a.x); }
-            if (on1) {
-                slo1 = pos ? slo1 + f.y : slo1 - f.y;
-                if (pos) { pp1 = pp1 + fmx((R)0, a.y); pm1 = pm1 + fmn((R)0, a.y); }
-                else     { pp1 = pp1 + fmx((R)0, -a.y); pm1 = pm1 + fmn((R)0, -a.y); }
-            }
+            const R ax = pos ? a.x : -a.x;
+            slo0 = slo0 + lo_flux((double)v.x * sg, T.x, To.x);
+            pp0 = pp0 + fmx((R)0, ax);
+            pm0 = pm0 + fmn((R)0, ax);
+            // second cell by selects, so that each pair stays one 16-byte load
+            const R ay = pos ? a.y : -a.y;
+            const double s1 = slo1 + lo_flux((double)v.y * sg, T.y, To.y);
+            const R q1 = pp1 + fmx((R)0, ay), r1 = pm1 + fmn((R)0, ay);
+            slo1 = on1 ? s1 : slo1;
+            pp1 = on1 ? q1 : pp1;
+            pm1 = on1 ? r1 : pm1;
         }
         // flux differences formed in FP64: each interface flux then leaves one
         // layer and enters the next with exactly opposite sign in every precision
@@ -657,9 +662,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_update(DMesh<R> m, double *_
             const V2<R> f = ld2(adf_h + te + (size_t)oe[j] + k0);
             const V2<R> rpo = ld2(Rp + tn + (size_t)oo[j] + k0), rmo = ld2(Rm + tn + (size_t)oo[j] + k0);
             // both cells evaluated unconditionally, the second one masked
-            // (keeps every pair load one 16-byte access)
-            R a0 = edge_alpha(pos, f.x, rp.x, rm.x, rpo.x, rmo.x);
-            R a1 = edge_alpha(pos, f.y, rp.y, rm.y, rpo.y, rmo.y);
+            // (keeps every pair load one 16-byte access). Oriented as the flux
+            // entering this node, fs = +-f, Zalesak's coefficient (b3) is
+            // min(1, R+ here, R- there) for fs >= 0 and min(1, R- here, R+ there)
+            // otherwise -- edge_alpha's choice for every f != 0, and for f == 0 the
+            // product with the flux is zero either way
+            const R fx = pos ? f.x : -f.x, fy = pos ? f.y : -f.y;
+            const bool in0 = fx >= (R)0, in1 = fy >= (R)0;
+            R a0 = fmn(fmn((R)1, in0 ? rp.x : rm.x), in0 ? rmo.x : rpo.x);
+            R a1 = fmn(fmn((R)1, in1 ? rp.y : rm.y), in1 ? rmo.y : rpo.y);
             if constexpr (LEARN) {
                 // the network's coefficient, Zalesak's where either node failed its bound check
                 const V2<R> al = ld2(alpha + te + (size_t)oe[j] + k0);
@@ -667,10 +678,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_update(DMesh<R> m, double *_
                 a0 = (us.x != (R)0 || uo.x != (R)0) ? fmn(al.x, a0) : al.x;
                 a1 = (us.y != (R)0 || uo.y != (R)0) ? fmn(al.y, a1) : al.y;
             }
-            const double fl0 = (double)(a0 * f.x);
-            const double fl1 = (double)(a1 * f.y);
-            sh0 = pos ? sh0 + fl0 : sh0 - fl0;
-            const double s1 = pos ? sh1 + fl1 : sh1 - fl1;
+            const double fl0 = (double)(a0 * fx);
+            const double fl1 = (double)(a1 * fy);
+            sh0 = sh0 + fl0;
+            const double s1 = sh1 + fl1;
             sh1 = on1 ? s1 : sh1;
         }
         // the edge sum times dt/areasvol once (the reference scales each edge's
