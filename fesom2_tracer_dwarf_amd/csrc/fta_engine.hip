@@ -336,7 +336,7 @@ __device__ __forceinline__ VCol2<R> node_vcol(const DMesh<R> &m, const Pair &p, 
 // (:257-283), and the raw antidiffusive sums P+/P- (lib/oce_adv_tra_fct.F90:232-346)
 // ===========================================================================
 template <class R, int VER, int NT, bool SAME>
-__global__ __launch_bounds__(kMaxBlock) void k_node_lo(DMesh<R> m, const R *__restrict__ Wv,
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && NT <= 2 && VER != VER_PPM ? 4 : 1))) void k_node_lo(DMesh<R> m, const R *__restrict__ Wv,
                                                     const R *__restrict__ WEv,
                                                     const double *__restrict__ ttf,
                                                     const double *__restrict__ ttfAB, size_t tsn,
