@@ -608,7 +608,7 @@ __device__ __forceinline__ R edge_alpha(bool pos, R f, R rp, R rm, R rpo, R rmo)
 // (lib/oce_adv_tra_driver.F90:298-450) and the update
 // values += (dttf_h + dttf_v)/hnode_new (src/fesom_analytic.F90:419-437)
 template <class R, int NT, bool LEARN>
-__global__ __launch_bounds__(kMaxBlock) void k_node_update(DMesh<R> m, double *__restrict__ ttf,
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && NT <= 2 && !LEARN ? 5 : 1))) void k_node_update(DMesh<R> m, double *__restrict__ ttf,
                                                         const double *__restrict__ LO, size_t tsn,
                                                         const R *__restrict__ adf_h, size_t tse,
                                                         const R *__restrict__ adf_v,
