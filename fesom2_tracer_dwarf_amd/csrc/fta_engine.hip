@@ -362,47 +362,53 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
     VCol2<double> lc{};
     lc.nlev = nlev;
     lc.k0 = k0;
-    V2<double> av{1, 1}, h{0, 0}, hn{1, 1};  // FP64 geometry: the LO solution feeds the conservative update
     if (p.live) {
         const V2<R> we = ld2(WEv + (size_t)c), ar = ld2(m.area + c);
         lc.W = {(double)we.x, (double)we.y};
         lc.area = {(double)ar.x, (double)ar.y};
     }
-    if (lay0) {
-        av = ld2(m.av_d + c);
-        h = ld2(m.h_d + c);
-        hn = ld2(m.hn_d + c);
+    // vertical fluxes of every tracer first: the column reconstruction's
+    // state (windows of Z, zbar, W) is dead before the edge loops start
+    V2<double> T[NT];
+    double fl0[NT], fl1[NT], fl2[NT];
+    R pp0[NT], pm0[NT], pp1[NT], pm1[NT];
+    {
+        VCol2<R> hc = node_vcol<R, VER>(m, p, nlev, Wv);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const size_t tn = t * tsn;
+            T[t] = V2<double>{0.0, 0.0};
+            if (p.live) T[t] = ld2(ttf + tn + c);
+            lc.A = tracer_window<double>(T[t], k0, nlev);
+            // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
+            fl0[t] = vflux_upw1<0>(lc, 0.0);
+            fl1[t] = vflux_upw1<1>(lc, 0.0);
+            fl2[t] = from_next(fl0[t]);
+            if constexpr (SAME) {
+                hc.A = tracer_window<R>(T[t], k0, nlev);
+            } else {
+                V2<double> B{0.0, 0.0};
+                if (p.live) B = ld2(ttfAB + tn + c);
+                hc.A = tracer_window<R>(B, k0, nlev);
+            }
+            R ad0, ad1;
+            ver_flux<R, VER>(hc, num_ord, (R)dt, (R)fl0[t], (R)fl1[t], ad0, ad1);  // HO - LO
+            const R ad2 = from_next(ad0);
+            if (p.live) st2(adf_v + tn + c, if0 ? ad0 : (R)0, if1 ? ad1 : (R)0);
+            pp0[t] = (R)0 + (fmx((R)0, ad0) + fmx((R)0, -ad1));
+            pm0[t] = (R)0 + (fmn((R)0, ad0) + fmn((R)0, -ad1));
+            pp1[t] = (R)0 + (fmx((R)0, ad1) + fmx((R)0, -ad2));
+            pm1[t] = (R)0 + (fmn((R)0, ad1) + fmn((R)0, -ad2));
+        }
     }
-    VCol2<R> hc = node_vcol<R, VER>(m, p, nlev, Wv);
+    if (!lay0) return;
 
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const size_t tn = t * tsn, te = t * tse;
-        V2<double> T{0.0, 0.0};
-        if (p.live) T = ld2(ttf + tn + c);
-        lc.A = tracer_window<double>(T, k0, nlev);
-        // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
-        const double fl0 = vflux_upw1<0>(lc, 0.0), fl1 = vflux_upw1<1>(lc, 0.0);
-        const double fl2 = from_next(fl0);
-        if constexpr (SAME) {
-            hc.A = tracer_window<R>(T, k0, nlev);
-        } else {
-            V2<double> B{0.0, 0.0};
-            if (p.live) B = ld2(ttfAB + tn + c);
-            hc.A = tracer_window<R>(B, k0, nlev);
-        }
-        R ad0, ad1;
-        ver_flux<R, VER>(hc, num_ord, (R)dt, (R)fl0, (R)fl1, ad0, ad1);  // HO - LO
-        const R ad2 = from_next(ad0);
-        if (p.live) st2(adf_v + tn + c, if0 ? ad0 : (R)0, if1 ? ad1 : (R)0);
-        if (!lay0) continue;
-
         // LO horizontal sum over incident edges, edge-index order
         double slo0 = 0.0, slo1 = 0.0;
-        R pp0 = (R)0 + (fmx((R)0, ad0) + fmx((R)0, -ad1));
-        R pm0 = (R)0 + (fmn((R)0, ad0) + fmn((R)0, -ad1));
-        R pp1 = (R)0 + (fmx((R)0, ad1) + fmx((R)0, -ad2));
-        R pm1 = (R)0 + (fmn((R)0, ad1) + fmn((R)0, -ad2));
+        R qp0 = pp0[t], qm0 = pm0[t], qp1 = pp1[t], qm1 = pm1[t];
 #pragma unroll
         for (int j = 0; j < kDeg; ++j) {
             const int nlm = info_nlmax(info[j]);
@@ -410,7 +416,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             const bool on1 = lay1 && k0 + 1 < nlm;
             const bool pos = info_pos(info[j]);
             // LO upwind flux of the edge (adv_tra_hor_upw1, lib/oce_adv_tra_hor.F90:64-261)
-            // from its transport and the tracer at edges(1), edges(2)
+            // from its transport and the tracer at edges(1), edges(2);
             // seen from edges(2) the flux enters with the opposite sign:
             // -lo_flux(v, To, T) == lo_flux(-v, T, To) exactly (sign flips and a
             // commuted two-term sum), so one sign on the transport orients it
@@ -419,26 +425,32 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             const V2<double> To = ld2(ttf + tn + (size_t)oo[j] + k0);
             const V2<R> a = ld2(adf_h + te + (size_t)oe[j] + k0);
             const R ax = pos ? a.x : -a.x;
-            slo0 = slo0 + lo_flux((double)v.x * sg, T.x, To.x);
-            pp0 = pp0 + fmx((R)0, ax);
-            pm0 = pm0 + fmn((R)0, ax);
+            slo0 = slo0 + lo_flux((double)v.x * sg, T[t].x, To.x);
+            qp0 = qp0 + fmx((R)0, ax);
+            qm0 = qm0 + fmn((R)0, ax);
             // second cell by selects, so that each pair stays one 16-byte load
             const R ay = pos ? a.y : -a.y;
-            const double s1 = slo1 + lo_flux((double)v.y * sg, T.y, To.y);
-            const R q1 = pp1 + fmx((R)0, ay), r1 = pm1 + fmn((R)0, ay);
+            const double s1 = slo1 + lo_flux((double)v.y * sg, T[t].y, To.y);
+            const R q1 = qp1 + fmx((R)0, ay), r1 = qm1 + fmn((R)0, ay);
             slo1 = on1 ? s1 : slo1;
-            pp1 = on1 ? q1 : pp1;
-            pm1 = on1 ? r1 : pm1;
+            qp1 = on1 ? q1 : qp1;
+            qm1 = on1 ? r1 : qm1;
         }
+        // FP64 geometry (the LO solution feeds the conservative update); without
+        // cavities areasvol equals area on every wet layer
+        V2<double> av;
+        if constexpr (std::is_same<R, double>::value) av = lc.area;
+        else av = ld2(m.av_d + c);
+        const V2<double> h = ld2(m.h_d + c), hn = ld2(m.hn_d + c);
         // flux differences formed in FP64: each interface flux then leaves one
         // layer and enters the next with exactly opposite sign in every precision
-        const double LO0 = (T.x * (double)h.x + (slo0 + (fl0 - fl1)) * dt / (double)av.x) / (double)hn.x;
-        double LO1 = 0.0;
-        if (lay1) LO1 = (T.y * (double)h.y + (slo1 + (fl1 - fl2)) * dt / (double)av.y) / (double)hn.y;
-        else { pp1 = 0; pm1 = 0; }
-        st2(LO + tn + c, LO0, LO1);
-        st2(Pp + tn + c, pp0, pp1);
-        st2(Pm + tn + c, pm0, pm1);
+        const double LO0 =
+            (T[t].x * (double)h.x + (slo0 + (fl0[t] - fl1[t])) * dt / (double)av.x) / (double)hn.x;
+        // second layer branch-free (selected away below a one-layer pair)
+        const double L1 = (T[t].y * (double)h.y + (slo1 + (fl1[t] - fl2[t])) * dt / (double)av.y) / (double)hn.y;
+        st2(LO + tn + c, LO0, lay1 ? L1 : 0.0);
+        st2(Pp + tn + c, qp0, lay1 ? qp1 : (R)0);
+        st2(Pm + tn + c, qm0, lay1 ? qm1 : (R)0);
     }
 }
 
