@@ -490,37 +490,41 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
             const V2<double> ls = ld2(LO + tn + c), ts = ld2(ttf + tn + c);
             const double mxs0 = fmx(ls.x, ts.x), mns0 = fmn(ls.x, ts.x);
             const double mxs1 = fmx(ls.y, ts.y), mns1 = fmn(ls.y, ts.y);
-            double mx0[kDeg], mn0[kDeg], mx1[kDeg], mn1[kDeg];
-#pragma unroll
-            for (int j = 0; j < kDeg; ++j) {
-                mx0[j] = 0; mn0[j] = 0; mx1[j] = 0; mn1[j] = 0;
-                if (rg.oo[j] >= 0 && k0 < (int)rg.lr[j]) {
-                    const V2<double> l = ld2(LO + tn + (size_t)rg.oo[j] + k0);
-                    const V2<double> v = ld2(ttf + tn + (size_t)rg.oo[j] + k0);
-                    mx0[j] = fmx(l.x, v.x); mn0[j] = fmn(l.x, v.x);
-                    mx1[j] = fmx(l.y, v.y); mn1[j] = fmn(l.y, v.y);
-                }
-            }
+            // one pass over the ring: fan element e lies between ring nodes e and
+            // e + 1 (ring node 0 where the fan closes, e + 1 == nr), so element
+            // j - 1 is complete once ring node j is read; only the previous and
+            // the first ring node are kept (max and min are exact, so the order
+            // in which the elements enter the bounds does not matter)
+            double fx0 = 0, fn0 = 0, fx1 = 0, fn1 = 0, px0 = 0, pn0 = 0, px1 = 0, pn1 = 0;
             double vmax0 = 0, vmin0 = 0, vmax1 = 0, vmin1 = 0;
             bool first = true;
 #pragma unroll
-            for (int j = 0; j < kDeg; ++j) {
-                const int le = rg.le[j];
-                if (le == 0) continue;
-                const bool wrap = j + 1 >= nr;
-                const double bx0 = wrap ? mx0[0] : mx0[(j + 1) & (kDeg - 1)];
-                const double bn0 = wrap ? mn0[0] : mn0[(j + 1) & (kDeg - 1)];
-                const double bx1 = wrap ? mx1[0] : mx1[(j + 1) & (kDeg - 1)];
-                const double bn1 = wrap ? mn1[0] : mn1[(j + 1) & (kDeg - 1)];
-                const double emax0 = k0 < le ? fmx(fmx(mxs0, mx0[j]), bx0) : -bignumber;
-                const double emin0 = k0 < le ? fmn(fmn(mns0, mn0[j]), bn0) : bignumber;
-                const double emax1 = k0 + 1 < le ? fmx(fmx(mxs1, mx1[j]), bx1) : -bignumber;
-                const double emin1 = k0 + 1 < le ? fmn(fmn(mns1, mn1[j]), bn1) : bignumber;
-                vmax0 = first ? emax0 : fmx(vmax0, emax0);
-                vmin0 = first ? emin0 : fmn(vmin0, emin0);
-                vmax1 = first ? emax1 : fmx(vmax1, emax1);
-                vmin1 = first ? emin1 : fmn(vmin1, emin1);
-                first = false;
+            for (int j = 0; j <= kDeg; ++j) {
+                double cx0 = 0, cn0 = 0, cx1 = 0, cn1 = 0;
+                if (j < kDeg && rg.oo[j] >= 0 && k0 < (int)rg.lr[j]) {
+                    const V2<double> l = ld2(LO + tn + (size_t)rg.oo[j] + k0);
+                    const V2<double> v = ld2(ttf + tn + (size_t)rg.oo[j] + k0);
+                    cx0 = fmx(l.x, v.x); cn0 = fmn(l.x, v.x);
+                    cx1 = fmx(l.y, v.y); cn1 = fmn(l.y, v.y);
+                }
+                if (j == 0) {
+                    fx0 = cx0; fn0 = cn0; fx1 = cx1; fn1 = cn1;
+                } else if (rg.le[j - 1] != 0) {
+                    const int le = rg.le[j - 1];
+                    const bool wrap = j >= nr;
+                    const double bx0 = wrap ? fx0 : cx0, bn0 = wrap ? fn0 : cn0;
+                    const double bx1 = wrap ? fx1 : cx1, bn1 = wrap ? fn1 : cn1;
+                    const double emax0 = k0 < le ? fmx(fmx(mxs0, px0), bx0) : -bignumber;
+                    const double emin0 = k0 < le ? fmn(fmn(mns0, pn0), bn0) : bignumber;
+                    const double emax1 = k0 + 1 < le ? fmx(fmx(mxs1, px1), bx1) : -bignumber;
+                    const double emin1 = k0 + 1 < le ? fmn(fmn(mns1, pn1), bn1) : bignumber;
+                    vmax0 = first ? emax0 : fmx(vmax0, emax0);
+                    vmin0 = first ? emin0 : fmn(vmin0, emin0);
+                    vmax1 = first ? emax1 : fmx(vmax1, emax1);
+                    vmin1 = first ? emin1 : fmn(vmin1, emin1);
+                    first = false;
+                }
+                px0 = cx0; pn0 = cn0; px1 = cx1; pn1 = cn1;
             }
             tvmax0[t] = vmax0; tvmin0[t] = vmin0;
             tvmax1[t] = lay1 ? vmax1 : 0.0; tvmin1[t] = lay1 ? vmin1 : 0.0;
