@@ -64,6 +64,9 @@ SIGNATURES = {
     "tracer_get_diagnostics": (c_int, [c_int, F64]),
     "tracer_mesh_array_info": (c_int, [c_char_p, P_ll, P_int]),
     "tracer_get_mesh_array": (c_int, [c_char_p, c_void_p, c_ll]),
+    "tracer_mesh_begin": (c_int, [c_int, c_int, c_int, c_int, c_int]),
+    "tracer_set_mesh_array": (c_int, [c_char_p, c_void_p, c_ll]),
+    "tracer_mesh_end": (c_int, []),
     "tracer_get_work_array": (c_int, [c_int, c_char_p, F64, c_ll]),
     "tracer_time_steps": (c_int, [c_int, c_double, P_double]),
     "tracer_time_kernels": (c_int, [c_double, c_int, c_char_p, c_int, F64, c_int]),

@@ -1294,60 +1294,13 @@ struct Engine::Impl {
     float gscale = 1.0f, ginv = 1.0f;
     size_t tsn = 0, tst = 0, tse = 0;
     int wslots = 1;  // tracers the work arrays hold
+    int fan_fault = -1;  // first node whose elements do not form one fan (-1: none)
     int tune_waves = kWavesPerBlock;   // waves per block of the chunk kernels
     int tune_nt = 4;                   // most tracers per launch
     // host copy of each tracer as last set (levels below the bottom are never
     // touched by a step, so get_values and the all-values statistics read them here)
     std::vector<std::vector<double>> host_vals;
 };
-
-// the elements around node n in fan order: ring[j], ring[(j+1) mod R] and
-// the node form element fan[j] (fan[j] = -1: open end of a boundary fan)
-static void node_fan(const HostMesh &m, int n, std::vector<int> &ring, std::vector<int> &fan)
-{
-    const int ne = m.nod_in_elem2D_num[n];
-    std::vector<int> ea(ne), eb(ne), el(ne);
-    for (int j = 0; j < ne; ++j) {
-        el[j] = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
-        int o[2], q = 0;
-        for (int v = 0; v < 3; ++v) {
-            const int node = m.elem2D_nodes[3 * el[j] + v];
-            if (node != n && q < 2) o[q++] = node;
-        }
-        if (q != 2) throw std::runtime_error("degenerate element around a node");
-        ea[j] = o[0];
-        eb[j] = o[1];
-    }
-    // start at a ring node met by one element only (open fan), else anywhere
-    int start = ne ? ea[0] : -1;
-    for (int j = 0; j < ne; ++j)
-        for (int s = 0; s < 2; ++s) {
-            const int v = s ? eb[j] : ea[j];
-            int cnt = 0;
-            for (int i = 0; i < ne; ++i) cnt += (ea[i] == v) + (eb[i] == v);
-            if (cnt == 1) start = v;
-        }
-    ring.assign(1, start);
-    fan.clear();
-    std::vector<char> used(ne, 0);
-    int cur = start;
-    for (int it = 0; it < ne; ++it) {
-        int pick = -1, nxt = -1;
-        for (int i = 0; i < ne && pick < 0; ++i) {
-            if (used[i]) continue;
-            if (ea[i] == cur) { pick = i; nxt = eb[i]; }
-            else if (eb[i] == cur) { pick = i; nxt = ea[i]; }
-        }
-        if (pick < 0) throw std::runtime_error("the elements around a node do not form one fan");
-        used[pick] = 1;
-        fan.push_back(el[pick]);
-        cur = nxt;
-        if (nxt == start && it == ne - 1) break;  // closed fan
-        ring.push_back(nxt);
-    }
-    if ((int)fan.size() != ne) throw std::runtime_error("the elements around a node do not form one fan");
-    if ((int)ring.size() > (int)fan.size()) fan.push_back(-1);  // open fan: no element after the last ring node
-}
 
 template <class R>
 static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
@@ -1426,7 +1379,13 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
             nt[n].le[j] = (unsigned char)(m.nlevels[el] - 1);
             na[kDeg * (size_t)n + j] = (R)m.elem_area[el];
         }
-        node_fan(m, n, ring, fan);
+        // the cluster bounds of FCT walk the fan; a node without one (a
+        // caller-supplied element list that is not a fan) leaves its ring
+        // empty and rules out the limiters that read it
+        if (!node_fan(m, n, ring, fan)) {
+            if (I.fan_fault < 0) I.fan_fault = n;
+            continue;
+        }
         if ((int)ring.size() > kDeg) throw std::runtime_error("1-ring above " + std::to_string(kDeg) + " nodes");
         for (size_t j = 0; j < ring.size(); ++j) {
             rg[n].oo[j] = (int)I.ln.off[ring[j]];
@@ -1561,9 +1520,12 @@ Engine::~Engine()
 
 void Engine::set_spec(int tr, const TracerSpec &sp)
 {
+    Impl &I = *p_;
+    if ((sp.lim == LIM_FCT || sp.lim == LIM_LEARNED) && I.fan_fault >= 0)
+        throw std::runtime_error("FCT and LEARNED need the elements around every node to form one fan; node " +
+                                 std::to_string(I.fan_fault + 1) + " does not");
     spec_.at(tr) = sp;
     graph_valid_ = false;
-    Impl &I = *p_;
     if (sp.lim == LIM_LEARNED && !I.alpha) {
         const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
         auto zalloc = [&](size_t bytes) {

@@ -663,19 +663,7 @@ void derive_all(HostMesh &m)
     if (nl < 3) throw std::runtime_error("vertical grid not set");
     if ((int)m.nlevels.size() != T) throw std::runtime_error("levels not set");
 
-    // find_levels_min_e2n (lib/oce_mesh.F90:1522-1560)
-    m.nlevels_nod2D_min.assign(N, 0);
-    m.ulevels_nod2D_max.assign(N, 0);
-    for (int n = 0; n < N; ++n) {
-        int mn = 1 << 30, mx = 0;
-        for (int k = 0; k < m.nod_in_elem2D_num[n]; ++k) {
-            int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + k];
-            mn = std::min(mn, m.nlevels[el]);
-            mx = std::max(mx, m.ulevels[el]);
-        }
-        m.nlevels_nod2D_min[n] = mn;
-        m.ulevels_nod2D_max[n] = mx;
-    }
+    derive_levels_min(m);
 
     if (m.cartesian) cartesian_geometry(m);
     else spherical_geometry(m);
@@ -738,9 +726,33 @@ void derive_all(HostMesh &m)
         }
     }
 
+    derive_boundary_lay(m);
+    derive_node_edges(m);
+}
+
+void derive_levels_min(HostMesh &m)
+{
+    // find_levels_min_e2n (lib/oce_mesh.F90:1522-1560)
+    const int N = m.nod2D;
+    m.nlevels_nod2D_min.assign(N, 0);
+    m.ulevels_nod2D_max.assign(N, 0);
+    for (int n = 0; n < N; ++n) {
+        int mn = 1 << 30, mx = 0;
+        for (int k = 0; k < m.nod_in_elem2D_num[n]; ++k) {
+            int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + k];
+            mn = std::min(mn, m.nlevels[el]);
+            mx = std::max(mx, m.ulevels[el]);
+        }
+        m.nlevels_nod2D_min[n] = mn;
+        m.ulevels_nod2D_max[n] = mx;
+    }
+}
+
+void derive_boundary_lay(HostMesh &m)
+{
     // nboundary_lay (lib/oce_muscl_adv.F90:68-104)
-    m.nboundary_lay.assign(N, nl - 1);
-    for (int e = 0; e < E; ++e) {
+    m.nboundary_lay.assign(m.nod2D, m.nl - 1);
+    for (int e = 0; e < m.edge2D; ++e) {
         int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
         int t1 = m.edge_tri[2 * e], t2 = m.edge_tri[2 * e + 1];
         if (t1 < 0 || t2 < 0) {
@@ -752,9 +764,13 @@ void derive_all(HostMesh &m)
             m.nboundary_lay[n2] = std::min(m.nboundary_lay[n2], mn);
         }
     }
+}
 
+void derive_node_edges(HostMesh &m)
+{
     // node -> incident edges, sorted by edge index (the order of the
     // reference's edge-loop scatters)
+    const int N = m.nod2D, E = m.edge2D;
     m.n2e_ptr.assign(N + 1, 0);
     for (int e = 0; e < E; ++e) {
         m.n2e_ptr[m.edges[2 * e] + 1]++;
@@ -767,6 +783,137 @@ void derive_all(HostMesh &m)
         m.n2e[fill[m.edges[2 * e]]++] = e + 1;
         m.n2e[fill[m.edges[2 * e + 1]]++] = -(e + 1);
     }
+}
+
+bool node_fan(const HostMesh &m, int n, std::vector<int> &ring, std::vector<int> &fan)
+{
+    const int ne = m.nod_in_elem2D_num[n];
+    std::vector<int> ea(ne), eb(ne), el(ne);
+    for (int j = 0; j < ne; ++j) {
+        el[j] = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
+        int o[2], q = 0;
+        for (int v = 0; v < 3; ++v) {
+            const int node = m.elem2D_nodes[3 * el[j] + v];
+            if (node != n && q < 2) o[q++] = node;
+        }
+        if (q != 2) return false;  // degenerate element, or n is not one of its nodes
+        ea[j] = o[0];
+        eb[j] = o[1];
+    }
+    // start at a ring node met by one element only (open fan), else anywhere
+    int start = ne ? ea[0] : -1;
+    for (int j = 0; j < ne; ++j)
+        for (int s = 0; s < 2; ++s) {
+            const int v = s ? eb[j] : ea[j];
+            int cnt = 0;
+            for (int i = 0; i < ne; ++i) cnt += (ea[i] == v) + (eb[i] == v);
+            if (cnt == 1) start = v;
+        }
+    ring.assign(1, start);
+    fan.clear();
+    std::vector<char> used(ne, 0);
+    int cur = start;
+    for (int it = 0; it < ne; ++it) {
+        int pick = -1, nxt = -1;
+        for (int i = 0; i < ne && pick < 0; ++i) {
+            if (used[i]) continue;
+            if (ea[i] == cur) { pick = i; nxt = eb[i]; }
+            else if (eb[i] == cur) { pick = i; nxt = ea[i]; }
+        }
+        if (pick < 0) return false;
+        used[pick] = 1;
+        fan.push_back(el[pick]);
+        cur = nxt;
+        if (nxt == start && it == ne - 1) break;  // closed fan
+        ring.push_back(nxt);
+    }
+    if ((int)fan.size() != ne) return false;
+    if ((int)ring.size() > (int)fan.size()) fan.push_back(-1);  // open fan: no element after the last ring node
+    return true;
+}
+
+void check_mesh(const HostMesh &m)
+{
+    const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl, A = m.maxAdj;
+    auto bad = [](const std::string &what, long long i) {
+        throw std::runtime_error("mesh array " + what + ": bad entry at " + std::to_string(i));
+    };
+    auto size_is = [](const char *what, size_t got, long long want) {
+        if ((long long)got != want)
+            throw std::runtime_error(std::string("mesh array ") + what + ": " + std::to_string(got) +
+                                     " entries, expected " + std::to_string(want));
+    };
+    if (nl < 3 || N < 1 || T < 1 || E < 1 || A < 1) throw std::runtime_error("mesh dimensions must be positive, nl >= 3");
+    size_is("edges", m.edges.size(), 2ll * E);
+    size_is("edge_tri", m.edge_tri.size(), 2ll * E);
+    size_is("elem2D_nodes", m.elem2D_nodes.size(), 3ll * T);
+    size_is("nlevels", m.nlevels.size(), T);
+    size_is("ulevels", m.ulevels.size(), T);
+    size_is("nlevels_nod2D", m.nlevels_nod2D.size(), N);
+    size_is("ulevels_nod2D", m.ulevels_nod2D.size(), N);
+    size_is("nlevels_nod2D_min", m.nlevels_nod2D_min.size(), N);
+    size_is("ulevels_nod2D_max", m.ulevels_nod2D_max.size(), N);
+    size_is("nod_in_elem2D_num", m.nod_in_elem2D_num.size(), N);
+    size_is("nod_in_elem2D", m.nod_in_elem2D.size(), (long long)A * N);
+    size_is("edge_up_dn_tri", m.edge_up_dn_tri.size(), 2ll * E);
+    size_is("nboundary_lay", m.nboundary_lay.size(), N);
+    for (long long i = 0; i < 2ll * E; ++i)
+        if (m.edges[i] < 0 || m.edges[i] >= N) bad("edges", i);
+    for (int e = 0; e < E; ++e) {
+        if (m.edges[2 * e] == m.edges[2 * e + 1]) bad("edges", 2ll * e);
+        if (m.edge_tri[2 * e] < 0 || m.edge_tri[2 * e] >= T) bad("edge_tri", 2ll * e);
+        if (m.edge_tri[2 * e + 1] < -1 || m.edge_tri[2 * e + 1] >= T) bad("edge_tri", 2ll * e + 1);
+        for (int s = 0; s < 2; ++s)
+            if (m.edge_up_dn_tri[2 * e + s] < -1 || m.edge_up_dn_tri[2 * e + s] >= T) bad("edge_up_dn_tri", 2ll * e + s);
+    }
+    for (int el = 0; el < T; ++el) {
+        if (m.ulevels[el] < 1 || m.nlevels[el] <= m.ulevels[el] || m.nlevels[el] > nl) bad("nlevels/ulevels", el);
+        for (int v = 0; v < 3; ++v) {
+            const int n = m.elem2D_nodes[3 * el + v];
+            if (n < 0 || n >= N) bad("elem2D_nodes", 3ll * el + v);
+            // an element's layers lie inside each of its nodes' columns
+            // (FESOM's node levels are the max over the adjacent elements)
+            if (m.nlevels[el] > m.nlevels_nod2D[n] || m.ulevels[el] < m.ulevels_nod2D[n]) bad("nlevels_nod2D", n);
+        }
+    }
+    for (int n = 0; n < N; ++n) {
+        if (m.ulevels_nod2D[n] < 1 || m.nlevels_nod2D[n] <= m.ulevels_nod2D[n] || m.nlevels_nod2D[n] > nl)
+            bad("nlevels_nod2D/ulevels_nod2D", n);
+        if (m.nlevels_nod2D_min[n] < 1 || m.nlevels_nod2D_min[n] > m.nlevels_nod2D[n]) bad("nlevels_nod2D_min", n);
+        if (m.nboundary_lay[n] < 0 || m.nboundary_lay[n] > nl - 1) bad("nboundary_lay", n);
+        const int k = m.nod_in_elem2D_num[n];
+        if (k < 0 || k > A) bad("nod_in_elem2D_num", n);
+        for (int j = 0; j < k; ++j) {
+            const int el = m.nod_in_elem2D[(size_t)n * A + j];
+            if (el < 0 || el >= T) bad("nod_in_elem2D", (long long)n * A + j);
+            // the nodal gradient reads the element's layers in the node's column
+            if (m.nlevels[el] > m.nlevels_nod2D[n]) bad("nod_in_elem2D", (long long)n * A + j);
+        }
+    }
+    for (int e = 0; e < E; ++e)
+        for (int s = 0; s < 2; ++s) {
+            const int el = m.edge_tri[2 * e + s];
+            if (el < 0) continue;
+            for (int q = 0; q < 2; ++q)
+                if (m.nlevels[el] > m.nlevels_nod2D[m.edges[2 * e + q]]) bad("edge_tri", 2ll * e + s);
+        }
+    size_is("zbar", m.zbar.size(), nl);
+    size_is("Z", m.Z.size(), nl - 1);
+    size_is("depth", m.depth.size(), N);
+    size_is("coord_nod2D", m.coord_nod2D.size(), 2ll * N);
+    size_is("elem_cos", m.elem_cos.size(), T);
+    size_is("elem_area", m.elem_area.size(), T);
+    size_is("gradient_sca", m.gradient_sca.size(), 6ll * T);
+    size_is("edge_dxdy", m.edge_dxdy.size(), 2ll * E);
+    size_is("edge_cross_dxdy", m.edge_cross_dxdy.size(), 4ll * E);
+    size_is("area", m.area.size(), (long long)nl * N);
+    size_is("areasvol", m.areasvol.size(), (long long)nl * N);
+    size_is("hnode", m.hnode.size(), (long long)(nl - 1) * N);
+    size_is("hnode_new", m.hnode_new.size(), (long long)(nl - 1) * N);
+    size_is("helem", m.helem.size(), (long long)(nl - 1) * T);
+    size_is("Z_3d_n", m.Z_3d_n.size(), (long long)(nl - 1) * N);
+    size_is("zbar_3d_n", m.zbar_3d_n.size(), (long long)nl * N);
+    size_is("n2e_ptr", m.n2e_ptr.size(), N + 1ll);
 }
 
 double make_velocity(const HostMesh &m, double u_bg, double v_bg, double cell_amp, double shear,

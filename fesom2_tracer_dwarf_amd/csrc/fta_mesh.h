@@ -98,6 +98,23 @@ void set_levels_from_elements(HostMesh &m, const int *nlevels_elem);
 // level min/max, mesh_areas, mesh_auxiliary_arrays, ALE thicknesses,
 // up/down-wind triangles, nboundary_lay and the node->edge lists.
 void derive_all(HostMesh &m);
+// Pieces of derive_all that need only topology and levels, for meshes whose
+// geometry a caller supplies: nlevels_nod2D_min / ulevels_nod2D_max,
+// nboundary_lay, and the node->edge lists.
+void derive_levels_min(HostMesh &m);
+void derive_boundary_lay(HostMesh &m);
+void derive_node_edges(HostMesh &m);
+
+// The elements around node n in fan order: ring[j], ring[(j+1) mod R] and the
+// node form element fan[j] (fan[j] = -1: open end of a boundary fan). False
+// when nod_in_elem2D of n does not form one fan (e.g. an element list that
+// misses the elements across a periodic seam).
+bool node_fan(const HostMesh &m, int n, std::vector<int> &ring, std::vector<int> &fan);
+
+// Sizes, index ranges and the level nesting the device path relies on
+// (element layers inside each of its nodes' columns); throws with the array
+// and entry at fault.
+void check_mesh(const HostMesh &m);
 
 // Analytic divergence-free velocity on elements, (2,nl-1,T), plus W (nl,N)
 // from discrete continuity. Returns the dt giving the requested CFL.

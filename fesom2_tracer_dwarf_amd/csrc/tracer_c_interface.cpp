@@ -4,6 +4,7 @@
 // (lib/tracer_c_interface.F90:28-31); not thread-safe.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -36,6 +37,9 @@ struct State {
     int tune_waves = 4, tune_nt = 4;  // launch schedule carried across engine rebuilds
     FesomFileOptions file_opt;         // mesh-file reader settings (force_rotation)
     std::vector<FILE *> frames;        // open scalar-output files, one per tracer
+    bool mesh_building = false;        // between tracer_mesh_begin and tracer_mesh_end
+    std::vector<std::string> mesh_given;  // arrays set since tracer_mesh_begin
+    int fan_fault = -1;                // first node without an element fan (caller-supplied meshes)
 };
 
 State &S()
@@ -504,6 +508,8 @@ int tracer_load_mesh_from_files(const char *mesh_path)
         HostMesh m;
         read_fesom_mesh(m, p, s.file_opt);
         s.mesh = std::move(m);
+        s.fan_fault = -1;
+        s.mesh_building = false;
     } catch (const std::exception &e) {
         s.mesh_ready = false;
         return fail(std::string("tracer_load_mesh_from_files: ") + e.what());
@@ -561,10 +567,166 @@ int tracer_generate_mesh(int nx, int ny, int periodic, double Lx, double Ly, int
         set_levels_random(m, lev_lo, lev_hi, seed);
         derive_all(m);
         s.mesh = std::move(m);
+        s.fan_fault = -1;
+        s.mesh_building = false;
     } catch (const std::exception &e) {
         s.mesh_ready = false;
         return fail(e.what());
     }
+    s.mesh_ready = true;
+    s.initialized = true;
+    adopt_mesh_dims();
+    s.ntr = 0;
+    s.values.clear();
+    s.valuesAB.clear();
+    s.specs.clear();
+    s.uv.clear();
+    s.w.clear();
+    s.we.clear();
+    ensure_host_arrays();
+    return 0;
+}
+
+namespace {
+// A caller-supplied mesh array by name: the destination vector and the
+// entry count the dimensions fix (names and layouts as tracer_get_mesh_array)
+struct Slot { std::vector<int> *i; std::vector<double> *d; long long n; };
+
+bool mesh_slot(HostMesh &m, const std::string &name, Slot &r)
+{
+    const long long N = m.nod2D, T = m.elem2D, E = m.edge2D, L = m.nl, A = m.maxAdj;
+#define SI(NAME, VEC, CNT) if (name == NAME) { r = {&VEC, nullptr, CNT}; return true; }
+#define SD(NAME, VEC, CNT) if (name == NAME) { r = {nullptr, &VEC, CNT}; return true; }
+    SI("edges", m.edges, 2 * E) SI("edge_tri", m.edge_tri, 2 * E) SI("elem2D_nodes", m.elem2D_nodes, 3 * T)
+    SI("nlevels", m.nlevels, T) SI("ulevels", m.ulevels, T) SI("nlevels_nod2D", m.nlevels_nod2D, N)
+    SI("ulevels_nod2D", m.ulevels_nod2D, N) SI("nlevels_nod2D_min", m.nlevels_nod2D_min, N)
+    SI("ulevels_nod2D_max", m.ulevels_nod2D_max, N) SI("nod_in_elem2D_num", m.nod_in_elem2D_num, N)
+    SI("nod_in_elem2D", m.nod_in_elem2D, A * N) SI("edge_up_dn_tri", m.edge_up_dn_tri, 2 * E)
+    SI("nboundary_lay", m.nboundary_lay, N)
+    SD("coord_nod2D", m.coord_nod2D, 2 * N) SD("zbar", m.zbar, L) SD("Z", m.Z, L - 1) SD("depth", m.depth, N)
+    SD("elem_cos", m.elem_cos, T) SD("elem_area", m.elem_area, T) SD("gradient_sca", m.gradient_sca, 6 * T)
+    SD("edge_dxdy", m.edge_dxdy, 2 * E) SD("edge_cross_dxdy", m.edge_cross_dxdy, 4 * E)
+    SD("area", m.area, L * N) SD("areasvol", m.areasvol, L * N) SD("hnode", m.hnode, (L - 1) * N)
+    SD("hnode_new", m.hnode_new, (L - 1) * N) SD("helem", m.helem, (L - 1) * T) SD("Z_3d_n", m.Z_3d_n, (L - 1) * N)
+    SD("zbar_3d_n", m.zbar_3d_n, L * N)
+#undef SI
+#undef SD
+    return false;
+}
+
+bool given(const std::string &name)
+{
+    const auto &g = S().mesh_given;
+    return std::find(g.begin(), g.end(), name) != g.end();
+}
+}  // namespace
+
+int tracer_mesh_begin(int nl, int nod2D, int elem2D, int edge2D, int maxAdj)
+{
+    State &s = S();
+    if (nl < 3 || nod2D < 1 || elem2D < 1 || edge2D < 1 || maxAdj < 1)
+        return fail("tracer_mesh_begin: dimensions must be positive and nl >= 3");
+    drop_engine();
+    HostMesh m;
+    m.nl = nl;
+    m.nod2D = nod2D;
+    m.elem2D = elem2D;
+    m.edge2D = edge2D;
+    m.maxAdj = maxAdj;
+    s.mesh = std::move(m);
+    s.mesh_ready = false;
+    s.mesh_building = true;
+    s.mesh_given.clear();
+    s.fan_fault = -1;
+    return 0;
+}
+
+int tracer_set_mesh_array(const char *name, const void *data, long long count)
+{
+    State &s = S();
+    const std::string n = name ? name : "";
+    if (!s.mesh_building) return fail("tracer_set_mesh_array: call tracer_mesh_begin first");
+    // a refused array ends the mesh: tracer_mesh_begin starts over
+    Slot r;
+    if (!mesh_slot(s.mesh, n, r)) {
+        s.mesh_building = false;
+        return fail("tracer_set_mesh_array: unknown mesh array " + n);
+    }
+    if (count != r.n || !data) {
+        s.mesh_building = false;
+        return fail("tracer_set_mesh_array: " + n + " takes " + std::to_string(r.n) + " entries, got " +
+                    (data ? std::to_string(count) : std::string("none")));
+    }
+    if (r.i) r.i->assign((const int *)data, (const int *)data + count);
+    else r.d->assign((const double *)data, (const double *)data + count);
+    if (!given(n)) s.mesh_given.push_back(n);
+    return 0;
+}
+
+int tracer_mesh_end(void)
+{
+    State &s = S();
+    if (!s.mesh_building) return fail("tracer_mesh_end: call tracer_mesh_begin first");
+    HostMesh &m = s.mesh;
+    static const char *required[] = {"edges", "edge_tri", "elem2D_nodes", "nlevels", "nlevels_nod2D",
+                                     "nod_in_elem2D_num", "nod_in_elem2D", "zbar", "elem_cos", "elem_area",
+                                     "gradient_sca", "edge_dxdy", "edge_cross_dxdy", "area", "hnode", "helem"};
+    for (const char *r : required)
+        if (!given(r)) return fail(std::string("tracer_mesh_end: mesh array ") + r + " was not set");
+    const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
+    // defaults for what FESOM derives from the arrays above: no cavities,
+    // areasvol = area, time-level thickness = hnode, 3-D depths from zbar
+    if (!given("ulevels")) m.ulevels.assign(T, 1);
+    if (!given("ulevels_nod2D")) m.ulevels_nod2D.assign(N, 1);
+    if (!given("edge_up_dn_tri")) m.edge_up_dn_tri.assign(2 * (size_t)E, -1);
+    if (!given("areasvol")) m.areasvol = m.area;
+    if (!given("hnode_new")) m.hnode_new = m.hnode;
+    if (!given("coord_nod2D")) m.coord_nod2D.assign(2 * (size_t)N, 0.0);
+    if (!given("Z")) {
+        m.Z.resize(nl - 1);
+        for (int k = 0; k < nl - 1; ++k) m.Z[k] = 0.5 * (m.zbar[k] + m.zbar[k + 1]);
+    }
+    if (!given("Z_3d_n")) {
+        m.Z_3d_n.resize((size_t)(nl - 1) * N);
+        for (int n = 0; n < N; ++n) std::copy(m.Z.begin(), m.Z.end(), m.Z_3d_n.begin() + (size_t)n * (nl - 1));
+    }
+    if (!given("zbar_3d_n")) {
+        m.zbar_3d_n.resize((size_t)nl * N);
+        for (int n = 0; n < N; ++n) std::copy(m.zbar.begin(), m.zbar.end(), m.zbar_3d_n.begin() + (size_t)n * nl);
+    }
+    // placeholders that pass the range checks, replaced below by derivation
+    // from the (then validated) topology
+    if (!given("nlevels_nod2D_min")) m.nlevels_nod2D_min = m.nlevels_nod2D;
+    if (!given("ulevels_nod2D_max")) m.ulevels_nod2D_max.assign(N, 1);
+    if (!given("nboundary_lay")) m.nboundary_lay.assign(N, nl - 1);
+    if (!given("depth")) m.depth.assign(N, 0.0);
+    m.n2e_ptr.assign(N + 1, 0);
+    try {
+        check_mesh(m);
+        if (!given("nlevels_nod2D_min") || !given("ulevels_nod2D_max")) {
+            std::vector<int> keep_min = m.nlevels_nod2D_min, keep_max = m.ulevels_nod2D_max;
+            derive_levels_min(m);
+            if (given("nlevels_nod2D_min")) m.nlevels_nod2D_min = keep_min;
+            if (given("ulevels_nod2D_max")) m.ulevels_nod2D_max = keep_max;
+        }
+        if (!given("nboundary_lay")) derive_boundary_lay(m);
+        if (!given("depth")) {
+            for (int n = 0; n < N; ++n) m.depth[n] = m.zbar[m.nlevels_nod2D[n] - 1];
+        }
+        derive_node_edges(m);
+        m.coord.resize(2 * (size_t)N);
+        for (size_t i = 0; i < m.coord.size(); ++i) m.coord[i] = m.coord_nod2D[i] * kREarth;
+        check_mesh(m);
+    } catch (const std::exception &e) {
+        s.mesh_building = false;
+        return fail(std::string("tracer_mesh_end: ") + e.what());
+    }
+    std::vector<int> ring, fan;
+    s.fan_fault = -1;
+    for (int n = 0; n < N && s.fan_fault < 0; ++n)
+        if (!node_fan(m, n, ring, fan)) s.fan_fault = n;
+    s.mesh_building = false;
+    s.mesh_given.clear();
     s.mesh_ready = true;
     s.initialized = true;
     adopt_mesh_dims();
@@ -601,8 +763,17 @@ int tracer_set_schemes(int tracer_id, const char *hor, const char *ver, const ch
     if (check_vertical_scheme(sp.ver)) return -1;
     sp.ph = ph;
     sp.pv = pv;
+    if ((sp.lim == LIM_FCT || sp.lim == LIM_LEARNED) && s.fan_fault >= 0)
+        return fail("FCT and LEARNED need the elements around every node to form one fan; node " +
+                    std::to_string(s.fan_fault + 1) + " does not");
     s.specs[tracer_id - 1] = sp;
-    if (s.eng) s.eng->set_spec(tracer_id - 1, sp);
+    if (s.eng) {
+        try {
+            s.eng->set_spec(tracer_id - 1, sp);
+        } catch (const std::exception &e) {
+            return fail(e.what());
+        }
+    }
     return 0;
 }
 

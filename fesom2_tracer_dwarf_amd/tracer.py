@@ -14,6 +14,11 @@ import numpy as np
 from . import _lib
 
 
+MESH_INT_ARRAYS = frozenset({
+    "edges", "edge_tri", "elem2D_nodes", "nlevels", "ulevels", "nlevels_nod2D", "ulevels_nod2D", "nlevels_nod2D_min",
+    "ulevels_nod2D_max", "nod_in_elem2D_num", "nod_in_elem2D", "edge_up_dn_tri", "nboundary_lay"})
+
+
 class FESOMTracerAdvection:
     """Handle on the library's single instance (python/demo_tracer.py:40)."""
 
@@ -96,6 +101,17 @@ class FESOMTracerAdvection:
         _lib.check(self.lib.tracer_get_mesh_array(name.encode(), out.ctypes.data, cnt.value),
                    f"mesh array {name}", self.lib)
         return out
+
+    def set_mesh(self, nl, nod2D, elem2D, edge2D, maxAdj, arrays):
+        """A caller-supplied mesh: ``arrays`` maps the names of :meth:`mesh_array` to arrays in its layout
+        (0-based indices, -1 for no element). Index arrays go in as int32, the rest as float64."""
+        _lib.check(self.lib.tracer_mesh_begin(nl, nod2D, elem2D, edge2D, maxAdj), "tracer_mesh_begin", self.lib)
+        for name, a in arrays.items():
+            a = np.ascontiguousarray(np.ravel(a, order="A"), dtype=np.int32 if name in MESH_INT_ARRAYS else np.float64)
+            _lib.check(self.lib.tracer_set_mesh_array(name.encode(), a.ctypes.data, a.size), f"mesh array {name}",
+                       self.lib)
+        _lib.check(self.lib.tracer_mesh_end(), "tracer_mesh_end", self.lib)
+        return self.dims()
 
     def allocate_tracers(self, num_tracers, AB_order=1):
         self.num_tracers = int(num_tracers)

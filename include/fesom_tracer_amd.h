@@ -183,6 +183,32 @@ int tracer_mesh_array_info(const char *name, long long *count, int *kind);
  * -1 for no element) */
 int tracer_get_mesh_array(const char *name, void *out, long long count);
 
+/* No reference counterpart (the reference's host fills its t_mesh in
+ * Fortran, lib/MOD_MESH.F90): a mesh whose arrays the caller supplies, e.g.
+ * a FESOM host handing over its own geometry. tracer_mesh_begin fixes the
+ * dimensions; tracer_set_mesh_array copies one array in, by the names,
+ * layouts and kinds of tracer_get_mesh_array (int32 or float64, 0-based
+ * indices, -1 for "no element"; count must be the exact size); and
+ * tracer_mesh_end validates and adopts the mesh, after which tracers are
+ * allocated and velocities set as after tracer_generate_mesh.
+ *   required: edges, edge_tri, elem2D_nodes, nlevels, nlevels_nod2D,
+ *     nod_in_elem2D_num, nod_in_elem2D, zbar, elem_cos, elem_area,
+ *     gradient_sca, edge_dxdy, edge_cross_dxdy, area, hnode, helem
+ *   optional, with FESOM's derivation or default when absent:
+ *     nlevels_nod2D_min, ulevels_nod2D_max (find_levels_min_e2n),
+ *     nboundary_lay, ulevels / ulevels_nod2D (1), edge_up_dn_tri (-1: no
+ *     up/down-wind element), areasvol (= area), hnode_new (= hnode),
+ *     Z, Z_3d_n, zbar_3d_n (from zbar), depth, coord_nod2D (0)
+ * tracer_set_mesh_array fails on an unknown name or a wrong count, and
+ * tracer_mesh_end on a missing array, an index out of range, or an element
+ * whose layers are not inside each of its nodes' columns; either failure
+ * leaves no mesh (begin again). FCT and LEARNED need nod_in_elem2D to list, for every node, the
+ * elements of one fan around it; tracer_set_schemes refuses them otherwise,
+ * and the other schemes run. */
+int tracer_mesh_begin(int nl, int nod2D, int elem2D, int edge2D, int maxAdj);
+int tracer_set_mesh_array(const char *name, const void *data, long long count);
+int tracer_mesh_end(void);
+
 /* No reference counterpart: a named GPU work array of the last step of one
  * tracer (1-based id) in the reference layout: fct_LO, adv_flux_hor (HO-LO
  * edge flux before limiting; the HO flux without a limiter), adv_flux_ver
