@@ -28,6 +28,10 @@ HIP_FLAGS = [
     # matrix-core results in ordinary VGPRs: the learned limiter's epilogue
     # reads every accumulator, and copies out of AGPRs would double its VALU work
     "-mllvm", "-amdgpu-mfma-vgpr-form",
+    # device code assumes no NaN arithmetic: max/min then skip the IEEE-mode
+    # quieting of each operand (one extra v_max per operand, a fifth of the
+    # limiter's instructions); NaN results are still counted by bit tests
+    "-Xarch_device", "-fno-honor-nans",
     "-Wno-unused-result",
 ]
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",

@@ -1114,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_diag_partial(DMesh<R> m, const double *
             if (k0 + q >= L) break;
             const long long i = 2 * s + q;
             const double x = v[i];
-            if (x != x) { a.nan += 1.0; continue; }
+            if (is_nan_bits(x)) { a.nan += 1.0; continue; }
             a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
             a.mass += x * m.hn_d[i] * m.av_d[i];
         }
@@ -2078,7 +2078,7 @@ Diag Engine::diagnostics(int tr)
     for (int n = 0; n < N_; ++n)
         for (int k = I.ln.len[n] - 1; k < L; ++k) {
             const double x = h[(size_t)n * L + k];
-            if (x != x) { d.nan_count += 1.0; continue; }
+            if (is_nan_bits(x)) { d.nan_count += 1.0; continue; }
             d.vmin = std::min(d.vmin, x);
             d.vmax = std::max(d.vmax, x);
             d.sum += x;

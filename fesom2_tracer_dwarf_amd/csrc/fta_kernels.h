@@ -151,6 +151,16 @@ __device__ __forceinline__ double fmn(double a, double b) { return __builtin_fmi
 __device__ __forceinline__ float fmx(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ float fmn(float a, float b) { return __builtin_fminf(a, b); }
 
+// NaN test on the bit pattern: device code is compiled assuming no NaN
+// arithmetic (so that max/min need no operand canonicalisation), under which
+// x != x may fold to false
+__host__ __device__ __forceinline__ bool is_nan_bits(double x)
+{
+    unsigned long long u;
+    __builtin_memcpy(&u, &x, 8);
+    return (u & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull;
+}
+
 // lane l receives lane l-1's / l+1's value (own value at the wave ends)
 template <class T> __device__ __forceinline__ T from_prev(T v) { return __shfl_up(v, 1, kWave); }
 template <class T> __device__ __forceinline__ T from_next(T v) { return __shfl_down(v, 1, kWave); }
