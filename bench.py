@@ -179,6 +179,7 @@ def main():
     cfg["nx"], cfg["ny"], cfg["nl"] = args.nx, args.ny, args.nl
     ntr = cfg["ntr"]
     t_setup = time.perf_counter()
+    api.set_option("device_geometry", 1)  # planar mesh: the same arrays as the host derivation, bit for bit
     api.generate_mesh(cfg["nx"], cfg["ny"], cfg["nl"], periodic=True, Lx=cfg["Lx"], Ly=cfg["Ly"], dz=cfg["dz"],
                       lev_lo=cfg["lev"][0], lev_hi=cfg["lev"][1], seed=0, sfc=True)
     api.allocate_tracers(ntr)

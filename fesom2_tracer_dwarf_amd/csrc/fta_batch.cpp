@@ -165,7 +165,7 @@ int tracer_batch_add_periodic_mesh(int nx, int ny, int nl, double Lx, double Ly,
         generate_periodic(b->mesh, nx, ny, Lx, Ly, 1);
         set_vertical_grid(b->mesh, nl, dz, 0.0);
         set_levels_random(b->mesh, lev_lo, lev_hi, seed);
-        derive_all(b->mesh);
+        derive_all_device(b->mesh);  // the batch runs on the GPU anyway
         b->Lx = Lx;
         b->Ly = Ly;
         std::vector<double> uv, w;

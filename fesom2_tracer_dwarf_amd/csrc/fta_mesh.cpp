@@ -458,7 +458,8 @@ void read_fesom_mesh(HostMesh &m, const std::string &dir_in, const FesomFileOpti
                     throw std::runtime_error("nlvls.out: node " + std::to_string(n + 1) +
                                              " is not the deepest of its elements");
     }
-    derive_all(m);
+    if (opt.device_geometry) derive_all_device(m);
+    else derive_all(m);
 }
 
 // Geometry of a planar mesh in metres (the generator's space): orientation
@@ -659,14 +660,26 @@ static void spherical_geometry(HostMesh &m)
 
 void derive_all(HostMesh &m)
 {
-    const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
-    if (nl < 3) throw std::runtime_error("vertical grid not set");
-    if ((int)m.nlevels.size() != T) throw std::runtime_error("levels not set");
+    check_derivable(m);
 
     derive_levels_min(m);
-
     if (m.cartesian) cartesian_geometry(m);
     else spherical_geometry(m);
+    derive_columns(m);
+    derive_up_down(m);
+    derive_boundary_lay(m);
+    derive_node_edges(m);
+}
+
+void check_derivable(const HostMesh &m)
+{
+    if (m.nl < 3) throw std::runtime_error("vertical grid not set");
+    if ((int)m.nlevels.size() != m.elem2D) throw std::runtime_error("levels not set");
+}
+
+void derive_columns(HostMesh &m)
+{
+    const int N = m.nod2D, T = m.elem2D, nl = m.nl;
     if ((int)m.depth.size() != N) {  // generated meshes: the bottom of each node's deepest layer
         m.depth.assign(N, 0.0);
         for (int n = 0; n < N; ++n) m.depth[n] = m.zbar[m.nlevels_nod2D[n] - 1];
@@ -688,7 +701,11 @@ void derive_all(HostMesh &m)
     }
     for (int el = 0; el < T; ++el)
         for (int k = 0; k < nl - 1; ++k) m.helem[(size_t)el * (nl - 1) + k] = m.zbar[k] - m.zbar[k + 1];
+}
 
+void derive_up_down(HostMesh &m)
+{
+    const int E = m.edge2D;
     // find_up_downwind_triangles (lib/oce_muscl_adv.F90:166-357) on the
     // radian coordinates; both axes trimmed; the last matching element wins
     {
@@ -725,9 +742,6 @@ void derive_all(HostMesh &m)
             }
         }
     }
-
-    derive_boundary_lay(m);
-    derive_node_edges(m);
 }
 
 void derive_levels_min(HostMesh &m)

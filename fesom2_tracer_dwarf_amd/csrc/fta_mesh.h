@@ -74,6 +74,7 @@ struct FesomFileOptions {
     bool force_rotation = true;
     double alpha_deg = 50.0, beta_deg = 15.0, gamma_deg = -90.0;  // lib/gen_modules_config.F90:85-88
     double cyclic_length_deg = 360.0;                             // lib/gen_modules_config.F90:83
+    bool device_geometry = false;  // derive_all_device instead of derive_all
 };
 void read_fesom_mesh(HostMesh &m, const std::string &dir, const FesomFileOptions &opt);
 
@@ -98,6 +99,18 @@ void set_levels_from_elements(HostMesh &m, const int *nlevels_elem);
 // level min/max, mesh_areas, mesh_auxiliary_arrays, ALE thicknesses,
 // up/down-wind triangles, nboundary_lay and the node->edge lists.
 void derive_all(HostMesh &m);
+// The same derivation with the geometry (mesh_areas, mesh_auxiliary_arrays)
+// run as GPU kernels on the current device; the integer, column and
+// up/down-wind steps stay on the host. Results equal derive_all's bit for
+// bit on planar meshes; on the sphere elem_cos and what depends on it may
+// differ in the last bits (device cos).
+void derive_all_device(HostMesh &m);
+
+// Stages of derive_all: vertical grid and levels present; depth, ALE
+// thicknesses and 3-D depths from zbar; find_up_downwind_triangles.
+void check_derivable(const HostMesh &m);
+void derive_columns(HostMesh &m);
+void derive_up_down(HostMesh &m);
 // Pieces of derive_all that need only topology and levels, for meshes whose
 // geometry a caller supplies: nlevels_nod2D_min / ulevels_nod2D_max,
 // nboundary_lay, and the node->edge lists.

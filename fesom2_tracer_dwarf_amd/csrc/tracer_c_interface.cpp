@@ -565,7 +565,8 @@ int tracer_generate_mesh(int nx, int ny, int periodic, double Lx, double Ly, int
         else generate_closed(m, nx, ny, Lx, Ly, sfc ? 1 : 0);
         set_vertical_grid(m, nl, dz, stretch);
         set_levels_random(m, lev_lo, lev_hi, seed);
-        derive_all(m);
+        if (s.file_opt.device_geometry) derive_all_device(m);
+        else derive_all(m);
         s.mesh = std::move(m);
         s.fan_fault = -1;
         s.mesh_building = false;
@@ -783,6 +784,10 @@ int tracer_set_option(const char *name, int value)
     std::string n = name ? name : "";
     if (n == "force_rotation") {  // mesh-file reader: rotate nod2d.out coordinates (default 1)
         s.file_opt.force_rotation = value != 0;
+        return 0;
+    }
+    if (n == "device_geometry") {  // mesh geometry and up/down-wind search as GPU kernels (default 0)
+        s.file_opt.device_geometry = value != 0;
         return 0;
     }
     if (n == "precision") {

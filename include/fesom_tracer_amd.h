@@ -166,7 +166,12 @@ int tracer_get_values_id(int tracer_id, double *values, int nz, int nn);
  *   "block_waves"        4 (default): waves per workgroup of the chunk kernels (1, 2, 4, 8)
  *   "batch_tracers"      4 (default): most tracers per launch (1, 2, 4)
  *   "force_rotation"     1 (default): tracer_load_mesh_from_files rotates the
- *                        node coordinates to the model grid; 0: as read */
+ *                        node coordinates to the model grid; 0: as read
+ *   "device_geometry"    0 (default); 1: tracer_generate_mesh and
+ *                        tracer_load_mesh_from_files derive the geometry
+ *                        (mesh_areas, mesh_auxiliary_arrays) with GPU
+ *                        kernels; planar meshes come out bit for bit as
+ *                        with 0 */
 int tracer_set_option(const char *name, int value);
 
 /* No reference counterpart: nsteps steps of ALL tracers */

@@ -144,6 +144,31 @@ def test_diagnostics_match_host(api):
     assert st["min"] == g1.min() and st["max"] == g1.max()
 
 
+def test_diagnostics_count_nans(api):
+    """NaN values (quiet and signalling patterns, either sign) in wet and dry cells are counted and left out
+    of min/max/sum, as the reference's isnan count (src/fesom_analytic.F90) does."""
+    print("[gpu] diagnostics NaN count", flush=True)
+    d, dt, trs = setup_case(api, MESHES["periodic_24x20x12_random_bottom"], ("MUSCL", "QR4C", "FCT"), ntr=1)
+    m = mesh_dict(api)
+    L = d["nl"] - 1
+    act = np.arange(L)[:, None] < (m["nlevels_nod2D"] - 1)[None, :]
+    t = np.array(trs[0], order="F", copy=True)
+    wet, dry = np.argwhere(act), np.argwhere(~act)
+    nans = np.array([0x7FF8000000000000, 0xFFF8000000000001, 0x7FF0000000000001, 0xFFF4000000000000],
+                    dtype=np.uint64).view(np.float64)
+    for q, (k, n) in enumerate(wet[[3, 50, 400]]):
+        t[k, n] = nans[q]
+    k, n = dry[7]
+    t[k, n] = nans[3]
+    api.set_values(1, t)
+    dg = api.diagnostics(1)
+    assert dg["nan_count"] == 4
+    ok = ~np.isnan(t)
+    assert dg["min"] == t[ok].min() and dg["max"] == t[ok].max()
+    assert abs(dg["sum"] - t[ok].sum()) <= 1e-13 * abs(t[ok]).sum()
+    assert dg["active_min"] == t[act & ok].min() and dg["active_max"] == t[act & ok].max()
+
+
 def test_schedule_does_not_change_results_and_autotune_restores(api):
     """Every launch schedule gives bitwise the same tracers; the autotuner picks one and restores the state."""
     print("[gpu] schedules and autotuner", flush=True)
