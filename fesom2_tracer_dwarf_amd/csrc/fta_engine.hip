@@ -1295,6 +1295,7 @@ struct Engine::Impl {
     size_t tsn = 0, tst = 0, tse = 0;
     int wslots = 1;  // tracers the work arrays hold
     int fan_fault = -1;  // first node whose elements do not form one fan (-1: none)
+    bool we_is_w = false;  // w_e equals w (no implicit vertical split): one device copy serves both
     int tune_waves = kWavesPerBlock;   // waves per block of the chunk kernels
     int tune_nt = 4;                   // most tracers per launch
     // host copy of each tracer as last set (levels below the bottom are never
@@ -1558,7 +1559,10 @@ void Engine::set_velocity(const double *uv, const double *w, const double *we)
     };
     put(I.vel, I.lt, uv, nl_ - 1, 2);
     put(I.W, I.ln, w, nl_, 1);
-    put(I.WE, I.ln, we, nl_, 1);
+    const bool same = w == we || std::memcmp(w, we, sizeof(double) * (size_t)nl_ * N_) == 0;
+    if (!same) put(I.WE, I.ln, we, nl_, 1);
+    if (same != I.we_is_w) graph_valid_ = false;  // the captured launches name the other array
+    I.we_is_w = same;
 }
 
 void Engine::set_values(int tr, const double *v)
@@ -1617,7 +1621,7 @@ template <class R, class G, int NT>
 static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &sp, double dt, int t0, bool refresh,
                          hipStream_t st, LaunchHook &hk)
 {
-    const R *vel = (const R *)I.vel, *W = (const R *)I.W, *WE = (const R *)I.WE;
+    const R *vel = (const R *)I.vel, *W = (const R *)I.W, *WE = (const R *)(I.we_is_w ? I.W : I.WE);
     const size_t tsn = I.tsn, tst = I.tst, tse = I.tse, tsg = 2 * I.tsn;
     double *vals = I.values + t0 * tsn;
     if (!refresh && !I.valuesAB) throw std::runtime_error("valuesAB was never set");

@@ -218,6 +218,35 @@ def test_stale_valuesAB_matches_oracle(api):
     api.set_option("refresh_valuesAB", 1)
 
 
+def test_distinct_w_e_then_equal_matches_oracle(api):
+    """w_e different from w (the LO vertical flux reads w_e, the HO one w), then w_e = w again, whose single
+    device copy serves both; graph-replayed steps follow the switch."""
+    import oracle as orc
+    print("[gpu] distinct w_e, then w_e = w", flush=True)
+    d, dt, trs = setup_case(api, MESHES["periodic_24x20x12_random_bottom"], ("MUSCL", "QR4C", "FCT"), ntr=2)
+    m = mesh_dict(api)
+    o = orc.Oracle(m)
+    uv, w = api.mesh_array("uv"), api.mesh_array("w")
+    L, N, T = d["nl"] - 1, d["nod2D"], d["elem2D"]
+    uv3 = uv.reshape(T, L, 2).transpose(2, 1, 0)
+    w2 = np.asfortranarray(w.reshape(N, d["nl"]).T)
+    ref = [np.array(t, order="F", copy=True) for t in trs]
+    for we2 in (np.asfortranarray(0.6 * w2), w2):
+        api.set_velocity_elem(uv3, w2, we2)
+        for step in range(2):
+            api.advect_steps(1, dt)
+            for i in range(2):
+                o.step(dt, uv, w, np.ravel(we2, order="F"), ref[i], hor="MUSCL", ver="QR4C", lim="FCT")
+                err = rel_err(api.get_tracer_values(i + 1), ref[i])
+                assert err <= RTOL, f"tracer {i + 1} step {step + 1}: rel err {err:.3e}"
+        api.advect_steps(3, dt)  # a replayed multi-step run under the current velocity
+        for i in range(2):
+            for _ in range(3):
+                o.step(dt, uv, w, np.ravel(we2, order="F"), ref[i], hor="MUSCL", ver="QR4C", lim="FCT")
+            err = rel_err(api.get_tracer_values(i + 1), ref[i])
+            assert err <= RTOL, f"tracer {i + 1} after replay: rel err {err:.3e}"
+
+
 @pytest.mark.parametrize("schemes", [("MUSCL", "QR4C", "FCT"), ("MFCT", "PPM", "FCT")], ids=lambda s: "-".join(s))
 def test_new_layer_thickness_matches_oracle(api, schemes):
     """tracer_set_thickness (lib/tracer_c_interface.F90:477-503) sets hnode_new only: the LO solution, the
