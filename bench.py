@@ -57,7 +57,7 @@ def step_alg_bytes(c, ntr, s_t, s_v, s_m):
             + c["E"] * (16 + 6 * s_m) + c["T"] * (20 + 7 * s_m) + 8 * c["N"] + 4 * c["maxAdj"] * c["N"])
 
 
-def kernel_alg_bytes(name, c, ntr, s_t, s_r, s_g):
+def kernel_alg_bytes(name, c, ntr, s_t, s_r, s_g, we_alias=True):
     """Compulsory bytes of one step's launch of a kernel (all ntr tracers
     batched into it): every array it reads or writes counted once over its
     active cells (C node layers, C+N node interfaces, Ce element layers, Ced
@@ -72,13 +72,18 @@ def kernel_alg_bytes(name, c, ntr, s_t, s_r, s_g):
         return ntr * (2 * s_g * Ce + 2 * s_g * C) + N * (48 + 8 * s_r)
     if name == "k_edge":          # tracer (T and AB share storage), gradients, vel, helem, edge records -> transport, HO-LO
         return ntr * (s_t * C + 2 * s_g * C + s_r * Ced) + s_r * Ced + 3 * s_r * Ce + E * (32 + 8 * s_r)
-    if name == "k_node_lo":       # edge transport, HO-LO edge fluxes, tracer, w, w_e, 6 geometry arrays -> LO, adf_v, P+, P-
+    if name == "k_node_lo":
+        # edge transport, HO-LO edge fluxes, tracer -> LO (FP64), adf_v, P+, P-; on interfaces w (w_e shares
+        # its storage when equal, as here), area, zbar_3d_n; on layers Z_3d_n and the FP64 hnode, hnode_new
+        # (and the FP64 areasvol copy in FP32; FP64 reads area, equal to it on wet layers)
+        n_if = 3 if we_alias else 4
+        geo = s_r * C + 16 * C + (8 * C if s_r == 4 else 0)
         return (ntr * (s_r * Ced + s_t * C + 8 * C + s_r * CI + 2 * s_r * C)
-                + s_r * Ced + 4 * s_r * CI + 4 * s_r * C + 80 * N)
+                + s_r * Ced + n_if * s_r * CI + geo + 80 * N)
     if name == "k_node_limit":    # LO, tracer, P+/P- -> R+/R-; areasvol, 1-ring records
         return ntr * (8 * C + s_t * C + 4 * s_r * C) + s_r * C + 48 * N
-    if name == "k_node_update":   # HO-LO edge flux, adf_v, R+/R- (own and 1-ring), LO, tracer -> tracer; 3 geometry arrays
-        return ntr * (s_r * Ced + s_r * CI + 2 * s_r * C + 8 * C + 2 * s_t * C) + 3 * s_r * C + 80 * N
+    if name == "k_node_update":   # HO-LO edge flux, adf_v, R+/R- (own and 1-ring), LO, tracer -> tracer; FP64 areasvol, hnode, hnode_new
+        return ntr * (s_r * Ced + s_r * CI + 2 * s_r * C + 8 * C + 2 * s_t * C) + 24 * C + 80 * N
     if name == "k_node_nolim":
         return ntr * (s_r * Ced + 2 * s_t * C) + 2 * s_r * CI + 2 * s_r * C + 48 * N
     return 0
