@@ -194,32 +194,29 @@ __global__ __launch_bounds__(kGeoBlock) void k_geo_edge(GeoMesh g, const double 
 }
 
 // area(nz, n): elem_area/3 summed over the node's elements that have layer
-// nz, in nod_in_elem2D order; areasvol = area on the node's own layers;
-// on the sphere all three are scaled by r_earth^2 afterwards (elem_area too)
+// nz, in nod_in_elem2D order (the host's order, so the sums agree bit for
+// bit); areasvol = area on the node's own layers; on the sphere all three are
+// scaled by r_earth^2 afterwards (elem_area too). One thread per (node,
+// level), consecutive threads on consecutive levels of a column.
 template <bool SPH>
-__global__ __launch_bounds__(kGeoBlock) void k_geo_node(GeoMesh g, double *__restrict__ elem_area,
+__global__ __launch_bounds__(kGeoBlock) void k_geo_node(GeoMesh g, const double *__restrict__ elem_area,
                                                     double *__restrict__ area, double *__restrict__ areasvol)
 {
-    const int n = blockIdx.x * kGeoBlock + threadIdx.x;
-    if (n >= g.N) return;
-    double *a = area + (size_t)n * g.nl;
-    double *v = areasvol + (size_t)n * g.nl;
-    for (int k = 0; k < g.nl; ++k) a[k] = 0.0;
+    const long long i = (long long)blockIdx.x * kGeoBlock + threadIdx.x;
+    if (i >= (long long)g.N * g.nl) return;
+    const int n = (int)(i / g.nl), nz = (int)(i % g.nl) + 1;
+    double x = 0.0;
     for (int j = 0; j < g.nien[n]; ++j) {
         const int el = g.nie[(size_t)n * g.maxAdj + j];
-        const double add = elem_area[el] / 3.0;
-        for (int nz = g.ulev[el]; nz <= g.nlev[el] - 1; ++nz) a[nz - 1] += add;
+        if (nz >= g.ulev[el] && nz <= g.nlev[el] - 1) x += elem_area[el] / 3.0;
     }
-    for (int k = 0; k < g.nl; ++k) {
-        const int nz = k + 1;
-        double x = a[k], y = (nz >= g.ulevn[n] && nz <= g.nlevn[n] - 1) ? x : 0.0;
-        if constexpr (SPH) {
-            x = x * kREarth * kREarth;
-            y = y * kREarth * kREarth;
-            a[k] = x;
-        }
-        v[k] = y;
+    double y = (nz >= g.ulevn[n] && nz <= g.nlevn[n] - 1) ? x : 0.0;
+    if constexpr (SPH) {
+        x = x * kREarth * kREarth;
+        y = y * kREarth * kREarth;
     }
+    area[i] = x;
+    areasvol[i] = y;
 }
 
 __global__ __launch_bounds__(kGeoBlock) void k_geo_scale_area(double *__restrict__ elem_area, int T)
@@ -312,7 +309,7 @@ void run(HostMesh &m)
         GEO_OK(hipGetLastError());
         k_geo_edge<SPH><<<blocks(E), kGeoBlock>>>(g, ec, ct, dx, cr, bad);
         GEO_OK(hipGetLastError());
-        k_geo_node<SPH><<<blocks(N), kGeoBlock>>>(g, ea, ar, av);
+        k_geo_node<SPH><<<dim3((unsigned)(((long long)N * nl + kGeoBlock - 1) / kGeoBlock)), kGeoBlock>>>(g, ea, ar, av);
         GEO_OK(hipGetLastError());
         if (SPH) {
             k_geo_scale_area<<<blocks(T), kGeoBlock>>>(ea, T);

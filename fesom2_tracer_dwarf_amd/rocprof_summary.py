@@ -25,6 +25,7 @@ COPY_BYTES = 1 << 30  # bench.py --calibrate-copy: 2^27 float64 each way per lau
 
 
 def short_name(kernel: str) -> str:
+    kernel = kernel.replace("(anonymous namespace)::", "")
     m = re.match(r"void fta::(\w+)<([^>]*)>", kernel)
     if m:
         return f"{m.group(1)}<{m.group(2).replace('fta::', '')}>"
