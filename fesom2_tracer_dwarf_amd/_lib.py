@@ -65,6 +65,7 @@ SIGNATURES = {
     "tracer_mesh_array_info": (c_int, [c_char_p, P_ll, P_int]),
     "tracer_get_mesh_array": (c_int, [c_char_p, c_void_p, c_ll]),
     "tracer_mesh_begin": (c_int, [c_int, c_int, c_int, c_int, c_int]),
+    "tracer_restart_file_copy": (c_int, [c_char_p, c_char_p, c_char_p]),
     "tracer_set_mesh_array": (c_int, [c_char_p, c_void_p, c_ll]),
     "tracer_mesh_end": (c_int, []),
     "tracer_get_work_array": (c_int, [c_int, c_char_p, F64, c_ll]),

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <numeric>
 #include <stdexcept>
 #include <unordered_map>
@@ -844,6 +845,59 @@ bool node_fan(const HostMesh &m, int n, std::vector<int> &ring, std::vector<int>
     if ((int)fan.size() != ne) return false;
     if ((int)ring.size() > (int)fan.size()) fan.push_back(-1);  // open fan: no element after the last ring node
     return true;
+}
+
+void complete_mesh(HostMesh &m, const std::function<bool(const std::string &)> &given)
+{
+    static const char *required[] = {"edges", "edge_tri", "elem2D_nodes", "nlevels", "nlevels_nod2D",
+                                     "nod_in_elem2D_num", "nod_in_elem2D", "zbar", "elem_cos", "elem_area",
+                                     "gradient_sca", "edge_dxdy", "edge_cross_dxdy", "area", "hnode", "helem"};
+    for (const char *r : required)
+        if (!given(r)) throw std::runtime_error(std::string("mesh array ") + r + " was not set");
+    const int N = m.nod2D, T = m.elem2D, E = m.edge2D, nl = m.nl;
+    if (nl < 3 || N < 1 || T < 1 || E < 1) throw std::runtime_error("mesh dimensions must be positive, nl >= 3");
+    if ((int)m.zbar.size() != nl) throw std::runtime_error("mesh array zbar: wrong size");
+    // defaults for what FESOM derives from the arrays above: no cavities,
+    // areasvol = area, time-level thickness = hnode, 3-D depths from zbar
+    if (!given("ulevels")) m.ulevels.assign(T, 1);
+    if (!given("ulevels_nod2D")) m.ulevels_nod2D.assign(N, 1);
+    if (!given("edge_up_dn_tri")) m.edge_up_dn_tri.assign(2 * (size_t)E, -1);
+    if (!given("areasvol")) m.areasvol = m.area;
+    if (!given("hnode_new")) m.hnode_new = m.hnode;
+    if (!given("coord_nod2D")) m.coord_nod2D.assign(2 * (size_t)N, 0.0);
+    if (!given("Z")) {
+        m.Z.resize(nl - 1);
+        for (int k = 0; k < nl - 1; ++k) m.Z[k] = 0.5 * (m.zbar[k] + m.zbar[k + 1]);
+    }
+    if (!given("Z_3d_n")) {
+        m.Z_3d_n.resize((size_t)(nl - 1) * N);
+        for (int n = 0; n < N; ++n) std::copy(m.Z.begin(), m.Z.end(), m.Z_3d_n.begin() + (size_t)n * (nl - 1));
+    }
+    if (!given("zbar_3d_n")) {
+        m.zbar_3d_n.resize((size_t)nl * N);
+        for (int n = 0; n < N; ++n) std::copy(m.zbar.begin(), m.zbar.end(), m.zbar_3d_n.begin() + (size_t)n * nl);
+    }
+    // placeholders that pass the range checks, replaced below by derivation
+    // from the (then validated) topology
+    if (!given("nlevels_nod2D_min")) m.nlevels_nod2D_min = m.nlevels_nod2D;
+    if (!given("ulevels_nod2D_max")) m.ulevels_nod2D_max.assign(N, 1);
+    if (!given("nboundary_lay")) m.nboundary_lay.assign(N, nl - 1);
+    if (!given("depth")) m.depth.assign(N, 0.0);
+    m.n2e_ptr.assign(N + 1, 0);
+    check_mesh(m);
+    if (!given("nlevels_nod2D_min") || !given("ulevels_nod2D_max")) {
+        std::vector<int> keep_min = m.nlevels_nod2D_min, keep_max = m.ulevels_nod2D_max;
+        derive_levels_min(m);
+        if (given("nlevels_nod2D_min")) m.nlevels_nod2D_min = keep_min;
+        if (given("ulevels_nod2D_max")) m.ulevels_nod2D_max = keep_max;
+    }
+    if (!given("nboundary_lay")) derive_boundary_lay(m);
+    if (!given("depth"))
+        for (int n = 0; n < N; ++n) m.depth[n] = m.zbar[m.nlevels_nod2D[n] - 1];
+    derive_node_edges(m);
+    m.coord.resize(2 * (size_t)N);
+    for (size_t i = 0; i < m.coord.size(); ++i) m.coord[i] = m.coord_nod2D[i] * kREarth;
+    check_mesh(m);
 }
 
 void check_mesh(const HostMesh &m)

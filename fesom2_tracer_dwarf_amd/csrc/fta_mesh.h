@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -123,6 +124,13 @@ void derive_node_edges(HostMesh &m);
 // when nod_in_elem2D of n does not form one fan (e.g. an element list that
 // misses the elements across a periodic seam).
 bool node_fan(const HostMesh &m, int n, std::vector<int> &ring, std::vector<int> &fan);
+
+// A mesh whose arrays a caller (or a restart file) supplied: given(name)
+// says which of the arrays named as in tracer_get_mesh_array were set. The
+// required ones must be; the rest get FESOM's derivation or default (no
+// cavities, areasvol = area, hnode_new = hnode, 3-D depths from zbar,
+// edge_up_dn_tri -1). Validates with check_mesh, before and after deriving.
+void complete_mesh(HostMesh &m, const std::function<bool(const std::string &)> &given);
 
 // Sizes, index ranges and the level nesting the device path relies on
 // (element layers inside each of its nodes' columns); throws with the array

@@ -68,6 +68,16 @@ class FESOMTracerAdvection:
                                                  seed, int(sfc)), "tracer_generate_mesh", self.lib)
         return self.dims()
 
+    def load_mesh_partition(self, restart_path):
+        """Mesh, partition and velocity from derived-type restarts under restart_path/np1 (no tracers)."""
+        _lib.check(self.lib.tracer_load_mesh_partition(str(restart_path).encode()), "tracer_load_mesh_partition",
+                   self.lib)
+        return self.dims()
+
+    def run_workflow(self, restart_path, nsteps, dt):
+        """tracer_init from restart_path, nsteps steps of tracer 1, tracer_finalize into restart_path."""
+        return self.lib.tracer_run_workflow(str(restart_path).encode(), nsteps, dt)
+
     def load_mesh_from_files(self, mesh_path):
         """FESOM mesh files (nod2d.out, elem2d.out, aux3d.out, elvls.out,
         nlvls.out, edges.out, edge_tri.out, edgenum.out) from a directory."""

@@ -32,9 +32,13 @@ int tracer_init_mpi(int mpi_comm);
 /* lib/tracer_c_interface.F90:89-106. Returns 0. */
 int tracer_finalize_mpi(void);
 
-/* lib/tracer_c_interface.F90:113-157. -1 if <restart_dir>/np1 is missing;
- * the derived-type restart reader is not part of this build, so an
- * existing directory also returns -1 with an explanatory error. */
+/* lib/tracer_c_interface.F90:113-157. Reads the derived-type binary
+ * restarts <restart_dir>/np1/t_mesh.0, t_partit.0, t_tracer.0, t_dynamics.0
+ * (lib/io_restart_derivedtype.F90:154-199; the layout of FESOM's default
+ * double-precision build, one rank): mesh, tracers (values, valuesAB,
+ * schemes, tra_adv_ph/pv), velocity. -1 with an error if the directory is
+ * missing, a file is unreadable, or its items do not fill its record
+ * exactly (another build or layout). */
 int tracer_init(const char *restart_dir);
 
 /* lib/tracer_c_interface.F90:163-220. One step of tracer 1 on the GPU:
@@ -53,11 +57,15 @@ int tracer_get_size(int *nz, int *nn);
 /* lib/tracer_c_interface.F90:280-302: tracer 1 values, (nz, nn) column-major */
 int tracer_get_values(double *values, int nz, int nn);
 
-/* lib/tracer_c_interface.F90:308-348. The restart writer is not part of
- * this build: marks the library uninitialised and returns -1. */
+/* lib/tracer_c_interface.F90:308-348. Writes the restarts that
+ * tracer_init reads to <output_dir>/np1/ (created if needed) and
+ * <output_dir>/meta.time, with the current tracers, and marks the library
+ * uninitialised. Fields the tracer path does not carry are written
+ * unallocated, scalars at FESOM's defaults. */
 int tracer_finalize(const char *output_dir);
 
-/* lib/tracer_c_interface.F90:356-386: init + nsteps + finalize */
+/* lib/tracer_c_interface.F90:356-386: init, nsteps of tracer_advect_step
+ * (printing min/max/sum after each), finalize into the same directory */
 int tracer_run_workflow(const char *restart_dir, int nsteps, double dt);
 
 /* lib/tracer_c_interface.F90:391-402: resets the flags (and frees the GPU state) */
@@ -96,7 +104,8 @@ int tracer_set_velocity(const double *uv, const double *w, int nl, int nod2D, in
 /* lib/tracer_c_interface.F90:749-759 */
 int tracer_init_complete(void);
 
-/* lib/tracer_c_interface.F90:769-821: -1 (restart-based, see tracer_init) */
+/* lib/tracer_c_interface.F90:769-821: mesh, partition and velocity from the
+ * restarts under <restart_dir>/np1 (no tracers; allocate them next) */
 int tracer_load_mesh_partition(const char *restart_dir);
 
 /* lib/tracer_c_interface.F90:830-868 (mesh_setup, lib/oce_mesh.F90:49-94 on
@@ -210,6 +219,12 @@ int tracer_get_mesh_array(const char *name, void *out, long long count);
  * leaves no mesh (begin again). FCT and LEARNED need nod_in_elem2D to list, for every node, the
  * elements of one fan around it; tracer_set_schemes refuses them otherwise,
  * and the other schemes run. */
+/* No reference counterpart: parse one derived-type restart file of the
+ * given kind ("t_mesh", "t_partit", "t_tracer", "t_dynamics") and write it
+ * back to out_path in canonical form (a format check; byte-identical for a
+ * file FESOM's default build wrote). -1 with an error otherwise. */
+int tracer_restart_file_copy(const char *kind, const char *in_path, const char *out_path);
+
 int tracer_mesh_begin(int nl, int nod2D, int elem2D, int edge2D, int maxAdj);
 int tracer_set_mesh_array(const char *name, const void *data, long long count);
 int tracer_mesh_end(void);
