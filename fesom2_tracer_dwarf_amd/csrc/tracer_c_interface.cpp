@@ -102,6 +102,19 @@ int ensure_engine()
     return 0;
 }
 
+// upload the host velocity to a live engine (no-op without one)
+int push_velocity()
+{
+    State &s = S();
+    if (!s.eng) return 0;
+    try {
+        s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
 void adopt_mesh_dims()
 {
     State &s = S();
@@ -535,7 +548,7 @@ int tracer_set_velocity_zero(int nl, int nod2D, int edge2D)
     s.uv.assign(2 * (size_t)(nl - 1) * s.mesh.elem2D, 0.0);
     s.w.assign((size_t)nl * nod2D, 0.0);
     s.we.assign((size_t)nl * nod2D, 0.0);
-    if (s.eng) s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
+    if (push_velocity()) return -1;
     return 0;
 }
 
@@ -548,7 +561,7 @@ int tracer_set_velocity(const double *uv, const double *w, int nl, int nod2D, in
     s.uv.assign(uv, uv + nv);
     s.w.assign(w, w + (size_t)nl * nod2D);
     s.we.assign((size_t)nl * nod2D, 0.0);
-    if (s.eng) s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
+    if (push_velocity()) return -1;
     return 0;
 }
 
@@ -560,7 +573,7 @@ int tracer_set_velocity_elem(const double *uv, const double *w, const double *w_
     s.uv.assign(uv, uv + 2 * (size_t)(nl - 1) * elem2D);
     s.w.assign(w, w + (size_t)nl * nod2D);
     s.we.assign(w_e, w_e + (size_t)nl * nod2D);
-    if (s.eng) s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
+    if (push_velocity()) return -1;
     return 0;
 }
 
@@ -571,7 +584,7 @@ int tracer_make_velocity(double u_bg, double v_bg, double cell_amp, double shear
     double dt = make_velocity(s.mesh, u_bg, v_bg, cell_amp, shear, cfl, s.uv, s.w);
     s.we = s.w;
     if (dt_out) *dt_out = dt;
-    if (s.eng) s.eng->set_velocity(s.uv.data(), s.w.data(), s.we.data());
+    if (push_velocity()) return -1;
     return 0;
 }
 
@@ -849,7 +862,11 @@ int tracer_set_option(const char *name, int value)
         if (value != 64 && value != 32 && value != 16) return fail("precision must be 64, 32 or 16");
         if (s.eng) {
             // carry the current tracer state over to the new engine
-            for (int t = 0; t < s.ntr; ++t) s.eng->get_values(t, s.values[t].data());
+            try {
+                for (int t = 0; t < s.ntr; ++t) s.eng->get_values(t, s.values[t].data());
+            } catch (const std::exception &e) {
+                return fail(e.what());
+            }
         }
         s.precision = value;
         drop_engine();
