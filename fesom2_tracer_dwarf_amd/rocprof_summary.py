@@ -2,6 +2,7 @@
 
     python -m fesom2_tracer_dwarf_amd.rocprof_summary stats <kernel_stats.csv>
     python -m fesom2_tracer_dwarf_amd.rocprof_summary traffic <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv>
+    python -m fesom2_tracer_dwarf_amd.rocprof_summary counters <counter_collection.csv> [kernel-name-prefix ...]
 
 `stats` prints the per-kernel duration table of a `--kernel-trace --stats`
 run. `traffic` turns two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of the
@@ -9,6 +10,10 @@ benchmark, plus the same two counters read on a plain device-to-device copy
 of known size (`bench.py --calibrate-copy`), into HBM bytes per dispatch:
 each counter is scaled by (bytes the copy moved) / (what the counter read for
 it), so the figures do not depend on the counter's unit or sampling.
+
+`counters` prints, per kernel (optionally only those whose short name starts
+with one of the prefixes), the dispatch count and the mean per dispatch of
+every counter in a `--pmc` run.
 
 `read_traffic(path)` parses a `traffic` table back (the benchmark reports the
 dominant kernel's measured bytes from it).
@@ -99,6 +104,22 @@ def main(argv: list[str]) -> None:
         print("|---|---|---|---|")
         for k, (r, w) in t.items():
             print(f"| {k} | {r / 1e9:.4f} | {w / 1e9:.4f} | {(r + w) / 1e9:.4f} |")
+        return
+    if len(argv) >= 2 and argv[0] == "counters":
+        acc, names = defaultdict(lambda: defaultdict(list)), []
+        for r in csv.DictReader(open(argv[1])):
+            k = short_name(r["Kernel_Name"])
+            if len(argv) > 2 and not any(k.startswith(p) for p in argv[2:]):
+                continue
+            if r["Counter_Name"] not in names:
+                names.append(r["Counter_Name"])
+            acc[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        print("| kernel | dispatches | " + " | ".join(names) + " |")
+        print("|---|---|" + "---|" * len(names))
+        for k, c in acc.items():
+            n = max(len(v) for v in c.values())
+            vals = [sum(c[x]) / len(c[x]) if c[x] else float("nan") for x in names]
+            print(f"| {k} | {n} | " + " | ".join(f"{v:.4g}" for v in vals) + " |")
         return
     raise SystemExit(__doc__)
 
