@@ -936,9 +936,11 @@ template <class R, int NT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_learn_alpha(DMesh<R> m, MlpDev w, LearnArgs<R, NT> a, double dt,
                                                         R *__restrict__ alpha, int iters)
 {
-    // tracers are staged two at a time: their gathers overlap, and the LDS
-    // of a block (36 KB) leaves room for more resident waves than four would
-    constexpr int G = NT < 2 ? NT : 2;
+    // one tracer staged at a time: a block's LDS (18 KB) then lets six waves
+    // per SIMD stay resident (the 84-VGPR limit), whose gathers hide each
+    // other's latency better than two tracers' gathers in one wave did
+    // (measured: 4.74 -> 4.56 ms at 4 tracers)
+    constexpr int G = 1;
     __shared__ uint4 sX[kWavesPerBlock][G][kChunk][2];  // 16 bf16 features per row
     __shared__ float sY[kWavesPerBlock][G][kChunk];
     const int wv = (int)(threadIdx.x >> 6), l = lane_id(), i = l & 15, g = l >> 4;
@@ -980,7 +982,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             wave_lds_sync();
 #pragma unroll 1
             for (int s = 0; s < G; ++s) {
-#pragma unroll 2
+#pragma unroll 4
                 for (int tile = 0; tile < kChunk / 16; ++tile) {
                     const mlp_v4s xb =
                         ld_bf4(reinterpret_cast<const unsigned short *>(&sX[wv][s][16 * tile + i][0]) + 4 * g);
