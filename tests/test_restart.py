@@ -130,3 +130,27 @@ def test_load_mesh_partition_and_errors(api, tmp_path):
 def api_error(api):
     from fesom2_tracer_dwarf_amd import _lib
     return _lib.last_error(api.lib)
+
+
+def test_round_trip_on_the_pi_mesh(api, pi_mesh_dir, tmp_path):
+    """The reference's pi mesh (coasts, so boundary edges with edge_tri(2) = 0 in the file; variable bottom)."""
+    api.set_option("force_rotation", 1)
+    d = api.load_mesh_from_files(pi_mesh_dir)
+    api.allocate_tracers(2)
+    api.make_velocity(0.1, 0.0, 0.2, 0.5, 0.3)
+    rng = np.random.default_rng(8)
+    vals = [np.asfortranarray(rng.uniform(-2, 25, (d["nl"] - 1, d["nod2D"]))) for _ in range(2)]
+    for i, v in enumerate(vals):
+        api.set_values(i + 1, v)
+        api.set_schemes(i + 1, "MUSCL", "PPM", "FCT")
+    before = {k: api.mesh_array(k) for k in MESH_KEYS}
+    assert np.any(before["edge_tri"].reshape(-1, 2)[:, 1] == -1)
+    r = _restart_dir(api, tmp_path, "pi")
+    raw = (r / "np1" / "t_mesh.0").read_bytes()
+    assert int.from_bytes(raw[:4], "little", signed=True) == len(raw) - 8
+    assert api.initialize(str(r)) == 0
+    assert api.dims() == d
+    for k in MESH_KEYS:
+        assert np.array_equal(api.mesh_array(k), before[k]), k
+    for i in range(2):
+        assert np.array_equal(api.get_tracer_values(i + 1), vals[i])
