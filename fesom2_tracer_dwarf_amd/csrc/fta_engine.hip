@@ -44,39 +44,47 @@ void set_error(const std::string &s) { g_last_error = s; }
 __device__ __forceinline__ int info_nlmax(unsigned i) { return (int)(i & 63u); }
 __device__ __forceinline__ bool info_pos(unsigned i) { return (i & 64u) != 0; }
 
-template <class R>
-__device__ __forceinline__ void load_node_edges(const DMesh<R> &m, int n, bool live, int (&oe)[kDeg],
-                                                unsigned (&info)[kDeg])
+template <int D, class R>
+__device__ __forceinline__ void load_node_edges(const DMesh<R> &m, int n, bool live, int (&oe)[D],
+                                                unsigned (&info)[D])
 {
     if (live) {
-        const int4 *q = reinterpret_cast<const int4 *>(&m.nedge[n]);
-        const int4 a = q[0], b = q[1];
-        oe[0] = a.x; oe[1] = a.y; oe[2] = a.z; oe[3] = a.w;
-        oe[4] = b.x; oe[5] = b.y; oe[6] = b.z; oe[7] = b.w;
-        const uint4 u = *reinterpret_cast<const uint4 *>(&m.ninfo[n]);
-        const unsigned w[4] = {u.x, u.y, u.z, u.w};
+        const int4 *q = reinterpret_cast<const int4 *>(&node_edges<D>(m)[n]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            info[2 * j] = w[j] & 0xFFFFu;
-            info[2 * j + 1] = w[j] >> 16;
+        for (int h = 0; h < D / 4; ++h) {
+            const int4 a = q[h];
+            oe[4 * h] = a.x; oe[4 * h + 1] = a.y; oe[4 * h + 2] = a.z; oe[4 * h + 3] = a.w;
+        }
+        const uint4 *u = reinterpret_cast<const uint4 *>(&node_info<D>(m)[n]);
+#pragma unroll
+        for (int h = 0; h < D / 8; ++h) {
+            const uint4 v = u[h];
+            const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                info[8 * h + 2 * j] = w[j] & 0xFFFFu;
+                info[8 * h + 2 * j + 1] = w[j] >> 16;
+            }
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < kDeg; ++j) { oe[j] = -1; info[j] = 0; }
+        for (int j = 0; j < D; ++j) { oe[j] = -1; info[j] = 0; }
     }
 }
 
-template <class R>
-__device__ __forceinline__ void load_other_nodes(const DMesh<R> &m, int n, bool live, int (&oo)[kDeg])
+template <int D, class R>
+__device__ __forceinline__ void load_other_nodes(const DMesh<R> &m, int n, bool live, int (&oo)[D])
 {
     if (live) {
-        const int4 *q = reinterpret_cast<const int4 *>(&m.nedge[n]);
-        const int4 a = q[2], b = q[3];
-        oo[0] = a.x; oo[1] = a.y; oo[2] = a.z; oo[3] = a.w;
-        oo[4] = b.x; oo[5] = b.y; oo[6] = b.z; oo[7] = b.w;
+        const int4 *q = reinterpret_cast<const int4 *>(&node_edges<D>(m)[n]) + D / 4;
+#pragma unroll
+        for (int h = 0; h < D / 4; ++h) {
+            const int4 a = q[h];
+            oo[4 * h] = a.x; oo[4 * h + 1] = a.y; oo[4 * h + 2] = a.z; oo[4 * h + 3] = a.w;
+        }
     } else {
 #pragma unroll
-        for (int j = 0; j < kDeg; ++j) oo[j] = 0;
+        for (int j = 0; j < D; ++j) oo[j] = 0;
     }
 }
 
@@ -113,7 +121,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_trxy(DMesh<R> m, const double *__
     }
 }
 
-template <class R, class G, int NT>
+template <class R, class G, int NT, int D>
 __global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const G *__restrict__ trxy, size_t tst,
                                                  G *__restrict__ gnod, size_t tsg, float gscale, float ginv)
 {
@@ -125,21 +133,29 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const G *__restr
     if (p.k0 >= L) return;  // the pair holds only the bottom interface
     const bool v1 = p.k0 + 1 < L;
     // node->element record and areas as whole 16-byte loads
-    int ot[kDeg], le[kDeg];
-    R ar[kDeg];
+    int ot[D], le[D];
+    R ar[D];
     {
-        const int4 *q = reinterpret_cast<const int4 *>(&m.nelem[p.col]);
-        const int4 a = q[0], b = q[1];
-        const uint2 l = *reinterpret_cast<const uint2 *>(&m.nelem[p.col].le[0]);
-        ot[0] = a.x; ot[1] = a.y; ot[2] = a.z; ot[3] = a.w; ot[4] = b.x; ot[5] = b.y; ot[6] = b.z; ot[7] = b.w;
+        const NodeElems<D> *rec = &node_elems<D>(m)[p.col];
+        const int4 *q = reinterpret_cast<const int4 *>(rec->ot);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            le[j] = (int)((l.x >> (8 * j)) & 255u);
-            le[j + 4] = (int)((l.y >> (8 * j)) & 255u);
+        for (int h = 0; h < D / 4; ++h) {
+            const int4 a = q[h];
+            ot[4 * h] = a.x; ot[4 * h + 1] = a.y; ot[4 * h + 2] = a.z; ot[4 * h + 3] = a.w;
         }
-        const R *ap = m.nelem_area + kDeg * (size_t)p.col;
+        const uint2 *lw = reinterpret_cast<const uint2 *>(rec->le);
 #pragma unroll
-        for (int j = 0; j < kDeg; j += 2) {
+        for (int h = 0; h < D / 8; ++h) {
+            const uint2 l = lw[h];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                le[8 * h + j] = (int)((l.x >> (8 * j)) & 255u);
+                le[8 * h + j + 4] = (int)((l.y >> (8 * j)) & 255u);
+            }
+        }
+        const R *ap = m.nelem_area + D * (size_t)p.col;
+#pragma unroll
+        for (int j = 0; j < D; j += 2) {
             const V2<R> v = ld2(ap + j);
             ar[j] = v.x;
             ar[j + 1] = v.y;
@@ -150,7 +166,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const G *__restr
 #pragma unroll
     for (int t = 0; t < NT; ++t) { tx0[t] = 0; ty0[t] = 0; tx1[t] = 0; ty1[t] = 0; }
 #pragma unroll
-    for (int j = 0; j < kDeg; ++j) {
+    for (int j = 0; j < D; ++j) {
         if (ot[j] < 0) break;
         if (p.k0 >= le[j]) continue;
         const bool a1 = p.k0 + 1 < le[j];
@@ -335,7 +351,7 @@ __device__ __forceinline__ VCol2<R> node_vcol(const DMesh<R> &m, const Pair &p, 
 // LO solution (lib/oce_adv_tra_driver.F90:95-211), HO-LO vertical flux
 // (:257-283), and the raw antidiffusive sums P+/P- (lib/oce_adv_tra_fct.F90:232-346)
 // ===========================================================================
-template <class R, int VER, int NT, bool SAME>
+template <class R, int VER, int NT, bool SAME, int D>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && NT <= 2 && VER != VER_PPM ? 4 : 1))) void k_node_lo(DMesh<R> m, const R *__restrict__ Wv,
                                                     const R *__restrict__ WEv,
                                                     const double *__restrict__ ttf,
@@ -354,10 +370,10 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
     const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
     const bool if0 = p.live && k0 < nlev, if1 = p.live && k0 + 1 < nlev;
     const size_t c = (size_t)p.c;
-    int oe[kDeg], oo[kDeg];
-    unsigned info[kDeg];
-    load_node_edges(m, p.col, p.live && lay0, oe, info);
-    load_other_nodes(m, p.col, p.live && lay0, oo);
+    int oe[D], oo[D];
+    unsigned info[D];
+    load_node_edges<D>(m, p.col, p.live && lay0, oe, info);
+    load_other_nodes<D>(m, p.col, p.live && lay0, oo);
     // the LO vertical flux in FP64 (see lo_flux)
     VCol2<double> lc{};
     lc.nlev = nlev;
@@ -410,7 +426,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
         double slo0 = 0.0, slo1 = 0.0;
         R qp0 = pp0[t], qm0 = pm0[t], qp1 = pp1[t], qm1 = pm1[t];
 #pragma unroll
-        for (int j = 0; j < kDeg; ++j) {
+        for (int j = 0; j < D; ++j) {
             const int nlm = info_nlmax(info[j]);
             if (oe[j] < 0 || k0 >= nlm) continue;
             const bool on1 = lay1 && k0 + 1 < nlm;
@@ -457,7 +473,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
 // Zalesak bounds a1-a3 and limiting factors b2 (lib/oce_adv_tra_fct.F90:94-217,361-374);
 // R+/R- overwrite P+/P- in place. a1 (max/min of LO and T) is formed once per
 // 1-ring node, a2 per fan element from the node and two consecutive ring nodes.
-template <class R, int NT, bool STOREQ>
+template <class R, int NT, bool STOREQ, int D>
 __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const double *__restrict__ ttf,
                                                        const double *__restrict__ LO, size_t tsn,
                                                        double dt, R *__restrict__ Pp_Rp,
@@ -480,10 +496,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
 #pragma unroll
     for (int t = 0; t < NT; ++t) { tvmax0[t] = 0; tvmin0[t] = 0; tvmax1[t] = 0; tvmin1[t] = 0; }
     if (lay0) {
-        const NodeRing rg = m.nring[p.col];
+        const NodeRing<D> rg = node_ring<D>(m)[p.col];
         int nr = 0;
 #pragma unroll
-        for (int j = 0; j < kDeg; ++j) nr += rg.oo[j] >= 0 ? 1 : 0;
+        for (int j = 0; j < D; ++j) nr += rg.oo[j] >= 0 ? 1 : 0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const size_t tn = t * tsn;
@@ -499,9 +515,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
             double vmax0 = 0, vmin0 = 0, vmax1 = 0, vmin1 = 0;
             bool first = true;
 #pragma unroll
-            for (int j = 0; j <= kDeg; ++j) {
+            for (int j = 0; j <= D; ++j) {
                 double cx0 = 0, cn0 = 0, cx1 = 0, cn1 = 0;
-                if (j < kDeg && rg.oo[j] >= 0 && k0 < (int)rg.lr[j]) {
+                if (j < D && rg.oo[j] >= 0 && k0 < (int)rg.lr[j]) {
                     const V2<double> l = ld2(LO + tn + (size_t)rg.oo[j] + k0);
                     const V2<double> v = ld2(ttf + tn + (size_t)rg.oo[j] + k0);
                     cx0 = fmx(l.x, v.x); cn0 = fmn(l.x, v.x);
@@ -623,7 +639,7 @@ __device__ __forceinline__ R edge_alpha(bool pos, R f, R rp, R rm, R rpo, R rmo)
 // limiting b3 (lib/oce_adv_tra_fct.F90:394-471), flux divergence
 // (lib/oce_adv_tra_driver.F90:298-450) and the update
 // values += (dttf_h + dttf_v)/hnode_new (src/fesom_analytic.F90:419-437)
-template <class R, int NT, bool LEARN>
+template <class R, int NT, bool LEARN, int D>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && NT <= 2 && !LEARN ? 5 : 1))) void k_node_update(DMesh<R> m, double *__restrict__ ttf,
                                                         const double *__restrict__ LO, size_t tsn,
                                                         const R *__restrict__ adf_h, size_t tse,
@@ -641,10 +657,10 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
     const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
     const bool if0 = p.live && k0 < nlev, if1 = p.live && k0 + 1 < nlev;
     const size_t c = (size_t)p.c;
-    int oe[kDeg], oo[kDeg];
-    unsigned info[kDeg];
-    load_node_edges(m, p.col, lay0, oe, info);
-    load_other_nodes(m, p.col, lay0, oo);
+    int oe[D], oo[D];
+    unsigned info[D];
+    load_node_edges<D>(m, p.col, lay0, oe, info);
+    load_other_nodes<D>(m, p.col, lay0, oo);
     V2<double> av{1, 1}, h{0, 0}, hn{1, 1};  // FP64 geometry: conservative accumulation
     if (lay0) {
         av = ld2(m.av_d + c);
@@ -670,7 +686,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
 
         double sh0 = 0.0, sh1 = 0.0;
 #pragma unroll
-        for (int j = 0; j < kDeg; ++j) {
+        for (int j = 0; j < D; ++j) {
             const int nlm = info_nlmax(info[j]);
             if (oe[j] < 0 || k0 >= nlm) continue;
             const bool on1 = lay1 && k0 + 1 < nlm;
@@ -723,7 +739,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
 }
 
 // lim = 'NONE': HO fluxes straight into the divergence (driver :291)
-template <class R, int VER, int NT>
+template <class R, int VER, int NT, int D>
 __global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *__restrict__ WEv,
                                                        double *__restrict__ ttf,
                                                        const double *__restrict__ ttfAB, size_t tsn,
@@ -737,9 +753,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *_
     const int L = nlev - 1, k0 = p.k0;
     const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
     const size_t c = (size_t)p.c;
-    int oe[kDeg];
-    unsigned info[kDeg];
-    load_node_edges(m, p.col, lay0, oe, info);
+    int oe[D];
+    unsigned info[D];
+    load_node_edges<D>(m, p.col, lay0, oe, info);
     V2<double> av{1, 1}, hn{1, 1};  // FP64 geometry: conservative accumulation
     if (lay0) {
         av = ld2(m.av_d + c);
@@ -758,7 +774,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *_
         if (!lay0) continue;
         double sh0 = 0.0, sh1 = 0.0;
 #pragma unroll
-        for (int j = 0; j < kDeg; ++j) {
+        for (int j = 0; j < D; ++j) {
             const int nlm = info_nlmax(info[j]);
             if (oe[j] < 0 || k0 >= nlm) continue;
             const bool on1 = lay1 && k0 + 1 < nlm;
@@ -1005,7 +1021,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // ones as Zalesak's) keep the node inside its bounds? Checked as Zalesak's
 // own sufficient condition, positive and negative contributions apart:
 // sum(max(0, c)) * dt / areasvol <= Q+ and sum(min(0, c)) * dt / areasvol >= Q-
-template <class R, int NT>
+template <class R, int NT, int D>
 __global__ __launch_bounds__(kMaxBlock) void k_learn_check(DMesh<R> m, const R *__restrict__ adf_h,
                                                         const R *__restrict__ alpha, size_t tse,
                                                         const R *__restrict__ adf_v, const R *__restrict__ Rp,
@@ -1021,9 +1037,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_check(DMesh<R> m, const R *
     const bool lay0 = k0 < L, lay1 = k0 + 1 < L;
     const bool if0 = p.live && k0 < nlev, if1 = p.live && k0 + 1 < nlev;
     const size_t c = (size_t)p.c;
-    int oe[kDeg];
-    unsigned info[kDeg];
-    load_node_edges(m, p.col, lay0, oe, info);
+    int oe[D];
+    unsigned info[D];
+    load_node_edges<D>(m, p.col, lay0, oe, info);
     V2<R> av{1, 1};
     if (lay0) av = ld2(m.areasvol + c);
 #pragma unroll
@@ -1044,7 +1060,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_check(DMesh<R> m, const R *
         double sp1 = fmax(0.0, (double)fvl1) + fmax(0.0, -(double)fvl2);
         double sn1 = fmin(0.0, (double)fvl1) + fmin(0.0, -(double)fvl2);
 #pragma unroll
-        for (int j = 0; j < kDeg; ++j) {
+        for (int j = 0; j < D; ++j) {
             const int nlm = info_nlmax(info[j]);
             if (oe[j] < 0 || k0 >= nlm) continue;
             const bool on1 = lay1 && k0 + 1 < nlm;
@@ -1305,6 +1321,60 @@ struct Engine::Impl {
     std::vector<std::vector<double>> host_vals;
 };
 
+// the per-node records (incident edges, 1-ring fan, elements) at width D
+template <int D, class R>
+static void build_node_records(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, const std::vector<int> &enlmax)
+{
+    const int N = m.nod2D;
+    std::vector<NodeEdges<D>> ne(N);
+    std::vector<NodeEdgeInfo<D>> ni(N);
+    std::vector<NodeRing<D>> rg(N);
+    std::vector<NodeElems<D>> nt(N);
+    std::vector<R> na(D * (size_t)N, (R)0);
+    std::vector<int> ring, fan;
+    for (int n = 0; n < N; ++n) {
+        const int deg = m.n2e_ptr[n + 1] - m.n2e_ptr[n];
+        for (int j = 0; j < D; ++j) {
+            ne[n].oe[j] = -1; ne[n].oo[j] = 0; ni[n].info[j] = 0;
+            rg[n].oo[j] = -1; rg[n].lr[j] = 0; rg[n].le[j] = 0;
+            nt[n].ot[j] = -1; nt[n].le[j] = 0;
+        }
+        for (int j = 0; j < deg; ++j) {
+            const int se = m.n2e[m.n2e_ptr[n] + j];
+            const int e = (se > 0 ? se : -se) - 1;
+            const int other = se > 0 ? m.edges[2 * e + 1] : m.edges[2 * e];
+            ne[n].oe[j] = (int)I.le.off[e];
+            ne[n].oo[j] = (int)I.ln.off[other];
+            ni[n].info[j] = (unsigned short)(enlmax[e] | (se > 0 ? 64 : 0) | ((m.nlevels_nod2D[other] - 1) << 7));
+        }
+        for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) {
+            const int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
+            nt[n].ot[j] = (int)I.lt.off[el];
+            nt[n].le[j] = (unsigned char)(m.nlevels[el] - 1);
+            na[D * (size_t)n + j] = (R)m.elem_area[el];
+        }
+        // the cluster bounds of FCT walk the fan; a node without one (a
+        // caller-supplied element list that is not a fan) leaves its ring
+        // empty and rules out the limiters that read it
+        if (!node_fan(m, n, ring, fan)) {
+            if (I.fan_fault < 0) I.fan_fault = n;
+            continue;
+        }
+        if ((int)ring.size() > D) throw std::runtime_error("1-ring above " + std::to_string(D) + " nodes");
+        for (size_t j = 0; j < ring.size(); ++j) {
+            rg[n].oo[j] = (int)I.ln.off[ring[j]];
+            rg[n].lr[j] = (unsigned char)(m.nlevels_nod2D[ring[j]] - 1);
+            rg[n].le[j] = fan[j] >= 0 ? (unsigned char)(m.nlevels[fan[j]] - 1) : 0;
+        }
+    }
+    d.deg = D;
+    d.nedge = I.track(dupload(ne));
+    d.ninfo = I.track(dupload(ni));
+    d.nring = I.track(dupload(rg));
+    d.nelem = I.track(dupload(nt));
+    d.nelem_area = I.track(dupload(na));
+}
+
 template <class R>
 static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
 {
@@ -1353,54 +1423,15 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
     }
     d.trec = I.track(dupload(tr));
     d.tgeo = I.track(dupload(tg));
-    std::vector<NodeEdges> ne(N);
-    std::vector<NodeEdgeInfo> ni(N);
-    std::vector<NodeRing> rg(N);
-    std::vector<NodeElems> nt(N);
-    std::vector<R> na(kDeg * (size_t)N, (R)0);
-    std::vector<int> ring, fan;
-    for (int n = 0; n < N; ++n) {
-        const int deg = m.n2e_ptr[n + 1] - m.n2e_ptr[n];
-        if (deg > kDeg || m.nod_in_elem2D_num[n] > kDeg)
-            throw std::runtime_error("node degree above " + std::to_string(kDeg) + " is not supported");
-        for (int j = 0; j < kDeg; ++j) {
-            ne[n].oe[j] = -1; ne[n].oo[j] = 0; ni[n].info[j] = 0;
-            rg[n].oo[j] = -1; rg[n].lr[j] = 0; rg[n].le[j] = 0;
-            nt[n].ot[j] = -1; nt[n].le[j] = 0;
-        }
-        for (int j = 0; j < deg; ++j) {
-            const int se = m.n2e[m.n2e_ptr[n] + j];
-            const int e = (se > 0 ? se : -se) - 1;
-            const int other = se > 0 ? m.edges[2 * e + 1] : m.edges[2 * e];
-            ne[n].oe[j] = (int)I.le.off[e];
-            ne[n].oo[j] = (int)I.ln.off[other];
-            ni[n].info[j] = (unsigned short)(enlmax[e] | (se > 0 ? 64 : 0) | ((m.nlevels_nod2D[other] - 1) << 7));
-        }
-        for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) {
-            const int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
-            nt[n].ot[j] = (int)I.lt.off[el];
-            nt[n].le[j] = (unsigned char)(m.nlevels[el] - 1);
-            na[kDeg * (size_t)n + j] = (R)m.elem_area[el];
-        }
-        // the cluster bounds of FCT walk the fan; a node without one (a
-        // caller-supplied element list that is not a fan) leaves its ring
-        // empty and rules out the limiters that read it
-        if (!node_fan(m, n, ring, fan)) {
-            if (I.fan_fault < 0) I.fan_fault = n;
-            continue;
-        }
-        if ((int)ring.size() > kDeg) throw std::runtime_error("1-ring above " + std::to_string(kDeg) + " nodes");
-        for (size_t j = 0; j < ring.size(); ++j) {
-            rg[n].oo[j] = (int)I.ln.off[ring[j]];
-            rg[n].lr[j] = (unsigned char)(m.nlevels_nod2D[ring[j]] - 1);
-            rg[n].le[j] = fan[j] >= 0 ? (unsigned char)(m.nlevels[fan[j]] - 1) : 0;
-        }
-    }
-    d.nedge = I.track(dupload(ne));
-    d.ninfo = I.track(dupload(ni));
-    d.nring = I.track(dupload(rg));
-    d.nelem = I.track(dupload(nt));
-    d.nelem_area = I.track(dupload(na));
+    // node record width: kDeg unless some node has more incident edges or
+    // elements (its 1-ring is never larger than its edge count)
+    int need = 0;
+    for (int n = 0; n < N; ++n)
+        need = std::max(need, std::max(m.n2e_ptr[n + 1] - m.n2e_ptr[n], m.nod_in_elem2D_num[n]));
+    static_assert(kDegWide == kMaxNodeDegree, "the host mesh check and the widest node record must agree");
+    if (need > kDegWide) throw std::runtime_error("node degree above " + std::to_string(kDegWide) + " is not supported");
+    if (need > kDeg) build_node_records<kDegWide>(I, d, m, enlmax);
+    else build_node_records<kDeg>(I, d, m, enlmax);
     d.area = I.track(dupload(I.ln.pack<R>(m.area.data(), nl, 0)));
     d.zbarn = I.track(dupload(I.ln.pack<R>(m.zbar_3d_n.data(), nl, 0)));
     d.areasvol = I.track(dupload(I.ln.pack<R>(m.areasvol.data(), nl, 1)));
@@ -1618,8 +1649,9 @@ static MlpDev mlp_dev(const Engine::Impl &I)
     return MlpDev{I.w1t, I.w2t, I.w2, I.w3, I.b1, I.b2, I.b3};
 }
 
-// one step of the NT tracers [t0, t0+NT), all with schemes sp
-template <class R, class G, int NT>
+// one step of the NT tracers [t0, t0+NT), all with schemes sp, on a mesh whose
+// node records are D wide
+template <class R, class G, int NT, int D>
 static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &sp, double dt, int t0, bool refresh,
                          hipStream_t st, LaunchHook &hk)
 {
@@ -1642,7 +1674,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
     if (sp.hor == HOR_MUSCL || sp.hor == HOR_MFCT) {
         hipLaunchKernelGGL((k_trxy<R, G, NT>), gt, blk, 0, st, m, ab, tsn, trxy, tst, I.gscale);
         hk.mark("k_trxy", st);
-        hipLaunchKernelGGL((k_gnod<R, G, NT>), gn, blk, 0, st, m, trxy, tst, gnod, tsg, I.gscale, I.ginv);
+        hipLaunchKernelGGL((k_gnod<R, G, NT, D>), gn, blk, 0, st, m, trxy, tst, gnod, tsg, I.gscale, I.ginv);
         hk.mark("k_gnod", st);
     }
 #define EDGE(HOR, F)                                                                                        \
@@ -1669,10 +1701,10 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
 #define NLO(V)                                                                                              \
     do {                                                                                                    \
         if (refresh)                                                                                        \
-            hipLaunchKernelGGL((k_node_lo<R, V, NT, true>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, vfl, \
+            hipLaunchKernelGGL((k_node_lo<R, V, NT, true, D>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, vfl, \
                                adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
         else                                                                                                \
-            hipLaunchKernelGGL((k_node_lo<R, V, NT, false>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, vfl, \
+            hipLaunchKernelGGL((k_node_lo<R, V, NT, false, D>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, vfl, \
                                adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
     } while (0)
         switch (sp.ver) {
@@ -1688,10 +1720,10 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
         R *Sp = learned || dump ? (R *)I.Sp : nullptr;
         R *Sm = learned || dump ? (R *)I.Sm : nullptr;
         if (learned || dump)
-            hipLaunchKernelGGL((k_node_limit<R, NT, true>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm, Qp, Qm,
+            hipLaunchKernelGGL((k_node_limit<R, NT, true, D>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm, Qp, Qm,
                                Sp, Sm);
         else
-            hipLaunchKernelGGL((k_node_limit<R, NT, false>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm, Qp, Qm,
+            hipLaunchKernelGGL((k_node_limit<R, NT, false, D>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm, Qp, Qm,
                                Sp, Sm);
         hk.mark("k_node_limit", st);
         const LearnArgs<R, NT> la{vals, LO, adf_h, vfl, Qp, Qm, Pp, Pm, Sp, Sm, tsn, tse};
@@ -1701,7 +1733,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
                 hipLaunchKernelGGL((k_learn_dump<R, NT>), ge, blk, 0, st, m, la, dt, dump->X, dump->Y, dump->row);
                 dump->row += (long long)NT * m.CE;
             }
-            hipLaunchKernelGGL((k_node_update<R, NT, false>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
+            hipLaunchKernelGGL((k_node_update<R, NT, false, D>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
                                Pp, Pm, dt, alpha, unsafe);
         } else if (learned) {
             const MlpDev w = mlp_dev(I);
@@ -1709,18 +1741,18 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
             const int iters = (m.nch_e + gb * kWavesPerBlock - 1) / (gb * kWavesPerBlock);
             hipLaunchKernelGGL((k_learn_alpha<R, NT>), dim3(gb), dim3(kBlock), 0, st, m, w, la, dt, alpha, iters);
             hk.mark("k_learn_alpha", st);
-            hipLaunchKernelGGL((k_learn_check<R, NT>), gn, blk, 0, st, m, adf_h, alpha, tse, adf_v, Pp, Pm, Qp, Qm,
+            hipLaunchKernelGGL((k_learn_check<R, NT, D>), gn, blk, 0, st, m, adf_h, alpha, tse, adf_v, Pp, Pm, Qp, Qm,
                                tsn, dt, unsafe);
             hk.mark("k_learn_check", st);
-            hipLaunchKernelGGL((k_node_update<R, NT, true>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
+            hipLaunchKernelGGL((k_node_update<R, NT, true, D>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
                                Pp, Pm, dt, alpha, unsafe);
         } else {
-            hipLaunchKernelGGL((k_node_update<R, NT, false>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
+            hipLaunchKernelGGL((k_node_update<R, NT, false, D>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
                                Pp, Pm, dt, alpha, unsafe);
         }
         hk.mark("k_node_update", st);
     } else {
-#define NNL(V) hipLaunchKernelGGL((k_node_nolim<R, V, NT>), gn, blk, 0, st, m, WE, vals, ab, tsn, adf_h, tse, dt, pv)
+#define NNL(V) hipLaunchKernelGGL((k_node_nolim<R, V, NT, D>), gn, blk, 0, st, m, WE, vals, ab, tsn, adf_h, tse, dt, pv)
         switch (sp.ver) {
         case VER_QR4C: NNL(VER_QR4C); break;
         case VER_PPM: NNL(VER_PPM); break;
@@ -1777,9 +1809,15 @@ static void launch_range(Engine::Impl &I, const DMesh<R> &m, const std::vector<T
         while (run > 0) {
             hk.mark("start", st);
             const int nb = batch_size(run, I.tune_nt);
-            if (nb == 4) launch_batch<R, G, 4>(I, m, spec[t], dt, t, refresh, st, hk);
-            else if (nb == 2) launch_batch<R, G, 2>(I, m, spec[t], dt, t, refresh, st, hk);
-            else launch_batch<R, G, 1>(I, m, spec[t], dt, t, refresh, st, hk);
+            if (m.deg == kDegWide) {
+                if (nb == 4) launch_batch<R, G, 4, kDegWide>(I, m, spec[t], dt, t, refresh, st, hk);
+                else if (nb == 2) launch_batch<R, G, 2, kDegWide>(I, m, spec[t], dt, t, refresh, st, hk);
+                else launch_batch<R, G, 1, kDegWide>(I, m, spec[t], dt, t, refresh, st, hk);
+            } else {
+                if (nb == 4) launch_batch<R, G, 4, kDeg>(I, m, spec[t], dt, t, refresh, st, hk);
+                else if (nb == 2) launch_batch<R, G, 2, kDeg>(I, m, spec[t], dt, t, refresh, st, hk);
+                else launch_batch<R, G, 1, kDeg>(I, m, spec[t], dt, t, refresh, st, hk);
+            }
             t += nb;
             run -= nb;
         }

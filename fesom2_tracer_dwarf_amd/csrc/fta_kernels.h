@@ -36,7 +36,8 @@ constexpr int kWavesPerBlock = 4;                  // default launch shape (and 
 constexpr int kBlock = kWave * kWavesPerBlock;
 constexpr int kMaxBlock = 512;                     // chunk kernels take 64..512 threads (autotuned)
 constexpr int kChunk = 2 * kWave;   // cells per wave chunk
-constexpr int kDeg = 8;             // padded node degree (incident edges / elements / 1-ring)
+constexpr int kDeg = 8;             // node record width (incident edges / elements / 1-ring) of ordinary meshes
+constexpr int kDegWide = 16;        // the width used when some node has more than kDeg
 constexpr int kColBits = 26;        // slot word: column | pair index << kColBits
 constexpr double kRE = 6367500.0;   // r_earth, lib/oce_modules.F90:24
 
@@ -56,32 +57,39 @@ struct alignas(16) ElemRec {
     int o0, o1, o2, nlev;
 };
 
-// node -> incident edges (sorted by edge index), kDeg slots:
+// Node records come in two widths D (kDeg, or kDegWide for meshes with a node
+// above kDeg); the node-side kernels are instantiated per width.
+//
+// node -> incident edges (sorted by edge index), D slots:
 // oe = edge offset (-1 = empty), oo = layer offset of the edge's other node,
 // info = nlmax | 64 if the node is edges(1) | (layers of the other node) << 7
+template <int D>
 struct alignas(16) NodeEdges {
-    int oe[kDeg];
-    int oo[kDeg];
+    int oe[D];
+    int oo[D];
 };
+template <int D>
 struct alignas(16) NodeEdgeInfo {
-    unsigned short info[kDeg];
+    unsigned short info[D];
 };
 
 // node -> 1-ring in cyclic (fan) order, for the Zalesak bounds: element j of
 // the fan is (node, ring j, ring (j+1) mod R); oo = ring node layer offsets
 // (-1 = empty), lr = ring node layer counts, le = layer count of fan element
 // j (0 = no element: the open end of a boundary fan)
+template <int D>
 struct alignas(16) NodeRing {
-    int oo[kDeg];
-    unsigned char lr[kDeg];
-    unsigned char le[kDeg];
+    int oo[D];
+    unsigned char lr[D];
+    unsigned char le[D];
 };
 
 // node -> elements (nod_in_elem2D order) for the nodal mean gradient:
 // ot = element layer offset (-1 = empty), le = element layer count
+template <int D>
 struct alignas(16) NodeElems {
-    int ot[kDeg];
-    unsigned char le[kDeg];
+    int ot[D];
+    unsigned char le[D];
 };
 
 template <class R>
@@ -95,11 +103,12 @@ struct DMesh {
     const R *egeo;                // (8,E): dx1 dy1 dx2 dy2 (edge_cross_dxdy) ex ey (edge_dxdy) a
     const ElemRec *trec;          // (T)
     const R *tgeo;                // (8,T): gradient_sca(1:6), 0, 0
-    const NodeEdges *nedge;       // (N)
-    const NodeEdgeInfo *ninfo;    // (N)
-    const NodeRing *nring;        // (N)
-    const NodeElems *nelem;       // (N)
-    const R *nelem_area;          // (8,N): elem_area of the node's elements
+    int deg;                      // node record width D (kDeg or kDegWide)
+    const void *nedge;            // (N) NodeEdges<D>
+    const void *ninfo;            // (N) NodeEdgeInfo<D>
+    const void *nring;            // (N) NodeRing<D>
+    const void *nelem;            // (N) NodeElems<D>
+    const R *nelem_area;          // (D,N): elem_area of the node's elements
     // node layout (interfaces 1..nlev and layers 1..nlev-1 share a column)
     const R *area, *zbarn;        // interface arrays
     const R *areasvol, *hnode, *hnode_new, *Zn;  // layer arrays
@@ -109,6 +118,27 @@ struct DMesh {
     // tracer mass is conserved to FP64 rounding in every precision form
     const double *av_d, *h_d, *hn_d;
 };
+
+template <int D, class R>
+__device__ __forceinline__ const NodeEdges<D> *node_edges(const DMesh<R> &m)
+{
+    return static_cast<const NodeEdges<D> *>(m.nedge);
+}
+template <int D, class R>
+__device__ __forceinline__ const NodeEdgeInfo<D> *node_info(const DMesh<R> &m)
+{
+    return static_cast<const NodeEdgeInfo<D> *>(m.ninfo);
+}
+template <int D, class R>
+__device__ __forceinline__ const NodeRing<D> *node_ring(const DMesh<R> &m)
+{
+    return static_cast<const NodeRing<D> *>(m.nring);
+}
+template <int D, class R>
+__device__ __forceinline__ const NodeElems<D> *node_elems(const DMesh<R> &m)
+{
+    return static_cast<const NodeElems<D> *>(m.nelem);
+}
 
 __device__ __forceinline__ int wave_chunk()
 {

@@ -791,7 +791,14 @@ void derive_node_edges(HostMesh &m)
         m.n2e_ptr[m.edges[2 * e] + 1]++;
         m.n2e_ptr[m.edges[2 * e + 1] + 1]++;
     }
-    for (int n = 0; n < N; ++n) m.n2e_ptr[n + 1] += m.n2e_ptr[n];
+    for (int n = 0; n < N; ++n) {
+        const int deg = std::max(m.n2e_ptr[n + 1], m.nod_in_elem2D_num[n]);
+        if (deg > kMaxNodeDegree)
+            throw std::runtime_error("node " + std::to_string(n + 1) + " has " + std::to_string(deg) +
+                                     " edges or elements; at most " + std::to_string(kMaxNodeDegree) +
+                                     " are supported");
+        m.n2e_ptr[n + 1] += m.n2e_ptr[n];
+    }
     m.n2e.assign(m.n2e_ptr[N], 0);
     std::vector<int> fill(m.n2e_ptr.begin(), m.n2e_ptr.end() - 1);
     for (int e = 0; e < E; ++e) {

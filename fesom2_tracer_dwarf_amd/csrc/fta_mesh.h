@@ -117,6 +117,9 @@ void derive_up_down(HostMesh &m);
 // nboundary_lay, and the node->edge lists.
 void derive_levels_min(HostMesh &m);
 void derive_boundary_lay(HostMesh &m);
+// Also refuses a node with more than kMaxNodeDegree edges or elements (the
+// widest per-node record the GPU kernels take).
+constexpr int kMaxNodeDegree = 16;
 void derive_node_edges(HostMesh &m);
 
 // The elements around node n in fan order: ring[j], ring[(j+1) mod R] and the

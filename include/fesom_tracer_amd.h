@@ -117,7 +117,7 @@ int tracer_load_mesh_partition(const char *restart_dir);
  * longitude is cyclic (360 degrees), and the geometry is spherical
  * (mesh_areas / mesh_auxiliary_arrays). nlvls.out must equal the deepest
  * level of each node's elements. -1 when the directory or a required file
- * is missing or inconsistent. Follow with tracer_allocate_tracers,
+ * is missing or inconsistent, or a node has more than 16 edges or elements. Follow with tracer_allocate_tracers,
  * tracer_set_values and a velocity, as after tracer_generate_mesh. */
 int tracer_load_mesh_from_files(const char *mesh_path);
 
@@ -215,19 +215,20 @@ int tracer_get_mesh_array(const char *name, void *out, long long count);
  *     Z, Z_3d_n, zbar_3d_n (from zbar), depth, coord_nod2D (0)
  * tracer_set_mesh_array fails on an unknown name or a wrong count, and
  * tracer_mesh_end on a missing array, an index out of range, or an element
- * whose layers are not inside each of its nodes' columns; either failure
- * leaves no mesh (begin again). FCT and LEARNED need nod_in_elem2D to list, for every node, the
+ * whose layers are not inside each of its nodes' columns, or a node with
+ * more than 16 edges or elements; either failure leaves no mesh (begin
+ * again). FCT and LEARNED need nod_in_elem2D to list, for every node, the
  * elements of one fan around it; tracer_set_schemes refuses them otherwise,
  * and the other schemes run. */
+int tracer_mesh_begin(int nl, int nod2D, int elem2D, int edge2D, int maxAdj);
+int tracer_set_mesh_array(const char *name, const void *data, long long count);
+int tracer_mesh_end(void);
+
 /* No reference counterpart: parse one derived-type restart file of the
  * given kind ("t_mesh", "t_partit", "t_tracer", "t_dynamics") and write it
  * back to out_path in canonical form (a format check; byte-identical for a
  * file FESOM's default build wrote). -1 with an error otherwise. */
 int tracer_restart_file_copy(const char *kind, const char *in_path, const char *out_path);
-
-int tracer_mesh_begin(int nl, int nod2D, int elem2D, int edge2D, int maxAdj);
-int tracer_set_mesh_array(const char *name, const void *data, long long count);
-int tracer_mesh_end(void);
 
 /* No reference counterpart: a named GPU work array of the last step of one
  * tracer (1-based id) in the reference layout: fct_LO, adv_flux_hor (HO-LO
