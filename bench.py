@@ -10,7 +10,9 @@ Metric: node-level tracer updates per second = n_tracers * C * steps / t,
 C = active (node, layer) cells of the generated mesh. The timed region is K
 graph-replayed steps bracketed by hipEvents on the library's stream (inputs
 already resident in HBM). FP64 is the headline (the parity path); the FP32
-form (FP64 node tracer/accumulation) is reported beside it.
+form (FP64 node tracer/accumulation) is reported beside it, and so is the
+rate of a caller that hands the tracers over as host arrays every step
+("host_round_trip": upload, one step, download over PCIe).
 
 Prints ONE JSON line.
 """
@@ -98,6 +100,30 @@ def run_precision(api, prec, dt, steps, warmup, ntr, tune=True):
     wall = time.perf_counter() - t0
     kern = api.time_kernels(dt, reps=3)
     return ms, wall, kern, table
+
+
+def host_round_trip(api, dt, trs, c, reps=3):
+    """The rate a caller sees when it hands the tracers over as host arrays every
+    step: tracer_set_values of every tracer, one step, tracer_get_values of every
+    tracer (pageable numpy arrays, FP64), timed on the wall clock. Reported beside
+    the headline, never as it: the headline keeps the inputs resident in HBM."""
+    ntr = len(trs)
+    host_bytes = 2 * sum(t.nbytes for t in trs)
+    out = None
+    t_total = 0.0
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        for i, t in enumerate(trs):
+            api.set_values(i + 1, t)
+        api.advect_steps(1, dt)
+        out = [api.get_tracer_values(i + 1) for i in range(ntr)]
+        if r > 0:  # the first pass warms the host pages
+            t_total += time.perf_counter() - t0
+    ms = t_total / reps * 1e3
+    assert all(np.all(np.isfinite(o)) for o in out)
+    return dict(updates_per_s=ntr * c["C"] / (ms * 1e-3), ms_per_step=ms, host_bytes_per_step=host_bytes,
+                host_gbs=host_bytes / (ms * 1e-3) / 1e9, steps=reps,
+                what="set_values + 1 step + get_values of every tracer, FP64, pageable host arrays")
 
 
 def cpu_baseline(api, dt, ntr, c, max_seconds=20.0):
@@ -219,6 +245,11 @@ def main():
         res[tag]["T_mass"] = diag["mass"]
         res[tag]["nan"] = diag["nan_count"]
 
+    log("host round trip (tracers handed over as host arrays every step)")
+    api.set_option("precision", 64)
+    host_rt = host_round_trip(api, dt, trs, c)
+    log(f"host round trip: {host_rt['ms_per_step']:.1f} ms/step")
+
     f64 = res["fp64"]
     dom = f64["dominant"]
     achieved = f64["kernels_gbs"][dom]
@@ -252,6 +283,7 @@ def main():
                           "frac": f64["step_hbm_frac"]},
         "variants": {k: {kk: vv for kk, vv in v.items() if kk not in ("wall_s",)} for k, v in res.items()},
         "fp32_over_fp64": res["fp32"]["updates_per_s"] / f64["updates_per_s"],
+        "host_round_trip": host_rt,
         "setup_s": t_setup,
     }
     if not args.no_cpu:

@@ -12,6 +12,7 @@
 // (without FCT the last three collapse into k_node_nolim). Every kernel runs
 // one wave per 128-cell chunk of its layout, two cells per lane.
 #include "fta_engine.h"
+#include "fta_hostcopy.h"
 #include "fta_kernels.h"
 #include "fta_mlp.h"
 #include "fta_vertical.h"
@@ -1115,21 +1116,64 @@ __device__ inline void diag_merge(DiagAcc &a, const DiagAcc &b)
     a.amin = fmin(a.amin, b.amin); a.amax = fmax(a.amax, b.amax); a.mass += b.mass; a.nan += b.nan;
 }
 
+// A tracer between the caller's dense (nl-1, nod2D) layout (levels fastest)
+// and the chunk-packed node layers: one thread per cell pair of the layout.
+// Packing writes every cell of the layout (0 outside the wet layers);
+// unpacking writes only the wet layers, so the dense copy keeps the levels
+// below the bottom as last set.
+__global__ __launch_bounds__(256) void k_pack_layers(const int *__restrict__ slot, const int *__restrict__ nlev,
+                                                     long long npair, const double *__restrict__ dense, int L,
+                                                     double *__restrict__ out)
+{
+    const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (s >= npair) return;
+    const int w = slot[s];
+    double a = 0.0, b = 0.0;
+    if (w >= 0) {
+        const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
+        const int Lc = nlev[n] - 1;
+        const double *src = dense + (size_t)n * L;
+        if (k0 < Lc) a = src[k0];
+        if (k0 + 1 < Lc) b = src[k0 + 1];
+    }
+    out[2 * s] = a;
+    out[2 * s + 1] = b;
+}
+
+__global__ __launch_bounds__(256) void k_unpack_layers(const int *__restrict__ slot, const int *__restrict__ nlev,
+                                                       long long npair, const double *__restrict__ in, int L,
+                                                       double *__restrict__ dense)
+{
+    const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (s >= npair) return;
+    const int w = slot[s];
+    if (w < 0) return;
+    const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
+    const int Lc = nlev[n] - 1;
+    double *dst = dense + (size_t)n * L;
+    if (k0 < Lc) dst[k0] = in[2 * s];
+    if (k0 + 1 < Lc) dst[k0 + 1] = in[2 * s + 1];
+}
+
 // reduction over the wet layers (node layout): min, max, sum, mass, NaN count
+// plus, for min / max / sum / NaN count over all values, the levels below each
+// column's bottom, read from the dense (nl-1, nod2D) copy (never stepped)
 template <class R>
 __global__ __launch_bounds__(256) void k_diag_partial(DMesh<R> m, const double *__restrict__ v,
+                                                      const double *__restrict__ dense, int L,
                                                       DiagAcc *__restrict__ part)
 {
     __shared__ DiagAcc sh[256];
     DiagAcc a{INFINITY, -INFINITY, 0.0, INFINITY, -INFINITY, 0.0, 0.0};
     const long long npair = m.CN / 2;
-    for (long long s = (long long)blockIdx.x * 256 + threadIdx.x; s < npair; s += (long long)gridDim.x * 256) {
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long s = (long long)blockIdx.x * 256 + threadIdx.x; s < npair; s += stride) {
         const int w = m.slot_n[s];
         if (w < 0) continue;
         const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
-        const int L = m.nlev[n] - 1;
+        const int Lc = m.nlev[n] - 1;
         for (int q = 0; q < 2; ++q) {
-            if (k0 + q >= L) break;
+            if (k0 + q >= Lc) break;
             const long long i = 2 * s + q;
             const double x = v[i];
             if (is_nan_bits(x)) { a.nan += 1.0; continue; }
@@ -1139,6 +1183,14 @@ __global__ __launch_bounds__(256) void k_diag_partial(DMesh<R> m, const double *
     }
     a.amin = a.vmin;
     a.amax = a.vmax;
+    const long long cells = (long long)m.N * L;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < cells; i += stride) {
+        const int n = (int)(i / L), k = (int)(i - (long long)n * L);
+        if (k < m.nlev[n] - 1) continue;
+        const double x = dense[i];
+        if (is_nan_bits(x)) { a.nan += 1.0; continue; }
+        a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
+    }
     sh[threadIdx.x] = a;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
@@ -1316,9 +1368,16 @@ struct Engine::Impl {
     bool we_is_w = false;  // w_e equals w (no implicit vertical split): one device copy serves both
     int tune_waves = kWavesPerBlock;   // waves per block of the chunk kernels
     int tune_nt = 4;                   // most tracers per launch
-    // host copy of each tracer as last set (levels below the bottom are never
-    // touched by a step, so get_values and the all-values statistics read them here)
-    std::vector<std::vector<double>> host_vals;
+    // each tracer in the caller's dense (nl-1, nod2D) layout, as last set and
+    // then updated in the wet layers by get_values; the levels below the
+    // bottom are never stepped, so get_values and the all-values statistics
+    // read them here. dense_ab stages set_valuesAB uploads.
+    double *dense = nullptr, *dense_ab = nullptr;
+    // two pinned host buffers that stage host <-> device transfers in chunks,
+    // the host copy of one chunk overlapping the DMA of the other
+    static constexpr size_t kStageBytes = (size_t)64 << 20;
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 // the per-node records (incident edges, 1-ring fan, elements) at width D
@@ -1534,7 +1593,7 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
         p[kMlpOffB3] = 2.0f;
         mlp_set(p.data());
     }
-    I.host_vals.assign(ntr_, std::vector<double>((size_t)(nl_ - 1) * N_, 0.0));
+    I.dense = (double *)zalloc((size_t)(nl_ - 1) * N_ * ntr_ * sizeof(double));
     // FP16 gradient scale: mean edge length (gradient * length = tracer increment)
     double len = 0.0;
     for (int e = 0; e < E_; ++e) len += std::hypot(m.edge_dxdy[2 * e], m.edge_dxdy[2 * e + 1]) * kRE;
@@ -1548,6 +1607,10 @@ Engine::~Engine()
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void *p : p_->allocs) (void)hipFree(p);
     if (p_->dsX) { (void)hipFree(p_->dsX); (void)hipFree(p_->dsY); }
+    for (int b = 0; b < 2; ++b) {
+        if (p_->stage_ev[b]) (void)hipEventDestroy(p_->stage_ev[b]);
+        if (p_->stage[b]) (void)hipHostFree(p_->stage[b]);
+    }
     if (stream_) (void)hipStreamDestroy(stream_);
     delete p_;
 }
@@ -1598,13 +1661,82 @@ void Engine::set_velocity(const double *uv, const double *w, const double *we)
     I.we_is_w = same;
 }
 
+// the node layout's pair slots and column heights (the same in both precisions)
+static void node_slots(const Engine::Impl &I, int prec, const int *&slot, const int *&nlev, long long &npair)
+{
+    if (prec == PREC_FP64) { slot = I.m64.slot_n; nlev = I.m64.nlev; npair = I.m64.CN / 2; }
+    else { slot = I.m32.slot_n; nlev = I.m32.nlev; npair = I.m32.CN / 2; }
+}
+
+static void ensure_staging(Engine::Impl &I, hipStream_t st)
+{
+    for (int b = 0; b < 2; ++b) {
+        if (I.stage[b]) continue;
+        HIP_OK(hipHostMalloc(&I.stage[b], Engine::Impl::kStageBytes, hipHostMallocDefault));
+        HIP_OK(hipEventCreateWithFlags(&I.stage_ev[b], hipEventDisableTiming));
+        HIP_OK(hipEventRecord(I.stage_ev[b], st));
+    }
+}
+
+// host (pageable) -> device through the pinned buffers, in stream order on st
+static void h2d_staged(Engine::Impl &I, hipStream_t st, void *dst, const void *src, size_t bytes)
+{
+    ensure_staging(I, st);
+    const size_t B = Engine::Impl::kStageBytes;
+    int b = 0;
+    for (size_t o = 0; o < bytes; o += B, b ^= 1) {
+        const size_t n = std::min(B, bytes - o);
+        HIP_OK(hipEventSynchronize(I.stage_ev[b]));  // the DMA that last read this buffer is done
+        copy_host(I.stage[b], (const char *)src + o, n);
+        HIP_OK(hipMemcpyAsync((char *)dst + o, I.stage[b], n, hipMemcpyHostToDevice, st));
+        HIP_OK(hipEventRecord(I.stage_ev[b], st));
+    }
+    HIP_OK(hipStreamSynchronize(st));
+}
+
+// device -> host (pageable) through the pinned buffers, after the work on st
+static void d2h_staged(Engine::Impl &I, hipStream_t st, void *dst, const void *src, size_t bytes)
+{
+    ensure_staging(I, st);
+    const size_t B = Engine::Impl::kStageBytes;
+    const size_t nchunk = (bytes + B - 1) / B;
+    auto issue = [&](size_t i) {
+        const size_t o = i * B, n = std::min(B, bytes - o);
+        HIP_OK(hipMemcpyAsync(I.stage[i & 1], (const char *)src + o, n, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipEventRecord(I.stage_ev[i & 1], st));
+    };
+    if (nchunk) issue(0);
+    for (size_t i = 0; i < nchunk; ++i) {
+        // the other buffer was last read by the host copy of chunk i-1, which has returned
+        if (i + 1 < nchunk) issue(i + 1);
+        HIP_OK(hipEventSynchronize(I.stage_ev[i & 1]));
+        const size_t o = i * B, n = std::min(B, bytes - o);
+        copy_host((char *)dst + o, I.stage[i & 1], n);
+    }
+    HIP_OK(hipStreamSynchronize(st));
+}
+
+// dense (nl-1, nod2D) host array -> device dense copy -> chunk layout at dst
+static void upload_layers(Engine::Impl &I, int prec, hipStream_t st, int N, int L, const double *v, double *dense,
+                          double *dst)
+{
+    const int *slot, *nlev;
+    long long npair;
+    node_slots(I, prec, slot, nlev, npair);
+    HIP_OK(hipStreamSynchronize(st));  // no step or earlier transfer still reads dense or dst
+    h2d_staged(I, st, dense, v, (size_t)L * N * sizeof(double));
+    hipLaunchKernelGGL(k_pack_layers, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, st, slot, nlev, npair,
+                       dense, L, dst);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));
+}
+
 void Engine::set_values(int tr, const double *v)
 {
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     Impl &I = *p_;
-    std::memcpy(I.host_vals[tr].data(), v, sizeof(double) * I.host_vals[tr].size());
-    auto a = I.ln.pack<double>(v, nl_ - 1, 1);
-    HIP_OK(hipMemcpy(I.values + tr * I.tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
+    const size_t cells = (size_t)(nl_ - 1) * N_;
+    upload_layers(I, prec_, stream_, N_, nl_ - 1, v, I.dense + tr * cells, I.values + tr * I.tsn);
 }
 
 void Engine::set_valuesAB(int tr, const double *v)
@@ -1612,24 +1744,27 @@ void Engine::set_valuesAB(int tr, const double *v)
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     Impl &I = *p_;
     if (!I.valuesAB) {
+        HIP_OK(hipStreamSynchronize(stream_));
         I.valuesAB = I.track(dalloc<double>(I.tsn * ntr_));
         HIP_OK(hipMemcpy(I.valuesAB, I.values, I.tsn * ntr_ * sizeof(double), hipMemcpyDeviceToDevice));
     }
-    auto a = I.ln.pack<double>(v, nl_ - 1, 1);
-    HIP_OK(hipMemcpy(I.valuesAB + tr * I.tsn, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!I.dense_ab) I.dense_ab = I.track(dalloc<double>((size_t)(nl_ - 1) * N_));
+    upload_layers(I, prec_, stream_, N_, nl_ - 1, v, I.dense_ab, I.valuesAB + tr * I.tsn);
 }
 
 void Engine::get_values(int tr, double *v)
 {
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     Impl &I = *p_;
-    std::vector<double> a(I.tsn);
-    HIP_OK(hipStreamSynchronize(stream_));
-    HIP_OK(hipMemcpy(a.data(), I.values + tr * I.tsn, a.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int *slot, *nlev;
+    long long npair;
+    node_slots(I, prec_, slot, nlev, npair);
     const int L = nl_ - 1;
-    std::memcpy(v, I.host_vals[tr].data(), sizeof(double) * I.host_vals[tr].size());
-    for (int n = 0; n < N_; ++n)
-        for (int k = 0; k < I.ln.len[n] - 1; ++k) v[(size_t)n * L + k] = a[(size_t)I.ln.off[n] + k];
+    double *dense = I.dense + tr * (size_t)L * N_;
+    hipLaunchKernelGGL(k_unpack_layers, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, stream_, slot, nlev,
+                       npair, I.values + tr * I.tsn, L, dense);
+    HIP_OK(hipGetLastError());
+    d2h_staged(I, stream_, v, dense, (size_t)L * N_ * sizeof(double));
 }
 
 struct LaunchHook {
@@ -2106,28 +2241,20 @@ Diag Engine::diagnostics(int tr)
     Impl &I = *p_;
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     const double *v = I.values + tr * I.tsn;
+    const int L = nl_ - 1;
+    const double *dense = I.dense + tr * (size_t)L * N_;
     if (prec_ == PREC_FP64)
-        hipLaunchKernelGGL((k_diag_partial<double>), dim3(I.ndiag), dim3(256), 0, stream_, I.m64, v, I.dpart);
+        hipLaunchKernelGGL((k_diag_partial<double>), dim3(I.ndiag), dim3(256), 0, stream_, I.m64, v, dense, L,
+                           I.dpart);
     else
-        hipLaunchKernelGGL((k_diag_partial<float>), dim3(I.ndiag), dim3(256), 0, stream_, I.m32, v, I.dpart);
+        hipLaunchKernelGGL((k_diag_partial<float>), dim3(I.ndiag), dim3(256), 0, stream_, I.m32, v, dense, L,
+                           I.dpart);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, stream_, I.dpart, I.ndiag, I.dout);
     HIP_OK(hipGetLastError());
     DiagAcc a;
     HIP_OK(hipMemcpyAsync(&a, I.dout, sizeof(a), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
-    // the levels below the bottom (never stepped) from the host copy
-    Diag d{a.vmin, a.vmax, a.sum, a.amin, a.amax, a.mass, a.nan};
-    const int L = nl_ - 1;
-    const std::vector<double> &h = I.host_vals[tr];
-    for (int n = 0; n < N_; ++n)
-        for (int k = I.ln.len[n] - 1; k < L; ++k) {
-            const double x = h[(size_t)n * L + k];
-            if (is_nan_bits(x)) { d.nan_count += 1.0; continue; }
-            d.vmin = std::min(d.vmin, x);
-            d.vmax = std::max(d.vmax, x);
-            d.sum += x;
-        }
-    return d;
+    return Diag{a.vmin, a.vmax, a.sum, a.amin, a.amax, a.mass, a.nan};
 }
 
 void Engine::difference(const Engine &ref, int tr, double out[4])

@@ -15,6 +15,7 @@
 
 #include "../../include/fesom_tracer_amd.h"
 #include "fta_engine.h"
+#include "fta_hostcopy.h"
 #include "fta_mesh.h"
 #include "fta_restart.h"
 
@@ -274,6 +275,15 @@ int tracer_get_values_id(int tracer_id, double *values, int nz, int nn)
     if (!s.initialized) return fail("tracer not initialized.");
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id");
     if (nz != s.nl - 1 || nn > s.nod2D || nn < 0) return fail("tracer_get_values: size mismatch");
+    if (s.eng && nn == s.nod2D) {
+        // the whole array: straight from the device into the caller's buffer
+        try {
+            s.eng->get_values(tracer_id - 1, values);
+        } catch (const std::exception &e) {
+            return fail(e.what());
+        }
+        return 0;
+    }
     const std::vector<double> *src = &s.values[tracer_id - 1];
     std::vector<double> tmp;
     if (s.eng) {
@@ -285,7 +295,7 @@ int tracer_get_values_id(int tracer_id, double *values, int nz, int nn)
         }
         src = &tmp;
     }
-    std::memcpy(values, src->data(), sizeof(double) * (size_t)nz * nn);
+    copy_host(values, src->data(), sizeof(double) * (size_t)nz * nn);
     return 0;
 }
 
@@ -510,8 +520,8 @@ int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D)
     }
     const size_t n = (size_t)(nl - 1) * nod2D;
     if (s.values[tracer_id - 1].size() != n) return fail("tracer_set_values: size mismatch");
-    std::memcpy(s.values[tracer_id - 1].data(), values, sizeof(double) * n);
-    std::memcpy(s.valuesAB[tracer_id - 1].data(), values, sizeof(double) * n);
+    copy_host(s.values[tracer_id - 1].data(), values, sizeof(double) * n);
+    copy_host(s.valuesAB[tracer_id - 1].data(), values, sizeof(double) * n);
     if (s.eng) {
         try {
             s.eng->set_values(tracer_id - 1, values);
@@ -529,7 +539,7 @@ int tracer_set_values_ab(int tracer_id, const double *values, int nl, int nod2D)
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id");
     const size_t n = (size_t)(nl - 1) * nod2D;
     if (s.valuesAB[tracer_id - 1].size() != n) return fail("tracer_set_values_ab: size mismatch");
-    std::memcpy(s.valuesAB[tracer_id - 1].data(), values, sizeof(double) * n);
+    copy_host(s.valuesAB[tracer_id - 1].data(), values, sizeof(double) * n);
     if (s.eng) {
         try {
             s.eng->set_valuesAB(tracer_id - 1, values);
