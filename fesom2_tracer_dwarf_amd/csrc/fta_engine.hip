@@ -1374,8 +1374,10 @@ struct Engine::Impl {
     // read them here. dense_ab stages set_valuesAB uploads.
     double *dense = nullptr, *dense_ab = nullptr;
     // two pinned host buffers that stage host <-> device transfers in chunks,
-    // the host copy of one chunk overlapping the DMA of the other
-    static constexpr size_t kStageBytes = (size_t)64 << 20;
+    // the host copy of one chunk overlapping the DMA of the other; each holds
+    // one tracer array (64 MB at most), so small meshes pin little
+    static constexpr size_t kStageMax = (size_t)64 << 20;
+    size_t stage_bytes = 0;
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
@@ -1668,11 +1670,13 @@ static void node_slots(const Engine::Impl &I, int prec, const int *&slot, const 
     else { slot = I.m32.slot_n; nlev = I.m32.nlev; npair = I.m32.CN / 2; }
 }
 
-static void ensure_staging(Engine::Impl &I, hipStream_t st)
+static void ensure_staging(Engine::Impl &I, hipStream_t st, size_t array_bytes)
 {
+    if (!I.stage_bytes)
+        I.stage_bytes = std::min(Engine::Impl::kStageMax, std::max<size_t>(4096, (array_bytes + 4095) & ~(size_t)4095));
     for (int b = 0; b < 2; ++b) {
         if (I.stage[b]) continue;
-        HIP_OK(hipHostMalloc(&I.stage[b], Engine::Impl::kStageBytes, hipHostMallocDefault));
+        HIP_OK(hipHostMalloc(&I.stage[b], I.stage_bytes, hipHostMallocDefault));
         HIP_OK(hipEventCreateWithFlags(&I.stage_ev[b], hipEventDisableTiming));
         HIP_OK(hipEventRecord(I.stage_ev[b], st));
     }
@@ -1681,8 +1685,8 @@ static void ensure_staging(Engine::Impl &I, hipStream_t st)
 // host (pageable) -> device through the pinned buffers, in stream order on st
 static void h2d_staged(Engine::Impl &I, hipStream_t st, void *dst, const void *src, size_t bytes)
 {
-    ensure_staging(I, st);
-    const size_t B = Engine::Impl::kStageBytes;
+    ensure_staging(I, st, bytes);
+    const size_t B = I.stage_bytes;
     int b = 0;
     for (size_t o = 0; o < bytes; o += B, b ^= 1) {
         const size_t n = std::min(B, bytes - o);
@@ -1697,8 +1701,8 @@ static void h2d_staged(Engine::Impl &I, hipStream_t st, void *dst, const void *s
 // device -> host (pageable) through the pinned buffers, after the work on st
 static void d2h_staged(Engine::Impl &I, hipStream_t st, void *dst, const void *src, size_t bytes)
 {
-    ensure_staging(I, st);
-    const size_t B = Engine::Impl::kStageBytes;
+    ensure_staging(I, st, bytes);
+    const size_t B = I.stage_bytes;
     const size_t nchunk = (bytes + B - 1) / B;
     auto issue = [&](size_t i) {
         const size_t o = i * B, n = std::min(B, bytes - o);
