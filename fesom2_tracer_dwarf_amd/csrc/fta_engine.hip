@@ -1183,13 +1183,15 @@ __global__ __launch_bounds__(256) void k_diag_partial(DMesh<R> m, const double *
     }
     a.amin = a.vmin;
     a.amax = a.vmax;
-    const long long cells = (long long)m.N * L;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < cells; i += stride) {
-        const int n = (int)(i / L), k = (int)(i - (long long)n * L);
-        if (k < m.nlev[n] - 1) continue;
-        const double x = dense[i];
-        if (is_nan_bits(x)) { a.nan += 1.0; continue; }
-        a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
+    // one wave per column, lanes over its levels below the bottom
+    const int waves = (int)gridDim.x * 4, lane = (int)(threadIdx.x & 63);
+    for (int n = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); n < m.N; n += waves) {
+        const double *col = dense + (size_t)n * L;
+        for (int k = m.nlev[n] - 1 + lane; k < L; k += 64) {
+            const double x = col[k];
+            if (is_nan_bits(x)) { a.nan += 1.0; continue; }
+            a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
+        }
     }
     sh[threadIdx.x] = a;
     __syncthreads();
