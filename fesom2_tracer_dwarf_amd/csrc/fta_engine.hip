@@ -1172,26 +1172,33 @@ __global__ __launch_bounds__(256) void k_diag_partial(DMesh<R> m, const double *
         if (w < 0) continue;
         const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
         const int Lc = m.nlev[n] - 1;
+        if (k0 >= Lc) continue;
+        const V2<double> x2 = ld2(v + 2 * s), h2 = ld2(m.hn_d + 2 * s), a2 = ld2(m.av_d + 2 * s);
+        const double xs[2] = {x2.x, x2.y}, hs[2] = {h2.x, h2.y}, as[2] = {a2.x, a2.y};
+#pragma unroll
         for (int q = 0; q < 2; ++q) {
             if (k0 + q >= Lc) break;
-            const long long i = 2 * s + q;
-            const double x = v[i];
+            const double x = xs[q];
             if (is_nan_bits(x)) { a.nan += 1.0; continue; }
             a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
-            a.mass += x * m.hn_d[i] * m.av_d[i];
+            a.mass += x * hs[q] * as[q];
         }
     }
     a.amin = a.vmin;
     a.amax = a.vmax;
-    // one wave per column, lanes over its levels below the bottom
-    const int waves = (int)gridDim.x * 4, lane = (int)(threadIdx.x & 63);
-    for (int n = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); n < m.N; n += waves) {
-        const double *col = dense + (size_t)n * L;
-        for (int k = m.nlev[n] - 1 + lane; k < L; k += 64) {
-            const double x = col[k];
-            if (is_nan_bits(x)) { a.nan += 1.0; continue; }
-            a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
-        }
+    // the dense copy cell by cell (coalesced); the column from a reciprocal,
+    // corrected by one either way
+    const long long cells = (long long)m.N * L;
+    const double invL = 1.0 / (double)L;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < cells; i += stride) {
+        int n = (int)((double)i * invL);
+        long long k = i - (long long)n * L;
+        if (k < 0) { --n; k += L; }
+        else if (k >= L) { ++n; k -= L; }
+        if (k < m.nlev[n] - 1) continue;
+        const double x = dense[i];
+        if (is_nan_bits(x)) { a.nan += 1.0; continue; }
+        a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
     }
     sh[threadIdx.x] = a;
     __syncthreads();
