@@ -2,7 +2,7 @@
 
 ctypes binding of oracle/libfesom_oracle.so (built from fesom_oracle.c by
 oracle/Makefile). Only tests, the smoke check and the benchmark's CPU
-baseline use it, always as the checker; the product never imports it.
+baseline use it, to compare against or to time; the product never imports it.
 """
 from __future__ import annotations
 

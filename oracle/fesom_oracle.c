@@ -2,8 +2,8 @@
  * fesom_oracle.c -- CPU FP64 restatement of FESOM2's do_oce_adv_tra.
  *
  * TEST INFRASTRUCTURE ONLY (the parity oracle). Only tests, the smoke check
- * and the benchmark's CPU-baseline leg call this code, as the checker. The
- * product library never links or loads it.
+ * and the benchmark's CPU-baseline leg call this code, to compare against or
+ * to time. The product library never links or loads it.
  *
  * Every routine restates one reference routine loop for loop, in the same
  * loop order and the same operand order, so that FP64 results agree with the

@@ -2,8 +2,8 @@
  * fesom_oracle.h -- CPU FP64 restatement of FESOM2's tracer-advection step.
  *
  * TEST INFRASTRUCTURE ONLY. This is the parity oracle: tests, the smoke
- * check and the CPU-baseline leg of the benchmark call it as the checker.
- * The product library never links, loads or calls it.
+ * check and the CPU-baseline leg of the benchmark call it, to compare
+ * against or to time. The product library never links, loads or calls it.
  *
  * Conventions (mirror the Fortran data model of the reference):
  *   - arrays are column-major exactly as declared in the Fortran:
