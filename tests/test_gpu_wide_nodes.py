@@ -7,7 +7,8 @@ Bars: FP64 relative max-norm <= 1e-12 after each of 3 steps, mass to 1e-13,
 no new extrema with FCT beyond 1e-12 of the values' magnitude; FP32 <= 1e-5
 and FP16 gradients <= 1e-3 against the oracle after 3 steps; the learned
 limiter conserves mass to 1e-13 and creates no new extrema (FP64 1e-12, FP32
-2^-23 per step, relative to the magnitude).
+2^-23 per step, relative to the magnitude). The same mesh handed over as
+arrays by a caller steps bit for bit as the file-loaded one.
 """
 import numpy as np
 import pytest
@@ -136,3 +137,29 @@ def test_wide_records_learned_limiter(api, wheel_dir, prec):
             assert g[act].min() >= lo - tol and g[act].max() <= hi + tol, f"tracer {i + 1}: new extrema"
     finally:
         api.set_option("precision", 64)
+
+
+def test_wide_records_through_caller_supplied_mesh(api, wheel_dir):
+    """The same mesh handed over as arrays (tracer_mesh_begin / tracer_set_mesh_array / tracer_mesh_end, required
+    arrays plus coordinates and up/down-wind triangles) steps bit for bit as the file-loaded one."""
+    from test_mesh_arrays import REQUIRED
+    print("[gpu] 16-wide node records, caller-supplied mesh", flush=True)
+    schemes = ("MUSCL", "QR4C", "FCT")
+    d, m, dt, trs, act, vol = _setup(api, wheel_dir, schemes)
+    given = {k: api.mesh_array(k) for k in REQUIRED + ["edge_up_dn_tri", "coord_nod2D", "areasvol", "hnode_new"]}
+    uv, w, we = api.mesh_array("uv"), api.mesh_array("w"), api.mesh_array("w_e")
+    api.advect_steps(2, dt)
+    ref = [api.get_tracer_values(i + 1) for i in range(NTR)]
+    got = api.set_mesh(d["nl"], d["nod2D"], d["elem2D"], d["edge2D"], d["maxAdj"], given)
+    assert got == d
+    api.allocate_tracers(NTR)
+    L = d["nl"] - 1
+    api.set_velocity_elem(uv.reshape(2, L, d["elem2D"], order="F"), w.reshape(d["nl"], d["nod2D"], order="F"),
+                          we.reshape(d["nl"], d["nod2D"], order="F"))
+    for i, t in enumerate(trs):
+        api.set_values(i + 1, t)
+        api.set_schemes(i + 1, *schemes)
+    api.advect_steps(2, dt)
+    for i in range(NTR):
+        g = api.get_tracer_values(i + 1)
+        assert np.array_equal(g.view(np.uint64), ref[i].view(np.uint64)), f"tracer {i + 1}"
