@@ -14,6 +14,15 @@ form (FP64 node tracer/accumulation) is reported beside it, and so is the
 rate of a caller that hands the tracers over as host arrays every step
 ("host_round_trip": upload, one step, download over PCIe).
 
+Roofline: the step's compulsory HBM bytes (every semantic input read once and
+every output written once, at its stored precision) over the measured step
+time, against the 8 TB/s peak; the measured step traffic (rocprofv3
+FETCH_SIZE + WRITE_SIZE, copy-calibrated), the busiest kernel's own figure and
+the rate of the library's hand-written streaming copy (the practical ceiling)
+are reported beside it. CPU baseline: the oracle (a C restatement of the
+reference's FP64 path) on the host cores -- all cores through its OpenMP build,
+one thread, and the 20x20x10 analytic case -- on bounded samples.
+
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -126,32 +135,82 @@ def host_round_trip(api, dt, trs, c, reps=3):
                 what="set_values + 1 step + get_values of every tracer, FP64, pageable host arrays")
 
 
-def cpu_baseline(api, dt, ntr, c, max_seconds=20.0):
-    log = globals()["log"]
-    """Oracle (the CPU FP64 restatement of the reference path), 1 thread, on the
-    same mesh: as many single-tracer steps as fit in ~max_seconds (at least one)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def host_threads():
+    """Threads for the all-cores leg: the CPU share this process may use."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def oracle_leg(api, dt, trs, threads, max_seconds, max_steps, hor="MUSCL", ver="QR4C", lim="FCT"):
+    """Time single-tracer oracle steps of the api's current mesh (tracers in
+    turn) until max_seconds or max_steps; returns (steps, seconds)."""
     import oracle as orc
-    from fesom2_tracer_dwarf_amd import configs
-    names = orc.MESH_INT + orc.MESH_DBL
-    mesh = {k: api.mesh_array(k) for k in names}
+    mesh = {k: api.mesh_array(k) for k in orc.MESH_INT + orc.MESH_DBL}
     mesh.update(api.dims())
-    o = orc.Oracle(mesh)
+    o = orc.Oracle(mesh, threads=threads)
     uv, w, we = api.mesh_array("uv"), api.mesh_array("w"), api.mesh_array("w_e")
-    cfg = configs.WORKLOADS["periodic_1000x1000x64"]
-    trs = configs.initial_tracers(api.mesh_array("coord"), mesh["nl"], cfg["Lx"], cfg["Ly"], ntr, seed=0)
+    vals = [np.array(t, order="F", copy=True) for t in trs]
     done, t_total = 0, 0.0
     while True:
-        v = trs[done % ntr]
         t0 = time.perf_counter()
-        o.step(dt, uv, w, we, v, hor="MUSCL", ver="QR4C", lim="FCT")
+        o.step(dt, uv, w, we, vals[done % len(vals)], hor=hor, ver=ver, lim=lim)
         t_total += time.perf_counter() - t0
         done += 1
-        log(f"cpu step {done}: {t_total:.1f} s")
-        if t_total >= max_seconds or done >= 4:
+        if t_total >= max_seconds or done >= max_steps:
             break
-    return dict(value=done * c["C"] / t_total, unit="updates/s", cores=1, kind="port",
-                sample=f"{done} single-tracer MUSCL+QR4C+FCT steps of the same 1000x1000x64 mesh "
-                       f"({t_total:.1f} s, 1 thread, {platform.processor() or platform.machine()})")
+    return done, t_total, o.threads
+
+
+def cpu_baseline(api, dt, ntr, c, max_seconds=20.0):
+    """The oracle (the CPU FP64 restatement of the reference path) on the host
+    cores, on bounded samples: config 1 on all cores (OpenMP build, bitwise the
+    serial result) and on one thread, and config 0 (the analytic dwarf, 20x20
+    periodic nodes x 10 levels, 2 tracers, 10 steps) on one thread. Leaves the
+    library holding the config-0 mesh."""
+    from fesom2_tracer_dwarf_amd import configs
+    cfg = configs.WORKLOADS["periodic_1000x1000x64"]
+    trs = configs.initial_tracers(api.mesh_array("coord"), cfg["nl"], cfg["Lx"], cfg["Ly"], ntr, seed=0)
+    model = cpu_model()
+    nthr = host_threads()
+    log(f"cpu baseline: all cores ({nthr} threads)")
+    n_all, t_all, used = oracle_leg(api, dt, trs, nthr, 0.6 * max_seconds, 16)
+    log(f"cpu all cores: {n_all} tracer-steps in {t_all:.1f} s")
+    log("cpu baseline: 1 thread")
+    n_one, t_one, _ = oracle_leg(api, dt, trs, None, 0.4 * max_seconds, 4)
+    log(f"cpu 1 thread: {n_one} tracer-steps in {t_one:.1f} s")
+    # config 0 on one thread, whole (10 steps of both tracers)
+    c0 = configs.WORKLOADS["analytic_20x20x10"]
+    api.generate_mesh(c0["nx"], c0["ny"], c0["nl"], periodic=True, Lx=c0["Lx"], Ly=c0["Ly"], dz=c0["dz"],
+                      lev_lo=c0["lev"][0], lev_hi=c0["lev"][1], seed=0, sfc=False)
+    api.allocate_tracers(c0["ntr"])
+    dt0 = api.make_velocity(0.1, 0.05, 0.2, 0.5, 0.3)
+    trs0 = configs.initial_tracers(api.mesh_array("coord"), c0["nl"], c0["Lx"], c0["Ly"], c0["ntr"], seed=0)
+    C0 = int((api.mesh_array("nlevels_nod2D") - 1).sum())
+    n0, t0, _ = oracle_leg(api, dt0, trs0, None, 1e9, c0["steps"] * c0["ntr"])
+    return dict(value=n_all * c["C"] / t_all, unit="updates/s", cores=used, kind="port",
+                cpu=model, nproc=os.cpu_count(),
+                sample=f"{n_all} single-tracer MUSCL+QR4C+FCT steps of periodic_1000x1000x64 ({t_all:.1f} s) on "
+                       f"{used} threads of the OpenMP oracle build (deterministic, equal to the serial build bit for "
+                       f"bit)",
+                one_thread=dict(value=n_one * c["C"] / t_one, unit="updates/s", cores=1,
+                                sample=f"{n_one} single-tracer steps of periodic_1000x1000x64 ({t_one:.1f} s), "
+                                       "serial oracle"),
+                config0=dict(value=n0 * C0 / t0, unit="updates/s", cores=1, ms_per_step=t0 / c0["steps"] * 1e3,
+                             sample=f"analytic_20x20x10: {c0['steps']} steps x {c0['ntr']} tracers, MUSCL+QR4C+FCT, "
+                                    f"serial oracle ({t0 * 1e3:.1f} ms)"))
 
 
 def log(msg):
@@ -172,23 +231,11 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     ap.add_argument("--copy-bandwidth", action="store_true",
-                    help="only time a 4 GiB device-to-device copy (the practical HBM ceiling)")
+                    help="only time the library's streaming copy of 4 GiB (the practical HBM ceiling)")
     args = ap.parse_args()
     if args.copy_bandwidth:
-        import torch
-        a = torch.ones(1 << 29, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record()
-        for _ in range(20):
-            b.copy_(a)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / 20
-        print(json.dumps({"copy_bytes_each_way": a.numel() * 8, "ms_per_copy": ms,
-                          "GBps_read_plus_write": 2 * a.numel() * 8 / ms / 1e6}))
+        from fesom2_tracer_dwarf_amd import FESOMTracerAdvection
+        print(json.dumps(FESOMTracerAdvection().hbm_copy_rate(4 << 30, 20)))
         return
     if args.calibrate_copy:
         import torch
@@ -252,11 +299,16 @@ def main():
 
     f64 = res["fp64"]
     dom = f64["dominant"]
-    achieved = f64["kernels_gbs"][dom]
-    traffic = None
+    kern_traffic = step_traffic = None
     if (cfg["nx"], cfg["ny"], cfg["nl"]) == (1000, 1000, 64):
         from fesom2_tracer_dwarf_amd.rocprof_summary import read_traffic
-        traffic = read_traffic(TRAFFIC_TABLE).get(dom, {}).get("fp64")
+        tt = read_traffic(TRAFFIC_TABLE)
+        kern_traffic = tt.get(dom, {}).get("fp64")
+        per = [tt.get(k, {}).get("fp64") for k in f64["kernels_ms"]]
+        step_traffic = sum(per) if per and all(v is not None for v in per) else None
+    log("copy ceiling (hand-written streaming copy)")
+    ceiling = api.hbm_copy_rate(4 << 30, 10)
+    log(f"copy ceiling: {ceiling}")
     line = {
         "metric": METRIC,
         "value": f64["updates_per_s"],
@@ -275,19 +327,26 @@ def main():
                    "bottom_levels": "U{3..64} seed 0", "tracers": ntr, "schemes": "MUSCL+QR4C+FCT",
                    "active_node_cells": c["C"], "dense_node_cells": c["N"] * (cfg["nl"] - 1),
                    "dt_s": dt, "cfl": 0.3},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes": kernel_alg_bytes(dom, c, ntr, 8, 8, 8),
-                     "traffic_source": str(TRAFFIC_TABLE.relative_to(ROOT)) if traffic else None},
-        "step_roofline": {"alg_bytes": f64["step_alg_bytes"], "achieved_gbs": f64["step_gbs"],
-                          "frac": f64["step_hbm_frac"]},
+        # BASELINE.md's definition: the step's compulsory bytes (each semantic
+        # input read once, each output written once) over the step time
+        "roofline": {"bound": "hbm", "achieved": f64["step_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": f64["step_gbs"] / HBM_PEAK_GBS, "traffic": step_traffic,
+                     "alg_bytes": f64["step_alg_bytes"], "scope": "whole step (all kernels, both tracers)",
+                     "traffic_source": str(TRAFFIC_TABLE.relative_to(ROOT)) if step_traffic else None,
+                     "copy_ceiling_gbs": ceiling["gbs"], "frac_of_copy_ceiling": f64["step_gbs"] / ceiling["gbs"]},
+        "kernel_roofline": {"kernel": dom, "achieved": f64["kernels_gbs"][dom], "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": f64["kernels_gbs"][dom] / HBM_PEAK_GBS,
+                            "alg_bytes": kernel_alg_bytes(dom, c, ntr, 8, 8, 8), "traffic": kern_traffic,
+                            "ms": f64["kernels_ms"][dom]},
+        "measured_traffic_gbs": (step_traffic / (f64["ms_per_step"] * 1e-3) / 1e9) if step_traffic else None,
+        "copy_ceiling": ceiling,
         "variants": {k: {kk: vv for kk, vv in v.items() if kk not in ("wall_s",)} for k, v in res.items()},
         "fp32_over_fp64": res["fp32"]["updates_per_s"] / f64["updates_per_s"],
         "host_round_trip": host_rt,
         "setup_s": t_setup,
     }
     if not args.no_cpu:
-        log("cpu baseline (oracle, 1 thread)")
+        log("cpu baseline (oracle)")
         api.set_option("precision", 64)
         line["cpu_baseline"] = cpu_baseline(api, dt, ntr, c, args.cpu_seconds)
     else:

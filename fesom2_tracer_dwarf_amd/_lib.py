@@ -82,6 +82,7 @@ SIGNATURES = {
     "tracer_mlp_gradient": (c_int, [F32, F32, c_longlong, c_float, F32, POINTER(c_double)]),
     "tracer_learn_fallback": (c_int, [c_int, POINTER(c_double)]),
     "tracer_autotune": (c_int, [c_double, c_int, F64, c_int]),
+    "tracer_hbm_copy_rate": (c_int, [c_longlong, c_int, POINTER(c_double), POINTER(c_int)]),
     "tracer_batch_create": (c_int, []),
     "tracer_batch_destroy": (c_int, []),
     "tracer_batch_add_periodic_mesh": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_int, c_int,

@@ -130,6 +130,11 @@ private:
     hipStream_t stream_ = nullptr;
 };
 
+// read+write rate (GB/s) of the best of a few launch shapes of a plain
+// 16-byte streaming copy of `bytes` (hand-written kernel, fta_stream.hip);
+// variant receives {workgroups, words in flight per lane, nontemporal}
+double hbm_copy_rate(long long bytes, int reps, int variant[3]);
+
 const char *last_error();
 void set_error(const std::string &s);
 

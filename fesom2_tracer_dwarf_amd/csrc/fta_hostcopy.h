@@ -21,12 +21,20 @@ inline void copy_host(void *dst, const void *src, size_t bytes)
     }
     const size_t part = (bytes / nt + 63) & ~(size_t)63;
     std::vector<std::thread> th;
-    for (int i = 0; i < nt; ++i) {
-        const size_t o = (size_t)i * part;
-        if (o >= bytes) break;
-        const size_t n = std::min(part, bytes - o);
-        th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, n); });
+    size_t done = 0;  // bytes handed to threads so far (from the start)
+    try {
+        th.reserve(nt);
+        for (int i = 0; i < nt; ++i) {
+            const size_t o = (size_t)i * part;
+            if (o >= bytes) break;
+            const size_t n = std::min(part, bytes - o);
+            th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, n); });
+            done = o + n;
+        }
+    } catch (...) {
+        // a thread could not be started: the caller's thread copies the rest
     }
+    if (done < bytes) std::memcpy((char *)dst + done, (const char *)src + done, bytes - done);
     for (std::thread &t : th) t.join();
 }
 

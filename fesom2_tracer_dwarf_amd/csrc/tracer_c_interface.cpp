@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -252,7 +253,8 @@ int tracer_get_stats(double *tmin, double *tmax, double *tsum)
     if (tracer_get_diagnostics(1, d)) return -1;
     *tmin = d[0];
     *tmax = d[1];
-    *tsum = d[2];
+    // the reference's sum(values) propagates a NaN (the diagnostics' sum skips them)
+    *tsum = d[6] > 0.0 ? std::nan("") : d[2];
     return 0;
 }
 
@@ -1178,6 +1180,18 @@ int tracer_autotune(double dt, int reps, double *table, int max_rows)
     } catch (const std::exception &e) {
         return fail(e.what());
     }
+}
+
+int tracer_hbm_copy_rate(long long bytes, int reps, double *gbs, int *variant)
+{
+    try {
+        int v[3] = {0, 0, 0};
+        *gbs = hbm_copy_rate(bytes, reps, v);
+        if (variant) for (int i = 0; i < 3; ++i) variant[i] = v[i];
+    } catch (const std::exception &e) {
+        return fail(std::string("tracer_hbm_copy_rate: ") + e.what());
+    }
+    return 0;
 }
 
 const char *tracer_last_error(void) { return last_error(); }

@@ -213,6 +213,15 @@ class FESOMTracerAdvection:
         rows = table[:3 * n].reshape(n, 3)
         return [dict(block_waves=int(w), batch_tracers=int(b), ms_per_step=float(ms)) for w, b, ms in rows]
 
+    def hbm_copy_rate(self, nbytes=4 << 30, reps=10):
+        """Practical HBM ceiling: best (read + write) GB/s of the library's
+        16-byte streaming copy over a few launch shapes, and that shape."""
+        gbs, v = ctypes.c_double(), (ctypes.c_int * 3)()
+        _lib.check(self.lib.tracer_hbm_copy_rate(int(nbytes), int(reps), ctypes.byref(gbs), v),
+                   "tracer_hbm_copy_rate", self.lib)
+        return dict(gbs=gbs.value, bytes_each_way=int(nbytes) // 16 * 16, workgroups=v[0], words_in_flight=v[1],
+                    nontemporal=bool(v[2]))
+
     # ---- learned limiter
     def mlp_param_count(self):
         return int(self.lib.tracer_mlp_param_count())

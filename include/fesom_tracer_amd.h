@@ -48,7 +48,9 @@ int tracer_init(const char *restart_dir);
 int tracer_advect_step(double dt);
 
 /* lib/tracer_c_interface.F90:228-248: min, max and sum of all
- * (nl-1) x nod2D values of tracer 1 (inactive levels included). */
+ * (nl-1) x nod2D values of tracer 1 (inactive levels included). As the
+ * reference's sum(values), the sum is NaN when a value is NaN; min and max
+ * skip NaN values (tracer_get_diagnostics counts them). */
 int tracer_get_stats(double *tmin, double *tmax, double *tsum);
 
 /* lib/tracer_c_interface.F90:255-272: nz = nl-1, nn = myDim_nod2D */
@@ -325,6 +327,13 @@ int tracer_write_mesh(const char *output_dir);
 int tracer_output_open(const char *output_dir, int nsteps);
 int tracer_output_step(void);
 int tracer_output_close(void);
+
+/* No reference counterpart: the practical HBM ceiling. Times reps copies of
+ * a `bytes` device buffer with a hand-written 16-byte streaming kernel over a
+ * few launch shapes; *gbs = best (read + write bytes) / time in GB/s, and
+ * variant[3] (may be NULL) = {workgroups, 16-byte words in flight per lane,
+ * nontemporal hint} of the best. Allocates and frees two buffers. */
+int tracer_hbm_copy_rate(long long bytes, int reps, double *gbs, int *variant);
 
 /* No reference counterpart: the last error message */
 const char *tracer_last_error(void);

@@ -1,8 +1,11 @@
 """CPU FP64 oracle for the tracer-advection step -- TEST INFRASTRUCTURE ONLY.
 
 ctypes binding of oracle/libfesom_oracle.so (built from fesom_oracle.c by
-oracle/Makefile). Only tests, the smoke check and the benchmark's CPU
-baseline use it, to compare against or to time; the product never imports it.
+oracle/Makefile) and of its OpenMP build libfesom_oracle_omp.so (the same
+source; the edge-to-node scatters run as gathers in edge order, so its
+results equal the serial build's bit for bit). Only tests, the smoke check
+and the benchmark's CPU baseline use it, to compare against or to time; the
+product never imports it.
 """
 from __future__ import annotations
 
@@ -14,6 +17,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "libfesom_oracle.so"
+LIB_OMP = HERE / "libfesom_oracle_omp.so"
 
 HOR = {"UPW1": 0, "MUSCL": 1, "MFCT": 2}
 VER = {"UPW1": 0, "QR4C": 1, "PPM": 2, "CDIFF": 3}
@@ -49,7 +53,7 @@ MESH_INT = ("edges", "edge_tri", "elem2D_nodes", "nlevels", "ulevels", "nlevels_
 MESH_DBL = ("coord_nod2D", "edge_dxdy", "edge_cross_dxdy", "elem_cos", "elem_area", "gradient_sca", "area",
             "areasvol", "hnode", "hnode_new", "helem", "Z_3d_n", "zbar_3d_n")
 
-_LIB = None
+_LIBS = {}
 
 
 def build() -> Path:
@@ -57,12 +61,13 @@ def build() -> Path:
     return LIB
 
 
-def load():
-    global _LIB
-    if _LIB is None:
-        if not LIB.exists():
+def load(omp: bool = False):
+    """The serial oracle, or its OpenMP build (omp=True)."""
+    if omp not in _LIBS:
+        path = LIB_OMP if omp else LIB
+        if not path.exists():
             build()
-        lib = ctypes.CDLL(str(LIB))
+        lib = ctypes.CDLL(str(path))
         P = ctypes.POINTER
         M, W = P(OracleMesh), P(OracleWork)
         d, i = ctypes.c_double, ctypes.c_int
@@ -82,8 +87,11 @@ def load():
         for nm in ("oracle_adv_tra_hor_muscl", "oracle_adv_tra_hor_mfct"):
             getattr(lib, nm).argtypes = [M, _dp, _dp, d, _dp, _dp, i]
         lib.oracle_tra_adv_fct.argtypes = [M, d, _dp, _dp, _dp, _dp, W]
-        _LIB = lib
-    return _LIB
+        lib.oracle_tra_adv_flux2dtracer.argtypes = [M, d, _dp, _dp, _dp, _dp, i, _dp, _dp]
+        lib.oracle_set_threads.argtypes = [i]
+        lib.oracle_set_threads.restype = i
+        _LIBS[omp] = lib
+    return _LIBS[omp]
 
 
 def _ptr(a, ct):
@@ -91,10 +99,13 @@ def _ptr(a, ct):
 
 
 class Oracle:
-    """Holds one mesh (dict of arrays in the reference layout) and work arrays."""
+    """Holds one mesh (dict of arrays in the reference layout) and work arrays.
+    threads=None runs the serial build; threads=n the OpenMP build on n threads
+    (0: OpenMP's default), whose results are bitwise the same."""
 
-    def __init__(self, mesh: dict):
-        self.lib = load()
+    def __init__(self, mesh: dict, threads: int | None = None):
+        self.lib = load(omp=threads is not None)
+        self.threads = self.lib.oracle_set_threads(threads) if threads is not None else 1
         self.arrays = {}
         for k in MESH_INT:
             self.arrays[k] = np.ascontiguousarray(mesh[k], dtype=np.int32)

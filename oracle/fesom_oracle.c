@@ -10,6 +10,14 @@
  * reference to rounding. Level loops keep the reference's 1-based bounds;
  * the macros below map (level, index) to the column-major arrays.
  * Reference files are cited relative to the reference repository root.
+ *
+ * Built twice (oracle/Makefile): serially, and with -fopenmp for the
+ * all-cores CPU baseline. The OpenMP build splits the loops whose iterations
+ * are independent, and runs the three edge-to-node scatters (the LO sum, the
+ * b1 sums of FCT, the divergence into dttf_h) as gathers over each node's
+ * incident edges in increasing edge index: every node receives the same
+ * additions in the same order as in the serial edge loop, so both builds give
+ * identical bits.
  */
 #include "fesom_oracle.h"
 
@@ -31,6 +39,67 @@ static inline double dmin(double a, double b) { return a < b ? a : b; }
 static inline int imax(int a, int b) { return a > b ? a : b; }
 static inline int imin(int a, int b) { return a < b ? a : b; }
 
+#ifdef _OPENMP
+#include <omp.h>
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#define OMP_PARALLEL _Pragma("omp parallel")
+#define OMP_FOR_IN _Pragma("omp for schedule(static)")
+#else
+#define OMP_FOR
+#define OMP_PARALLEL
+#define OMP_FOR_IN
+#endif
+
+int oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+#ifdef _OPENMP
+/* node -> incident edges in increasing edge index, +(e+1) where the node is
+ * edges(1), -(e+1) where it is edges(2); rebuilt when the mesh changes */
+static struct { const int *edges; int N, E; int *ptr, *idx; } g_n2e;
+
+static void n2e_lists(const oracle_mesh *m, const int **ptr, const int **idx)
+{
+    if (g_n2e.edges != m->edges || g_n2e.N != m->nod2D || g_n2e.E != m->edge2D || !g_n2e.ptr) {
+        free(g_n2e.ptr);
+        free(g_n2e.idx);
+        int *p = (int *)calloc((size_t)m->nod2D + 1, sizeof(int));
+        int *x = (int *)malloc(sizeof(int) * (size_t)(2 * m->edge2D + 1));
+        for (int e = 0; e < m->edge2D; ++e) { p[m->edges[2 * e] + 1]++; p[m->edges[2 * e + 1] + 1]++; }
+        for (int n = 0; n < m->nod2D; ++n) p[n + 1] += p[n];
+        int *fill = (int *)malloc(sizeof(int) * (size_t)(m->nod2D + 1));
+        memcpy(fill, p, sizeof(int) * (size_t)m->nod2D);
+        for (int e = 0; e < m->edge2D; ++e) {
+            x[fill[m->edges[2 * e]]++] = e + 1;
+            x[fill[m->edges[2 * e + 1]]++] = -(e + 1);
+        }
+        free(fill);
+        g_n2e.edges = m->edges; g_n2e.N = m->nod2D; g_n2e.E = m->edge2D; g_n2e.ptr = p; g_n2e.idx = x;
+    }
+    *ptr = g_n2e.ptr;
+    *idx = g_n2e.idx;
+}
+
+/* the level range nu12..nl12 of an edge over which the scatters run */
+static inline void edge_scatter_range(const oracle_mesh *m, int e, int *nu12, int *nl12)
+{
+    int el1 = m->edge_tri[2 * e], el2 = m->edge_tri[2 * e + 1];
+    int nl1 = m->nlevels[el1] - 1, nu1 = m->ulevels[el1], nl2 = 0, nu2 = 0;
+    if (el2 >= 0) { nl2 = m->nlevels[el2] - 1; nu2 = m->ulevels[el2]; }
+    *nl12 = nl1 > nl2 ? nl1 : nl2;
+    *nu12 = nu1;
+    if (nu2 > 0) *nu12 = nu1 < nu2 ? nu1 : nu2;
+}
+#endif
+
 /* ------------------------------------------------------------------------
  * Elemental tracer gradient tr_xy(1:2, nz, elem). The reference declares
  * tr_xy (lib/oce_modules.F90:212) but vendors no routine that fills it; this
@@ -40,6 +109,7 @@ static inline int imin(int a, int b) { return a < b ? a : b; }
  * ---------------------------------------------------------------------- */
 void oracle_tracer_gradient_elements(const oracle_mesh *m, const double *ttf, double *tr_xy)
 {
+    OMP_FOR
     for (int elem = 0; elem < m->elem2D; ++elem) {
         const int *en = &m->elem2D_nodes[3 * elem];
         const double *g = &m->gradient_sca[6 * elem];
@@ -71,8 +141,9 @@ static void node_mean_grad(const oracle_mesh *m, const double *tr_xy, int node, 
 /* fill_up_dn_grad: lib/oce_muscl_adv.F90:361-530 */
 void oracle_fill_up_dn_grad(const oracle_mesh *m, const double *tr_xy, double *grad)
 {
-    double gx, gy;
+    OMP_FOR
     for (int edge = 0; edge < m->edge2D; ++edge) {
+        double gx, gy;
         int n1 = m->edges[2 * edge], n2 = m->edges[2 * edge + 1];
         int up = m->edge_up_dn_tri[2 * edge], dn = m->edge_up_dn_tri[2 * edge + 1];
         if (up >= 0 && dn >= 0) {
@@ -150,6 +221,7 @@ void oracle_adv_tra_hor_upw1(const oracle_mesh *m, const double *vel, const doub
                              double *flux, int init_zero)
 {
     if (init_zero) memset(flux, 0, sizeof(double) * (size_t)(m->nl - 1) * m->edge2D);
+    OMP_FOR
     for (int edge = 0; edge < m->edge2D; ++edge) {
         edge_ctx c; edge_setup(m, edge, &c);
         double vflux;
@@ -172,6 +244,7 @@ static void hor_muscl_common(const oracle_mesh *m, const double *vel, const doub
                              int use_clo)
 {
     if (init_zero) memset(flux, 0, sizeof(double) * (size_t)(m->nl - 1) * m->edge2D);
+    OMP_FOR
     for (int edge = 0; edge < m->edge2D; ++edge) {
         edge_ctx c; edge_setup(m, edge, &c);
         const double ex = m->edge_dxdy[2 * edge], ey = m->edge_dxdy[2 * edge + 1];
@@ -228,6 +301,7 @@ void oracle_adv_tra_ver_upw1(const oracle_mesh *m, const double *W, const double
                              double *flux, int init_zero)
 {
     if (init_zero) zero_vflux(m, flux);
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
         int nz = nzmin;
@@ -248,6 +322,7 @@ void oracle_adv_tra_ver_qr4c(const oracle_mesh *m, const double *W, const double
                              double num_ord, double *flux, int init_zero)
 {
     if (init_zero) zero_vflux(m, flux);
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
         int nz = nzmin;
@@ -278,8 +353,11 @@ void oracle_adv_tra_vert_ppm(const oracle_mesh *m, double dt, const double *W, c
                              double *flux, int init_zero)
 {
     if (init_zero) zero_vflux(m, flux);
+    OMP_PARALLEL
+    {
     double *tv = (double *)malloc(sizeof(double) * (size_t)(m->nl + 1));
     double *tvert = (double *)malloc(sizeof(double) * (size_t)(m->nl + 1));
+    OMP_FOR_IN
     for (int n = 0; n < m->nod2D; ++n) {
         int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
         const double *h = m->hnode_new;
@@ -341,6 +419,7 @@ void oracle_adv_tra_vert_ppm(const oracle_mesh *m, double dt, const double *W, c
     }
     free(tv);
     free(tvert);
+    }
 }
 
 /* adv_tra_ver_cdiff: lib/oce_adv_tra_ver.F90:648-712 */
@@ -348,7 +427,10 @@ void oracle_adv_tra_ver_cdiff(const oracle_mesh *m, const double *W, const doubl
                               double *flux, int init_zero)
 {
     if (init_zero) zero_vflux(m, flux);
+    OMP_PARALLEL
+    {
     double *tvert = (double *)malloc(sizeof(double) * (size_t)(m->nl + 1));
+    OMP_FOR_IN
     for (int n = 0; n < m->nod2D; ++n) {
         int nzmax = m->nlevels_nod2D[n] - 1, nzmin = m->ulevels_nod2D[n];
         tvert[nzmin] = -LI(W, nzmin, n) * L1(ttf, nzmin, n) * LI(m->area, nzmin, n);
@@ -360,6 +442,7 @@ void oracle_adv_tra_ver_cdiff(const oracle_mesh *m, const double *W, const doubl
         for (int k = nzmin; k <= nzmax; ++k) LI(flux, k, n) = tvert[k] - LI(flux, k, n);
     }
     free(tvert);
+    }
 }
 
 /* ------------------------------------------------------------------------
@@ -378,6 +461,7 @@ void oracle_tra_adv_fct(const oracle_mesh *m, double dt, const double *ttf, cons
     double *tvert_max = wk->tvert_max, *tvert_min = wk->tvert_min;
 
     /* a1 (:94-114) */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
         for (int nz = nu1; nz <= nl1 - 1; ++nz) {
@@ -386,6 +470,7 @@ void oracle_tra_adv_fct(const oracle_mesh *m, double dt, const double *ttf, cons
         }
     }
     /* a2 (:127-147) */
+    OMP_FOR
     for (int elem = 0; elem < m->elem2D; ++elem) {
         const int *en = &m->elem2D_nodes[3 * elem];
         int nu1 = m->ulevels[elem], nl1 = m->nlevels[elem];
@@ -397,6 +482,7 @@ void oracle_tra_adv_fct(const oracle_mesh *m, double dt, const double *ttf, cons
             for (int nz = nl1; nz <= m->nl - 1; ++nz) { AUX(1, nz, elem) = -bignumber; AUX(2, nz, elem) = bignumber; }
     }
     /* a3 (:162-184) */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
         const int *nie = &m->nod_in_elem2D[(size_t)n * m->maxAdj];
@@ -410,6 +496,7 @@ void oracle_tra_adv_fct(const oracle_mesh *m, double dt, const double *ttf, cons
         }
     }
     /* admissible increments (:194-217) */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
         L1(tmax, nu1, n) = L1(tvert_max, nu1, n) - L1(lo, nu1, n);
@@ -423,10 +510,12 @@ void oracle_tra_adv_fct(const oracle_mesh *m, double dt, const double *ttf, cons
         L1(tmin, nz, n) = L1(tvert_min, nz, n) - L1(lo, nz, n);
     }
     /* b1 (:232-346) */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
         for (int nz = nu1; nz <= nl1 - 1; ++nz) { L1(plus, nz, n) = 0.0; L1(minus, nz, n) = 0.0; }
     }
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
         for (int nz = nu1; nz <= nl1 - 1; ++nz) {
@@ -434,6 +523,7 @@ void oracle_tra_adv_fct(const oracle_mesh *m, double dt, const double *ttf, cons
             L1(minus, nz, n) = L1(minus, nz, n) + (dmin(0.0, LI(adf_v, nz, n)) + dmin(0.0, -LI(adf_v, nz + 1, n)));
         }
     }
+#ifndef _OPENMP
     for (int edge = 0; edge < m->edge2D; ++edge) {
         int n1 = m->edges[2 * edge], n2 = m->edges[2 * edge + 1];
         int el1 = m->edge_tri[2 * edge], el2 = m->edge_tri[2 * edge + 1];
@@ -450,7 +540,25 @@ void oracle_tra_adv_fct(const oracle_mesh *m, double dt, const double *ttf, cons
             L1(minus, nz, n2) = L1(minus, nz, n2) + dmin(0.0, -L1(adf_h, nz, edge));
         }
     }
+#else
+    {
+        const int *ep, *ei;
+        n2e_lists(m, &ep, &ei);
+        OMP_FOR
+        for (int n = 0; n < m->nod2D; ++n)
+            for (int j = ep[n]; j < ep[n + 1]; ++j) {
+                int edge = (ei[j] > 0 ? ei[j] : -ei[j]) - 1, nu12, nl12;
+                double sg = ei[j] > 0 ? 1.0 : -1.0;
+                edge_scatter_range(m, edge, &nu12, &nl12);
+                for (int nz = nu12; nz <= nl12; ++nz) {
+                    L1(plus, nz, n) = L1(plus, nz, n) + dmax(0.0, sg * L1(adf_h, nz, edge));
+                    L1(minus, nz, n) = L1(minus, nz, n) + dmin(0.0, sg * L1(adf_h, nz, edge));
+                }
+            }
+    }
+#endif
     /* b2 (:361-374) */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
         for (int nz = nu1; nz <= nl1 - 1; ++nz) {
@@ -461,6 +569,7 @@ void oracle_tra_adv_fct(const oracle_mesh *m, double dt, const double *ttf, cons
         }
     }
     /* b3 vertical (:394-427) */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
         int nz = nu1;
@@ -476,6 +585,7 @@ void oracle_tra_adv_fct(const oracle_mesh *m, double dt, const double *ttf, cons
         }
     }
     /* b3 horizontal (:437-471) */
+    OMP_FOR
     for (int edge = 0; edge < m->edge2D; ++edge) {
         int n1 = m->edges[2 * edge], n2 = m->edges[2 * edge + 1];
         int el1 = m->edge_tri[2 * edge], el2 = m->edge_tri[2 * edge + 1];
@@ -499,6 +609,7 @@ void oracle_tra_adv_flux2dtracer(const oracle_mesh *m, double dt, double *dttf_h
                                  const double *ttf, const double *lo)
 {
     if (use_lo) {
+        OMP_FOR
         for (int n = 0; n < m->nod2D; ++n) {
             int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
             for (int nz = nu1; nz <= nl1 - 1; ++nz)
@@ -506,11 +617,13 @@ void oracle_tra_adv_flux2dtracer(const oracle_mesh *m, double dt, double *dttf_h
                                     L1(lo, nz, n) * L1(m->hnode_new, nz, n);
         }
     }
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
         for (int nz = nu1; nz <= nl1 - 1; ++nz)
             L1(dttf_v, nz, n) = L1(dttf_v, nz, n) + (LI(flux_v, nz, n) - LI(flux_v, nz + 1, n)) * dt / LI(m->areasvol, nz, n);
     }
+#ifndef _OPENMP
     for (int edge = 0; edge < m->edge2D; ++edge) {
         int n1 = m->edges[2 * edge], n2 = m->edges[2 * edge + 1];
         int el1 = m->edge_tri[2 * edge], el2 = m->edge_tri[2 * edge + 1];
@@ -523,6 +636,24 @@ void oracle_tra_adv_flux2dtracer(const oracle_mesh *m, double dt, double *dttf_h
         for (int nz = nu12; nz <= nl12; ++nz)
             L1(dttf_h, nz, n2) = L1(dttf_h, nz, n2) - L1(flux_h, nz, edge) * dt / LI(m->areasvol, nz, n2);
     }
+#else
+    {
+        const int *ep, *ei;
+        n2e_lists(m, &ep, &ei);
+        OMP_FOR
+        for (int n = 0; n < m->nod2D; ++n)
+            for (int j = ep[n]; j < ep[n + 1]; ++j) {
+                int edge = (ei[j] > 0 ? ei[j] : -ei[j]) - 1, nu12, nl12;
+                edge_scatter_range(m, edge, &nu12, &nl12);
+                if (ei[j] > 0)
+                    for (int nz = nu12; nz <= nl12; ++nz)
+                        L1(dttf_h, nz, n) = L1(dttf_h, nz, n) + L1(flux_h, nz, edge) * dt / LI(m->areasvol, nz, n);
+                else
+                    for (int nz = nu12; nz <= nl12; ++nz)
+                        L1(dttf_h, nz, n) = L1(dttf_h, nz, n) - L1(flux_h, nz, edge) * dt / LI(m->areasvol, nz, n);
+            }
+    }
+#endif
 }
 
 /* do_oce_adv_tra: lib/oce_adv_tra_driver.F90:23-294 (use_wsplit = .false.,
@@ -539,6 +670,7 @@ int oracle_do_oce_adv_tra(const oracle_mesh *m, double dt, const double *vel, co
     if (lim == OR_LIM_FCT) {
         oracle_adv_tra_hor_upw1(m, vel, ttf, fh, 1);                     /* :95 */
         memset(fct_LO, 0, sizeof(double) * (size_t)(m->nl - 1) * m->nod2D); /* :102-106 */
+#ifndef _OPENMP
         for (int e = 0; e < m->edge2D; ++e) {                            /* :122-181 */
             int n1 = m->edges[2 * e], n2 = m->edges[2 * e + 1];
             int el1 = m->edge_tri[2 * e], el2 = m->edge_tri[2 * e + 1];
@@ -551,7 +683,24 @@ int oracle_do_oce_adv_tra(const oracle_mesh *m, double dt, const double *vel, co
                 L1(fct_LO, nz, n2) = L1(fct_LO, nz, n2) - L1(fh, nz, e);
             }
         }
+#else
+        {
+            const int *ep, *ei;
+            n2e_lists(m, &ep, &ei);
+            OMP_FOR
+            for (int n = 0; n < m->nod2D; ++n)
+                for (int j = ep[n]; j < ep[n + 1]; ++j) {
+                    int e = (ei[j] > 0 ? ei[j] : -ei[j]) - 1, nu12, nl12;
+                    edge_scatter_range(m, e, &nu12, &nl12);
+                    if (ei[j] > 0)
+                        for (int nz = nu12; nz <= nl12; ++nz) L1(fct_LO, nz, n) = L1(fct_LO, nz, n) + L1(fh, nz, e);
+                    else
+                        for (int nz = nu12; nz <= nl12; ++nz) L1(fct_LO, nz, n) = L1(fct_LO, nz, n) - L1(fh, nz, e);
+                }
+        }
+#endif
         oracle_adv_tra_ver_upw1(m, we, ttf, fv, 1);                      /* :194 */
+        OMP_FOR
         for (int n = 0; n < m->nod2D; ++n) {                             /* :202-211 */
             int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
             for (int nz = nu1; nz <= nl1 - 1; ++nz)
@@ -600,11 +749,13 @@ int oracle_step_tracer(const oracle_mesh *m, double dt, const double *vel, const
     }
     int st = oracle_do_oce_adv_tra(m, dt, vel, w, we, values, valuesAB, hor, ver, lim, opth, optv, wk);
     if (st) return st;
+    OMP_FOR
     for (int k = 0; k < m->nod2D; ++k) {
         int nzmax = m->nlevels_nod2D[k] - 1, nzmin = m->ulevels_nod2D[k];
         for (int kz = nzmin; kz <= nzmax; ++kz)
             L1(wk->del_ttf, kz, k) = L1(wk->del_ttf, kz, k) + L1(wk->dttf_h, kz, k) + L1(wk->dttf_v, kz, k);
     }
+    OMP_FOR
     for (int k = 0; k < m->nod2D; ++k) {
         int nzmax = m->nlevels_nod2D[k] - 1, nzmin = m->ulevels_nod2D[k];
         for (int kz = nzmin; kz <= nzmax; ++kz)

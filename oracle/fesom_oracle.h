@@ -104,6 +104,10 @@ int oracle_step_tracer(const oracle_mesh *m, double dt, const double *vel, const
                        int hor, int ver, int lim, double opth, double optv,
                        oracle_work *wk);
 
+/* threads of the OpenMP build (n > 0 sets them); returns the count in use
+ * (1 in the serial build) */
+int oracle_set_threads(int n);
+
 /* setup-time restatements */
 void oracle_find_up_downwind_triangles(const oracle_mesh *m, double cyclic_x, double cyclic_y,
                                        int *edge_up_dn_tri);
