@@ -93,6 +93,7 @@ SIGNATURES = {
     "tracer_batch_mesh_info": (c_int, [c_int, P_int, P_int, P_int, P_ll, P_double]),
     "tracer_batch_get_values": (c_int, [c_int, c_int, F64F]),
     "tracer_batch_diagnostics": (c_int, [c_int, c_int, F64]),
+    "tracer_batch_get_mesh_array": (c_int, [c_int, c_char_p, c_void_p, c_ll, P_ll, P_int]),
     "tracer_last_error": (c_char_p, []),
 }
 REFERENCE_SYMBOLS = list(SIGNATURES)[:23]

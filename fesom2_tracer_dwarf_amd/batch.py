@@ -52,5 +52,22 @@ class TracerBatch:
         _lib.check(self.lib.tracer_batch_diagnostics(mesh, tracer_id, out), "tracer_batch_diagnostics", self.lib)
         return dict(zip(("min", "max", "sum", "active_min", "active_max", "mass", "nan_count"), out.tolist()))
 
+    def mesh_array(self, mesh, name):
+        """A named array of a batch mesh (names as FESOMTracerAdvection.mesh_array, plus uv, w, w_e)."""
+        n, kind = ctypes.c_longlong(), ctypes.c_int()
+        _lib.check(self.lib.tracer_batch_get_mesh_array(mesh, name.encode(), None, 0, ctypes.byref(n),
+                                                        ctypes.byref(kind)), "tracer_batch_get_mesh_array", self.lib)
+        buf = np.zeros(n.value, dtype=np.float64 if kind.value else np.int32)
+        _lib.check(self.lib.tracer_batch_get_mesh_array(mesh, name.encode(), buf.ctypes.data, n.value, None, None),
+                   "tracer_batch_get_mesh_array", self.lib)
+        return buf
+
+    def dims(self, mesh):
+        """Dimensions of a batch mesh, as FESOMTracerAdvection.dims."""
+        d = self.info(mesh)
+        return dict(nl=d["nl"], nod2D=d["nod2D"], elem2D=int(self.mesh_array(mesh, "nlevels").size),
+                    edge2D=int(self.mesh_array(mesh, "edges").size // 2),
+                    maxAdj=int(self.mesh_array(mesh, "nod_in_elem2D").size // d["nod2D"]))
+
     def close(self):
         self.lib.tracer_batch_destroy()

@@ -13,6 +13,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -32,6 +33,7 @@ struct BatchMesh {
     double dt = 0.0;
     int ntr = 0;
     double Lx = 0.0, Ly = 0.0;
+    std::vector<double> uv, w;  // the velocity the engine holds (w_e = w)
 };
 
 struct Batch {
@@ -168,7 +170,7 @@ int tracer_batch_add_periodic_mesh(int nx, int ny, int nl, double Lx, double Ly,
         derive_all_device(b->mesh);  // the batch runs on the GPU anyway
         b->Lx = Lx;
         b->Ly = Ly;
-        std::vector<double> uv, w;
+        std::vector<double> &uv = b->uv, &w = b->w;
         b->dt = make_velocity(b->mesh, 0.1, 0.05, 0.2, 0.5, cfl, uv, w);
         const int prec = precision == 32 ? PREC_FP32 : (precision == 16 ? PREC_FP32_H16 : PREC_FP64);
         b->eng.reset(new Engine(b->mesh, ntracers, prec));
@@ -184,7 +186,6 @@ int tracer_batch_add_periodic_mesh(int nx, int ny, int nl, double Lx, double Ly,
             b->eng->set_spec(t, sp);
             fill_tracer(*b, t, seed, vals);
             b->eng->set_values(t, vals.data());
-            b->eng->set_valuesAB(t, vals.data());
         }
         if (dt_out) *dt_out = b->dt;
         g_batch->meshes.push_back(std::move(b));
@@ -276,6 +277,25 @@ int tracer_batch_get_values(int mesh, int tracer_id, double *values)
     if (!b) return bfail("invalid batch mesh index");
     if (tracer_id < 1 || tracer_id > b->ntr) return bfail("invalid tracer_id");
     try { b->eng->get_values(tracer_id - 1, values); } catch (const std::exception &e) { return bfail(e.what()); }
+    return 0;
+}
+
+int tracer_batch_get_mesh_array(int mesh, const char *name, void *out, long long count, long long *size, int *kind_out)
+{
+    BatchMesh *b = get_mesh(mesh);
+    if (!b) return bfail("invalid batch mesh index");
+    const std::string n = name ? name : "";
+    const void *p = nullptr;
+    long long cnt = 0;
+    int kind = 1;
+    if (n == "uv") { p = b->uv.data(); cnt = (long long)b->uv.size(); }
+    else if (n == "w" || n == "w_e") { p = b->w.data(); cnt = (long long)b->w.size(); }
+    else if (!named_mesh_array(b->mesh, n, p, cnt, kind)) return bfail("unknown mesh array " + n);
+    if (size) *size = cnt;
+    if (kind_out) *kind_out = kind;
+    if (!out) return 0;
+    if (count < cnt) return bfail("tracer_batch_get_mesh_array: buffer too small");
+    std::memcpy(out, p, (size_t)cnt * (kind ? sizeof(double) : sizeof(int)));
     return 0;
 }
 

@@ -1128,4 +1128,24 @@ int64_t active_elem_cells(const HostMesh &m)
     return c;
 }
 
+bool named_mesh_array(const HostMesh &m, const std::string &name, const void *&data, long long &count, int &kind)
+{
+#define IA(NAME, VEC) if (name == NAME) { data = VEC.data(); count = (long long)VEC.size(); kind = 0; return true; }
+#define DA(NAME, VEC) if (name == NAME) { data = VEC.data(); count = (long long)VEC.size(); kind = 1; return true; }
+    IA("edges", m.edges) IA("edge_tri", m.edge_tri) IA("elem2D_nodes", m.elem2D_nodes)
+    IA("nlevels", m.nlevels) IA("ulevels", m.ulevels) IA("nlevels_nod2D", m.nlevels_nod2D)
+    IA("ulevels_nod2D", m.ulevels_nod2D) IA("nlevels_nod2D_min", m.nlevels_nod2D_min)
+    IA("ulevels_nod2D_max", m.ulevels_nod2D_max) IA("nod_in_elem2D_num", m.nod_in_elem2D_num)
+    IA("nod_in_elem2D", m.nod_in_elem2D) IA("edge_up_dn_tri", m.edge_up_dn_tri)
+    IA("nboundary_lay", m.nboundary_lay)
+    DA("coord", m.coord) DA("coord_nod2D", m.coord_nod2D) DA("zbar", m.zbar) DA("Z", m.Z) DA("depth", m.depth)
+    DA("elem_cos", m.elem_cos) DA("elem_area", m.elem_area) DA("gradient_sca", m.gradient_sca)
+    DA("edge_dxdy", m.edge_dxdy) DA("edge_cross_dxdy", m.edge_cross_dxdy) DA("area", m.area)
+    DA("areasvol", m.areasvol) DA("hnode", m.hnode) DA("hnode_new", m.hnode_new) DA("helem", m.helem)
+    DA("Z_3d_n", m.Z_3d_n) DA("zbar_3d_n", m.zbar_3d_n)
+#undef IA
+#undef DA
+    return false;
+}
+
 }  // namespace fta

@@ -146,6 +146,11 @@ double make_velocity(const HostMesh &m, double u_bg, double v_bg, double cell_am
                      double shear, double cfl, std::vector<double> &uv, std::vector<double> &w);
 
 // counts of active node-cells and element-cells
+// a named array of m, by the names and layouts of tracer_get_mesh_array
+// (coord, edges, nlevels_nod2D, area, ...): its data, entry count and kind
+// (0 int32, 1 float64); false for a name m does not hold
+bool named_mesh_array(const HostMesh &m, const std::string &name, const void *&data, long long &count, int &kind);
+
 int64_t active_node_cells(const HostMesh &m);
 int64_t active_elem_cells(const HostMesh &m);
 

@@ -941,30 +941,15 @@ bool mesh_array(const std::string &name, ArrRef &r)
     State &s = S();
     HostMesh &m = s.mesh;
     static std::vector<int> dims;
-    static std::vector<double> zb;
-#define IA(NAME, VEC) if (name == NAME) { r = {VEC.data(), (long long)VEC.size(), 0}; return true; }
-#define DA(NAME, VEC) if (name == NAME) { r = {VEC.data(), (long long)VEC.size(), 1}; return true; }
     if (name == "dims") {
         dims = {m.nl, m.nod2D, m.elem2D, m.edge2D, m.maxAdj};
         r = {dims.data(), (long long)dims.size(), 0};
         return true;
     }
-    IA("edges", m.edges) IA("edge_tri", m.edge_tri) IA("elem2D_nodes", m.elem2D_nodes)
-    IA("nlevels", m.nlevels) IA("ulevels", m.ulevels) IA("nlevels_nod2D", m.nlevels_nod2D)
-    IA("ulevels_nod2D", m.ulevels_nod2D) IA("nlevels_nod2D_min", m.nlevels_nod2D_min)
-    IA("ulevels_nod2D_max", m.ulevels_nod2D_max) IA("nod_in_elem2D_num", m.nod_in_elem2D_num)
-    IA("nod_in_elem2D", m.nod_in_elem2D) IA("edge_up_dn_tri", m.edge_up_dn_tri)
-    IA("nboundary_lay", m.nboundary_lay)
-    DA("coord", m.coord) DA("coord_nod2D", m.coord_nod2D) DA("zbar", m.zbar) DA("Z", m.Z) DA("depth", m.depth)
-    DA("elem_cos", m.elem_cos) DA("elem_area", m.elem_area) DA("gradient_sca", m.gradient_sca)
-    DA("edge_dxdy", m.edge_dxdy) DA("edge_cross_dxdy", m.edge_cross_dxdy) DA("area", m.area)
-    DA("areasvol", m.areasvol) DA("hnode", m.hnode) DA("hnode_new", m.hnode_new) DA("helem", m.helem)
-    DA("Z_3d_n", m.Z_3d_n) DA("zbar_3d_n", m.zbar_3d_n)
-    DA("uv", s.uv) DA("w", s.w) DA("w_e", s.we)
-#undef IA
-#undef DA
-    (void)zb;
-    return false;
+    if (name == "uv") { r = {s.uv.data(), (long long)s.uv.size(), 1}; return true; }
+    if (name == "w") { r = {s.w.data(), (long long)s.w.size(), 1}; return true; }
+    if (name == "w_e") { r = {s.we.data(), (long long)s.we.size(), 1}; return true; }
+    return named_mesh_array(m, name, r.p, r.n, r.kind);
 }
 }  // namespace
 

@@ -296,6 +296,11 @@ int tracer_batch_time(int nsteps, double *ms_out);
 int tracer_batch_mesh_info(int mesh, int *nl, int *nod2D, int *ntracers, long long *active_cells, double *dt);
 int tracer_batch_get_values(int mesh, int tracer_id, double *values);
 int tracer_batch_diagnostics(int mesh, int tracer_id, double *out);
+/* a named array of batch mesh `mesh`, by the names, layouts and kinds of
+ * tracer_get_mesh_array (mesh arrays, "uv", "w", "w_e"): *size (entries)
+ * and *kind (0 int32, 1 float64) when not NULL, then, when out is not NULL,
+ * the data (count >= *size) */
+int tracer_batch_get_mesh_array(int mesh, const char *name, void *out, long long count, long long *size, int *kind);
 
 /* No reference counterpart: the autotuner. Times every launch schedule of
  * the chunk kernels (waves per workgroup 2, 4, 8 x tracers per launch 4, 2,
