@@ -17,6 +17,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG.parent / "_build"
 LIBDIR = PKG / "lib"
 LIBNAME = "libfesom_tracer_amd.so"
+ALIAS = "libfesom_tracer_Fortran.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = os.environ.get("FTA_OFFLOAD_ARCH", "gfx950")
 
@@ -64,17 +65,28 @@ def build(verbose: bool = False) -> Path:
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs,
           f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, out)
+    # the file name the reference's Python wrapper loads (python/demo_tracer.py:27-37:
+    # <its directory>/lib/libfesom_tracer_Fortran.so): the same library under that name
+    alias = LIBDIR / ALIAS
+    if alias.is_symlink() or alias.exists():
+        alias.unlink()
+    alias.symlink_to(LIBNAME)
     build_fortran(verbose)
     return out
 
 
 FORTRAN_DIR = PKG / "fortran"
 DRIVER = LIBDIR / "fesom_tracer_amd_driver"
+# the FESOM-style host that tests the drop-in do_oce_adv_tra module, built
+# against the minimal FESOM type modules of tests/fortran
+TEST_FORTRAN = PKG.parent / "tests" / "fortran"
+CHECKER = TEST_FORTRAN / "bin" / "check_do_oce_adv_tra"
 
 
 def build_fortran(verbose: bool = False) -> Path | None:
-    """The ISO_C_BINDING module and the Fortran analytic driver (amdflang), if
-    the compiler is present; the driver links the library with an $ORIGIN rpath."""
+    """The ISO_C_BINDING module, the Fortran analytic driver and (when the
+    tests are present) the do_oce_adv_tra checker, with amdflang if it is
+    installed; the programs link the library with $ORIGIN rpaths."""
     fc = shutil.which("amdflang") or (str(ROCM / "bin" / "amdflang") if (ROCM / "bin" / "amdflang").exists() else None)
     if not fc:
         return None
@@ -82,6 +94,18 @@ def build_fortran(verbose: bool = False) -> Path | None:
     cmds = [[fc, "-O2", "-c", str(FORTRAN_DIR / "fesom_tracer_amd.F90"), "-J", str(BUILD), "-o", str(mod)],
             [fc, "-O2", "-I", str(BUILD), str(FORTRAN_DIR / "analytic_driver.F90"), str(mod), f"-L{LIBDIR}",
              "-lfesom_tracer_amd", "-Wl,-rpath,$ORIGIN", "-o", str(DRIVER)]]
+    if (TEST_FORTRAN / "fesom_types_min.F90").exists():
+        tb = BUILD / "fortran_check"
+        tb.mkdir(exist_ok=True)
+        CHECKER.parent.mkdir(exist_ok=True)
+        types_o, drv_o = tb / "fesom_types_min.o", tb / "oce_adv_tra_driver.o"
+        cmds += [[fc, "-O2", "-c", str(TEST_FORTRAN / "fesom_types_min.F90"), "-J", str(tb), "-o", str(types_o)],
+                 [fc, "-O2", "-c", str(FORTRAN_DIR / "oce_adv_tra_driver.F90"), "-I", str(tb), "-I", str(BUILD),
+                  "-J", str(tb), "-o", str(drv_o)],
+                 [fc, "-O2", "-I", str(tb), "-I", str(BUILD), str(TEST_FORTRAN / "check_do_oce_adv_tra.F90"),
+                  str(types_o), str(drv_o),
+                  str(mod), f"-L{LIBDIR}", "-lfesom_tracer_amd",
+                  "-Wl,-rpath,$ORIGIN/../../../fesom2_tracer_dwarf_amd/lib", "-o", str(CHECKER)]]
     for c in cmds:
         if verbose:
             print(" ".join(c))

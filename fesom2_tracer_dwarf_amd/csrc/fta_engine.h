@@ -31,6 +31,12 @@ enum Lim { LIM_NONE = 0, LIM_FCT = 1, LIM_LEARNED = 2 };
 // solution and divergence accumulation kept in double. FP32_H16: FP32 with
 // the elemental/nodal gradients stored as half.
 enum Prec { PREC_FP64 = 0, PREC_FP32 = 1, PREC_FP32_H16 = 2 };
+// how a step ends at each node layer: the tracer update of the analytic
+// driver (del_ttf zeroed every step), the same with del_ttf carried over
+// between steps as the reference's C interface does, or the tendencies
+// dttf_h / dttf_v returned with the tracer unchanged (do_oce_adv_tra's own
+// contract)
+enum Out { OUT_UPDATE = 0, OUT_ACCUM = 1, OUT_TEND = 2 };
 
 struct TracerSpec {
     int hor = HOR_UPW1, ver = VER_UPW1, lim = LIM_NONE;
@@ -60,6 +66,15 @@ public:
     void get_values(int tr, double *v);
     void set_spec(int tr, const TracerSpec &s);
     const TracerSpec &spec(int tr) const { return spec_.at(tr); }
+
+    // del_ttf carried over between steps (one array per tracer, zeroed when
+    // switched on), as the reference's tracer_advect_step never zeroes it
+    void set_accumulate(bool on);
+    bool accumulate() const;
+    // one do_oce_adv_tra of tracer tr: its horizontal and vertical tendencies
+    // are ADDED into the host arrays dttf_h, dttf_v ((nl-1, nod2D), wet layers)
+    // and the tracer is left unchanged
+    void tendencies(int tr, double dt, double *dttf_h, double *dttf_v);
 
     // refresh_ab: valuesAB := values at the start of each step (AB1)
     void set_refresh_ab(bool r) { refresh_ab_ = r; graph_valid_ = false; }

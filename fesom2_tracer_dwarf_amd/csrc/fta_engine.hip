@@ -637,10 +637,32 @@ __device__ __forceinline__ R edge_alpha(bool pos, R f, R rp, R rm, R rpo, R rmo)
     return ae;
 }
 
+// the end of a step at one node layer (OUT): del_ttf = (del + dttf_h) + dttf_v
+// with del = 0 and values += del_ttf / hnode_new (OUT_UPDATE, the analytic
+// driver, src/fesom_analytic.F90:405-437), the same with del carried over from
+// the last step in acc_h (OUT_ACCUM, the reference C interface's del_ttf,
+// lib/tracer_c_interface.F90:194-212), or dttf_h -> acc_h and dttf_v -> acc_v
+// with the tracer left alone (OUT_TEND, what do_oce_adv_tra itself returns)
+template <int OUT>
+__device__ __forceinline__ void node_out(double *__restrict__ ttf, double *__restrict__ acc_h,
+                                         double *__restrict__ acc_v, size_t i, bool lay1, V2<double> T,
+                                         double sh0, double dv0, double sh1, double dv1, V2<double> hn)
+{
+    if constexpr (OUT == OUT_TEND) {
+        st2(acc_h + i, 0.0 + sh0, lay1 ? 0.0 + sh1 : 0.0);
+        st2(acc_v + i, dv0, lay1 ? dv1 : 0.0);
+    } else {
+        V2<double> d{0.0, 0.0};
+        if constexpr (OUT == OUT_ACCUM) d = ld2(acc_h + i);
+        const double del0 = (d.x + sh0) + dv0, del1 = (d.y + sh1) + dv1;
+        if constexpr (OUT == OUT_ACCUM) st2(acc_h + i, del0, lay1 ? del1 : 0.0);
+        st2(ttf + i, T.x + del0 / hn.x, lay1 ? T.y + del1 / hn.y : T.y);
+    }
+}
+
 // limiting b3 (lib/oce_adv_tra_fct.F90:394-471), flux divergence
-// (lib/oce_adv_tra_driver.F90:298-450) and the update
-// values += (dttf_h + dttf_v)/hnode_new (src/fesom_analytic.F90:419-437)
-template <class R, int NT, bool LEARN, int D>
+// (lib/oce_adv_tra_driver.F90:298-450) and the end of the step (node_out)
+template <class R, int NT, bool LEARN, int D, int OUT>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && NT <= 2 && !LEARN ? 5 : 1))) void k_node_update(DMesh<R> m, double *__restrict__ ttf,
                                                         const double *__restrict__ LO, size_t tsn,
                                                         const R *__restrict__ adf_h, size_t tse,
@@ -648,7 +670,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
                                                         const R *__restrict__ Rp,
                                                         const R *__restrict__ Rm, double dt,
                                                         const R *__restrict__ alpha,
-                                                        const R *__restrict__ unsafe)
+                                                        const R *__restrict__ unsafe,
+                                                        double *__restrict__ acc_h, double *__restrict__ acc_v)
 {
     const int ch = wave_chunk();
     if (ch >= m.nch_n) return;
@@ -722,30 +745,22 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
         sh0 = sh0 * dt / (double)av.x;
         sh1 = sh1 * dt / (double)av.y;
         const V2<double> T = ld2(ttf + tn + c), lo = ld2(LO + tn + c);
-        double n0, n1;
-        {
-            double dv = 0.0 - T.x * (double)h.x + lo.x * (double)hn.x;
-            dv = dv + ((double)fvl0 - (double)fvl1) * dt / (double)av.x;
-            const double del = (0.0 + sh0) + dv;
-            n0 = T.x + del / (double)hn.x;
-        }
-        {
-            double dv = 0.0 - T.y * (double)h.y + lo.y * (double)hn.y;
-            dv = dv + ((double)fvl1 - (double)fvl2) * dt / (double)av.y;
-            const double del = (0.0 + sh1) + dv;
-            n1 = T.y + del / (double)hn.y;
-        }
-        st2(ttf + tn + c, n0, lay1 ? n1 : T.y);
+        double dv0 = 0.0 - T.x * (double)h.x + lo.x * (double)hn.x;
+        dv0 = dv0 + ((double)fvl0 - (double)fvl1) * dt / (double)av.x;
+        double dv1 = 0.0 - T.y * (double)h.y + lo.y * (double)hn.y;
+        dv1 = dv1 + ((double)fvl1 - (double)fvl2) * dt / (double)av.y;
+        node_out<OUT>(ttf, acc_h, acc_v, tn + c, lay1, T, sh0, dv0, sh1, dv1, hn);
     }
 }
 
 // lim = 'NONE': HO fluxes straight into the divergence (driver :291)
-template <class R, int VER, int NT, int D>
+template <class R, int VER, int NT, int D, int OUT>
 __global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *__restrict__ WEv,
                                                        double *__restrict__ ttf,
                                                        const double *__restrict__ ttfAB, size_t tsn,
                                                        const R *__restrict__ flux_h, size_t tse,
-                                                       double dt, R num_ord)
+                                                       double dt, R num_ord,
+                                                       double *__restrict__ acc_h, double *__restrict__ acc_v)
 {
     const int ch = wave_chunk();
     if (ch >= m.nch_n) return;
@@ -789,18 +804,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *_
         sh0 = sh0 * dt / (double)av.x;
         sh1 = sh1 * dt / (double)av.y;
         const V2<double> T = ld2(ttf + tn + c);
-        double n0, n1;
-        {
-            const double dv = 0.0 + ((double)fv0 - (double)fv1) * dt / (double)av.x;
-            const double del = (0.0 + sh0) + dv;
-            n0 = T.x + del / (double)hn.x;
-        }
-        {
-            const double dv = 0.0 + ((double)fv1 - (double)fv2) * dt / (double)av.y;
-            const double del = (0.0 + sh1) + dv;
-            n1 = T.y + del / (double)hn.y;
-        }
-        st2(ttf + tn + c, n0, lay1 ? n1 : T.y);
+        const double dv0 = 0.0 + ((double)fv0 - (double)fv1) * dt / (double)av.x;
+        const double dv1 = 0.0 + ((double)fv1 - (double)fv2) * dt / (double)av.y;
+        node_out<OUT>(ttf, acc_h, acc_v, tn + c, lay1, T, sh0, dv0, sh1, dv1, hn);
     }
 }
 
@@ -1155,6 +1161,23 @@ __global__ __launch_bounds__(256) void k_unpack_layers(const int *__restrict__ s
     if (k0 + 1 < Lc) dst[k0 + 1] = in[2 * s + 1];
 }
 
+// dense += the wet layers of a chunk-packed node array (the caller's tendency
+// arrays accumulate what a step returns, as do_oce_adv_tra adds into them)
+__global__ __launch_bounds__(256) void k_unpack_add_layers(const int *__restrict__ slot, const int *__restrict__ nlev,
+                                                           long long npair, const double *__restrict__ in, int L,
+                                                           double *__restrict__ dense)
+{
+    const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (s >= npair) return;
+    const int w = slot[s];
+    if (w < 0) return;
+    const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
+    const int Lc = nlev[n] - 1;
+    double *dst = dense + (size_t)n * L;
+    if (k0 < Lc) dst[k0] = dst[k0] + in[2 * s];
+    if (k0 + 1 < Lc) dst[k0 + 1] = dst[k0 + 1] + in[2 * s + 1];
+}
+
 // reduction over the wet layers (node layout): min, max, sum, mass, NaN count
 // plus, for min / max / sum / NaN count over all values, the levels below each
 // column's bottom, read from the dense (nl-1, nod2D) copy (never stepped)
@@ -1375,6 +1398,11 @@ struct Engine::Impl {
     int wslots = 1;  // tracers the work arrays hold
     int fan_fault = -1;  // first node whose elements do not form one fan (-1: none)
     bool we_is_w = false;  // w_e equals w (no implicit vertical split): one device copy serves both
+    // how a step ends (OUT_UPDATE, OUT_ACCUM with del_ttf per tracer [ntr][CN],
+    // OUT_TEND into tend_h / tend_v [CN]); dense_t stages a caller's tendency
+    // array in its (nl-1, nod2D) layout
+    int out_mode = OUT_UPDATE;
+    double *del_ttf = nullptr, *tend_h = nullptr, *tend_v = nullptr, *dense_t = nullptr;
     int tune_waves = kWavesPerBlock;   // waves per block of the chunk kernels
     int tune_nt = 4;                   // most tracers per launch
     // each tracer in the caller's dense (nl-1, nod2D) layout, as last set and
@@ -1780,6 +1808,53 @@ void Engine::get_values(int tr, double *v)
     d2h_staged(I, stream_, v, dense, (size_t)L * N_ * sizeof(double));
 }
 
+void Engine::set_accumulate(bool on)
+{
+    Impl &I = *p_;
+    HIP_OK(hipStreamSynchronize(stream_));
+    if (on) {
+        if (!I.del_ttf) I.del_ttf = I.track(dalloc<double>(I.tsn * ntr_));
+        HIP_OK(hipMemset(I.del_ttf, 0, I.tsn * ntr_ * sizeof(double)));
+    }
+    I.out_mode = on ? OUT_ACCUM : OUT_UPDATE;
+    graph_valid_ = false;
+}
+
+bool Engine::accumulate() const { return p_->out_mode == OUT_ACCUM; }
+
+void Engine::tendencies(int tr, double dt, double *dttf_h, double *dttf_v)
+{
+    if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
+    Impl &I = *p_;
+    if (spec_[tr].lim == LIM_LEARNED) throw std::runtime_error("tendencies are not returned for the LEARNED limiter");
+    const size_t L = nl_ - 1;
+    if (!I.tend_h) {
+        I.tend_h = I.track(dalloc<double>(I.tsn));
+        I.tend_v = I.track(dalloc<double>(I.tsn));
+    }
+    if (!I.dense_t) I.dense_t = I.track(dalloc<double>(L * N_));
+    const int saved = I.out_mode;
+    I.out_mode = OUT_TEND;
+    try {
+        enqueue_step(dt, tr, 1);
+    } catch (...) {
+        I.out_mode = saved;
+        throw;
+    }
+    I.out_mode = saved;
+    const int *slot, *nlev;
+    long long npair;
+    node_slots(I, prec_, slot, nlev, npair);
+    for (int q = 0; q < 2; ++q) {
+        double *host = q == 0 ? dttf_h : dttf_v;
+        h2d_staged(I, stream_, I.dense_t, host, L * N_ * sizeof(double));
+        hipLaunchKernelGGL(k_unpack_add_layers, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, stream_, slot,
+                           nlev, npair, q == 0 ? I.tend_h : I.tend_v, (int)L, I.dense_t);
+        HIP_OK(hipGetLastError());
+        d2h_staged(I, stream_, host, I.dense_t, L * N_ * sizeof(double));
+    }
+}
+
 struct LaunchHook {
     std::vector<std::pair<std::string, hipEvent_t>> *ev = nullptr;
     void mark(const char *name, hipStream_t st)
@@ -1819,6 +1894,14 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
     const bool fct = sp.lim == LIM_FCT || learned;
     Engine::Impl::Dump *dump = I.dump;
     const R ph = (R)sp.ph, pv = (R)sp.pv;
+    // how the step ends: update the tracer, update it carrying del_ttf over
+    // from the last step (one array per tracer), or only return the tendencies
+    const int out = I.out_mode;
+    if (out != OUT_UPDATE && learned) throw std::runtime_error("the LEARNED limiter updates the tracer only");
+    if (out == OUT_TEND && NT != 1) throw std::runtime_error("tendencies are returned one tracer at a time");
+    double *acc_h = out == OUT_ACCUM ? I.del_ttf + t0 * tsn : (out == OUT_TEND ? I.tend_h : nullptr);
+    double *acc_v = out == OUT_TEND ? I.tend_v : nullptr;
+    if (out != OUT_UPDATE && !acc_h) throw std::runtime_error("accumulation arrays not allocated");
     if (sp.hor == HOR_MUSCL || sp.hor == HOR_MFCT) {
         hipLaunchKernelGGL((k_trxy<R, G, NT>), gt, blk, 0, st, m, ab, tsn, trxy, tst, I.gscale);
         hk.mark("k_trxy", st);
@@ -1881,8 +1964,8 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
                 hipLaunchKernelGGL((k_learn_dump<R, NT>), ge, blk, 0, st, m, la, dt, dump->X, dump->Y, dump->row);
                 dump->row += (long long)NT * m.CE;
             }
-            hipLaunchKernelGGL((k_node_update<R, NT, false, D>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
-                               Pp, Pm, dt, alpha, unsafe);
+            hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_UPDATE>), gn, blk, 0, st, m, vals, LO, tsn, adf_h,
+                               tse, adf_v, Pp, Pm, dt, alpha, unsafe, nullptr, nullptr);
         } else if (learned) {
             const MlpDev w = mlp_dev(I);
             const int gb = std::min((m.nch_e + kWavesPerBlock - 1) / kWavesPerBlock, 2048);
@@ -1892,15 +1975,33 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
             hipLaunchKernelGGL((k_learn_check<R, NT, D>), gn, blk, 0, st, m, adf_h, alpha, tse, adf_v, Pp, Pm, Qp, Qm,
                                tsn, dt, unsafe);
             hk.mark("k_learn_check", st);
-            hipLaunchKernelGGL((k_node_update<R, NT, true, D>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
-                               Pp, Pm, dt, alpha, unsafe);
+            hipLaunchKernelGGL((k_node_update<R, NT, true, D, OUT_UPDATE>), gn, blk, 0, st, m, vals, LO, tsn, adf_h,
+                               tse, adf_v, Pp, Pm, dt, alpha, unsafe, nullptr, nullptr);
+        } else if (out == OUT_ACCUM) {
+            hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_ACCUM>), gn, blk, 0, st, m, vals, LO, tsn, adf_h,
+                               tse, adf_v, Pp, Pm, dt, alpha, unsafe, acc_h, nullptr);
+        } else if (out == OUT_TEND) {
+            if constexpr (NT == 1)
+                hipLaunchKernelGGL((k_node_update<R, 1, false, D, OUT_TEND>), gn, blk, 0, st, m, vals, LO, tsn,
+                                   adf_h, tse, adf_v, Pp, Pm, dt, alpha, unsafe, acc_h, acc_v);
         } else {
-            hipLaunchKernelGGL((k_node_update<R, NT, false, D>), gn, blk, 0, st, m, vals, LO, tsn, adf_h, tse, adf_v,
-                               Pp, Pm, dt, alpha, unsafe);
+            hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_UPDATE>), gn, blk, 0, st, m, vals, LO, tsn, adf_h,
+                               tse, adf_v, Pp, Pm, dt, alpha, unsafe, nullptr, nullptr);
         }
         hk.mark("k_node_update", st);
     } else {
-#define NNL(V) hipLaunchKernelGGL((k_node_nolim<R, V, NT, D>), gn, blk, 0, st, m, WE, vals, ab, tsn, adf_h, tse, dt, pv)
+#define NNL(V)                                                                                                      \
+    do {                                                                                                            \
+        if (out == OUT_ACCUM)                                                                                       \
+            hipLaunchKernelGGL((k_node_nolim<R, V, NT, D, OUT_ACCUM>), gn, blk, 0, st, m, WE, vals, ab, tsn, adf_h, \
+                               tse, dt, pv, acc_h, nullptr);                                                        \
+        else if (out != OUT_TEND)                                                                                   \
+            hipLaunchKernelGGL((k_node_nolim<R, V, NT, D, OUT_UPDATE>), gn, blk, 0, st, m, WE, vals, ab, tsn, adf_h, \
+                               tse, dt, pv, nullptr, nullptr);                                                      \
+        else if constexpr (NT == 1)                                                                                 \
+            hipLaunchKernelGGL((k_node_nolim<R, V, 1, D, OUT_TEND>), gn, blk, 0, st, m, WE, vals, ab, tsn, adf_h,   \
+                               tse, dt, pv, acc_h, acc_v);                                                          \
+    } while (0)
         switch (sp.ver) {
         case VER_QR4C: NNL(VER_QR4C); break;
         case VER_PPM: NNL(VER_PPM); break;

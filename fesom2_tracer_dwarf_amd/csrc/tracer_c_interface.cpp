@@ -43,6 +43,7 @@ struct State {
     bool mesh_building = false;        // between tracer_mesh_begin and tracer_mesh_end
     std::vector<std::string> mesh_given;  // arrays set since tracer_mesh_begin
     int fan_fault = -1;                // first node without an element fan (caller-supplied meshes)
+    bool accumulate = false;           // del_ttf carried over between steps
 };
 
 State &S()
@@ -92,6 +93,7 @@ int ensure_engine()
         s.eng->set_refresh_ab(s.refresh != 0);
         if (!s.mlp.empty()) s.eng->mlp_set(s.mlp.data());
         s.eng->set_schedule(s.tune_waves, s.tune_nt);
+        if (s.accumulate) s.eng->set_accumulate(true);
         for (int t = 0; t < s.ntr; ++t) {
             s.eng->set_spec(t, s.specs[t]);
             s.eng->set_values(t, s.values[t].data());
@@ -234,6 +236,22 @@ int tracer_advect_steps(int nsteps, double dt)
     try {
         s.eng->run_steps(nsteps, dt, s.use_graph != 0);
         s.eng->synchronize();
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int tracer_do_oce_adv_tra(int tracer_id, double dt, double *dttf_h, double *dttf_v, int nl, int nod2D)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id");
+    if (nl != s.nl || nod2D != s.nod2D) return fail("tracer_do_oce_adv_tra: size mismatch");
+    if (!dttf_h || !dttf_v) return fail("tracer_do_oce_adv_tra: null tendency array");
+    if (ensure_engine()) return -1;
+    try {
+        s.eng->tendencies(tracer_id - 1, dt, dttf_h, dttf_v);
     } catch (const std::exception &e) {
         return fail(e.what());
     }
@@ -910,8 +928,15 @@ int tracer_set_option(const char *name, int value)
         s.use_graph = value ? 1 : 0;
         return 0;
     }
-    if (n == "accumulate_del_ttf") {
-        if (value) return fail("accumulate_del_ttf is not supported by the device path");
+    if (n == "accumulate_del_ttf") {  // del_ttf carried over between steps (reference C interface, Q6)
+        s.accumulate = value != 0;
+        if (s.eng) {
+            try {
+                s.eng->set_accumulate(s.accumulate);
+            } catch (const std::exception &e) {
+                return fail(e.what());
+            }
+        }
         return 0;
     }
     return fail("unknown option " + n);

@@ -9,7 +9,7 @@
 module fesom_tracer_amd
   use iso_c_binding
   implicit none
-  private :: c_char, c_int, c_double, c_long_long
+  private :: c_char, c_int, c_double, c_long_long, c_ptr
 
   interface
     ! ---- entry points of lib/tracer_c_interface.F90
@@ -119,6 +119,40 @@ module fesom_tracer_amd
       integer(c_int), value :: tracer_id
       real(c_double), intent(out) :: out(7)
     end function
+    integer(c_int) function tracer_mesh_begin(nl, nod2d, elem2d, edge2d, maxadj) bind(C, name="tracer_mesh_begin")
+      import :: c_int
+      integer(c_int), value :: nl, nod2d, elem2d, edge2d, maxadj
+    end function
+    integer(c_int) function tracer_set_mesh_array(name, data, count) bind(C, name="tracer_set_mesh_array")
+      import :: c_int, c_char, c_long_long
+      character(kind=c_char), intent(in) :: name(*)
+      type(*), dimension(*), intent(in) :: data
+      integer(c_long_long), value :: count
+    end function
+    integer(c_int) function tracer_mesh_end() bind(C, name="tracer_mesh_end")
+      import :: c_int
+    end function
+    integer(c_int) function tracer_set_velocity_elem(uv, w, w_e, nl, nod2d, elem2d) &
+        bind(C, name="tracer_set_velocity_elem")
+      import :: c_int, c_double
+      integer(c_int), value :: nl, nod2d, elem2d
+      real(c_double), intent(in) :: uv(2, nl - 1, elem2d), w(nl, nod2d), w_e(nl, nod2d)
+    end function
+    integer(c_int) function tracer_set_values_ab(tracer_id, values, nl, nod2d) bind(C, name="tracer_set_values_ab")
+      import :: c_int, c_double
+      integer(c_int), value :: tracer_id, nl, nod2d
+      real(c_double), intent(in) :: values(nl - 1, nod2d)
+    end function
+    integer(c_int) function tracer_do_oce_adv_tra(tracer_id, dt, dttf_h, dttf_v, nl, nod2d) &
+        bind(C, name="tracer_do_oce_adv_tra")
+      import :: c_int, c_double
+      integer(c_int), value :: tracer_id, nl, nod2d
+      real(c_double), value :: dt
+      real(c_double), intent(inout) :: dttf_h(nl - 1, nod2d), dttf_v(nl - 1, nod2d)
+    end function
+    type(c_ptr) function tracer_last_error() bind(C, name="tracer_last_error")
+      import :: c_ptr
+    end function
   end interface
 
 contains
@@ -129,5 +163,24 @@ contains
     character(kind=c_char, len=len_trim(s) + 1) :: c
     c = trim(s)//c_null_char
   end function cstr
+
+  !> the library's last error message as a Fortran string
+  function last_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: p(:)
+    type(c_ptr) :: cp
+    integer :: n
+    cp = tracer_last_error()
+    msg = ""
+    if (.not. c_associated(cp)) return
+    call c_f_pointer(cp, p, [4096])
+    n = 0
+    do while (n < 4096)
+      if (p(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(p(1:n), msg)
+  end function last_error_message
 
 end module fesom_tracer_amd

@@ -169,6 +169,21 @@ class FESOMTracerAdvection:
     def advect_steps(self, nsteps, dt):
         _lib.check(self.lib.tracer_advect_steps(nsteps, dt), "tracer_advect_steps", self.lib)
 
+    def do_oce_adv_tra(self, tracer_id, dt, dttf_h=None, dttf_v=None):
+        """One do_oce_adv_tra of a tracer: its tendencies added into dttf_h /
+        dttf_v ((nl-1, nod2D) F-order float64, zeros when not given), which are
+        returned; the tracer is unchanged."""
+        d = self.dims()
+        shape = (d["nl"] - 1, d["nod2D"])
+        h = np.zeros(shape, order="F") if dttf_h is None else dttf_h
+        v = np.zeros(shape, order="F") if dttf_v is None else dttf_v
+        for a in (h, v):
+            if a.shape != shape or a.dtype != np.float64 or not a.flags.f_contiguous:
+                raise ValueError(f"tendency arrays must be F-contiguous float64 {shape}")
+        _lib.check(self.lib.tracer_do_oce_adv_tra(tracer_id, float(dt), h, v, d["nl"], d["nod2D"]),
+                   "tracer_do_oce_adv_tra", self.lib)
+        return h, v
+
     def diagnostics(self, tracer_id=1):
         out = np.zeros(7)
         _lib.check(self.lib.tracer_get_diagnostics(tracer_id, out), "tracer_get_diagnostics", self.lib)

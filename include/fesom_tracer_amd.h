@@ -44,7 +44,8 @@ int tracer_init(const char *restart_dir);
 /* lib/tracer_c_interface.F90:163-220. One step of tracer 1 on the GPU:
  * zero the advective tendencies, do_oce_adv_tra, del_ttf += dttf_h+dttf_v,
  * values += del_ttf/hnode_new. del_ttf is zeroed every step unless the
- * option "accumulate_del_ttf" is set (the reference never zeroes it). */
+ * option "accumulate_del_ttf" is set, which reproduces the reference (it
+ * never zeroes it). */
 int tracer_advect_step(double dt);
 
 /* lib/tracer_c_interface.F90:228-248: min, max and sum of all
@@ -172,7 +173,14 @@ int tracer_get_values_id(int tracer_id, double *values, int nz, int nn);
 /* No reference counterpart. Options:
  *   "precision"          64 (default), 32, 16 (FP32 with FP16 gradients)
  *   "refresh_valuesAB"   1 (default): valuesAB := values each step; 0: as set
- *   "accumulate_del_ttf" 0 (default); 1 is refused (the device path zeroes del_ttf each step)
+ *   "accumulate_del_ttf" 0 (default): del_ttf is zeroed every step, as the
+ *                        analytic driver does (src/fesom_analytic.F90:407-413);
+ *                        1: it is carried over from step to step, as the
+ *                        reference's tracer_advect_step does (it never zeroes
+ *                        it, lib/tracer_c_interface.F90:194-212), one del_ttf
+ *                        per tracer, starting from zero when switched on (and
+ *                        when the device state is rebuilt, e.g. by a precision
+ *                        switch); not with the LEARNED limiter
  *   "use_graph"          1 (default): replay multi-step runs from a hipGraph
  *   "block_waves"        4 (default): waves per workgroup of the chunk kernels (1, 2, 4, 8)
  *   "batch_tracers"      4 (default): most tracers per launch (1, 2, 4)
@@ -184,6 +192,16 @@ int tracer_get_values_id(int tracer_id, double *values, int nz, int nn);
  *                        kernels; planar meshes come out bit for bit as
  *                        with 0 */
 int tracer_set_option(const char *name, int value);
+
+/* lib/oce_adv_tra_driver.F90:23-294 (do_oce_adv_tra), as its Fortran callers
+ * use it: one advection of tracer tracer_id (1-based) on the GPU from its
+ * current values and valuesAB, the velocity and the schemes set, whose
+ * horizontal and vertical tendencies are ADDED into dttf_h and dttf_v
+ * ((nl-1, nod2D) column-major, the wet layers of each column; the caller's
+ * del_ttf_advhoriz / del_ttf_advvert), with the tracer left unchanged. The
+ * work arrays of tracer_get_work_array then hold this call's fct_LO and
+ * fluxes. Not with the LEARNED limiter. */
+int tracer_do_oce_adv_tra(int tracer_id, double dt, double *dttf_h, double *dttf_v, int nl, int nod2D);
 
 /* No reference counterpart: nsteps steps of ALL tracers */
 int tracer_advect_steps(int nsteps, double dt);

@@ -73,6 +73,8 @@ def load(omp: bool = False):
         d, i = ctypes.c_double, ctypes.c_int
         lib.oracle_step_tracer.argtypes = [M, d, _dp, _dp, _dp, _dp, _dp, i, i, i, d, d, W]
         lib.oracle_step_tracer.restype = i
+        lib.oracle_step_tracer_ex.argtypes = [M, d, _dp, _dp, _dp, _dp, _dp, i, i, i, d, d, W, i]
+        lib.oracle_step_tracer_ex.restype = i
         lib.oracle_do_oce_adv_tra.argtypes = [M, d, _dp, _dp, _dp, _dp, _dp, i, i, i, d, d, W]
         lib.oracle_do_oce_adv_tra.restype = i
         lib.oracle_tracer_gradient_elements.argtypes = [M, _dp, _dp]
@@ -129,21 +131,52 @@ class Oracle:
         self.work_arrays = {k: np.zeros(v) for k, v in shapes.items()}
         self.work = OracleWork(**{k: _ptr(v, ctypes.c_double) for k, v in self.work_arrays.items()})
 
-    def step(self, dt, vel, w, we, values, valuesAB=None, hor="MUSCL", ver="QR4C", lim="FCT", ph=1.0, pv=1.0):
-        """One analytic-driver step of one tracer; values (nl-1, nod2D) F-order is updated in place."""
+    def step(self, dt, vel, w, we, values, valuesAB=None, hor="MUSCL", ver="QR4C", lim="FCT", ph=1.0, pv=1.0,
+             keep_del_ttf=False):
+        """One analytic-driver step of one tracer; values (nl-1, nod2D) F-order is updated in place.
+        keep_del_ttf: del_ttf carried over from this oracle's previous step (the reference C interface)."""
         assert values.flags.f_contiguous and values.dtype == np.float64
         ab = values if valuesAB is None else np.asfortranarray(valuesAB, dtype=np.float64)
         vel = np.ascontiguousarray(np.ravel(vel, order="F"))
         w = np.ascontiguousarray(np.ravel(w, order="F"))
         we = np.ascontiguousarray(np.ravel(we, order="F"))
         ab_copy = np.array(ab, order="F", copy=True)  # values is updated in place; AB must stay the old field
-        st = self.lib.oracle_step_tracer(ctypes.byref(self.mesh), dt, _ptr(vel, ctypes.c_double),
-                                         _ptr(w, ctypes.c_double), _ptr(we, ctypes.c_double),
-                                         _ptr(values, ctypes.c_double), _ptr(ab_copy, ctypes.c_double),
-                                         HOR[hor], VER[ver], LIM[lim], ph, pv, ctypes.byref(self.work))
+        st = self.lib.oracle_step_tracer_ex(ctypes.byref(self.mesh), dt, _ptr(vel, ctypes.c_double),
+                                            _ptr(w, ctypes.c_double), _ptr(we, ctypes.c_double),
+                                            _ptr(values, ctypes.c_double), _ptr(ab_copy, ctypes.c_double),
+                                            HOR[hor], VER[ver], LIM[lim], ph, pv, ctypes.byref(self.work),
+                                            1 if keep_del_ttf else 0)
         if st != 0:
             raise ValueError("oracle: unknown scheme")
         return values
+
+    def tendencies(self, dt, vel, w, we, values, valuesAB=None, hor="MUSCL", ver="QR4C", lim="FCT", ph=1.0,
+                   pv=1.0, dttf_h=None, dttf_v=None):
+        """do_oce_adv_tra of one tracer (MUSCL/MFCT prerequisites from valuesAB first, as FESOM's
+        caller computes them): returns dttf_h, dttf_v (nl-1, nod2D) after the routine added its
+        tendencies into the given starting arrays (zeros when None); values is not changed."""
+        L, N = self.nl - 1, self.N
+        ab = values if valuesAB is None else valuesAB
+        vals = np.asfortranarray(values, dtype=np.float64)
+        ab = np.asfortranarray(ab, dtype=np.float64)
+        vel = np.ascontiguousarray(np.ravel(vel, order="F"))
+        w = np.ascontiguousarray(np.ravel(w, order="F"))
+        we = np.ascontiguousarray(np.ravel(we, order="F"))
+        wk = self.work_arrays
+        wk["dttf_h"][:] = 0.0 if dttf_h is None else np.ravel(dttf_h, order="F")
+        wk["dttf_v"][:] = 0.0 if dttf_v is None else np.ravel(dttf_v, order="F")
+        if hor in ("MUSCL", "MFCT"):
+            self.lib.oracle_tracer_gradient_elements(ctypes.byref(self.mesh), _ptr(ab, ctypes.c_double),
+                                                     _ptr(wk["tr_xy"], ctypes.c_double))
+            self.lib.oracle_fill_up_dn_grad(ctypes.byref(self.mesh), _ptr(wk["tr_xy"], ctypes.c_double),
+                                            _ptr(wk["edge_up_dn_grad"], ctypes.c_double))
+        st = self.lib.oracle_do_oce_adv_tra(ctypes.byref(self.mesh), dt, _ptr(vel, ctypes.c_double),
+                                            _ptr(w, ctypes.c_double), _ptr(we, ctypes.c_double),
+                                            _ptr(vals, ctypes.c_double), _ptr(ab, ctypes.c_double),
+                                            HOR[hor], VER[ver], LIM[lim], ph, pv, ctypes.byref(self.work))
+        if st != 0:
+            raise ValueError("oracle: unknown scheme")
+        return (wk["dttf_h"].reshape(N, L).T.copy(order="F"), wk["dttf_v"].reshape(N, L).T.copy(order="F"))
 
     def up_downwind(self, cyc_x_rad, cyc_y_rad):
         out = np.zeros(2 * self.E, dtype=np.int32)

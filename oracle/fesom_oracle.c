@@ -739,10 +739,21 @@ int oracle_step_tracer(const oracle_mesh *m, double dt, const double *vel, const
                        int hor, int ver, int lim, double opth, double optv,
                        oracle_work *wk)
 {
+    return oracle_step_tracer_ex(m, dt, vel, w, we, values, valuesAB, hor, ver, lim, opth, optv, wk, 0);
+}
+
+/* keep_del_ttf = 1: the reference C interface's tracer_advect_step, which
+ * zeroes the two tendencies but never del_ttf (lib/tracer_c_interface.F90:
+ * 176-212), so del_ttf grows from step to step */
+int oracle_step_tracer_ex(const oracle_mesh *m, double dt, const double *vel, const double *w,
+                          const double *we, double *values, const double *valuesAB,
+                          int hor, int ver, int lim, double opth, double optv,
+                          oracle_work *wk, int keep_del_ttf)
+{
     size_t nn = (size_t)(m->nl - 1) * m->nod2D;
     memset(wk->dttf_h, 0, sizeof(double) * nn);
     memset(wk->dttf_v, 0, sizeof(double) * nn);
-    memset(wk->del_ttf, 0, sizeof(double) * nn);
+    if (!keep_del_ttf) memset(wk->del_ttf, 0, sizeof(double) * nn);
     if (hor == OR_HOR_MUSCL || hor == OR_HOR_MFCT) {
         oracle_tracer_gradient_elements(m, valuesAB, wk->tr_xy);
         oracle_fill_up_dn_grad(m, wk->tr_xy, wk->edge_up_dn_grad);

@@ -104,6 +104,13 @@ int oracle_step_tracer(const oracle_mesh *m, double dt, const double *vel, const
                        int hor, int ver, int lim, double opth, double optv,
                        oracle_work *wk);
 
+/* the same, with del_ttf kept from the previous call when keep_del_ttf != 0
+ * (the reference C interface's tracer_advect_step) */
+int oracle_step_tracer_ex(const oracle_mesh *m, double dt, const double *vel, const double *w,
+                          const double *we, double *values, const double *valuesAB,
+                          int hor, int ver, int lim, double opth, double optv,
+                          oracle_work *wk, int keep_del_ttf);
+
 /* threads of the OpenMP build (n > 0 sets them); returns the count in use
  * (1 in the serial build) */
 int oracle_set_threads(int n);
