@@ -62,6 +62,7 @@ SIGNATURES = {
     "tracer_set_option": (c_int, [c_char_p, c_int]),
     "tracer_advect_steps": (c_int, [c_int, c_double]),
     "tracer_do_oce_adv_tra": (c_int, [c_int, c_double, F64F, F64F, c_int, c_int]),
+    "tracer_do_oce_adv_tra_noalloc": (c_int, [c_int, c_double, c_void_p, c_void_p, c_int, c_int]),
     "tracer_get_diagnostics": (c_int, [c_int, F64]),
     "tracer_mesh_array_info": (c_int, [c_char_p, P_ll, P_int]),
     "tracer_get_mesh_array": (c_int, [c_char_p, c_void_p, c_ll]),

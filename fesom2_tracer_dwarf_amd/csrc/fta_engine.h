@@ -75,6 +75,11 @@ public:
     // are ADDED into the host arrays dttf_h, dttf_v ((nl-1, nod2D), wet layers)
     // and the tracer is left unchanged
     void tendencies(int tr, double dt, double *dttf_h, double *dttf_v);
+    // the same into DEVICE arrays dttf_h, dttf_v ((nl-1, nod2D)), with the two
+    // packed tendency arrays in the caller's device scratch (2 * node cells
+    // doubles at most scratch_doubles, else an error); allocates nothing
+    void tendencies_device(int tr, double dt, double *dttf_h, double *dttf_v, double *scratch,
+                           size_t scratch_doubles);
 
     // refresh_ab: valuesAB := values at the start of each step (AB1)
     void set_refresh_ab(bool r) { refresh_ab_ = r; graph_valid_ = false; }
@@ -149,6 +154,10 @@ private:
 // 16-byte streaming copy of `bytes` (hand-written kernel, fta_stream.hip);
 // variant receives {workgroups, words in flight per lane, nontemporal}
 double hbm_copy_rate(long long bytes, int reps, int variant[3]);
+
+// this GPU's copy of the allocation-free entry point's device scratch
+// (fta_noalloc.hip) and its size in doubles
+double *noalloc_scratch(size_t *doubles);
 
 const char *last_error();
 void set_error(const std::string &s);

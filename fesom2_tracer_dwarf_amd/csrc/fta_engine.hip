@@ -1855,6 +1855,41 @@ void Engine::tendencies(int tr, double dt, double *dttf_h, double *dttf_v)
     }
 }
 
+void Engine::tendencies_device(int tr, double dt, double *dttf_h, double *dttf_v, double *scratch,
+                               size_t scratch_doubles)
+{
+    if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
+    Impl &I = *p_;
+    if (spec_[tr].lim == LIM_LEARNED) throw std::runtime_error("tendencies are not returned for the LEARNED limiter");
+    if (2 * I.tsn > scratch_doubles)
+        throw std::runtime_error("the mesh needs " + std::to_string(2 * I.tsn * sizeof(double) >> 20) +
+                                 " MiB of tendency scratch, more than the " +
+                                 std::to_string(scratch_doubles * sizeof(double) >> 20) + " MiB available");
+    // the step writes every node layer it covers into the two scratch halves
+    // (nothing needs to start at zero); the engine's own tendency arrays are
+    // left as they are
+    double *own_h = I.tend_h, *own_v = I.tend_v;
+    const int saved = I.out_mode;
+    I.tend_h = scratch;
+    I.tend_v = scratch + I.tsn;
+    I.out_mode = OUT_TEND;
+    try {
+        enqueue_step(dt, tr, 1);
+    } catch (...) {
+        I.tend_h = own_h; I.tend_v = own_v; I.out_mode = saved;
+        throw;
+    }
+    I.tend_h = own_h; I.tend_v = own_v; I.out_mode = saved;
+    const int *slot, *nlev;
+    long long npair;
+    node_slots(I, prec_, slot, nlev, npair);
+    for (int q = 0; q < 2; ++q)
+        hipLaunchKernelGGL(k_unpack_add_layers, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, stream_, slot,
+                           nlev, npair, scratch + (size_t)q * I.tsn, nl_ - 1, q == 0 ? dttf_h : dttf_v);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
 struct LaunchHook {
     std::vector<std::pair<std::string, hipEvent_t>> *ev = nullptr;
     void mark(const char *name, hipStream_t st)

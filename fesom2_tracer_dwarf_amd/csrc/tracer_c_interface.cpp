@@ -258,6 +258,26 @@ int tracer_do_oce_adv_tra(int tracer_id, double dt, double *dttf_h, double *dttf
     return 0;
 }
 
+int tracer_do_oce_adv_tra_noalloc(int tracer_id, double dt, double *dttf_h, double *dttf_v, int nl, int nod2D)
+{
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id");
+    if (nl != s.nl || nod2D != s.nod2D) return fail("tracer_do_oce_adv_tra_noalloc: size mismatch");
+    if (!dttf_h || !dttf_v) return fail("tracer_do_oce_adv_tra_noalloc: null tendency array");
+    // building the device state allocates: it must exist already
+    if (!s.eng) return fail("tracer_do_oce_adv_tra_noalloc: no device state yet (run tracer_init_for_advection, "
+                            "a step or tracer_do_oce_adv_tra first)");
+    try {
+        size_t n = 0;
+        double *scratch = noalloc_scratch(&n);
+        s.eng->tendencies_device(tracer_id - 1, dt, dttf_h, dttf_v, scratch, n);
+    } catch (const std::exception &e) {
+        return fail(std::string("tracer_do_oce_adv_tra_noalloc: ") + e.what());
+    }
+    return 0;
+}
+
 int tracer_get_stats(double *tmin, double *tmax, double *tsum)
 {
     State &s = S();

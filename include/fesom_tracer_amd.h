@@ -203,6 +203,22 @@ int tracer_set_option(const char *name, int value);
  * fluxes. Not with the LEARNED limiter. */
 int tracer_do_oce_adv_tra(int tracer_id, double dt, double *dttf_h, double *dttf_v, int nl, int nod2D);
 
+/* No reference counterpart: the allocation-free form of tracer_do_oce_adv_tra,
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). dttf_h and dttf_v are
+ * DEVICE pointers to (nl-1, nod2D) column-major doubles, into whose wet
+ * layers the tendencies are added on the GPU. The call allocates nothing: its
+ * scratch (the two packed tendency arrays) lives in a file-scope device
+ * array of 64 MiB that each GPU holds until the process ends, so it takes
+ * meshes of at most 4,194,304 packed node layers (each column's wet layers
+ * rounded up to even, columns packed whole into 128-layer chunks) and refuses
+ * larger ones with -1 and an error; and the device state must exist already
+ * (-1 otherwise). Every call on one GPU shares that scratch: calls that
+ * overlap on one GPU race. The call runs on the library's own stream: work
+ * other streams still have pending on dttf_h / dttf_v must be finished
+ * first. Returns after the GPU work is done. */
+int tracer_do_oce_adv_tra_noalloc(int tracer_id, double dt, double *dttf_h, double *dttf_v, int nl, int nod2D);
+
 /* No reference counterpart: nsteps steps of ALL tracers */
 int tracer_advect_steps(int nsteps, double dt);
 
