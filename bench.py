@@ -77,12 +77,11 @@ def kernel_alg_bytes(name, c, ntr, s_t, s_r, s_g, we_alias=True):
     replace them on shared levels), so the figure is a lower bound."""
     C, Ce, Ced, N, T, E = c["C"], c["Ce"], c["Ced"], c["N"], c["T"], c["E"]
     CI = C + N
-    if name == "k_trxy":          # valuesAB at the element nodes -> elemental gradients (2 comps)
-        return ntr * (s_t * C + 2 * s_g * Ce) + T * (16 + 8 * s_r)
-    if name == "k_gnod":          # elemental gradients -> nodal means; node->element records
-        return ntr * (2 * s_g * Ce + 2 * s_g * C) + N * (48 + 8 * s_r)
-    if name == "k_edge":          # tracer (T and AB share storage), gradients, vel, helem, edge records -> transport, HO-LO
-        return ntr * (s_t * C + 2 * s_g * C + s_r * Ced) + s_r * Ced + 3 * s_r * Ce + E * (32 + 8 * s_r)
+    if name == "k_gnod":          # valuesAB -> nodal mean gradients (2 comps); element geometry, node->element records
+        return ntr * (s_t * C + 2 * s_g * C) + T * 8 * s_r + N * 112
+    if name == "k_edge":          # tracer (T and AB share storage), nodal gradients, vel, helem, edge records and
+                                  # up/down-wind weights -> transport, HO-LO
+        return ntr * (s_t * C + 2 * s_g * C + s_r * Ced) + s_r * Ced + 3 * s_r * Ce + E * (48 + 16 * s_r)
     if name == "k_node_lo":
         # edge transport, HO-LO edge fluxes, tracer -> LO (FP64), adf_v, P+, P-; on interfaces w (w_e shares
         # its storage when equal, as here), area, zbar_3d_n; on layers Z_3d_n and the FP64 hnode, hnode_new

@@ -124,6 +124,10 @@ public:
     // fraction of wet layers of tracer tr whose Zalesak fallback was taken in the last step
     double learn_fallback(int tr);
 
+    // nodal mean gradients over each node's fan (default; meshes whose nodes
+    // all have one) or element by element (the same values to rounding)
+    void set_gradient_fan(bool on);
+
     // launch schedule of the chunk kernels: waves per workgroup (1, 2, 4, 8)
     // and most tracers per launch (1, 2, 4); results do not depend on it
     void set_schedule(int waves, int max_nt);

@@ -2,9 +2,9 @@
 //
 // One step (reference: do_oce_adv_tra, lib/oce_adv_tra_driver.F90:23-294,
 // plus the caller's update, src/fesom_analytic.F90:405-437) for a batch of
-// NT tracers that share their schemes is six launches:
-//   k_trxy        per element-level: elemental gradient of valuesAB (MUSCL/MFCT)
-//   k_gnod        per node-level: area-weighted mean of those gradients
+// NT tracers that share their schemes is five launches:
+//   k_gnod        per node-level: area-weighted mean of the elemental
+//                 gradients of valuesAB around the node (MUSCL/MFCT)
 //   k_edge        per edge-level: LO upwind flux and HO-LO antidiffusive flux
 //   k_node_lo     per node-level: LO solution, vertical HO-LO flux, P+/P- sums
 //   k_node_limit  per node-level: Zalesak bounds (a1-a3) and R+/R- (b2)
@@ -90,41 +90,25 @@ __device__ __forceinline__ void load_other_nodes(const DMesh<R> &m, int n, bool 
 }
 
 // ===========================================================================
-// MUSCL prerequisites (fill_up_dn_grad, lib/oce_muscl_adv.F90:361-530):
-// elemental gradients of valuesAB (tracer_gradient_elements:
-// gradient_sca(1:3)*T(nodes), gradient_sca(4:6)*T(nodes), node order 1,2,3)
-// and their area-weighted mean over the node's elements that have the layer
-// (:397-407)
+// MUSCL prerequisites (fill_up_dn_grad, lib/oce_muscl_adv.F90:361-530): the
+// area-weighted mean (:397-407) over the node's elements that have the layer
+// of their elemental gradients of valuesAB (tracer_gradient_elements:
+// gradient_sca(1:3)*T(nodes), gradient_sca(4:6)*T(nodes), node order 1,2,3),
+// each element's gradient formed here from the values at its three nodes and
+// never stored: the sums run in the same order with the same operands as when
+// the elemental gradients were a separate array, so the result is the same
 // ===========================================================================
-template <class R, class G, int NT>
-__global__ __launch_bounds__(kMaxBlock) void k_trxy(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
-                                                 G *__restrict__ trxy, size_t tst, float gscale)
+template <class R>
+__device__ __forceinline__ void ld_tgeo(const R *g, R (&c)[6], R &area)
 {
-    const int ch = wave_chunk();
-    if (ch >= m.nch_t) return;
-    const Pair p = lane_pair(m.slot_t, ch);
-    if (!p.live) return;
-    const ElemRec er = m.trec[p.col];
-    const bool v1 = p.k0 + 1 < er.nlev - 1;
-    const R *g = m.tgeo + 8 * (size_t)p.col;
-    const R g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const double *A = ab + t * tsn + p.k0;
-        const V2<double> a0 = ld2(A + er.o0), a1 = ld2(A + er.o1), a2 = ld2(A + er.o2);
-        // both cells computed unconditionally (a branch on the second cell
-        // would let the compiler split each 16-byte pair load in two)
-        const R gx0 = g0 * (R)a0.x + g1 * (R)a1.x + g2 * (R)a2.x;
-        const R gy0 = g3 * (R)a0.x + g4 * (R)a1.x + g5 * (R)a2.x;
-        const R gx1 = g0 * (R)a0.y + g1 * (R)a1.y + g2 * (R)a2.y;
-        const R gy1 = g3 * (R)a0.y + g4 * (R)a1.y + g5 * (R)a2.y;
-        GradIO<G>::st(trxy + t * tst + 2 * p.c, gx0, gy0, v1 ? gx1 : (R)0, v1 ? gy1 : (R)0, gscale);
-    }
+    const V4<R> a = ld4(g), b = ld4(g + 4);
+    c[0] = a.x0; c[1] = a.y0; c[2] = a.x1; c[3] = a.y1; c[4] = b.x0; c[5] = b.y0;
+    area = b.x1;
 }
 
 template <class R, class G, int NT, int D>
-__global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const G *__restrict__ trxy, size_t tst,
-                                                 G *__restrict__ gnod, size_t tsg, float gscale, float ginv)
+__global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
+                                                 G *__restrict__ gnod, size_t tsg, float gscale)
 {
     const int ch = wave_chunk();
     if (ch >= m.nch_n) return;
@@ -133,56 +117,149 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const G *__restr
     const int L = m.nlev[p.col] - 1;
     if (p.k0 >= L) return;  // the pair holds only the bottom interface
     const bool v1 = p.k0 + 1 < L;
-    // node->element record and areas as whole 16-byte loads
-    int ot[D], le[D];
-    R ar[D];
+    int ei[D], oa[D], ob[D], le[D], pos[D];
     {
         const NodeElems<D> *rec = &node_elems<D>(m)[p.col];
-        const int4 *q = reinterpret_cast<const int4 *>(rec->ot);
+        const int4 *q = reinterpret_cast<const int4 *>(rec->ei);
 #pragma unroll
         for (int h = 0; h < D / 4; ++h) {
-            const int4 a = q[h];
-            ot[4 * h] = a.x; ot[4 * h + 1] = a.y; ot[4 * h + 2] = a.z; ot[4 * h + 3] = a.w;
+            const int4 a = q[h], b = q[D / 4 + h], c = q[D / 2 + h];
+            ei[4 * h] = a.x; ei[4 * h + 1] = a.y; ei[4 * h + 2] = a.z; ei[4 * h + 3] = a.w;
+            oa[4 * h] = b.x; oa[4 * h + 1] = b.y; oa[4 * h + 2] = b.z; oa[4 * h + 3] = b.w;
+            ob[4 * h] = c.x; ob[4 * h + 1] = c.y; ob[4 * h + 2] = c.z; ob[4 * h + 3] = c.w;
         }
-        const uint2 *lw = reinterpret_cast<const uint2 *>(rec->le);
+        // le[0..D) then pos[0..D), four bytes to a word
+        const uint4 *lw = reinterpret_cast<const uint4 *>(rec->le);
+        if constexpr (D == 8) {
+            const uint4 u = lw[0];
+            const unsigned w[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
-        for (int h = 0; h < D / 8; ++h) {
-            const uint2 l = lw[h];
+            for (int j = 0; j < 8; ++j) {
+                le[j] = (int)((w[j >> 2] >> (8 * (j & 3))) & 255u);
+                pos[j] = (int)((w[2 + (j >> 2)] >> (8 * (j & 3))) & 255u);
+            }
+        } else {
+            const uint4 u0 = lw[0], u1 = lw[1];
+            const unsigned w[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                le[8 * h + j] = (int)((l.x >> (8 * j)) & 255u);
-                le[8 * h + j + 4] = (int)((l.y >> (8 * j)) & 255u);
+            for (int j = 0; j < D; ++j) {
+                le[j] = (int)((w[j >> 2] >> (8 * (j & 3))) & 255u);
+                pos[j] = (int)((w[(D >> 2) + (j >> 2)] >> (8 * (j & 3))) & 255u);
             }
         }
-        const R *ap = m.nelem_area + D * (size_t)p.col;
-#pragma unroll
-        for (int j = 0; j < D; j += 2) {
-            const V2<R> v = ld2(ap + j);
-            ar[j] = v.x;
-            ar[j + 1] = v.y;
-        }
     }
+    V2<double> Tn[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) Tn[t] = ld2(ab + t * tsn + p.c);
     R tvol0 = 0, tvol1 = 0;
     R tx0[NT], ty0[NT], tx1[NT], ty1[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) { tx0[t] = 0; ty0[t] = 0; tx1[t] = 0; ty1[t] = 0; }
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        if (ot[j] < 0) break;
+        if (ei[j] < 0) break;
         if (p.k0 >= le[j]) continue;
         const bool a1 = p.k0 + 1 < le[j];
-        const R a = ar[j];
+        R g[6], a;
+        ld_tgeo(m.tgeo + 8 * (size_t)ei[j], g, a);
         tvol0 = tvol0 + a;
         tvol1 = a1 ? tvol1 + a : tvol1;
+        const int ps = pos[j];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const auto gg = GradIO<G>::ld(trxy + t * tst + 2 * ((size_t)ot[j] + p.k0), ginv);
-            tx0[t] = tx0[t] + (R)gg.x0 * a;
-            ty0[t] = ty0[t] + (R)gg.y0 * a;
-            const R x1 = tx1[t] + (R)gg.x1 * a, y1 = ty1[t] + (R)gg.y1 * a;
+            const double *A = ab + t * tsn + p.k0;
+            const V2<double> Ta = ld2(A + oa[j]), Tb = ld2(A + ob[j]);
+            // the element's node values in its elem2D_nodes order (this node at ps)
+            const V2<double> T0 = ps == 0 ? Tn[t] : Ta;
+            const V2<double> T1 = ps == 0 ? Ta : (ps == 1 ? Tn[t] : Tb);
+            const V2<double> T2 = ps == 2 ? Tn[t] : Tb;
+            const R gx0 = g[0] * (R)T0.x + g[1] * (R)T1.x + g[2] * (R)T2.x;
+            const R gy0 = g[3] * (R)T0.x + g[4] * (R)T1.x + g[5] * (R)T2.x;
+            const R gx1 = g[0] * (R)T0.y + g[1] * (R)T1.y + g[2] * (R)T2.y;
+            const R gy1 = g[3] * (R)T0.y + g[4] * (R)T1.y + g[5] * (R)T2.y;
+            tx0[t] = tx0[t] + gx0 * a;
+            ty0[t] = ty0[t] + gy0 * a;
+            const R x1 = tx1[t] + gx1 * a, y1 = ty1[t] + gy1 * a;
             tx1[t] = a1 ? x1 : tx1[t];
             ty1[t] = a1 ? y1 : ty1[t];
         }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+        GradIO<G>::st(gnod + t * tsg + 2 * p.c, tx0[t] / tvol0, ty0[t] / tvol0, v1 ? tx1[t] / tvol1 : (R)0,
+                      v1 ? ty1[t] / tvol1 : (R)0, gscale);
+}
+
+// the same mean over the node's fan (ring node j and j+1 bound element j):
+// every ring value is loaded once, and each element's gradient times its area
+// is the weighted sum of its two ring values' differences to this node's
+// value (a triangle's gradient coefficients sum to zero). Sums in fan order;
+// the result equals the element-by-element form to rounding.
+template <class R, class G, int NT, int D>
+__global__ __launch_bounds__(kMaxBlock) void k_gnod_fan(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
+                                                     G *__restrict__ gnod, size_t tsg, float gscale)
+{
+    const int ch = wave_chunk();
+    if (ch >= m.nch_n) return;
+    const Pair p = lane_pair(m.slot_n, ch);
+    if (!p.live) return;
+    const int L = m.nlev[p.col] - 1, k0 = p.k0;
+    if (k0 >= L) return;  // the pair holds only the bottom interface
+    const bool v1 = k0 + 1 < L;
+    const NodeRing<D> rg = node_ring<D>(m)[p.col];
+    int nr = 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) nr += rg.oo[j] >= 0 ? 1 : 0;
+    V2<double> Tn[NT], first[NT], prev[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        Tn[t] = ld2(ab + t * tsn + p.c);
+        first[t] = prev[t] = V2<double>{0.0, 0.0};
+    }
+    R tvol0 = 0, tvol1 = 0;
+    R tx0[NT], ty0[NT], tx1[NT], ty1[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) { tx0[t] = 0; ty0[t] = 0; tx1[t] = 0; ty1[t] = 0; }
+    const R *W = m.ngw + (size_t)8 * D * p.col;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) {
+        V2<double> cur[NT];
+        if (j < D && rg.oo[j] >= 0) {
+            // a ring node shallower than this pair borders only elements that
+            // are dry here (masked below): its column's first pair stands in
+            const int kk = k0 < (int)rg.lr[j] ? k0 : 0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cur[t] = ld2(ab + t * tsn + rg.oo[j] + kk);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cur[t] = V2<double>{0.0, 0.0};
+        }
+        if (j == 0) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) first[t] = cur[t];
+        } else if (rg.le[j - 1] != 0) {
+            const int le = rg.le[j - 1];
+            const bool m0 = k0 < le, m1 = k0 + 1 < le;
+            const bool wrap = j >= nr;
+            const V2<R> wa = ld2(W + 8 * (j - 1)), wb = ld2(W + 8 * (j - 1) + 2), wc = ld2(W + 8 * (j - 1) + 4);
+            const R a = wc.x;
+            tvol0 = m0 ? tvol0 + a : tvol0;
+            tvol1 = m1 ? tvol1 + a : tvol1;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const V2<double> b = wrap ? first[t] : cur[t];
+                const R da0 = (R)(prev[t].x - Tn[t].x), db0 = (R)(b.x - Tn[t].x);
+                const R da1 = (R)(prev[t].y - Tn[t].y), db1 = (R)(b.y - Tn[t].y);
+                const R gx0 = wa.x * da0 + wa.y * db0, gy0 = wb.x * da0 + wb.y * db0;
+                const R gx1 = wa.x * da1 + wa.y * db1, gy1 = wb.x * da1 + wb.y * db1;
+                tx0[t] = m0 ? tx0[t] + gx0 : tx0[t];
+                ty0[t] = m0 ? ty0[t] + gy0 : ty0[t];
+                tx1[t] = m1 ? tx1[t] + gx1 : tx1[t];
+                ty1[t] = m1 ? ty1[t] + gy1 : ty1[t];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) prev[t] = cur[t];
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t)
@@ -211,7 +288,6 @@ template <class R, class G, int HOR, bool FCT, int NT, bool SAME>
 __global__ __launch_bounds__(kMaxBlock) void k_edge(DMesh<R> m, const R *__restrict__ vel,
                                                  const double *__restrict__ ttf,
                                                  const double *__restrict__ ttfAB, size_t tsn,
-                                                 const G *__restrict__ trxy, size_t tst,
                                                  const G *__restrict__ gnod, size_t tsg,
                                                  R num_ord, float ginv,
                                                  R *__restrict__ vfl, R *__restrict__ flux, size_t tse)
@@ -251,7 +327,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_edge(DMesh<R> m, const R *__restr
     // MUSCL/MFCT: reconstruction geometry, gradient source per level
     R ex = 0, ey = 0, a = 0, clo10 = 1, clo20 = 1, clo11 = 1, clo21 = 1;
     bool sh0 = false, sh1 = false;
+    int4 ud{0, 0, 0, 0};
+    V4<R> wud{0, 0, 0, 0};
     if constexpr (HOR != HOR_UPW1) {
+        // independent of the edge record: issued with it
+        ud = m.eud[p.col];
+        wud = ld4(m.egud + 4 * (size_t)p.col);
         ex = geo[4]; ey = geo[5]; a = geo[6];
         if constexpr (HOR == HOR_MUSCL) {
             const int nb1 = er.nb & 0xFFFF, nb2 = er.nb >> 16;
@@ -292,32 +373,38 @@ __global__ __launch_bounds__(kMaxBlock) void k_edge(DMesh<R> m, const R *__restr
             ho0 = (R)(-0.5) * (A10 * (vflux0 + fabs(vflux0)) + A20 * (vflux0 - fabs(vflux0)));
             ho1 = (R)(-0.5) * (A11 * (vflux1 + fabs(vflux1)) + A21 * (vflux1 - fabs(vflux1)));
         } else {
-            V4<R> gu{0, 0, 0, 0}, gd{0, 0, 0, 0}, n1{0, 0, 0, 0}, n2{0, 0, 0, 0};
+            // the gradient along the edge, ex*a*d/dx + ey*r_earth*d/dy: of the
+            // up/down-wind triangle on the levels all elements around both
+            // nodes share (sum w T over its nodes), of the nodal means below
+            R du0 = 0, dd0 = 0, du1 = 0, dd1 = 0;
             if (sh0) {
-                const auto u = GradIO<G>::ld(trxy + t * tst + 2 * ((size_t)er.tu + k0), ginv);
-                const auto d = GradIO<G>::ld(trxy + t * tst + 2 * ((size_t)er.td + k0), ginv);
-                gu = {(R)u.x0, (R)u.y0, (R)u.x1, (R)u.y1};
-                gd = {(R)d.x0, (R)d.y0, (R)d.x1, (R)d.y1};
+                const double *A = (SAME ? ttf : ttfAB) + tn + k0;
+                const V2<double> ua = ld2(A + ud.x), ub = ld2(A + ud.y), da = ld2(A + ud.z), db = ld2(A + ud.w);
+                du0 = wud.x0 * (R)(ua.x - B1.x) + wud.y0 * (R)(ub.x - B1.x);
+                dd0 = wud.x1 * (R)(da.x - B2.x) + wud.y1 * (R)(db.x - B2.x);
+                du1 = wud.x0 * (R)(ua.y - B1.y) + wud.y0 * (R)(ub.y - B1.y);
+                dd1 = wud.x1 * (R)(da.y - B2.y) + wud.y1 * (R)(db.y - B2.y);
             }
+            V4<R> n1{0, 0, 0, 0}, n2{0, 0, 0, 0};
             if (!sh0 || (valid1 && !sh1)) {
                 const auto u = GradIO<G>::ld(gnod + t * tsg + 2 * ((size_t)er.o1 + k0), ginv);
                 const auto d = GradIO<G>::ld(gnod + t * tsg + 2 * ((size_t)er.o2 + k0), ginv);
                 n1 = {(R)u.x0, (R)u.y0, (R)u.x1, (R)u.y1};
                 n2 = {(R)d.x0, (R)d.y0, (R)d.x1, (R)d.y1};
             }
-            const R g1x0 = sh0 ? gu.x0 : n1.x0, g1y0 = sh0 ? gu.y0 : n1.y0;
-            const R g2x0 = sh0 ? gd.x0 : n2.x0, g2y0 = sh0 ? gd.y0 : n2.y0;
-            const R g1x1 = sh1 ? gu.x1 : n1.x1, g1y1 = sh1 ? gu.y1 : n1.y1;
-            const R g2x1 = sh1 ? gd.x1 : n2.x1, g2y1 = sh1 ? gd.y1 : n2.y1;
+            const R dot10 = sh0 ? du0 : ex * a * n1.x0 + ey * (R)kRE * n1.y0;
+            const R dot20 = sh0 ? dd0 : ex * a * n2.x0 + ey * (R)kRE * n2.y0;
+            const R dot11 = sh1 ? du1 : ex * a * n1.x1 + ey * (R)kRE * n1.y1;
+            const R dot21 = sh1 ? dd1 : ex * a * n2.x1 + ey * (R)kRE * n2.y1;
             {
-                const R Tmean2 = A20 - ((R)2 * (A20 - A10) + ex * a * g2x0 + ey * (R)kRE * g2y0) / (R)6 * clo20;
-                const R Tmean1 = A10 + ((R)2 * (A20 - A10) + ex * a * g1x0 + ey * (R)kRE * g1y0) / (R)6 * clo10;
+                const R Tmean2 = A20 - ((R)2 * (A20 - A10) + dot20) / (R)6 * clo20;
+                const R Tmean1 = A10 + ((R)2 * (A20 - A10) + dot10) / (R)6 * clo10;
                 const R cHO = (vflux0 + fabs(vflux0)) * Tmean1 + (vflux0 - fabs(vflux0)) * Tmean2;
                 ho0 = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux0 * num_ord * (R)0.5 * (Tmean1 + Tmean2);
             }
             {
-                const R Tmean2 = A21 - ((R)2 * (A21 - A11) + ex * a * g2x1 + ey * (R)kRE * g2y1) / (R)6 * clo21;
-                const R Tmean1 = A11 + ((R)2 * (A21 - A11) + ex * a * g1x1 + ey * (R)kRE * g1y1) / (R)6 * clo11;
+                const R Tmean2 = A21 - ((R)2 * (A21 - A11) + dot21) / (R)6 * clo21;
+                const R Tmean1 = A11 + ((R)2 * (A21 - A11) + dot11) / (R)6 * clo11;
                 const R cHO = (vflux1 + fabs(vflux1)) * Tmean1 + (vflux1 - fabs(vflux1)) * Tmean2;
                 ho1 = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux1 * num_ord * (R)0.5 * (Tmean1 + Tmean2);
             }
@@ -1374,7 +1461,6 @@ struct Engine::Impl {
     void *adf_v = nullptr;                                 // [ntr][CN] (interfaces)
     void *Pp = nullptr, *Pm = nullptr;                     // [ntr][CN]
     double *LO = nullptr;                                  // [ntr][CN]
-    void *trxy = nullptr;                                  // [ntr][2*CT]
     void *gnod = nullptr;                                  // [ntr][2*CN]
     DiagAcc *dpart = nullptr, *dout = nullptr;
     ErrAcc *epart = nullptr;
@@ -1394,9 +1480,10 @@ struct Engine::Impl {
     struct Dump { unsigned short *X; float *Y; long long row; long long cap; } *dump = nullptr;
     int ndiag = 1024;
     float gscale = 1.0f, ginv = 1.0f;
-    size_t tsn = 0, tst = 0, tse = 0;
+    size_t tsn = 0, tse = 0;
     int wslots = 1;  // tracers the work arrays hold
     int fan_fault = -1;  // first node whose elements do not form one fan (-1: none)
+    bool grad_fan = true;  // nodal gradients over the fan (when every node has one), else element by element
     bool we_is_w = false;  // w_e equals w (no implicit vertical split): one device copy serves both
     // how a step ends (OUT_UPDATE, OUT_ACCUM with del_ttf per tracer [ntr][CN],
     // OUT_TEND into tend_h / tend_v [CN]); dense_t stages a caller's tendency
@@ -1428,14 +1515,14 @@ static void build_node_records(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, 
     std::vector<NodeEdgeInfo<D>> ni(N);
     std::vector<NodeRing<D>> rg(N);
     std::vector<NodeElems<D>> nt(N);
-    std::vector<R> na(D * (size_t)N, (R)0);
+    std::vector<R> gw((size_t)8 * D * N, (R)0);
     std::vector<int> ring, fan;
     for (int n = 0; n < N; ++n) {
         const int deg = m.n2e_ptr[n + 1] - m.n2e_ptr[n];
         for (int j = 0; j < D; ++j) {
             ne[n].oe[j] = -1; ne[n].oo[j] = 0; ni[n].info[j] = 0;
             rg[n].oo[j] = -1; rg[n].lr[j] = 0; rg[n].le[j] = 0;
-            nt[n].ot[j] = -1; nt[n].le[j] = 0;
+            nt[n].ei[j] = -1; nt[n].oa[j] = 0; nt[n].ob[j] = 0; nt[n].le[j] = 0; nt[n].pos[j] = 0;
         }
         for (int j = 0; j < deg; ++j) {
             const int se = m.n2e[m.n2e_ptr[n] + j];
@@ -1447,9 +1534,14 @@ static void build_node_records(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, 
         }
         for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) {
             const int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
-            nt[n].ot[j] = (int)I.lt.off[el];
+            const int *en = &m.elem2D_nodes[3 * (size_t)el];
+            const int ps = en[0] == n ? 0 : (en[1] == n ? 1 : 2);
+            if (en[ps] != n) throw std::runtime_error("nod_in_elem2D lists an element without the node");
+            nt[n].ei[j] = el;
+            nt[n].oa[j] = (int)I.ln.off[en[ps == 0 ? 1 : 0]];
+            nt[n].ob[j] = (int)I.ln.off[en[ps == 2 ? 1 : 2]];
             nt[n].le[j] = (unsigned char)(m.nlevels[el] - 1);
-            na[D * (size_t)n + j] = (R)m.elem_area[el];
+            nt[n].pos[j] = (unsigned char)ps;
         }
         // the cluster bounds of FCT walk the fan; a node without one (a
         // caller-supplied element list that is not a fan) leaves its ring
@@ -1463,6 +1555,20 @@ static void build_node_records(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, 
             rg[n].oo[j] = (int)I.ln.off[ring[j]];
             rg[n].lr[j] = (unsigned char)(m.nlevels_nod2D[ring[j]] - 1);
             rg[n].le[j] = fan[j] >= 0 ? (unsigned char)(m.nlevels[fan[j]] - 1) : 0;
+            // fan element j between ring j and ring j+1: its area-weighted
+            // gradient coefficients of those two nodes
+            const int el = fan[j];
+            if (el < 0) continue;
+            const int ra = ring[j], rb = ring[(j + 1) % ring.size()];
+            const int *en = &m.elem2D_nodes[3 * (size_t)el];
+            const int pa = en[0] == ra ? 0 : (en[1] == ra ? 1 : 2), pb = en[0] == rb ? 0 : (en[1] == rb ? 1 : 2);
+            const double A = m.elem_area[el], *gs = &m.gradient_sca[6 * (size_t)el];
+            R *w = &gw[(size_t)8 * D * n + 8 * j];
+            w[0] = (R)(A * gs[pa]);
+            w[1] = (R)(A * gs[pb]);
+            w[2] = (R)(A * gs[3 + pa]);
+            w[3] = (R)(A * gs[3 + pb]);
+            w[4] = (R)A;
         }
     }
     d.deg = D;
@@ -1470,7 +1576,7 @@ static void build_node_records(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, 
     d.ninfo = I.track(dupload(ni));
     d.nring = I.track(dupload(rg));
     d.nelem = I.track(dupload(nt));
-    d.nelem_area = I.track(dupload(na));
+    d.ngw = I.track(dupload(gw));
 }
 
 template <class R>
@@ -1486,7 +1592,8 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
     d.nlev = I.track(dupload(m.nlevels_nod2D));
     std::vector<EdgeRec> er(E);
     std::vector<int> enlmax(E);
-    std::vector<R> eg(8 * (size_t)E, (R)0);
+    std::vector<R> eg(8 * (size_t)E, (R)0), egud(4 * (size_t)E, (R)0);
+    std::vector<int4> eud(E, make_int4(0, 0, 0, 0));
     for (int e = 0; e < E; ++e) {
         EdgeRec &r = er[e];
         const int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
@@ -1509,15 +1616,42 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
         g[4] = (R)m.edge_dxdy[2 * e];
         g[5] = (R)m.edge_dxdy[2 * e + 1];
         g[6] = (R)a;
+        // the up/down-wind triangles: ex*a*d/dx + ey*r_earth*d/dy of the triangle's
+        // gradient as weights of its node values, the edge's own node first
+        if (up >= 0 && dn >= 0) {
+            const double ex = m.edge_dxdy[2 * e], ey = m.edge_dxdy[2 * e + 1];
+            int off[4];
+            double wt[4];
+            for (int side = 0; side < 2; ++side) {
+                const int tri = side == 0 ? up : dn, own = side == 0 ? n1 : n2;
+                const int *en = &m.elem2D_nodes[3 * (size_t)tri];
+                const int ps = en[0] == own ? 0 : (en[1] == own ? 1 : 2);
+                if (en[ps] != own)
+                    throw std::runtime_error("edge " + std::to_string(e + 1) + ": its " +
+                                             (side == 0 ? "up" : "down") + "-wind triangle lacks the edge's node");
+                const int ia = ps == 0 ? 1 : 0, ib = ps == 2 ? 1 : 2;
+                const double *gs = &m.gradient_sca[6 * (size_t)tri];
+                auto w = [&](int i) { return ex * a * gs[i] + ey * kRE * gs[3 + i]; };
+                wt[2 * side] = w(ia);
+                wt[2 * side + 1] = w(ib);
+                off[2 * side] = (int)I.ln.off[en[ia]];
+                off[2 * side + 1] = (int)I.ln.off[en[ib]];
+            }
+            eud[e] = make_int4(off[0], off[1], off[2], off[3]);
+            for (int q = 0; q < 4; ++q) egud[4 * (size_t)e + q] = (R)wt[q];
+        }
     }
     d.erec = I.track(dupload(er));
     d.egeo = I.track(dupload(eg));
+    d.eud = I.track(dupload(eud));
+    d.egud = I.track(dupload(egud));
     std::vector<ElemRec> tr(T);
     std::vector<R> tg(8 * (size_t)T, (R)0);
     for (int el = 0; el < T; ++el) {
         const int *en = &m.elem2D_nodes[3 * el];
         tr[el] = ElemRec{(int)I.ln.off[en[0]], (int)I.ln.off[en[1]], (int)I.ln.off[en[2]], m.nlevels[el]};
         for (int q = 0; q < 6; ++q) tg[8 * (size_t)el + q] = (R)m.gradient_sca[6 * el + q];
+        tg[8 * (size_t)el + 6] = (R)m.elem_area[el];
     }
     d.trec = I.track(dupload(tr));
     d.tgeo = I.track(dupload(tg));
@@ -1574,7 +1708,6 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     if (prec_ == PREC_FP64) build_dmesh(I, I.m64, m);
     else build_dmesh(I, I.m32, m);
     I.tsn = (size_t)I.ln.total;
-    I.tst = 2 * (size_t)I.lt.total;
     I.tse = (size_t)I.le.total;
     // work arrays hold one tracer batch (at most 4 tracers); batches run in
     // stream order, so every batch reuses them
@@ -1595,7 +1728,6 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     I.Pp = zalloc(I.tsn * I.wslots * rs);
     I.Pm = zalloc(I.tsn * I.wslots * rs);
     I.LO = (double *)zalloc(I.tsn * I.wslots * sizeof(double));
-    I.trxy = zalloc(I.tst * I.wslots * gs);
     I.gnod = zalloc(2 * I.tsn * I.wslots * gs);
     I.dpart = I.track(dalloc<DiagAcc>(I.ndiag));
     I.dout = I.track(dalloc<DiagAcc>(1));
@@ -1914,17 +2046,16 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
                          hipStream_t st, LaunchHook &hk)
 {
     const R *vel = (const R *)I.vel, *W = (const R *)I.W, *WE = (const R *)(I.we_is_w ? I.W : I.WE);
-    const size_t tsn = I.tsn, tst = I.tst, tse = I.tse, tsg = 2 * I.tsn;
+    const size_t tsn = I.tsn, tse = I.tse, tsg = 2 * I.tsn;
     double *vals = I.values + t0 * tsn;
     if (!refresh && !I.valuesAB) throw std::runtime_error("valuesAB was never set");
     const double *ab = refresh ? vals : I.valuesAB + t0 * tsn;
     R *vfl = (R *)I.vfl, *adf_h = (R *)I.adf_h;
     R *adf_v = (R *)I.adf_v, *Pp = (R *)I.Pp, *Pm = (R *)I.Pm;
     double *LO = I.LO;
-    G *trxy = (G *)I.trxy;
     G *gnod = (G *)I.gnod;
     const int wv = I.tune_waves;
-    const dim3 blk(kWave * wv), ge(grid_for(m.nch_e, wv)), gn(grid_for(m.nch_n, wv)), gt(grid_for(m.nch_t, wv));
+    const dim3 blk(kWave * wv), ge(grid_for(m.nch_e, wv)), gn(grid_for(m.nch_n, wv));
     const bool learned = sp.lim == LIM_LEARNED;
     const bool fct = sp.lim == LIM_FCT || learned;
     Engine::Impl::Dump *dump = I.dump;
@@ -1938,18 +2069,19 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
     double *acc_v = out == OUT_TEND ? I.tend_v : nullptr;
     if (out != OUT_UPDATE && !acc_h) throw std::runtime_error("accumulation arrays not allocated");
     if (sp.hor == HOR_MUSCL || sp.hor == HOR_MFCT) {
-        hipLaunchKernelGGL((k_trxy<R, G, NT>), gt, blk, 0, st, m, ab, tsn, trxy, tst, I.gscale);
-        hk.mark("k_trxy", st);
-        hipLaunchKernelGGL((k_gnod<R, G, NT, D>), gn, blk, 0, st, m, trxy, tst, gnod, tsg, I.gscale, I.ginv);
+        if (I.grad_fan && I.fan_fault < 0)
+            hipLaunchKernelGGL((k_gnod_fan<R, G, NT, D>), gn, blk, 0, st, m, ab, tsn, gnod, tsg, I.gscale);
+        else
+            hipLaunchKernelGGL((k_gnod<R, G, NT, D>), gn, blk, 0, st, m, ab, tsn, gnod, tsg, I.gscale);
         hk.mark("k_gnod", st);
     }
 #define EDGE(HOR, F)                                                                                        \
     do {                                                                                                    \
         if (refresh)                                                                                        \
-            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, true>), ge, blk, 0, st, m, vel, vals, ab, tsn, trxy, tst, \
+            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, true>), ge, blk, 0, st, m, vel, vals, ab, tsn,             \
                                gnod, tsg, ph, I.ginv, vfl, adf_h, tse);                                 \
         else                                                                                                \
-            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, false>), ge, blk, 0, st, m, vel, vals, ab, tsn, trxy, tst, \
+            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, false>), ge, blk, 0, st, m, vel, vals, ab, tsn,            \
                                gnod, tsg, ph, I.ginv, vfl, adf_h, tse);                                 \
     } while (0)
     if (fct) {
@@ -2316,6 +2448,12 @@ double Engine::learn_fallback(int tr)
     HIP_OK(hipMemcpyAsync(h, I.cnt, sizeof(h), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
     return h[1] ? (double)h[0] / (double)h[1] : 0.0;
+}
+
+void Engine::set_gradient_fan(bool on)
+{
+    p_->grad_fan = on;
+    graph_valid_ = false;
 }
 
 void Engine::set_schedule(int waves, int max_nt)

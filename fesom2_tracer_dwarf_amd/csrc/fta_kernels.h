@@ -84,12 +84,17 @@ struct alignas(16) NodeRing {
     unsigned char le[D];
 };
 
-// node -> elements (nod_in_elem2D order) for the nodal mean gradient:
-// ot = element layer offset (-1 = empty), le = element layer count
+// node -> elements (nod_in_elem2D order) for the nodal mean gradient, formed
+// from the node values: ei = element index (-1 = empty), oa / ob = layer
+// offsets of the element's other two nodes in its elem2D_nodes order,
+// le = element layer count, pos = this node's place (0..2) in elem2D_nodes
 template <int D>
 struct alignas(16) NodeElems {
-    int ot[D];
+    int ei[D];
+    int oa[D];
+    int ob[D];
     unsigned char le[D];
+    unsigned char pos[D];
 };
 
 template <class R>
@@ -102,13 +107,27 @@ struct DMesh {
     const EdgeRec *erec;          // (E)
     const R *egeo;                // (8,E): dx1 dy1 dx2 dy2 (edge_cross_dxdy) ex ey (edge_dxdy) a
     const ElemRec *trec;          // (T)
-    const R *tgeo;                // (8,T): gradient_sca(1:6), 0, 0
+    const R *tgeo;                // (8,T): gradient_sca(1:6), elem_area, 0
+    // MUSCL on the levels where the up/down-wind triangles give the gradient:
+    // eud = layer offsets of the up-wind triangle's two nodes a, b other than
+    // edges(1) and of the down-wind triangle's two nodes c, d other than
+    // edges(2); egud = (4,E): weights with which ex*a*d/dx + ey*r_earth*d/dy of
+    // the up-wind triangle's gradient is w0 (T_a - T_1) + w1 (T_b - T_1) and of
+    // the down-wind one w2 (T_c - T_2) + w3 (T_d - T_2) (a triangle's gradient
+    // coefficients sum to zero, so the edge's own node enters through the
+    // differences)
+    const int4 *eud;              // (E)
+    const R *egud;                // (4,E)
+    // the nodal mean gradient over a node's fan: slot j (the element between
+    // ring nodes j and j+1) holds A*c of ring j and ring j+1 for d/dx, the same
+    // for d/dy, and A (elem_area), with c the element's gradient coefficients,
+    // so that its gradient times A is the weighted differences to this node
+    const R *ngw;                 // (8*D,N), unused slots 0
     int deg;                      // node record width D (kDeg or kDegWide)
     const void *nedge;            // (N) NodeEdges<D>
     const void *ninfo;            // (N) NodeEdgeInfo<D>
     const void *nring;            // (N) NodeRing<D>
     const void *nelem;            // (N) NodeElems<D>
-    const R *nelem_area;          // (D,N): elem_area of the node's elements
     // node layout (interfaces 1..nlev and layers 1..nlev-1 share a column)
     const R *area, *zbarn;        // interface arrays
     const R *areasvol, *hnode, *hnode_new, *Zn;  // layer arrays

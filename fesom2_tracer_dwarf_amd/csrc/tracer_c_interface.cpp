@@ -44,6 +44,7 @@ struct State {
     std::vector<std::string> mesh_given;  // arrays set since tracer_mesh_begin
     int fan_fault = -1;                // first node without an element fan (caller-supplied meshes)
     bool accumulate = false;           // del_ttf carried over between steps
+    bool grad_fan = true;              // nodal gradients over the node fans
 };
 
 State &S()
@@ -94,6 +95,7 @@ int ensure_engine()
         if (!s.mlp.empty()) s.eng->mlp_set(s.mlp.data());
         s.eng->set_schedule(s.tune_waves, s.tune_nt);
         if (s.accumulate) s.eng->set_accumulate(true);
+        if (!s.grad_fan) s.eng->set_gradient_fan(false);
         for (int t = 0; t < s.ntr; ++t) {
             s.eng->set_spec(t, s.specs[t]);
             s.eng->set_values(t, s.values[t].data());
@@ -946,6 +948,11 @@ int tracer_set_option(const char *name, int value)
     }
     if (n == "use_graph") {
         s.use_graph = value ? 1 : 0;
+        return 0;
+    }
+    if (n == "gradient_fan") {  // nodal mean gradients over the node fans (1, default) or element by element
+        s.grad_fan = value != 0;
+        if (s.eng) s.eng->set_gradient_fan(s.grad_fan);
         return 0;
     }
     if (n == "accumulate_del_ttf") {  // del_ttf carried over between steps (reference C interface, Q6)

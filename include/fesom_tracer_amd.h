@@ -182,6 +182,10 @@ int tracer_get_values_id(int tracer_id, double *values, int nz, int nn);
  *                        when the device state is rebuilt, e.g. by a precision
  *                        switch); not with the LEARNED limiter
  *   "use_graph"          1 (default): replay multi-step runs from a hipGraph
+ *   "gradient_fan"       1 (default): MUSCL's nodal mean gradients are formed
+ *                        over each node's fan of elements (when every node
+ *                        has one); 0: element by element (the same values to
+ *                        rounding)
  *   "block_waves"        4 (default): waves per workgroup of the chunk kernels (1, 2, 4, 8)
  *   "batch_tracers"      4 (default): most tracers per launch (1, 2, 4)
  *   "force_rotation"     1 (default): tracer_load_mesh_from_files rotates the

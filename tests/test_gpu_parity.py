@@ -273,3 +273,34 @@ def test_new_layer_thickness_matches_oracle(api, schemes):
             o.step(dt, uv, w, we, ref[i], hor=schemes[0], ver=schemes[1], lim=schemes[2])
             err = rel_err(api.get_tracer_values(i + 1), ref[i])
             assert err <= RTOL, f"tracer {i + 1} step {step + 1}: rel err {err:.3e}"
+
+
+@pytest.mark.parametrize("mesh_name", ["periodic_24x20x12_random_bottom", "closed_16x14x10_random_bottom"])
+@pytest.mark.parametrize("schemes", [("MUSCL", "QR4C", "FCT"), ("MFCT", "PPM", "NONE")], ids=lambda s: "-".join(s))
+def test_elementwise_gradient_path_matches_oracle(api, mesh_name, schemes):
+    """The nodal mean gradients formed element by element (the path of meshes
+    whose nodes lack an element fan) against the oracle, and equal to the fan
+    path to rounding."""
+    import oracle as orc
+    print(f"[gpu] element-wise nodal gradients {mesh_name} {schemes}", flush=True)
+    d, dt, trs = setup_case(api, MESHES[mesh_name], schemes)
+    api.set_option("gradient_fan", 0)
+    try:
+        m = mesh_dict(api)
+        o = orc.Oracle(m)
+        uv, w, we = api.mesh_array("uv"), api.mesh_array("w"), api.mesh_array("w_e")
+        ref = [np.array(t, order="F", copy=True) for t in trs]
+        for step in range(2):
+            api.advect_steps(1, dt)
+            for i in range(len(ref)):
+                o.step(dt, uv, w, we, ref[i], hor=schemes[0], ver=schemes[1], lim=schemes[2])
+                assert rel_err(api.get_tracer_values(i + 1), ref[i]) <= RTOL
+        elem = [api.get_tracer_values(i + 1) for i in range(len(ref))]
+        for i, t in enumerate(trs):
+            api.set_values(i + 1, t)
+        api.set_option("gradient_fan", 1)
+        api.advect_steps(2, dt)
+        for i in range(len(ref)):
+            assert rel_err(api.get_tracer_values(i + 1), elem[i]) <= 1e-13
+    finally:
+        api.set_option("gradient_fan", 1)
