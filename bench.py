@@ -21,7 +21,9 @@ FETCH_SIZE + WRITE_SIZE, copy-calibrated), the busiest kernel's own figure and
 the rate of the library's hand-written streaming copy (the practical ceiling)
 are reported beside it. CPU baseline: the oracle (a C restatement of the
 reference's FP64 path) on the host cores -- all cores through its OpenMP build,
-one thread, and the 20x20x10 analytic case -- on bounded samples.
+one thread, and the 20x20x10 analytic case -- on bounded samples. Beside the
+headline the line also carries the batched form (8 meshes x 32 tracers, one
+graph launch per step) and the learned limiter's step time against Zalesak's.
 
 Prints ONE JSON line.
 """
@@ -212,6 +214,90 @@ def cpu_baseline(api, dt, ntr, c, max_seconds=20.0):
                                     f"serial oracle ({t0 * 1e3:.1f} ms)"))
 
 
+BATCH_SIZES = [(256, 32), (384, 40), (512, 48), (640, 56), (768, 64), (896, 48), (960, 40), (1024, 64)]
+
+
+def batched_form(steps):
+    """The batched form: 8 periodic meshes of 256^2 .. 1024^2 nodes with 32..64
+    levels, 32 tracers each (T, S, blobs, 0/1 steps, white noise), FP32,
+    MUSCL+QR4C+FCT, one graph launch per step for all meshes."""
+    from fesom2_tracer_dwarf_amd.batch import TracerBatch
+    t0 = time.perf_counter()
+    b = TracerBatch()
+    cells = 0
+    for k, (n, nl) in enumerate(BATCH_SIZES):
+        i, _ = b.add_periodic_mesh(n, n, nl, 2.0e3 * n, 2.0e3 * n, seed=10 + k, ntracers=32, precision=32)
+        cells += b.info(i)["active_cells"]
+    setup = time.perf_counter() - t0
+    b.run(2)  # warm-up (captures the graph)
+    ms = b.time(steps)
+    b.close()
+    return dict(workload="8 periodic meshes 256^2..1024^2 nodes, 32..64 levels, 32 tracers each, FP32",
+                steps=steps, ms_per_step=ms / steps, updates_per_s=32 * cells * steps / (ms * 1e-3),
+                active_cells=cells, graph_launches_per_step=1, setup_s=setup)
+
+
+def learned_form(api, steps):
+    """The learned limiter on the 1000x1000x64 mesh with 4 tracers (T, S, a
+    0/1 step, a blob), FP32, after training on the FCT's own pairs of a
+    256x256x32 mesh of random fields; timed beside Zalesak's FCT in FP64 and
+    FP32 from the same state, and its distance to Zalesak FP32 after the
+    same number of steps."""
+    from fesom2_tracer_dwarf_amd import configs
+    t0 = time.perf_counter()
+    api.generate_mesh(256, 256, 32, periodic=True, Lx=5.12e5, Ly=5.12e5, dz=10.0, lev_lo=3, lev_hi=32, seed=5,
+                      sfc=True)
+    api.set_option("precision", 32)
+    api.allocate_tracers(4)
+    dtt = api.make_velocity(0.1, 0.05, 0.2, 0.5, 0.3)
+    coord = api.mesh_array("coord")
+    rng = np.random.default_rng(123)
+    fields = [configs.tracer_noise(coord, 31, rng), configs.tracer_step(coord, 31, 5.12e5, 5.12e5, rng),
+              configs.tracer_blob(coord, 31, 5.12e5, 5.12e5, rng, depth_decay=False),
+              configs.tracer_noise(coord, 31, rng)]
+    for i, f in enumerate(fields):
+        api.set_values(i + 1, f)
+        api.set_schemes(i + 1, "MUSCL", "QR4C", "FCT")
+    api.learn_clear()
+    rows = api.learn_collect(2, dtt, 40_000_000)
+    losses = api.learn_train(4, batch=65536, lr=2e-3, seed=0)
+    params = api.mlp_params().copy()
+    t_train = time.perf_counter() - t0
+    cfg = configs.WORKLOADS["periodic_1000x1000x64"]
+    api.generate_mesh(cfg["nx"], cfg["ny"], cfg["nl"], periodic=True, Lx=cfg["Lx"], Ly=cfg["Ly"], dz=cfg["dz"],
+                      lev_lo=cfg["lev"][0], lev_hi=cfg["lev"][1], seed=0, sfc=True)
+    api.allocate_tracers(4)
+    dt = api.make_velocity(0.1, 0.05, 0.2, 0.5, 0.3)
+    trs = configs.initial_tracers(api.mesh_array("coord"), cfg["nl"], cfg["Lx"], cfg["Ly"], 4, seed=0)
+    api.set_mlp_params(params)
+    C = int((api.mesh_array("nlevels_nod2D") - 1).sum())
+
+    def run(prec, lim):
+        api.set_option("precision", prec)
+        for i, t in enumerate(trs):
+            api.set_values(i + 1, t)
+            api.set_schemes(i + 1, "MUSCL", "QR4C", lim)
+        api.advect_steps(steps, dt)
+        vals = [api.get_tracer_values(i + 1) for i in range(4)]
+        fb = [api.learn_fallback(i + 1) for i in range(4)] if lim == "LEARNED" else None
+        ms = api.time_steps(steps, dt) / steps
+        return vals, fb, ms
+
+    _, _, ms64 = run(64, "FCT")
+    z32, _, ms32 = run(32, "FCT")
+    ln, fb, msl = run(32, "LEARNED")
+    api.set_option("precision", 64)
+    return dict(workload="periodic_1000x1000x64, 4 tracers (T, S, 0/1 step, blob), FP32", steps=steps,
+                training=dict(mesh="periodic_256x256x32 random fields", rows=int(rows), epochs=4,
+                              loss_first_last=[float(losses[0]), float(losses[-1])], seconds=t_train),
+                ms_per_step=dict(zalesak_fp64=ms64, zalesak_fp32=ms32, learned_fp32=msl),
+                updates_per_s=dict(zalesak_fp64=4 * C / (ms64 * 1e-3), zalesak_fp32=4 * C / (ms32 * 1e-3),
+                                   learned_fp32=4 * C / (msl * 1e-3)),
+                learned_speed_over_fp64=ms64 / msl, learned_speed_over_zalesak_fp32=ms32 / msl,
+                l2_rel_to_zalesak_fp32=[float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(ln, z32)],
+                fallback_fraction=fb)
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -228,7 +314,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedule (no autotuning)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the host round trip and the copy ceiling (profiled runs of the step kernels)")
+                    help="skip the host round trip, the copy ceiling, the batched form and the learned limiter "
+                         "(profiled runs of the step kernels)")
+    ap.add_argument("--extra-steps", type=int, default=10,
+                    help="timed steps of the batched form and of the learned-limiter comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     ap.add_argument("--copy-bandwidth", action="store_true",
@@ -359,6 +448,15 @@ def main():
         line["cpu_baseline"] = cpu_baseline(api, dt, ntr, c, args.cpu_seconds)
     else:
         line["cpu_baseline"] = None
+    if not args.no_extras:
+        log("batched form (8 meshes x 32 tracers, FP32)")
+        line["batched_form"] = batched_form(args.extra_steps)
+        line["batched_form"]["per_cell_rate_over_single_mesh_fp32"] = \
+            line["batched_form"]["updates_per_s"] / res["fp32"]["updates_per_s"]
+        log(f"batched: {line['batched_form']['ms_per_step']:.1f} ms/step")
+        log("learned limiter (1000x1000x64, 4 tracers)")
+        line["learned_limiter"] = learned_form(api, args.extra_steps)
+        log(f"learned: {line['learned_limiter']['ms_per_step']}")
     print(json.dumps(line))
 
 

@@ -1802,7 +1802,8 @@ void Engine::set_spec(int tr, const TracerSpec &sp)
             return p;
         };
         I.alpha = zalloc(I.tse * I.wslots * rs);
-        I.unsafe = zalloc(I.tsn * I.wslots * rs);
+        // the fallback flags stay per tracer (tracer_learn_fallback reads them after the step)
+        I.unsafe = zalloc(I.tsn * ntr_ * rs);
         if (!I.Qp) {
             I.Qp = zalloc(I.tsn * I.wslots * rs);
             I.Qm = zalloc(I.tsn * I.wslots * rs);
@@ -2125,7 +2126,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
                                Sp, Sm);
         hk.mark("k_node_limit", st);
         const LearnArgs<R, NT> la{vals, LO, adf_h, vfl, Qp, Qm, Pp, Pm, Sp, Sm, tsn, tse};
-        R *alpha = learned ? (R *)I.alpha : nullptr, *unsafe = learned ? (R *)I.unsafe : nullptr;
+        R *alpha = learned ? (R *)I.alpha : nullptr, *unsafe = learned ? (R *)I.unsafe + t0 * tsn : nullptr;
         if (dump) {
             if (dump->row + (long long)NT * m.CE <= dump->cap) {
                 hipLaunchKernelGGL((k_learn_dump<R, NT>), ge, blk, 0, st, m, la, dt, dump->X, dump->Y, dump->row);
@@ -2438,7 +2439,7 @@ double Engine::learn_fallback(int tr)
     if (!I.unsafe || tr < 0 || tr >= ntr_) return 0.0;
     HIP_OK(hipMemsetAsync(I.cnt, 0, 2 * sizeof(unsigned long long), stream_));
     const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
-    const char *u = (const char *)I.unsafe + work_slot(spec_, tr, p_->tune_nt) * I.tsn * rs;
+    const char *u = (const char *)I.unsafe + (size_t)tr * I.tsn * rs;
     if (prec_ == PREC_FP64)
         hipLaunchKernelGGL((k_count_flags<double>), dim3(256), dim3(256), 0, stream_, I.m64, (const double *)u, I.cnt);
     else

@@ -113,3 +113,33 @@ def test_training_gains_over_untrained_and_low_order(api):
     assert res["trained"][0] < res["low_order"][0]
     api.set_mlp_params(untrained)
     api.set_option("precision", 64)
+
+
+def test_fallback_fraction_is_per_tracer(api):
+    """tracer_learn_fallback reports each tracer's own flags, whichever launch
+    batch (and slot within it) the tracer ran in: with two tracers per launch,
+    fields [A, B, A, C] report fractions [a, b, a, c] with b != c (the flags of
+    one launch slot are not overwritten by the next launch), and the fields
+    permuted to [C, A, B, A] report [c, a, b, a]."""
+    print("[gpu] learned fallback per tracer", flush=True)
+    d, dt, trs = setup_case(api, MESH, ("MUSCL", "QR4C", "LEARNED"), ntr=4, seed=4)
+    api.set_option("precision", 32)
+    api.set_option("batch_tracers", 2)
+    try:
+        A, B, C = trs[2], np.full_like(trs[0], 3.5), trs[3]
+
+        def fractions(fields):
+            for i, f in enumerate(fields):
+                api.set_values(i + 1, f)
+            api.advect_steps(1, dt)
+            return [api.learn_fallback(i + 1) for i in range(4)]
+
+        fb = fractions([A, B, A, C])
+        print(f"[gpu]   fallback fractions {fb}", flush=True)
+        assert fb[0] == fb[2] and fb[1] != fb[3]
+        fb2 = fractions([C, A, B, A])
+        print(f"[gpu]   permuted {fb2}", flush=True)
+        assert fb2 == [fb[3], fb[0], fb[1], fb[0]]
+    finally:
+        api.set_option("batch_tracers", 4)
+        api.set_option("precision", 64)
