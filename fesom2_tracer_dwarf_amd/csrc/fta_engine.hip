@@ -195,6 +195,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const double *__
 // is the weighted sum of its two ring values' differences to this node's
 // value (a triangle's gradient coefficients sum to zero). Sums in fan order;
 // the result equals the element-by-element form to rounding.
+// the same mean over the node's fan (ring node j and j+1 bound element j):
+// every ring value is loaded once, and each element's gradient times its area
+// is the weighted sum of its two ring values' differences to this node's
+// value (a triangle's gradient coefficients sum to zero). Sums in fan order;
+// the result equals the element-by-element form to rounding.
 template <class R, class G, int NT, int D>
 __global__ __launch_bounds__(kMaxBlock) void k_gnod_fan(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
                                                      G *__restrict__ gnod, size_t tsg, float gscale)
@@ -285,7 +290,7 @@ __device__ __forceinline__ double lo_flux(double v, double T1, double T2)
 }
 
 template <class R, class G, int HOR, bool FCT, int NT, bool SAME>
-__global__ __launch_bounds__(kMaxBlock) void k_edge(DMesh<R> m, const R *__restrict__ vel,
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && NT <= 2 ? 7 : 1))) void k_edge(DMesh<R> m, const R *__restrict__ vel,
                                                  const double *__restrict__ ttf,
                                                  const double *__restrict__ ttfAB, size_t tsn,
                                                  const G *__restrict__ gnod, size_t tsg,
@@ -295,14 +300,41 @@ __global__ __launch_bounds__(kMaxBlock) void k_edge(DMesh<R> m, const R *__restr
     const int ch = wave_chunk();
     if (ch >= m.nch_e) return;
     const Pair p = lane_pair(m.slot_e, ch);
+    // the chunk's edges are consecutive columns; when there are at most
+    // kStageEdges of them the wave loads their packed records into its LDS
+    // slice with cooperative 16-byte loads (one instruction moves 64 units)
+    // and every lane reads its edge's record from there
+    constexpr int U = kEdgeUnits<R>;
+    extern __shared__ uint4 stage_all[];
+    uint4 *stage = stage_all + (threadIdx.x >> 6) * (kStageEdges * U);
+    const StageRange sr = stage_range(p, kStageEdges);
+    if (sr.staged) {
+        stage_copy<U>(m.epack, stage, sr);
+        stage_sync();
+    }
     if (!p.live) return;
-    const EdgeRec er = m.erec[p.col];
+    uint4 rec[U];
+    if (sr.staged) {
+        const uint4 *q = stage + (p.col - sr.c0) * U;
+#pragma unroll
+        for (int u = 0; u < U; ++u) rec[u] = q[u];
+    } else {
+        const uint4 *q = m.epack + (size_t)p.col * U;
+#pragma unroll
+        for (int u = 0; u < U; ++u) rec[u] = q[u];
+    }
+    EdgeRec er;
+    R geo[8], gud[4];
+    int4 eud4;
+    __builtin_memcpy(&er, &rec[0], 32);
+    __builtin_memcpy(geo, &rec[2], 8 * sizeof(R));
+    __builtin_memcpy(&eud4, reinterpret_cast<const char *>(rec) + 32 + 8 * sizeof(R), 16);
+    __builtin_memcpy(gud, reinterpret_cast<const char *>(rec) + 48 + 8 * sizeof(R), 4 * sizeof(R));
     const int nl1 = er.nl & 255, nl2 = (er.nl >> 8) & 255, nzmax_e = (er.nl >> 16) & 255;
     const int nlmax = (int)((unsigned)er.nl >> 24);
     const int nl12 = min(nl1, nl2);
     const int k0 = p.k0, nz0 = k0 + 1, nz1 = k0 + 2;
     const bool valid1 = nz1 <= nlmax;
-    const R *geo = m.egeo + 8 * (size_t)p.col;
     const R dx1 = geo[0], dy1 = geo[1], dx2 = geo[2], dy2 = geo[3];
     const bool el2 = er.t2 >= 0;
     const bool h10 = nz0 <= nl1, h11 = valid1 && nz1 <= nl1;
@@ -330,9 +362,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_edge(DMesh<R> m, const R *__restr
     int4 ud{0, 0, 0, 0};
     V4<R> wud{0, 0, 0, 0};
     if constexpr (HOR != HOR_UPW1) {
-        // independent of the edge record: issued with it
-        ud = m.eud[p.col];
-        wud = ld4(m.egud + 4 * (size_t)p.col);
+        ud = eud4;
+        wud = V4<R>{gud[0], gud[1], gud[2], gud[3]};
         ex = geo[4]; ey = geo[5]; a = geo[6];
         if constexpr (HOR == HOR_MUSCL) {
             const int nb1 = er.nb & 0xFFFF, nb2 = er.nb >> 16;
@@ -1645,6 +1676,19 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
     d.egeo = I.track(dupload(eg));
     d.eud = I.track(dupload(eud));
     d.egud = I.track(dupload(egud));
+    {
+        constexpr int U = kEdgeUnits<R>;
+        static_assert(sizeof(EdgeRec) == 32, "EdgeRec is two 16-byte units");
+        std::vector<uint4> pk((size_t)U * E);
+        for (int e = 0; e < E; ++e) {
+            char *q = reinterpret_cast<char *>(&pk[(size_t)U * e]);
+            std::memcpy(q, &er[e], 32);
+            std::memcpy(q + 32, &eg[8 * (size_t)e], 8 * sizeof(R));
+            std::memcpy(q + 32 + 8 * sizeof(R), &eud[e], 16);
+            std::memcpy(q + 48 + 8 * sizeof(R), &egud[4 * (size_t)e], 4 * sizeof(R));
+        }
+        d.epack = I.track(dupload(pk));
+    }
     std::vector<ElemRec> tr(T);
     std::vector<R> tg(8 * (size_t)T, (R)0);
     for (int el = 0; el < T; ++el) {
@@ -2076,13 +2120,14 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
             hipLaunchKernelGGL((k_gnod<R, G, NT, D>), gn, blk, 0, st, m, ab, tsn, gnod, tsg, I.gscale);
         hk.mark("k_gnod", st);
     }
+    const size_t elds = (size_t)wv * kStageEdges * kEdgeUnits<R> * sizeof(uint4);  // k_edge's LDS slices
 #define EDGE(HOR, F)                                                                                        \
     do {                                                                                                    \
         if (refresh)                                                                                        \
-            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, true>), ge, blk, 0, st, m, vel, vals, ab, tsn,             \
+            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, true>), ge, blk, elds, st, m, vel, vals, ab, tsn,          \
                                gnod, tsg, ph, I.ginv, vfl, adf_h, tse);                                 \
         else                                                                                                \
-            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, false>), ge, blk, 0, st, m, vel, vals, ab, tsn,            \
+            hipLaunchKernelGGL((k_edge<R, G, HOR, F, NT, false>), ge, blk, elds, st, m, vel, vals, ab, tsn,         \
                                gnod, tsg, ph, I.ginv, vfl, adf_h, tse);                                 \
     } while (0)
     if (fct) {

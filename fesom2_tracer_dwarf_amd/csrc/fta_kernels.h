@@ -97,6 +97,13 @@ struct alignas(16) NodeElems {
     unsigned char pos[D];
 };
 
+// 16-byte units of one edge's packed record: EdgeRec (2), egeo (8 R),
+// eud (1), egud (4 R)
+template <class R>
+constexpr int kEdgeUnits = 3 + (12 * (int)sizeof(R)) / 16;
+// edges a wave stages into LDS (chunks with more fall back to global loads)
+constexpr int kStageEdges = 16;
+
 template <class R>
 struct DMesh {
     int N, T, E, nl;
@@ -118,6 +125,10 @@ struct DMesh {
     // differences)
     const int4 *eud;              // (E)
     const R *egud;                // (4,E)
+    // the same per-edge records packed edge by edge in 16-byte units (erec,
+    // egeo, eud, egud; kEdgeUnits<R> per edge), so that a wave can stage its
+    // chunk's edges into LDS with a few cooperative loads
+    const uint4 *epack;           // (kEdgeUnits<R>, E)
     // the nodal mean gradient over a node's fan: slot j (the element between
     // ring nodes j and j+1) holds A*c of ring j and ring j+1 for d/dx, the same
     // for d/dy, and A (elem_area), with c the element's gradient coefficients,
@@ -189,6 +200,45 @@ __device__ __forceinline__ Pair lane_pair(const int *slot, int chunk)
     p.k0 = p.live ? 2 * (int)((unsigned)w >> kColBits) : 0;
     p.c = 2 * s;
     return p;
+}
+
+// the columns of a chunk are consecutive: [c0, c0 + ncol). A wave stages
+// per-column records into its LDS slice only when it can see that range
+// whole (lane 0 live, every live lane inside it) and ncol <= cap; the result
+// is wave-uniform
+struct StageRange {
+    int c0, ncol;
+    bool staged;
+};
+
+__device__ __forceinline__ StageRange stage_range(const Pair &p, int cap)
+{
+    const unsigned long long lv = __ballot(p.live);
+    StageRange r;
+    r.c0 = __builtin_amdgcn_readfirstlane(p.col);
+    const int cmax = __builtin_amdgcn_readfirstlane(__shfl(p.col, __popcll(lv) - 1));
+    r.ncol = cmax - r.c0 + 1;
+    bool ok = (lv & 1ull) && cmax >= r.c0 && r.ncol <= cap;
+    ok = ok && __ballot(p.live && (p.col < r.c0 || p.col > cmax)) == 0ull;
+    r.staged = ok;
+    return r;
+}
+
+// copies the packed records (U 16-byte units per column) of the range into
+// the wave's slice with cooperative loads; call stage_sync() before reading
+template <int U>
+__device__ __forceinline__ void stage_copy(const uint4 *__restrict__ pack, uint4 *slice, const StageRange &r)
+{
+    const uint4 *src = pack + (size_t)r.c0 * U;
+    const int nu = r.ncol * U;
+    for (int u = (int)(threadIdx.x & 63); u < nu; u += 64) slice[u] = src[u];
+}
+
+__device__ __forceinline__ void stage_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Fortran max/min as the reference compiles them (a > b ? a : b)
