@@ -45,48 +45,84 @@ void set_error(const std::string &s) { g_last_error = s; }
 __device__ __forceinline__ int info_nlmax(unsigned i) { return (int)(i & 63u); }
 __device__ __forceinline__ bool info_pos(unsigned i) { return (i & 64u) != 0; }
 
+// node -> edge records from their 16-byte units (global memory or LDS)
+template <int D>
+__device__ __forceinline__ void decode_node_edges(const int4 *qe, const uint4 *qi, int (&oe)[D], unsigned (&info)[D])
+{
+#pragma unroll
+    for (int h = 0; h < D / 4; ++h) {
+        const int4 a = qe[h];
+        oe[4 * h] = a.x; oe[4 * h + 1] = a.y; oe[4 * h + 2] = a.z; oe[4 * h + 3] = a.w;
+    }
+#pragma unroll
+    for (int h = 0; h < D / 8; ++h) {
+        const uint4 v = qi[h];
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            info[8 * h + 2 * j] = w[j] & 0xFFFFu;
+            info[8 * h + 2 * j + 1] = w[j] >> 16;
+        }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void decode_other_nodes(const int4 *qo, int (&oo)[D])
+{
+#pragma unroll
+    for (int h = 0; h < D / 4; ++h) {
+        const int4 a = qo[h];
+        oo[4 * h] = a.x; oo[4 * h + 1] = a.y; oo[4 * h + 2] = a.z; oo[4 * h + 3] = a.w;
+    }
+}
+
+// the chunk's node -> edge records (NodeEdges, then NodeEdgeInfo, per node)
+// staged into the wave's LDS slice when the chunk has at most kStageNodes
+// nodes; the wave must call this before any lane leaves
+template <int D, class R>
+__device__ __forceinline__ StageRange stage_node_edges(const DMesh<R> &m, const Pair &p, uint4 *slice)
+{
+    constexpr int UE = (int)sizeof(NodeEdges<D>) / 16, UI = (int)sizeof(NodeEdgeInfo<D>) / 16;
+    const StageRange sr = stage_range(p, kStageNodes);
+    if (sr.staged) {
+        stage_copy<UE>(static_cast<const uint4 *>(m.nedge), slice, sr);
+        stage_copy<UI>(static_cast<const uint4 *>(m.ninfo), slice + kStageNodes * UE, sr);
+        stage_sync();
+    }
+    return sr;
+}
+
+// incident edges of node n (edge-index order) and, with OO, their other nodes
+template <int D, bool OO, class R>
+__device__ __forceinline__ void load_node_edges(const DMesh<R> &m, const StageRange &sr, const uint4 *slice, int n,
+                                                bool live, int (&oe)[D], unsigned (&info)[D], int (&oo)[D])
+{
+    constexpr int UE = (int)sizeof(NodeEdges<D>) / 16, UI = (int)sizeof(NodeEdgeInfo<D>) / 16;
+    if (live) {
+        if (sr.staged) {
+            const uint4 *q = slice + (n - sr.c0) * UE;
+            decode_node_edges<D>(reinterpret_cast<const int4 *>(q), slice + kStageNodes * UE + (n - sr.c0) * UI, oe,
+                                 info);
+            if constexpr (OO) decode_other_nodes<D>(reinterpret_cast<const int4 *>(q) + D / 4, oo);
+        } else {
+            const int4 *q = reinterpret_cast<const int4 *>(&node_edges<D>(m)[n]);
+            decode_node_edges<D>(q, reinterpret_cast<const uint4 *>(&node_info<D>(m)[n]), oe, info);
+            if constexpr (OO) decode_other_nodes<D>(q + D / 4, oo);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < D; ++j) { oe[j] = -1; info[j] = 0; oo[j] = 0; }
+    }
+}
+
+// records without staging (kernels that do not stage)
 template <int D, class R>
 __device__ __forceinline__ void load_node_edges(const DMesh<R> &m, int n, bool live, int (&oe)[D],
                                                 unsigned (&info)[D])
 {
-    if (live) {
-        const int4 *q = reinterpret_cast<const int4 *>(&node_edges<D>(m)[n]);
-#pragma unroll
-        for (int h = 0; h < D / 4; ++h) {
-            const int4 a = q[h];
-            oe[4 * h] = a.x; oe[4 * h + 1] = a.y; oe[4 * h + 2] = a.z; oe[4 * h + 3] = a.w;
-        }
-        const uint4 *u = reinterpret_cast<const uint4 *>(&node_info<D>(m)[n]);
-#pragma unroll
-        for (int h = 0; h < D / 8; ++h) {
-            const uint4 v = u[h];
-            const unsigned w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                info[8 * h + 2 * j] = w[j] & 0xFFFFu;
-                info[8 * h + 2 * j + 1] = w[j] >> 16;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < D; ++j) { oe[j] = -1; info[j] = 0; }
-    }
-}
-
-template <int D, class R>
-__device__ __forceinline__ void load_other_nodes(const DMesh<R> &m, int n, bool live, int (&oo)[D])
-{
-    if (live) {
-        const int4 *q = reinterpret_cast<const int4 *>(&node_edges<D>(m)[n]) + D / 4;
-#pragma unroll
-        for (int h = 0; h < D / 4; ++h) {
-            const int4 a = q[h];
-            oo[4 * h] = a.x; oo[4 * h + 1] = a.y; oo[4 * h + 2] = a.z; oo[4 * h + 3] = a.w;
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < D; ++j) oo[j] = 0;
-    }
+    int oo[D];
+    const StageRange none{0, 0, false};
+    load_node_edges<D, false>(m, none, nullptr, n, live, oe, info, oo);
 }
 
 // ===========================================================================
@@ -491,8 +527,10 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
     const size_t c = (size_t)p.c;
     int oe[D], oo[D];
     unsigned info[D];
-    load_node_edges<D>(m, p.col, p.live && lay0, oe, info);
-    load_other_nodes<D>(m, p.col, p.live && lay0, oo);
+    extern __shared__ uint4 stage_all[];
+    uint4 *slice = stage_all + (threadIdx.x >> 6) * (kStageNodes * kNodeEdgeUnits<D>);
+    const StageRange sr = stage_node_edges<D>(m, p, slice);
+    load_node_edges<D, true>(m, sr, slice, p.col, p.live && lay0, oe, info, oo);
     // the LO vertical flux in FP64 (see lo_flux)
     VCol2<double> lc{};
     lc.nlev = nlev;
@@ -801,8 +839,10 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
     const size_t c = (size_t)p.c;
     int oe[D], oo[D];
     unsigned info[D];
-    load_node_edges<D>(m, p.col, lay0, oe, info);
-    load_other_nodes<D>(m, p.col, lay0, oo);
+    extern __shared__ uint4 stage_all[];
+    uint4 *slice = stage_all + (threadIdx.x >> 6) * (kStageNodes * kNodeEdgeUnits<D>);
+    const StageRange sr = stage_node_edges<D>(m, p, slice);
+    load_node_edges<D, true>(m, sr, slice, p.col, lay0, oe, info, oo);
     V2<double> av{1, 1}, h{0, 0}, hn{1, 1};  // FP64 geometry: conservative accumulation
     if (lay0) {
         av = ld2(m.av_d + c);
@@ -2121,6 +2161,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
         hk.mark("k_gnod", st);
     }
     const size_t elds = (size_t)wv * kStageEdges * kEdgeUnits<R> * sizeof(uint4);  // k_edge's LDS slices
+    const size_t nlds = (size_t)wv * kStageNodes * kNodeEdgeUnits<D> * sizeof(uint4);  // k_node_lo / k_node_update
 #define EDGE(HOR, F)                                                                                        \
     do {                                                                                                    \
         if (refresh)                                                                                        \
@@ -2145,10 +2186,10 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
 #define NLO(V)                                                                                              \
     do {                                                                                                    \
         if (refresh)                                                                                        \
-            hipLaunchKernelGGL((k_node_lo<R, V, NT, true, D>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, vfl, \
+            hipLaunchKernelGGL((k_node_lo<R, V, NT, true, D>), gn, blk, nlds, st, m, W, WE, vals, ab, tsn, vfl, \
                                adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
         else                                                                                                \
-            hipLaunchKernelGGL((k_node_lo<R, V, NT, false, D>), gn, blk, 0, st, m, W, WE, vals, ab, tsn, vfl, \
+            hipLaunchKernelGGL((k_node_lo<R, V, NT, false, D>), gn, blk, nlds, st, m, W, WE, vals, ab, tsn, vfl, \
                                adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
     } while (0)
         switch (sp.ver) {
@@ -2177,7 +2218,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
                 hipLaunchKernelGGL((k_learn_dump<R, NT>), ge, blk, 0, st, m, la, dt, dump->X, dump->Y, dump->row);
                 dump->row += (long long)NT * m.CE;
             }
-            hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_UPDATE>), gn, blk, 0, st, m, vals, LO, tsn, adf_h,
+            hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_UPDATE>), gn, blk, nlds, st, m, vals, LO, tsn, adf_h,
                                tse, adf_v, Pp, Pm, dt, alpha, unsafe, nullptr, nullptr);
         } else if (learned) {
             const MlpDev w = mlp_dev(I);
@@ -2188,17 +2229,17 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
             hipLaunchKernelGGL((k_learn_check<R, NT, D>), gn, blk, 0, st, m, adf_h, alpha, tse, adf_v, Pp, Pm, Qp, Qm,
                                tsn, dt, unsafe);
             hk.mark("k_learn_check", st);
-            hipLaunchKernelGGL((k_node_update<R, NT, true, D, OUT_UPDATE>), gn, blk, 0, st, m, vals, LO, tsn, adf_h,
+            hipLaunchKernelGGL((k_node_update<R, NT, true, D, OUT_UPDATE>), gn, blk, nlds, st, m, vals, LO, tsn, adf_h,
                                tse, adf_v, Pp, Pm, dt, alpha, unsafe, nullptr, nullptr);
         } else if (out == OUT_ACCUM) {
-            hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_ACCUM>), gn, blk, 0, st, m, vals, LO, tsn, adf_h,
+            hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_ACCUM>), gn, blk, nlds, st, m, vals, LO, tsn, adf_h,
                                tse, adf_v, Pp, Pm, dt, alpha, unsafe, acc_h, nullptr);
         } else if (out == OUT_TEND) {
             if constexpr (NT == 1)
-                hipLaunchKernelGGL((k_node_update<R, 1, false, D, OUT_TEND>), gn, blk, 0, st, m, vals, LO, tsn,
+                hipLaunchKernelGGL((k_node_update<R, 1, false, D, OUT_TEND>), gn, blk, nlds, st, m, vals, LO, tsn,
                                    adf_h, tse, adf_v, Pp, Pm, dt, alpha, unsafe, acc_h, acc_v);
         } else {
-            hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_UPDATE>), gn, blk, 0, st, m, vals, LO, tsn, adf_h,
+            hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_UPDATE>), gn, blk, nlds, st, m, vals, LO, tsn, adf_h,
                                tse, adf_v, Pp, Pm, dt, alpha, unsafe, nullptr, nullptr);
         }
         hk.mark("k_node_update", st);

@@ -103,6 +103,10 @@ template <class R>
 constexpr int kEdgeUnits = 3 + (12 * (int)sizeof(R)) / 16;
 // edges a wave stages into LDS (chunks with more fall back to global loads)
 constexpr int kStageEdges = 16;
+// nodes a wave stages into LDS (their NodeEdges + NodeEdgeInfo)
+constexpr int kStageNodes = 16;
+template <int D>
+constexpr int kNodeEdgeUnits = (int)(sizeof(NodeEdges<D>) + sizeof(NodeEdgeInfo<D>)) / 16;
 
 template <class R>
 struct DMesh {
