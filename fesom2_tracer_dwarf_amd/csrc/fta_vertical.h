@@ -78,9 +78,13 @@ __device__ __forceinline__ R vflux_upw1(const VCol2<R> &c, R fin)
 }
 
 // adv_tra_ver_qr4c (lib/oce_adv_tra_ver.F90:339-444) including the double
-// write of interface nzmin+1 on two-layer columns
+// write of interface nzmin+1 on two-layer columns. The three slopes of
+// interface nz are qc(nz) = (T(nz-1) - T(nz)) / (Z(nz-1) - Z(nz)), qu = qc(nz+1)
+// and qd = qc(nz-1): the same quotient of the same operands as the
+// neighbouring interface's qc, so each lane divides once per interface and
+// takes the others from its own second interface or the lanes beside it
 template <int Q, class R>
-__device__ __forceinline__ R vflux_qr4c(const VCol2<R> &c, R num_ord, R fin)
+__device__ __forceinline__ R vflux_qr4c(const VCol2<R> &c, R num_ord, R fin, R qd, R qc, R qu)
 {
     const int nz = c.k0 + Q + 1, nlev = c.nlev;
     const R t0 = c.A.v[2 + Q], tm1 = c.A.v[1 + Q];
@@ -94,16 +98,24 @@ __device__ __forceinline__ R vflux_qr4c(const VCol2<R> &c, R num_ord, R fin)
         return cen - fin;
     }
     // 3 <= nz <= nlev-2
-    const R tm2 = c.A.v[Q], tp1 = c.A.v[3 + Q];
-    const R Z0 = c.Z.v[2 + Q], Zm1 = c.Z.v[1 + Q], Zm2 = c.Z.v[Q], Zp1 = c.Z.v[3 + Q];
-    const R qc = (tm1 - t0) / (Zm1 - Z0);
-    const R qu = (t0 - tp1) / (Z0 - Zp1);
-    const R qd = (tm2 - tm1) / (Zm2 - Zm1);
+    const R Z0 = c.Z.v[2 + Q], Zm1 = c.Z.v[1 + Q];
     const R zb = cell<Q>(c.zbar);
     const R Tmean1 = t0 + ((R)2 * qc + qu) * (zb - Z0) / (R)3;
     const R Tmean2 = tm1 + ((R)2 * qc + qd) * (zb - Zm1) / (R)3;
     const R Tmean = (w + fabs(w)) * Tmean1 + (w - fabs(w)) * Tmean2;
     return ((R)(-0.5) * ((R)1 - num_ord) * Tmean - num_ord * ((R)0.5 * (Tmean1 + Tmean2)) * w) * ar - fin;
+}
+
+// both interfaces of the lane; all lanes must call (cross-lane moves)
+template <class R>
+__device__ __forceinline__ void vflux_qr4c2(const VCol2<R> &c, R num_ord, R fin0, R fin1, R &f0, R &f1)
+{
+    const R qc0 = (c.A.v[1] - c.A.v[2]) / (c.Z.v[1] - c.Z.v[2]);
+    const R qc1 = (c.A.v[2] - c.A.v[3]) / (c.Z.v[2] - c.Z.v[3]);
+    const R qd0 = from_prev(qc1);  // qc of interface k0 (the previous lane's second)
+    const R qu1 = from_next(qc0);  // qc of interface k0+3 (the next lane's first)
+    f0 = vflux_qr4c<0>(c, num_ord, fin0, qd0, qc0, qc1);
+    f1 = vflux_qr4c<1>(c, num_ord, fin1, qc0, qc1, qu1);
 }
 
 // adv_tra_ver_cdiff (lib/oce_adv_tra_ver.F90:648-712); interface nlev untouched
@@ -214,8 +226,7 @@ template <class R, int VER>
 __device__ __forceinline__ void ver_flux(const VCol2<R> &c, R num_ord, R dt, R fin0, R fin1, R &f0, R &f1)
 {
     if constexpr (VER == VER_QR4C) {
-        f0 = vflux_qr4c<0>(c, num_ord, fin0);
-        f1 = vflux_qr4c<1>(c, num_ord, fin1);
+        vflux_qr4c2(c, num_ord, fin0, fin1, f0, f1);
     } else if constexpr (VER == VER_PPM) {
         vflux_ppm(c, dt, fin0, fin1, f0, f1);
     } else if constexpr (VER == VER_CDIFF) {
