@@ -6,8 +6,8 @@
 //   k_gnod        per node-level: area-weighted mean of the elemental
 //                 gradients of valuesAB around the node (MUSCL/MFCT)
 //   k_edge        per edge-level: LO upwind flux and HO-LO antidiffusive flux
-//   k_node_lo     per node-level: LO solution, vertical HO-LO flux, P+/P- sums
-//   k_node_limit  per node-level: Zalesak bounds (a1-a3) and R+/R- (b2)
+//   k_node_lo     per node-level: LO solution, vertical HO-LO flux
+//   k_node_limit  per node-level: P+/P- sums, Zalesak bounds (a1-a3), R+/R- (b2)
 //   k_node_update per node-level: limited fluxes (b3), divergence, T update
 // (without FCT the last three collapse into k_node_nolim). Every kernel runs
 // one wave per 128-cell chunk of its layout, two cells per lane.
@@ -503,8 +503,10 @@ __device__ __forceinline__ VCol2<R> node_vcol(const DMesh<R> &m, const Pair &p, 
 }
 
 // ===========================================================================
-// LO solution (lib/oce_adv_tra_driver.F90:95-211), HO-LO vertical flux
-// (:257-283), and the raw antidiffusive sums P+/P- (lib/oce_adv_tra_fct.F90:232-346)
+// LO solution (lib/oce_adv_tra_driver.F90:95-211) and HO-LO vertical flux
+// (:257-283). The antidiffusive sums P+/P- that the reference forms next
+// (lib/oce_adv_tra_fct.F90:232-346) are formed in k_node_limit, which reads
+// the HO-LO fluxes straight away instead of a stored copy of the sums
 // ===========================================================================
 template <class R, int VER, int NT, bool SAME, int D>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && NT <= 2 && VER != VER_PPM ? 4 : 1))) void k_node_lo(DMesh<R> m, const R *__restrict__ Wv,
@@ -512,10 +514,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
                                                     const double *__restrict__ ttf,
                                                     const double *__restrict__ ttfAB, size_t tsn,
                                                     const R *__restrict__ vfl,
-                                                    const R *__restrict__ adf_h, size_t tse,
                                                     double dt, R num_ord,
-                                                    double *__restrict__ LO, R *__restrict__ adf_v,
-                                                    R *__restrict__ Pp, R *__restrict__ Pm)
+                                                    double *__restrict__ LO, R *__restrict__ adf_v)
 {
     const int ch = wave_chunk();
     if (ch >= m.nch_n) return;
@@ -544,7 +544,6 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
     // state (windows of Z, zbar, W) is dead before the edge loops start
     V2<double> T[NT];
     double fl0[NT], fl1[NT], fl2[NT];
-    R pp0[NT], pm0[NT], pp1[NT], pm1[NT];
     {
         VCol2<R> hc = node_vcol<R, VER>(m, p, nlev, Wv);
 #pragma unroll
@@ -566,22 +565,26 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             }
             R ad0, ad1;
             ver_flux<R, VER>(hc, num_ord, (R)dt, (R)fl0[t], (R)fl1[t], ad0, ad1);  // HO - LO
-            const R ad2 = from_next(ad0);
             if (p.live) st2(adf_v + tn + c, if0 ? ad0 : (R)0, if1 ? ad1 : (R)0);
-            pp0[t] = (R)0 + (fmx((R)0, ad0) + fmx((R)0, -ad1));
-            pm0[t] = (R)0 + (fmn((R)0, ad0) + fmn((R)0, -ad1));
-            pp1[t] = (R)0 + (fmx((R)0, ad1) + fmx((R)0, -ad2));
-            pm1[t] = (R)0 + (fmn((R)0, ad1) + fmn((R)0, -ad2));
         }
     }
     if (!lay0) return;
 
+    // FP64 geometry (the LO solution feeds the conservative update); without
+    // cavities areasvol equals area on every wet layer
+    V2<double> av;
+    if constexpr (std::is_same<R, double>::value) av = lc.area;
+    else av = ld2(m.av_d + c);
+    const V2<double> h = ld2(m.h_d + c), hn = ld2(m.hn_d + c);
+    constexpr int TG = NT < 2 ? NT : 2;  // tracers whose gathers share a round trip
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const size_t tn = t * tsn, te = t * tse;
-        // LO horizontal sum over incident edges, edge-index order
-        double slo0 = 0.0, slo1 = 0.0;
-        R qp0 = pp0[t], qm0 = pm0[t], qp1 = pp1[t], qm1 = pm1[t];
+    for (int t0 = 0; t0 < NT; t0 += TG) {
+        // LO horizontal sums over incident edges, edge-index order; edge slots
+        // outer and the group's tracers inner, so that one memory round trip
+        // per edge carries its transport and the other node's value of TG tracers
+        double slo0[TG], slo1[TG];
+#pragma unroll
+        for (int u = 0; u < TG; ++u) { slo0[u] = 0.0; slo1[u] = 0.0; }
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             const int nlm = info_nlmax(info[j]);
@@ -595,46 +598,45 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             // commuted two-term sum), so one sign on the transport orients it
             const double sg = pos ? 1.0 : -1.0;
             const V2<R> v = ld2(vfl + (size_t)oe[j] + k0);
-            const V2<double> To = ld2(ttf + tn + (size_t)oo[j] + k0);
-            const V2<R> a = ld2(adf_h + te + (size_t)oe[j] + k0);
-            const R ax = pos ? a.x : -a.x;
-            slo0 = slo0 + lo_flux((double)v.x * sg, T[t].x, To.x);
-            qp0 = qp0 + fmx((R)0, ax);
-            qm0 = qm0 + fmn((R)0, ax);
-            // second cell by selects, so that each pair stays one 16-byte load
-            const R ay = pos ? a.y : -a.y;
-            const double s1 = slo1 + lo_flux((double)v.y * sg, T[t].y, To.y);
-            const R q1 = qp1 + fmx((R)0, ay), r1 = qm1 + fmn((R)0, ay);
-            slo1 = on1 ? s1 : slo1;
-            qp1 = on1 ? q1 : qp1;
-            qm1 = on1 ? r1 : qm1;
+            V2<double> To[TG];
+#pragma unroll
+            for (int u = 0; u < TG; ++u) To[u] = ld2(ttf + (t0 + u) * tsn + (size_t)oo[j] + k0);
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                slo0[u] = slo0[u] + lo_flux((double)v.x * sg, T[t0 + u].x, To[u].x);
+                // second cell by selects, so that each pair stays one 16-byte load
+                const double s1 = slo1[u] + lo_flux((double)v.y * sg, T[t0 + u].y, To[u].y);
+                slo1[u] = on1 ? s1 : slo1[u];
+            }
         }
-        // FP64 geometry (the LO solution feeds the conservative update); without
-        // cavities areasvol equals area on every wet layer
-        V2<double> av;
-        if constexpr (std::is_same<R, double>::value) av = lc.area;
-        else av = ld2(m.av_d + c);
-        const V2<double> h = ld2(m.h_d + c), hn = ld2(m.hn_d + c);
-        // flux differences formed in FP64: each interface flux then leaves one
-        // layer and enters the next with exactly opposite sign in every precision
-        const double LO0 =
-            (T[t].x * (double)h.x + (slo0 + (fl0[t] - fl1[t])) * dt / (double)av.x) / (double)hn.x;
-        // second layer branch-free (selected away below a one-layer pair)
-        const double L1 = (T[t].y * (double)h.y + (slo1 + (fl1[t] - fl2[t])) * dt / (double)av.y) / (double)hn.y;
-        st2(LO + tn + c, LO0, lay1 ? L1 : 0.0);
-        st2(Pp + tn + c, qp0, lay1 ? qp1 : (R)0);
-        st2(Pm + tn + c, qm0, lay1 ? qm1 : (R)0);
+#pragma unroll
+        for (int u = 0; u < TG; ++u) {
+            const int t = t0 + u;
+            // flux differences formed in FP64: each interface flux then leaves one
+            // layer and enters the next with exactly opposite sign in every precision
+            const double LO0 =
+                (T[t].x * (double)h.x + (slo0[u] + (fl0[t] - fl1[t])) * dt / (double)av.x) / (double)hn.x;
+            // second layer branch-free (selected away below a one-layer pair)
+            const double L1 =
+                (T[t].y * (double)h.y + (slo1[u] + (fl1[t] - fl2[t])) * dt / (double)av.y) / (double)hn.y;
+            st2(LO + t * tsn + c, LO0, lay1 ? L1 : 0.0);
+        }
     }
 }
 
-// Zalesak bounds a1-a3 and limiting factors b2 (lib/oce_adv_tra_fct.F90:94-217,361-374);
-// R+/R- overwrite P+/P- in place. a1 (max/min of LO and T) is formed once per
-// 1-ring node, a2 per fan element from the node and two consecutive ring nodes.
+// The raw antidiffusive sums P+/P- (lib/oce_adv_tra_fct.F90:232-346: the
+// vertical HO-LO fluxes of the layer's two interfaces, then the horizontal
+// ones of the incident edges in edge-index order, each oriented into the
+// node), Zalesak bounds a1-a3 and limiting factors b2 (:94-217,361-374).
+// a1 (max/min of LO and T) is formed once per 1-ring node and a2/a3 as one
+// running max/min over the ring.
 template <class R, int NT, bool STOREQ, int D>
 __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const double *__restrict__ ttf,
                                                        const double *__restrict__ LO, size_t tsn,
-                                                       double dt, R *__restrict__ Pp_Rp,
-                                                       R *__restrict__ Pm_Rm, R *__restrict__ Qp_out,
+                                                       const R *__restrict__ adf_h, size_t tse,
+                                                       const R *__restrict__ adf_v,
+                                                       double dt, R *__restrict__ Rp,
+                                                       R *__restrict__ Rm, R *__restrict__ Qp_out,
                                                        R *__restrict__ Qm_out, R *__restrict__ Pp_out,
                                                        R *__restrict__ Pm_out)
 {
@@ -648,115 +650,183 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
     // bounds in FP64 whatever R is: they are selections among FP64 tracer
     // values, so no new extrema appear beyond the flux arithmetic's rounding
     const double bignumber = 1000.0;
+    constexpr int TG = NT < 2 ? NT : 2;  // tracers whose gathers share a round trip
+    static_assert(NT % TG == 0, "tracer batches are 1, 2 or 4");
+
+    // the chunk's node->edge records go to LDS first (every lane of the wave
+    // takes part); the bounds come next, then per tracer group the P+/P- sums
+    // and R+/R-, so that the sums are live only briefly
+    extern __shared__ uint4 stage_all[];
+    uint4 *slice = stage_all + (threadIdx.x >> 6) * (kStageNodes * kNodeEdgeUnits<D>);
+    const StageRange sr = stage_node_edges<D>(m, p, slice);
 
     double tvmax0[NT], tvmin0[NT], tvmax1[NT], tvmin1[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) { tvmax0[t] = 0; tvmin0[t] = 0; tvmax1[t] = 0; tvmin1[t] = 0; }
     if (lay0) {
+        // a2/a3 as one max/min per level: element e's bound is the max (min)
+        // of max(LO, T) (min(LO, T)) over its three nodes when it has the
+        // level, -bignumber (+bignumber) when it does not; the node's bound is
+        // the max (min) over its elements. The node itself and every ring
+        // node next to an element that has the level enter it, plus the
+        // padding value when some element lacks the level (max and min are
+        // exact, so this equals the element-by-element form bit for bit).
+        // Fan element j lies between ring nodes j and j + 1 (ring node 0 where
+        // the fan closes, j + 1 == nr); le[j] = 0 where a boundary fan is open
         const NodeRing<D> rg = node_ring<D>(m)[p.col];
-        int nr = 0;
+        int nr = 0, le_last = 0, lemin = 255;
 #pragma unroll
         for (int j = 0; j < D; ++j) nr += rg.oo[j] >= 0 ? 1 : 0;
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const size_t tn = t * tsn;
-            const V2<double> ls = ld2(LO + tn + c), ts = ld2(ttf + tn + c);
-            const double mxs0 = fmx(ls.x, ts.x), mns0 = fmn(ls.x, ts.x);
-            const double mxs1 = fmx(ls.y, ts.y), mns1 = fmn(ls.y, ts.y);
-            // one pass over the ring: fan element e lies between ring nodes e and
-            // e + 1 (ring node 0 where the fan closes, e + 1 == nr), so element
-            // j - 1 is complete once ring node j is read; only the previous and
-            // the first ring node are kept (max and min are exact, so the order
-            // in which the elements enter the bounds does not matter)
-            double fx0 = 0, fn0 = 0, fx1 = 0, fn1 = 0, px0 = 0, pn0 = 0, px1 = 0, pn1 = 0;
-            double vmax0 = 0, vmin0 = 0, vmax1 = 0, vmin1 = 0;
-            bool first = true;
+        for (int j = 0; j < D; ++j) {
+            le_last = j == nr - 1 ? (int)rg.le[j] : le_last;
+            lemin = rg.le[j] != 0 ? min(lemin, (int)rg.le[j]) : lemin;
+        }
+        int lw[D];  // layers of the deeper element next to ring node j
 #pragma unroll
-            for (int j = 0; j <= D; ++j) {
-                double cx0 = 0, cn0 = 0, cx1 = 0, cn1 = 0;
-                if (j < D && rg.oo[j] >= 0 && k0 < (int)rg.lr[j]) {
-                    const V2<double> l = ld2(LO + tn + (size_t)rg.oo[j] + k0);
-                    const V2<double> v = ld2(ttf + tn + (size_t)rg.oo[j] + k0);
-                    cx0 = fmx(l.x, v.x); cn0 = fmn(l.x, v.x);
-                    cx1 = fmx(l.y, v.y); cn1 = fmn(l.y, v.y);
-                }
-                if (j == 0) {
-                    fx0 = cx0; fn0 = cn0; fx1 = cx1; fn1 = cn1;
-                } else if (rg.le[j - 1] != 0) {
-                    const int le = rg.le[j - 1];
-                    const bool wrap = j >= nr;
-                    const double bx0 = wrap ? fx0 : cx0, bn0 = wrap ? fn0 : cn0;
-                    const double bx1 = wrap ? fx1 : cx1, bn1 = wrap ? fn1 : cn1;
-                    const double emax0 = k0 < le ? fmx(fmx(mxs0, px0), bx0) : -bignumber;
-                    const double emin0 = k0 < le ? fmn(fmn(mns0, pn0), bn0) : bignumber;
-                    const double emax1 = k0 + 1 < le ? fmx(fmx(mxs1, px1), bx1) : -bignumber;
-                    const double emin1 = k0 + 1 < le ? fmn(fmn(mns1, pn1), bn1) : bignumber;
-                    vmax0 = first ? emax0 : fmx(vmax0, emax0);
-                    vmin0 = first ? emin0 : fmn(vmin0, emin0);
-                    vmax1 = first ? emax1 : fmx(vmax1, emax1);
-                    vmin1 = first ? emin1 : fmn(vmin1, emin1);
-                    first = false;
-                }
-                px0 = cx0; pn0 = cn0; px1 = cx1; pn1 = cn1;
+        for (int j = 0; j < D; ++j) lw[j] = rg.oo[j] >= 0 ? max(j == 0 ? le_last : (int)rg.le[j - 1], (int)rg.le[j]) : 0;
+        const bool dry0 = k0 >= lemin, dry1 = k0 + 1 >= lemin;
+        // ring nodes outer, tracers inner (groups of TG): one memory round
+        // trip per ring node carries the LO and tracer pairs of TG tracers
+#pragma unroll
+        for (int t0 = 0; t0 < NT; t0 += TG) {
+            double vmax0[TG], vmin0[TG], vmax1[TG], vmin1[TG];
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                const size_t tn = (t0 + u) * tsn;
+                const V2<double> ls = ld2(LO + tn + c), ts = ld2(ttf + tn + c);
+                vmax0[u] = fmx(ls.x, ts.x); vmin0[u] = fmn(ls.x, ts.x);
+                vmax1[u] = fmx(ls.y, ts.y); vmin1[u] = fmn(ls.y, ts.y);
             }
-            tvmax0[t] = vmax0; tvmin0[t] = vmin0;
-            tvmax1[t] = lay1 ? vmax1 : 0.0; tvmin1[t] = lay1 ? vmin1 : 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                if (k0 >= lw[j]) continue;
+                const bool on1 = k0 + 1 < lw[j];
+                V2<double> l[TG], v[TG];
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    const size_t o = (t0 + u) * tsn + (size_t)rg.oo[j] + k0;
+                    l[u] = ld2(LO + o);
+                    v[u] = ld2(ttf + o);
+                }
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    vmax0[u] = fmx(vmax0[u], fmx(l[u].x, v[u].x));
+                    vmin0[u] = fmn(vmin0[u], fmn(l[u].x, v[u].x));
+                    const double x1 = fmx(vmax1[u], fmx(l[u].y, v[u].y));
+                    const double n1 = fmn(vmin1[u], fmn(l[u].y, v[u].y));
+                    vmax1[u] = on1 ? x1 : vmax1[u];
+                    vmin1[u] = on1 ? n1 : vmin1[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                tvmax0[t0 + u] = dry0 ? fmx(vmax0[u], -bignumber) : vmax0[u];
+                tvmin0[t0 + u] = dry0 ? fmn(vmin0[u], bignumber) : vmin0[u];
+                tvmax1[t0 + u] = lay1 ? (dry1 ? fmx(vmax1[u], -bignumber) : vmax1[u]) : 0.0;
+                tvmin1[t0 + u] = lay1 ? (dry1 ? fmn(vmin1[u], bignumber) : vmin1[u]) : 0.0;
+            }
         }
     }
+    int oe[D], oo[D];
+    unsigned info[D];
+    load_node_edges<D, true>(m, sr, slice, p.col, lay0, oe, info, oo);
     V2<R> av{1, 1};
     if (lay0) av = ld2(m.areasvol + c);
     const R flux_eps = (R)1e-16;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const double upx0 = from_prev(tvmax1[t]), upn0 = from_prev(tvmin1[t]);
-        const double dnx1 = from_next(tvmax0[t]), dnn1 = from_next(tvmin0[t]);
-        if (!lay0) continue;
-        const size_t tn = t * tsn;
-        const V2<double> lo = ld2(LO + tn + c);
-        const V2<R> P = ld2(Pp_Rp + tn + c), M = ld2(Pm_Rm + tn + c);
-        R rp0, rm0, rp1 = 0, rm1 = 0;
-        double qp0 = 0.0, qm0 = 0.0, qp1 = 0.0, qm1 = 0.0;
-        {
-            const int nz = k0 + 1;
-            double qp, qm;
-            if (nz == 1 || nz == L) {
-                qp = tvmax0[t] - lo.x;
-                qm = tvmin0[t] - lo.x;
-            } else {
-                qp = fmx(fmx(upx0, tvmax0[t]), tvmax1[t]) - lo.x;
-                qm = fmn(fmn(upn0, tvmin0[t]), tvmin1[t]) - lo.x;
-            }
-            R flux = P.x * (R)dt / av.x + flux_eps;
-            rp0 = fmn((R)1, (R)qp / flux);
-            flux = M.x * (R)dt / av.x - flux_eps;
-            rm0 = fmn((R)1, (R)qm / flux);
-            qp0 = qp;
-            qm0 = qm;
+    for (int t0 = 0; t0 < NT; t0 += TG) {
+        // P+/P-: the vertical HO-LO fluxes of interfaces k0+1, k0+2 (layer k0)
+        // and k0+2, k0+3 (layer k0+1; the last one in the next lane: where it
+        // is used the layer below exists, so that lane holds the same column),
+        // then the incident edges in edge-index order, each oriented into the
+        // node; edge slots outer and the group's tracers inner, so that one
+        // memory round trip per edge carries the flux pairs of TG tracers
+        R pp0[TG], pm0[TG], pp1[TG], pm1[TG];
+#pragma unroll
+        for (int u = 0; u < TG; ++u) {
+            V2<R> ad{0, 0};
+            if (p.live) ad = ld2(adf_v + (t0 + u) * tsn + c);
+            const R ad0 = ad.x, ad1 = ad.y, ad2 = from_next(ad0);
+            pp0[u] = (R)0 + (fmx((R)0, ad0) + fmx((R)0, -ad1));
+            pm0[u] = (R)0 + (fmn((R)0, ad0) + fmn((R)0, -ad1));
+            pp1[u] = (R)0 + (fmx((R)0, ad1) + fmx((R)0, -ad2));
+            pm1[u] = (R)0 + (fmn((R)0, ad1) + fmn((R)0, -ad2));
         }
-        if (lay1) {
-            const int nz = k0 + 2;
-            double qp, qm;
-            if (nz == L) {
-                qp = tvmax1[t] - lo.y;
-                qm = tvmin1[t] - lo.y;
-            } else {
-                qp = fmx(fmx(tvmax0[t], tvmax1[t]), dnx1) - lo.y;
-                qm = fmn(fmn(tvmin0[t], tvmin1[t]), dnn1) - lo.y;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const int nlm = info_nlmax(info[j]);
+            if (oe[j] < 0 || k0 >= nlm) continue;
+            const bool on1 = lay1 && k0 + 1 < nlm;
+            const bool pos = info_pos(info[j]);
+            V2<R> a[TG];
+#pragma unroll
+            for (int u = 0; u < TG; ++u) a[u] = ld2(adf_h + (t0 + u) * tse + (size_t)oe[j] + k0);
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                const R ax = pos ? a[u].x : -a[u].x;
+                pp0[u] = pp0[u] + fmx((R)0, ax);
+                pm0[u] = pm0[u] + fmn((R)0, ax);
+                // second cell by selects, so that each pair stays one 16-byte load
+                const R ay = pos ? a[u].y : -a[u].y;
+                const R q1 = pp1[u] + fmx((R)0, ay), r1 = pm1[u] + fmn((R)0, ay);
+                pp1[u] = on1 ? q1 : pp1[u];
+                pm1[u] = on1 ? r1 : pm1[u];
             }
-            R flux = P.y * (R)dt / av.y + flux_eps;
-            rp1 = fmn((R)1, (R)qp / flux);
-            flux = M.y * (R)dt / av.y - flux_eps;
-            rm1 = fmn((R)1, (R)qm / flux);
-            qp1 = qp;
-            qm1 = qm;
         }
-        st2(Pp_Rp + tn + c, rp0, rp1);
-        st2(Pm_Rm + tn + c, rm0, rm1);
-        if constexpr (STOREQ) {  // Q+/Q- and P+/P- for the learned limiter's features and bound check
-            st2(Qp_out + tn + c, (R)qp0, (R)qp1);
-            st2(Qm_out + tn + c, (R)qm0, (R)qm1);
-            st2(Pp_out + tn + c, P.x, lay1 ? P.y : (R)0);
-            st2(Pm_out + tn + c, M.x, lay1 ? M.y : (R)0);
+#pragma unroll
+        for (int u = 0; u < TG; ++u) {
+            const int t = t0 + u;
+            const double upx0 = from_prev(tvmax1[t]), upn0 = from_prev(tvmin1[t]);
+            const double dnx1 = from_next(tvmax0[t]), dnn1 = from_next(tvmin0[t]);
+            if (!lay0) continue;
+            const size_t tn = t * tsn;
+            const V2<double> lo = ld2(LO + tn + c);
+            const V2<R> P{pp0[u], lay1 ? pp1[u] : (R)0}, M{pm0[u], lay1 ? pm1[u] : (R)0};
+            R rp0, rm0, rp1 = 0, rm1 = 0;
+            double qp0 = 0.0, qm0 = 0.0, qp1 = 0.0, qm1 = 0.0;
+            {
+                const int nz = k0 + 1;
+                double qp, qm;
+                if (nz == 1 || nz == L) {
+                    qp = tvmax0[t] - lo.x;
+                    qm = tvmin0[t] - lo.x;
+                } else {
+                    qp = fmx(fmx(upx0, tvmax0[t]), tvmax1[t]) - lo.x;
+                    qm = fmn(fmn(upn0, tvmin0[t]), tvmin1[t]) - lo.x;
+                }
+                R flux = P.x * (R)dt / av.x + flux_eps;
+                rp0 = fmn((R)1, (R)qp / flux);
+                flux = M.x * (R)dt / av.x - flux_eps;
+                rm0 = fmn((R)1, (R)qm / flux);
+                qp0 = qp;
+                qm0 = qm;
+            }
+            if (lay1) {
+                const int nz = k0 + 2;
+                double qp, qm;
+                if (nz == L) {
+                    qp = tvmax1[t] - lo.y;
+                    qm = tvmin1[t] - lo.y;
+                } else {
+                    qp = fmx(fmx(tvmax0[t], tvmax1[t]), dnx1) - lo.y;
+                    qm = fmn(fmn(tvmin0[t], tvmin1[t]), dnn1) - lo.y;
+                }
+                R flux = P.y * (R)dt / av.y + flux_eps;
+                rp1 = fmn((R)1, (R)qp / flux);
+                flux = M.y * (R)dt / av.y - flux_eps;
+                rm1 = fmn((R)1, (R)qm / flux);
+                qp1 = qp;
+                qm1 = qm;
+            }
+            st2(Rp + tn + c, rp0, rp1);
+            st2(Rm + tn + c, rm0, rm1);
+            if constexpr (STOREQ) {  // Q+/Q- and P+/P- for the learned limiter's features and bound check
+                st2(Qp_out + tn + c, (R)qp0, (R)qp1);
+                st2(Qm_out + tn + c, (R)qm0, (R)qm1);
+                st2(Pp_out + tn + c, P.x, P.y);
+                st2(Pm_out + tn + c, M.x, M.y);
+            }
         }
     }
 }
@@ -850,64 +920,90 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
         hn = ld2(m.hn_d + c);
     }
 
+    constexpr int TG = NT < 2 ? NT : 2;  // tracers whose gathers share a round trip
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const size_t tn = t * tsn, te = t * tse;
-        V2<R> fv{0, 0}, rp{0, 0}, rm{0, 0};
-        if (if0) fv = ld2(adf_v + tn + c);
-        if (!if1) fv.y = 0;
-        if (lay0) { rp = ld2(Rp + tn + c); rm = ld2(Rm + tn + c); }
-        if (!lay1) { rp.y = 0; rm.y = 0; }
-        const R rpu0 = from_prev(rp.y), rmu0 = from_prev(rm.y);
-        const R fvl0 = limit_vertical(k0 + 1, lay0, fv.x, rp.x, rm.x, rpu0, rmu0);
-        const R fvl1 = limit_vertical(k0 + 2, lay1, fv.y, rp.y, rm.y, rp.x, rm.x);
-        const R fvl2 = from_next(fvl0);
+    for (int t0 = 0; t0 < NT; t0 += TG) {
+        V2<R> rp[TG], rm[TG], us[TG];
+        R fvl0[TG], fvl1[TG], fvl2[TG];
+#pragma unroll
+        for (int u = 0; u < TG; ++u) {
+            const size_t tn = (t0 + u) * tsn;
+            V2<R> fv{0, 0};
+            rp[u] = V2<R>{0, 0};
+            rm[u] = V2<R>{0, 0};
+            us[u] = V2<R>{0, 0};
+            if (if0) fv = ld2(adf_v + tn + c);
+            if (!if1) fv.y = 0;
+            if (lay0) { rp[u] = ld2(Rp + tn + c); rm[u] = ld2(Rm + tn + c); }
+            if (!lay1) { rp[u].y = 0; rm[u].y = 0; }
+            const R rpu0 = from_prev(rp[u].y), rmu0 = from_prev(rm[u].y);
+            fvl0[u] = limit_vertical(k0 + 1, lay0, fv.x, rp[u].x, rm[u].x, rpu0, rmu0);
+            fvl1[u] = limit_vertical(k0 + 2, lay1, fv.y, rp[u].y, rm[u].y, rp[u].x, rm[u].x);
+            fvl2[u] = from_next(fvl0[u]);
+            if constexpr (LEARN) { if (lay0) us[u] = ld2(unsafe + tn + c); }
+        }
         if (!lay0) continue;
-        V2<R> us{0, 0};
-        if constexpr (LEARN) us = ld2(unsafe + tn + c);
 
-        double sh0 = 0.0, sh1 = 0.0;
+        // edge slots outer and the group's tracers inner: one memory round trip
+        // per incident edge carries its flux and the other node's R+/R- of TG tracers
+        double sh0[TG], sh1[TG];
+#pragma unroll
+        for (int u = 0; u < TG; ++u) { sh0[u] = 0.0; sh1[u] = 0.0; }
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             const int nlm = info_nlmax(info[j]);
             if (oe[j] < 0 || k0 >= nlm) continue;
             const bool on1 = lay1 && k0 + 1 < nlm;
             const bool pos = info_pos(info[j]);  // this node is edges(1)
-            const V2<R> f = ld2(adf_h + te + (size_t)oe[j] + k0);
-            const V2<R> rpo = ld2(Rp + tn + (size_t)oo[j] + k0), rmo = ld2(Rm + tn + (size_t)oo[j] + k0);
-            // both cells evaluated unconditionally, the second one masked
-            // (keeps every pair load one 16-byte access). Oriented as the flux
-            // entering this node, fs = +-f, Zalesak's coefficient (b3) is
-            // min(1, R+ here, R- there) for fs >= 0 and min(1, R- here, R+ there)
-            // otherwise -- edge_alpha's choice for every f != 0, and for f == 0 the
-            // product with the flux is zero either way
-            const R fx = pos ? f.x : -f.x, fy = pos ? f.y : -f.y;
-            const bool in0 = fx >= (R)0, in1 = fy >= (R)0;
-            R a0 = fmn(fmn((R)1, in0 ? rp.x : rm.x), in0 ? rmo.x : rpo.x);
-            R a1 = fmn(fmn((R)1, in1 ? rp.y : rm.y), in1 ? rmo.y : rpo.y);
-            if constexpr (LEARN) {
-                // the network's coefficient, Zalesak's where either node failed its bound check
-                const V2<R> al = ld2(alpha + te + (size_t)oe[j] + k0);
-                const V2<R> uo = ld2(unsafe + tn + (size_t)oo[j] + k0);
-                a0 = (us.x != (R)0 || uo.x != (R)0) ? fmn(al.x, a0) : al.x;
-                a1 = (us.y != (R)0 || uo.y != (R)0) ? fmn(al.y, a1) : al.y;
+            V2<R> f[TG], rpo[TG], rmo[TG];
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                const size_t tn = (t0 + u) * tsn, te = (t0 + u) * tse;
+                f[u] = ld2(adf_h + te + (size_t)oe[j] + k0);
+                rpo[u] = ld2(Rp + tn + (size_t)oo[j] + k0);
+                rmo[u] = ld2(Rm + tn + (size_t)oo[j] + k0);
             }
-            const double fl0 = (double)(a0 * fx);
-            const double fl1 = (double)(a1 * fy);
-            sh0 = sh0 + fl0;
-            const double s1 = sh1 + fl1;
-            sh1 = on1 ? s1 : sh1;
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                // both cells evaluated unconditionally, the second one masked
+                // (keeps every pair load one 16-byte access). Oriented as the flux
+                // entering this node, fs = +-f, Zalesak's coefficient (b3) is
+                // min(1, R+ here, R- there) for fs >= 0 and min(1, R- here, R+ there)
+                // otherwise -- edge_alpha's choice for every f != 0, and for f == 0 the
+                // product with the flux is zero either way
+                const R fx = pos ? f[u].x : -f[u].x, fy = pos ? f[u].y : -f[u].y;
+                const bool in0 = fx >= (R)0, in1 = fy >= (R)0;
+                R a0 = fmn(fmn((R)1, in0 ? rp[u].x : rm[u].x), in0 ? rmo[u].x : rpo[u].x);
+                R a1 = fmn(fmn((R)1, in1 ? rp[u].y : rm[u].y), in1 ? rmo[u].y : rpo[u].y);
+                if constexpr (LEARN) {
+                    // the network's coefficient, Zalesak's where either node failed its bound check
+                    const size_t tn = (t0 + u) * tsn, te = (t0 + u) * tse;
+                    const V2<R> al = ld2(alpha + te + (size_t)oe[j] + k0);
+                    const V2<R> uo = ld2(unsafe + tn + (size_t)oo[j] + k0);
+                    a0 = (us[u].x != (R)0 || uo.x != (R)0) ? fmn(al.x, a0) : al.x;
+                    a1 = (us[u].y != (R)0 || uo.y != (R)0) ? fmn(al.y, a1) : al.y;
+                }
+                const double fl0 = (double)(a0 * fx);
+                const double fl1 = (double)(a1 * fy);
+                sh0[u] = sh0[u] + fl0;
+                const double s1 = sh1[u] + fl1;
+                sh1[u] = on1 ? s1 : sh1[u];
+            }
         }
-        // the edge sum times dt/areasvol once (the reference scales each edge's
-        // flux before summing; the difference is rounding)
-        sh0 = sh0 * dt / (double)av.x;
-        sh1 = sh1 * dt / (double)av.y;
-        const V2<double> T = ld2(ttf + tn + c), lo = ld2(LO + tn + c);
-        double dv0 = 0.0 - T.x * (double)h.x + lo.x * (double)hn.x;
-        dv0 = dv0 + ((double)fvl0 - (double)fvl1) * dt / (double)av.x;
-        double dv1 = 0.0 - T.y * (double)h.y + lo.y * (double)hn.y;
-        dv1 = dv1 + ((double)fvl1 - (double)fvl2) * dt / (double)av.y;
-        node_out<OUT>(ttf, acc_h, acc_v, tn + c, lay1, T, sh0, dv0, sh1, dv1, hn);
+#pragma unroll
+        for (int u = 0; u < TG; ++u) {
+            const size_t tn = (t0 + u) * tsn;
+            // the edge sum times dt/areasvol once (the reference scales each edge's
+            // flux before summing; the difference is rounding)
+            const double s0 = sh0[u] * dt / (double)av.x;
+            const double s1 = sh1[u] * dt / (double)av.y;
+            const V2<double> T = ld2(ttf + tn + c), lo = ld2(LO + tn + c);
+            double dv0 = 0.0 - T.x * (double)h.x + lo.x * (double)hn.x;
+            dv0 = dv0 + ((double)fvl0[u] - (double)fvl1[u]) * dt / (double)av.x;
+            double dv1 = 0.0 - T.y * (double)h.y + lo.y * (double)hn.y;
+            dv1 = dv1 + ((double)fvl1[u] - (double)fvl2[u]) * dt / (double)av.y;
+            node_out<OUT>(ttf, acc_h, acc_v, tn + c, lay1, T, s0, dv0, s1, dv1, hn);
+        }
     }
 }
 
@@ -2161,7 +2257,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
         hk.mark("k_gnod", st);
     }
     const size_t elds = (size_t)wv * kStageEdges * kEdgeUnits<R> * sizeof(uint4);  // k_edge's LDS slices
-    const size_t nlds = (size_t)wv * kStageNodes * kNodeEdgeUnits<D> * sizeof(uint4);  // k_node_lo / k_node_update
+    const size_t nlds = (size_t)wv * kStageNodes * kNodeEdgeUnits<D> * sizeof(uint4);  // node kernels' LDS slices
 #define EDGE(HOR, F)                                                                                        \
     do {                                                                                                    \
         if (refresh)                                                                                        \
@@ -2187,10 +2283,10 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
     do {                                                                                                    \
         if (refresh)                                                                                        \
             hipLaunchKernelGGL((k_node_lo<R, V, NT, true, D>), gn, blk, nlds, st, m, W, WE, vals, ab, tsn, vfl, \
-                               adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
+                               dt, pv, LO, adf_v);                                                          \
         else                                                                                                \
             hipLaunchKernelGGL((k_node_lo<R, V, NT, false, D>), gn, blk, nlds, st, m, W, WE, vals, ab, tsn, vfl, \
-                               adf_h, tse, dt, pv, LO, adf_v, Pp, Pm);                                      \
+                               dt, pv, LO, adf_v);                                                          \
     } while (0)
         switch (sp.ver) {
         case VER_QR4C: NLO(VER_QR4C); break;
@@ -2204,12 +2300,13 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
         R *Qm = learned || dump ? (R *)I.Qm : nullptr;
         R *Sp = learned || dump ? (R *)I.Sp : nullptr;
         R *Sm = learned || dump ? (R *)I.Sm : nullptr;
+        // R+/R- go to the arrays named Pp/Pm (fct_plus/fct_minus)
         if (learned || dump)
-            hipLaunchKernelGGL((k_node_limit<R, NT, true, D>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm, Qp, Qm,
-                               Sp, Sm);
+            hipLaunchKernelGGL((k_node_limit<R, NT, true, D>), gn, blk, nlds, st, m, vals, LO, tsn, adf_h, tse, adf_v,
+                               dt, Pp, Pm, Qp, Qm, Sp, Sm);
         else
-            hipLaunchKernelGGL((k_node_limit<R, NT, false, D>), gn, blk, 0, st, m, vals, LO, tsn, dt, Pp, Pm, Qp, Qm,
-                               Sp, Sm);
+            hipLaunchKernelGGL((k_node_limit<R, NT, false, D>), gn, blk, nlds, st, m, vals, LO, tsn, adf_h, tse, adf_v,
+                               dt, Pp, Pm, Qp, Qm, Sp, Sm);
         hk.mark("k_node_limit", st);
         const LearnArgs<R, NT> la{vals, LO, adf_h, vfl, Qp, Qm, Pp, Pm, Sp, Sm, tsn, tse};
         R *alpha = learned ? (R *)I.alpha : nullptr, *unsafe = learned ? (R *)I.unsafe + t0 * tsn : nullptr;
