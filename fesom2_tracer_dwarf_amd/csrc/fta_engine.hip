@@ -231,11 +231,45 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const double *__
 // is the weighted sum of its two ring values' differences to this node's
 // value (a triangle's gradient coefficients sum to zero). Sums in fan order;
 // the result equals the element-by-element form to rounding.
-// the same mean over the node's fan (ring node j and j+1 bound element j):
-// every ring value is loaded once, and each element's gradient times its area
-// is the weighted sum of its two ring values' differences to this node's
-// value (a triangle's gradient coefficients sum to zero). Sums in fan order;
-// the result equals the element-by-element form to rounding.
+//
+// The chunk's fan weights (8*D values per column) are staged into the wave's
+// LDS slice by cooperative 16-byte loads when the chunk has at most
+// kStageFanBytes / (column's bytes) columns: every lane of a column reads the
+// same weights, which from global memory cost one vector-memory instruction
+// per lane and slot.
+template <class R, int D>
+struct FanW {
+    static constexpr int kUnits = 8 * D * (int)sizeof(R) / 16;     // 16-byte units per column
+    static constexpr int kCols = kStageFanBytes / (kUnits * 16);   // columns a wave stages
+};
+
+template <class R>
+__device__ __forceinline__ void fan_weights(const R *w, R &ax, R &bx, R &ay, R &by, R &a)
+{
+    const V2<R> wa = ld2(w), wb = ld2(w + 2), wc = ld2(w + 4);
+    ax = wa.x; bx = wa.y; ay = wb.x; by = wb.y; a = wc.x;
+}
+
+__device__ __forceinline__ void fan_weights_lds(const uint4 *q, double &ax, double &bx, double &ay, double &by,
+                                                double &a)
+{
+    const uint4 u0 = q[0], u1 = q[1], u2 = q[2];
+    double v[6];
+    __builtin_memcpy(&v[0], &u0, 16);
+    __builtin_memcpy(&v[2], &u1, 16);
+    __builtin_memcpy(&v[4], &u2, 16);
+    ax = v[0]; bx = v[1]; ay = v[2]; by = v[3]; a = v[4];
+}
+
+__device__ __forceinline__ void fan_weights_lds(const uint4 *q, float &ax, float &bx, float &ay, float &by, float &a)
+{
+    const uint4 u0 = q[0], u1 = q[1];
+    float v[8];
+    __builtin_memcpy(&v[0], &u0, 16);
+    __builtin_memcpy(&v[4], &u1, 16);
+    ax = v[0]; bx = v[1]; ay = v[2]; by = v[3]; a = v[4];
+}
+
 template <class R, class G, int NT, int D>
 __global__ __launch_bounds__(kMaxBlock) void k_gnod_fan(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
                                                      G *__restrict__ gnod, size_t tsg, float gscale)
@@ -243,6 +277,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod_fan(DMesh<R> m, const double
     const int ch = wave_chunk();
     if (ch >= m.nch_n) return;
     const Pair p = lane_pair(m.slot_n, ch);
+    constexpr int WU = FanW<R, D>::kUnits, WC = FanW<R, D>::kCols;
+    extern __shared__ uint4 stage_all[];
+    uint4 *stage = stage_all + (threadIdx.x >> 6) * (WC * WU);
+    const StageRange sr = stage_range(p, WC);
+    if (sr.staged) {
+        stage_copy<WU>(reinterpret_cast<const uint4 *>(m.ngw), stage, sr);
+        stage_sync();
+    }
     if (!p.live) return;
     const int L = m.nlev[p.col] - 1, k0 = p.k0;
     if (k0 >= L) return;  // the pair holds only the bottom interface
@@ -262,6 +304,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod_fan(DMesh<R> m, const double
 #pragma unroll
     for (int t = 0; t < NT; ++t) { tx0[t] = 0; ty0[t] = 0; tx1[t] = 0; ty1[t] = 0; }
     const R *W = m.ngw + (size_t)8 * D * p.col;
+    const uint4 *WS = stage + (p.col - sr.c0) * WU;
 #pragma unroll
     for (int j = 0; j <= D; ++j) {
         V2<double> cur[NT];
@@ -282,8 +325,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod_fan(DMesh<R> m, const double
             const int le = rg.le[j - 1];
             const bool m0 = k0 < le, m1 = k0 + 1 < le;
             const bool wrap = j >= nr;
-            const V2<R> wa = ld2(W + 8 * (j - 1)), wb = ld2(W + 8 * (j - 1) + 2), wc = ld2(W + 8 * (j - 1) + 4);
-            const R a = wc.x;
+            R wax, wbx, way, wby, a;
+            if (sr.staged) fan_weights_lds(WS + (j - 1) * (8 * (int)sizeof(R) / 16), wax, wbx, way, wby, a);
+            else fan_weights(W + 8 * (j - 1), wax, wbx, way, wby, a);
             tvol0 = m0 ? tvol0 + a : tvol0;
             tvol1 = m1 ? tvol1 + a : tvol1;
 #pragma unroll
@@ -291,8 +335,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod_fan(DMesh<R> m, const double
                 const V2<double> b = wrap ? first[t] : cur[t];
                 const R da0 = (R)(prev[t].x - Tn[t].x), db0 = (R)(b.x - Tn[t].x);
                 const R da1 = (R)(prev[t].y - Tn[t].y), db1 = (R)(b.y - Tn[t].y);
-                const R gx0 = wa.x * da0 + wa.y * db0, gy0 = wb.x * da0 + wb.y * db0;
-                const R gx1 = wa.x * da1 + wa.y * db1, gy1 = wb.x * da1 + wb.y * db1;
+                const R gx0 = wax * da0 + wbx * db0, gy0 = way * da0 + wby * db0;
+                const R gx1 = wax * da1 + wbx * db1, gy1 = way * da1 + wby * db1;
                 tx0[t] = m0 ? tx0[t] + gx0 : tx0[t];
                 ty0[t] = m0 ? ty0[t] + gy0 : ty0[t];
                 tx1[t] = m1 ? tx1[t] + gx1 : tx1[t];
@@ -2250,8 +2294,10 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
     double *acc_v = out == OUT_TEND ? I.tend_v : nullptr;
     if (out != OUT_UPDATE && !acc_h) throw std::runtime_error("accumulation arrays not allocated");
     if (sp.hor == HOR_MUSCL || sp.hor == HOR_MFCT) {
+        using FW = FanW<R, D>;
+        const size_t flds = (size_t)wv * FW::kCols * FW::kUnits * sizeof(uint4);  // k_gnod_fan's LDS slices
         if (I.grad_fan && I.fan_fault < 0)
-            hipLaunchKernelGGL((k_gnod_fan<R, G, NT, D>), gn, blk, 0, st, m, ab, tsn, gnod, tsg, I.gscale);
+            hipLaunchKernelGGL((k_gnod_fan<R, G, NT, D>), gn, blk, flds, st, m, ab, tsn, gnod, tsg, I.gscale);
         else
             hipLaunchKernelGGL((k_gnod<R, G, NT, D>), gn, blk, 0, st, m, ab, tsn, gnod, tsg, I.gscale);
         hk.mark("k_gnod", st);

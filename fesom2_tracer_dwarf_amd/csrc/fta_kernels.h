@@ -105,6 +105,8 @@ constexpr int kEdgeUnits = 3 + (12 * (int)sizeof(R)) / 16;
 constexpr int kStageEdges = 16;
 // nodes a wave stages into LDS (their NodeEdges + NodeEdgeInfo)
 constexpr int kStageNodes = 16;
+// bytes of nodal-gradient fan weights a wave stages into LDS
+constexpr int kStageFanBytes = 4096;
 template <int D>
 constexpr int kNodeEdgeUnits = (int)(sizeof(NodeEdges<D>) + sizeof(NodeEdgeInfo<D>)) / 16;
 
