@@ -70,32 +70,38 @@ def step_alg_bytes(c, ntr, s_t, s_v, s_m):
             + c["E"] * (16 + 6 * s_m) + c["T"] * (20 + 7 * s_m) + 8 * c["N"] + 4 * c["maxAdj"] * c["N"])
 
 
-def kernel_alg_bytes(name, c, ntr, s_t, s_r, s_g, we_alias=True):
+def kernel_alg_bytes(name, c, ntr, s_t, s_r, s_g, we_alias=True, h_alias=True):
     """Compulsory bytes of one step's launch of a kernel (all ntr tracers
     batched into it): every array it reads or writes counted once over its
     active cells (C node layers, C+N node interfaces, Ce element layers, Ced
     edge levels); mesh and velocity terms once, tracer terms ntr times.
     Gradients count the nodal means only (the up/down-wind triangle values
-    replace them on shared levels), so the figure is a lower bound."""
+    replace them on shared levels), so the figure is a lower bound. w_e shares
+    w's storage and hnode_new hnode's when they are equal (we_alias, h_alias),
+    as on the benchmark mesh; the FP64 copies of areasvol, hnode, hnode_new
+    that the FP32 form keeps for its FP64 accumulation are counted as read."""
     C, Ce, Ced, N, T, E = c["C"], c["Ce"], c["Ced"], c["N"], c["T"], c["E"]
     CI = C + N
-    if name == "k_gnod":          # valuesAB -> nodal mean gradients (2 comps); element geometry, node->element records
-        return ntr * (s_t * C + 2 * s_g * C) + T * 8 * s_r + N * 112
+    h64 = 8 * C if h_alias else 16 * C          # FP64 hnode (and hnode_new)
+    if name == "k_gnod":          # valuesAB -> nodal mean gradients (2 comps); 5 fan weights per element around
+                                  # each node (3T in all), 1-ring records
+        return ntr * (s_t * C + 2 * s_g * C) + 5 * s_r * 3 * T + 48 * N
     if name == "k_edge":          # tracer (T and AB share storage), nodal gradients, vel, helem, edge records and
                                   # up/down-wind weights -> transport, HO-LO
         return ntr * (s_t * C + 2 * s_g * C + s_r * Ced) + s_r * Ced + 3 * s_r * Ce + E * (48 + 16 * s_r)
     if name == "k_node_lo":
-        # edge transport, HO-LO edge fluxes, tracer -> LO (FP64), adf_v, P+, P-; on interfaces w (w_e shares
-        # its storage when equal, as here), area, zbar_3d_n; on layers Z_3d_n and the FP64 hnode, hnode_new
-        # (and the FP64 areasvol copy in FP32; FP64 reads area, equal to it on wet layers)
+        # edge transport, tracer -> LO (FP64), adf_v; on interfaces w (w_e shares its storage when equal),
+        # area, zbar_3d_n; on layers Z_3d_n and the FP64 hnode (hnode_new), and the FP64 areasvol copy in FP32
+        # (FP64 reads area, equal to it on wet layers); node->edge records
         n_if = 3 if we_alias else 4
-        geo = s_r * C + 16 * C + (8 * C if s_r == 4 else 0)
-        return (ntr * (s_r * Ced + s_t * C + 8 * C + s_r * CI + 2 * s_r * C)
-                + s_r * Ced + n_if * s_r * CI + geo + 80 * N)
-    if name == "k_node_limit":    # LO, tracer, P+/P- -> R+/R-; areasvol, 1-ring records
-        return ntr * (8 * C + s_t * C + 4 * s_r * C) + s_r * C + 48 * N
-    if name == "k_node_update":   # HO-LO edge flux, adf_v, R+/R- (own and 1-ring), LO, tracer -> tracer; FP64 areasvol, hnode, hnode_new
-        return ntr * (s_r * Ced + s_r * CI + 2 * s_r * C + 8 * C + 2 * s_t * C) + 24 * C + 80 * N
+        geo = s_r * C + h64 + (8 * C if s_r == 4 else 0)
+        return ntr * (s_t * C + 8 * C + s_r * CI) + s_r * Ced + n_if * s_r * CI + geo + 80 * N
+    if name == "k_node_limit":    # HO-LO edge fluxes, adf_v, LO, tracer -> R+/R-; areasvol, 1-ring and
+                                  # node->edge records
+        return ntr * (s_r * Ced + s_r * CI + 8 * C + s_t * C + 2 * s_r * C) + s_r * C + 128 * N
+    if name == "k_node_update":   # HO-LO edge flux, adf_v, R+/R- (own and 1-ring), LO, tracer -> tracer;
+                                  # FP64 areasvol, hnode (hnode_new)
+        return ntr * (s_r * Ced + s_r * CI + 2 * s_r * C + 8 * C + 2 * s_t * C) + 8 * C + h64 + 80 * N
     if name == "k_node_nolim":
         return ntr * (s_r * Ced + 2 * s_t * C) + 2 * s_r * CI + 2 * s_r * C + 48 * N
     return 0

@@ -92,6 +92,8 @@ SIGNATURES = {
                                                POINTER(c_double)]),
     "tracer_batch_run": (c_int, [c_int, c_int]),
     "tracer_batch_time": (c_int, [c_int, POINTER(c_double)]),
+    "tracer_batch_autotune": (c_int, [c_int, F64, c_int]),
+    "tracer_batch_set_schedule": (c_int, [c_int, c_int]),
     "tracer_batch_mesh_info": (c_int, [c_int, P_int, P_int, P_int, P_ll, P_double]),
     "tracer_batch_get_values": (c_int, [c_int, c_int, F64F]),
     "tracer_batch_diagnostics": (c_int, [c_int, c_int, F64]),

@@ -36,6 +36,18 @@ class TracerBatch:
     def run(self, nsteps, use_graph=True):
         _lib.check(self.lib.tracer_batch_run(int(nsteps), 1 if use_graph else 0), "tracer_batch_run", self.lib)
 
+    def autotune(self, reps=3):
+        """Each mesh keeps its fastest launch schedule; returns [(waves, tracers per launch)] per mesh."""
+        out = np.zeros(2 * max(len(self.meshes), 1))
+        n = self.lib.tracer_batch_autotune(int(reps), out, len(self.meshes))
+        if n < 0:
+            raise RuntimeError(f"tracer_batch_autotune failed: {_lib.last_error(self.lib)}")
+        return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(n)]
+
+    def set_schedule(self, waves, max_nt):
+        _lib.check(self.lib.tracer_batch_set_schedule(int(waves), int(max_nt)), "tracer_batch_set_schedule",
+                   self.lib)
+
     def time(self, nsteps):
         ms = ctypes.c_double()
         _lib.check(self.lib.tracer_batch_time(int(nsteps), ctypes.byref(ms)), "tracer_batch_time", self.lib)

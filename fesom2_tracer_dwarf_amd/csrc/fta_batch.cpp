@@ -259,6 +259,44 @@ int tracer_batch_time(int nsteps, double *ms_out)
     return 0;
 }
 
+int tracer_batch_autotune(int reps, double *table, int max_rows)
+{
+    if (!g_batch || g_batch->meshes.empty()) return bfail("empty batch");
+    Batch &B = *g_batch;
+    try {
+        // each mesh's engine times its own schedules alone (its stream, its
+        // tracers restored afterwards) and keeps the fastest
+        B.drop_graph();
+        int rows = 0;
+        for (size_t i = 0; i < B.meshes.size(); ++i) {
+            BatchMesh &b = *B.meshes[i];
+            b.eng->autotune(b.dt, reps);
+            int w = 0, nt = 0;
+            b.eng->schedule(w, nt);
+            if (table && rows < max_rows) {
+                table[2 * rows] = (double)w;
+                table[2 * rows + 1] = (double)nt;
+            }
+            ++rows;
+        }
+        return rows;
+    } catch (const std::exception &e) {
+        return bfail(e.what());
+    }
+}
+
+int tracer_batch_set_schedule(int waves, int max_nt)
+{
+    if (!g_batch || g_batch->meshes.empty()) return bfail("empty batch");
+    try {
+        g_batch->drop_graph();
+        for (auto &b : g_batch->meshes) b->eng->set_schedule(waves, max_nt);
+    } catch (const std::exception &e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
+
 int tracer_batch_mesh_info(int mesh, int *nl, int *nod2D, int *ntracers, long long *active_cells, double *dt)
 {
     BatchMesh *b = get_mesh(mesh);

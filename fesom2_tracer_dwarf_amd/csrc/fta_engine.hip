@@ -537,7 +537,7 @@ __device__ __forceinline__ VCol2<R> node_vcol(const DMesh<R> &m, const Pair &p, 
         W = ld2(Wv + i);
         ar = ld2(m.area + i);
         if constexpr (VER == VER_QR4C) { zb = ld2(m.zbarn + i); Z = ld2(m.Zn + i); }
-        if constexpr (VER == VER_PPM) { h = ld2(m.hnode + i); hn = ld2(m.hnode_new + i); }
+        if constexpr (VER == VER_PPM) { h = ld2(m.hnode + i); hn = ld_same(m.hnode_new, m.hnode, i, h); }
     }
     c.W = W;
     c.area = ar;
@@ -619,7 +619,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
     V2<double> av;
     if constexpr (std::is_same<R, double>::value) av = lc.area;
     else av = ld2(m.av_d + c);
-    const V2<double> h = ld2(m.h_d + c), hn = ld2(m.hn_d + c);
+    const V2<double> h = ld2(m.h_d + c), hn = ld_same(m.hn_d, m.h_d, c, h);
     constexpr int TG = NT < 2 ? NT : 2;  // tracers whose gathers share a round trip
 #pragma unroll
     for (int t0 = 0; t0 < NT; t0 += TG) {
@@ -961,7 +961,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
     if (lay0) {
         av = ld2(m.av_d + c);
         h = ld2(m.h_d + c);
-        hn = ld2(m.hn_d + c);
+        hn = ld_same(m.hn_d, m.h_d, c, h);
     }
 
     constexpr int TG = NT < 2 ? NT : 2;  // tracers whose gathers share a round trip
@@ -1892,7 +1892,11 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
     d.zbarn = I.track(dupload(I.ln.pack<R>(m.zbar_3d_n.data(), nl, 0)));
     d.areasvol = I.track(dupload(I.ln.pack<R>(m.areasvol.data(), nl, 1)));
     d.hnode = I.track(dupload(I.ln.pack<R>(m.hnode.data(), nl - 1, 1)));
-    d.hnode_new = I.track(dupload(I.ln.pack<R>(m.hnode_new.data(), nl - 1, 1)));
+    // layers that keep their thickness over the step (hnode_new == hnode, no
+    // tracer_set_thickness): one device copy, which the node kernels then
+    // read once (ld_same)
+    const bool same_h = m.hnode_new == m.hnode;
+    d.hnode_new = same_h ? d.hnode : I.track(dupload(I.ln.pack<R>(m.hnode_new.data(), nl - 1, 1)));
     d.Zn = I.track(dupload(I.ln.pack<R>(m.Z_3d_n.data(), nl - 1, 1)));
     d.helem = I.track(dupload(I.lt.pack<R>(m.helem.data(), nl - 1, 0)));
     if constexpr (std::is_same<R, double>::value) {
@@ -1900,7 +1904,7 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
     } else {
         d.av_d = I.track(dupload(I.ln.pack<double>(m.areasvol.data(), nl, 1)));
         d.h_d = I.track(dupload(I.ln.pack<double>(m.hnode.data(), nl - 1, 1)));
-        d.hn_d = I.track(dupload(I.ln.pack<double>(m.hnode_new.data(), nl - 1, 1)));
+        d.hn_d = same_h ? d.h_d : I.track(dupload(I.ln.pack<double>(m.hnode_new.data(), nl - 1, 1)));
     }
 }
 

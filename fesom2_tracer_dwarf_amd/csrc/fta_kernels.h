@@ -295,6 +295,16 @@ __device__ __forceinline__ void st2(float *p, float a, float b)
     *reinterpret_cast<float2 *>(p) = make_float2(a, b);
 }
 
+// the pair at i of array b, or `known` (the pair at i of array a) when b is
+// a's storage (the host shares one copy between arrays that are equal); the
+// test is on kernel arguments, so the branch is uniform
+template <class T>
+__device__ __forceinline__ V2<T> ld_same(const T *b, const T *a, size_t i, V2<T> known)
+{
+    if (b == a) return known;
+    return ld2(b + i);
+}
+
 // two cells x two components (x0 y0 x1 y1), gradients and velocity
 template <class T>
 struct V4 {

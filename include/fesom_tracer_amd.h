@@ -331,6 +331,13 @@ int tracer_batch_add_periodic_mesh(int nx, int ny, int nl, double Lx, double Ly,
 int tracer_batch_run(int nsteps, int use_graph);
 /* hipEvent time of nsteps graph launches, milliseconds */
 int tracer_batch_time(int nsteps, double *ms_out);
+/* every mesh's engine runs the autotuner (tracer_autotune) on reps steps of
+ * its own and keeps its fastest schedule; table[2*i..] = {waves, tracers per
+ * launch} of mesh i (up to max_rows meshes); returns the number of meshes */
+int tracer_batch_autotune(int reps, double *table, int max_rows);
+/* one launch schedule for every mesh (waves per workgroup 2, 4, 8; tracers
+ * per launch 1, 2, 4) */
+int tracer_batch_set_schedule(int waves, int max_nt);
 int tracer_batch_mesh_info(int mesh, int *nl, int *nod2D, int *ntracers, long long *active_cells, double *dt);
 int tracer_batch_get_values(int mesh, int tracer_id, double *values);
 int tracer_batch_diagnostics(int mesh, int tracer_id, double *out);
