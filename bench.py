@@ -235,12 +235,14 @@ def batched_form(steps):
         i, _ = b.add_periodic_mesh(n, n, nl, 2.0e3 * n, 2.0e3 * n, seed=10 + k, ntracers=32, precision=32)
         cells += b.info(i)["active_cells"]
     setup = time.perf_counter() - t0
+    sched = b.autotune(2)  # each mesh keeps its fastest launch schedule (tracers restored)
     b.run(2)  # warm-up (captures the graph)
     ms = b.time(steps)
     b.close()
     return dict(workload="8 periodic meshes 256^2..1024^2 nodes, 32..64 levels, 32 tracers each, FP32",
                 steps=steps, ms_per_step=ms / steps, updates_per_s=32 * cells * steps / (ms * 1e-3),
-                active_cells=cells, graph_launches_per_step=1, setup_s=setup)
+                active_cells=cells, graph_launches_per_step=1, setup_s=setup,
+                schedules=[dict(block_waves=w, batch_tracers=nt) for w, nt in sched])
 
 
 def learned_form(api, steps):

@@ -584,45 +584,41 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
         lc.W = {(double)we.x, (double)we.y};
         lc.area = {(double)ar.x, (double)ar.y};
     }
-    // vertical fluxes of every tracer first: the column reconstruction's
-    // state (windows of Z, zbar, W) is dead before the edge loops start
-    V2<double> T[NT];
-    double fl0[NT], fl1[NT], fl2[NT];
-    {
-        VCol2<R> hc = node_vcol<R, VER>(m, p, nlev, Wv);
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const size_t tn = t * tsn;
-            T[t] = V2<double>{0.0, 0.0};
-            if (p.live) T[t] = ld2(ttf + tn + c);
-            lc.A = tracer_window<double>(T[t], k0, nlev);
-            // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
-            fl0[t] = vflux_upw1<0>(lc, 0.0);
-            fl1[t] = vflux_upw1<1>(lc, 0.0);
-            fl2[t] = from_next(fl0[t]);
-            if constexpr (SAME) {
-                hc.A = tracer_window<R>(T[t], k0, nlev);
-            } else {
-                V2<double> B{0.0, 0.0};
-                if (p.live) B = ld2(ttfAB + tn + c);
-                hc.A = tracer_window<R>(B, k0, nlev);
-            }
-            R ad0, ad1;
-            ver_flux<R, VER>(hc, num_ord, (R)dt, (R)fl0[t], (R)fl1[t], ad0, ad1);  // HO - LO
-            if (p.live) st2(adf_v + tn + c, if0 ? ad0 : (R)0, if1 ? ad1 : (R)0);
-        }
-    }
-    if (!lay0) return;
-
-    // FP64 geometry (the LO solution feeds the conservative update); without
-    // cavities areasvol equals area on every wet layer
-    V2<double> av;
-    if constexpr (std::is_same<R, double>::value) av = lc.area;
-    else av = ld2(m.av_d + c);
-    const V2<double> h = ld2(m.h_d + c), hn = ld_same(m.hn_d, m.h_d, c, h);
+    // tracers in groups of TG, each group start to end (vertical fluxes, then
+    // the edge sums), so that only one group's state is live at a time
     constexpr int TG = NT < 2 ? NT : 2;  // tracers whose gathers share a round trip
 #pragma unroll
     for (int t0 = 0; t0 < NT; t0 += TG) {
+        // vertical fluxes of the group's tracers first: the column
+        // reconstruction's state (windows of Z, zbar, W) is dead before the
+        // edge loop starts (every lane takes part: the windows cross lanes)
+        V2<double> T[TG];
+        double fl0[TG], fl1[TG], fl2[TG];
+        {
+            VCol2<R> hc = node_vcol<R, VER>(m, p, nlev, Wv);
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                const size_t tn = (t0 + u) * tsn;
+                T[u] = V2<double>{0.0, 0.0};
+                if (p.live) T[u] = ld2(ttf + tn + c);
+                lc.A = tracer_window<double>(T[u], k0, nlev);
+                // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
+                fl0[u] = vflux_upw1<0>(lc, 0.0);
+                fl1[u] = vflux_upw1<1>(lc, 0.0);
+                fl2[u] = from_next(fl0[u]);
+                if constexpr (SAME) {
+                    hc.A = tracer_window<R>(T[u], k0, nlev);
+                } else {
+                    V2<double> B{0.0, 0.0};
+                    if (p.live) B = ld2(ttfAB + tn + c);
+                    hc.A = tracer_window<R>(B, k0, nlev);
+                }
+                R ad0, ad1;
+                ver_flux<R, VER>(hc, num_ord, (R)dt, (R)fl0[u], (R)fl1[u], ad0, ad1);  // HO - LO
+                if (p.live) st2(adf_v + tn + c, if0 ? ad0 : (R)0, if1 ? ad1 : (R)0);
+            }
+        }
+        if (!lay0) continue;
         // LO horizontal sums over incident edges, edge-index order; edge slots
         // outer and the group's tracers inner, so that one memory round trip
         // per edge carries its transport and the other node's value of TG tracers
@@ -647,23 +643,28 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             for (int u = 0; u < TG; ++u) To[u] = ld2(ttf + (t0 + u) * tsn + (size_t)oo[j] + k0);
 #pragma unroll
             for (int u = 0; u < TG; ++u) {
-                slo0[u] = slo0[u] + lo_flux((double)v.x * sg, T[t0 + u].x, To[u].x);
+                slo0[u] = slo0[u] + lo_flux((double)v.x * sg, T[u].x, To[u].x);
                 // second cell by selects, so that each pair stays one 16-byte load
-                const double s1 = slo1[u] + lo_flux((double)v.y * sg, T[t0 + u].y, To[u].y);
+                const double s1 = slo1[u] + lo_flux((double)v.y * sg, T[u].y, To[u].y);
                 slo1[u] = on1 ? s1 : slo1[u];
             }
         }
+        // FP64 geometry (the LO solution feeds the conservative update); without
+        // cavities areasvol equals area on every wet layer
+        V2<double> av;
+        if constexpr (std::is_same<R, double>::value) av = lc.area;
+        else av = ld2(m.av_d + c);
+        const V2<double> h = ld2(m.h_d + c), hn = ld_same(m.hn_d, m.h_d, c, h);
 #pragma unroll
         for (int u = 0; u < TG; ++u) {
-            const int t = t0 + u;
             // flux differences formed in FP64: each interface flux then leaves one
             // layer and enters the next with exactly opposite sign in every precision
             const double LO0 =
-                (T[t].x * (double)h.x + (slo0[u] + (fl0[t] - fl1[t])) * dt / (double)av.x) / (double)hn.x;
+                (T[u].x * (double)h.x + (slo0[u] + (fl0[u] - fl1[u])) * dt / (double)av.x) / (double)hn.x;
             // second layer branch-free (selected away below a one-layer pair)
             const double L1 =
-                (T[t].y * (double)h.y + (slo1[u] + (fl1[t] - fl2[t])) * dt / (double)av.y) / (double)hn.y;
-            st2(LO + t * tsn + c, LO0, lay1 ? L1 : 0.0);
+                (T[u].y * (double)h.y + (slo1[u] + (fl1[u] - fl2[u])) * dt / (double)av.y) / (double)hn.y;
+            st2(LO + (t0 + u) * tsn + c, LO0, lay1 ? L1 : 0.0);
         }
     }
 }
@@ -698,16 +699,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
     static_assert(NT % TG == 0, "tracer batches are 1, 2 or 4");
 
     // the chunk's node->edge records go to LDS first (every lane of the wave
-    // takes part); the bounds come next, then per tracer group the P+/P- sums
-    // and R+/R-, so that the sums are live only briefly
+    // takes part); then per group of TG tracers the bounds, the P+/P- sums
+    // and R+/R-, so that only one group's state is live at a time
     extern __shared__ uint4 stage_all[];
     uint4 *slice = stage_all + (threadIdx.x >> 6) * (kStageNodes * kNodeEdgeUnits<D>);
     const StageRange sr = stage_node_edges<D>(m, p, slice);
 
-    double tvmax0[NT], tvmin0[NT], tvmax1[NT], tvmin1[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) { tvmax0[t] = 0; tvmin0[t] = 0; tvmax1[t] = 0; tvmin1[t] = 0; }
-    if (lay0) {
+    NodeRing<D> rg;
+    int lw[D];  // layers of the deeper element next to ring node j
+    bool dry0 = false, dry1 = false;
+    {
         // a2/a3 as one max/min per level: element e's bound is the max (min)
         // of max(LO, T) (min(LO, T)) over its three nodes when it has the
         // level, -bignumber (+bignumber) when it does not; the node's bound is
@@ -717,7 +718,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
         // exact, so this equals the element-by-element form bit for bit).
         // Fan element j lies between ring nodes j and j + 1 (ring node 0 where
         // the fan closes, j + 1 == nr); le[j] = 0 where a boundary fan is open
-        const NodeRing<D> rg = node_ring<D>(m)[p.col];
+        if (lay0) rg = node_ring<D>(m)[p.col];
+        else {
+#pragma unroll
+            for (int j = 0; j < D; ++j) { rg.oo[j] = -1; rg.lr[j] = 0; rg.le[j] = 0; }
+        }
         int nr = 0, le_last = 0, lemin = 255;
 #pragma unroll
         for (int j = 0; j < D; ++j) nr += rg.oo[j] >= 0 ? 1 : 0;
@@ -726,14 +731,20 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
             le_last = j == nr - 1 ? (int)rg.le[j] : le_last;
             lemin = rg.le[j] != 0 ? min(lemin, (int)rg.le[j]) : lemin;
         }
-        int lw[D];  // layers of the deeper element next to ring node j
 #pragma unroll
         for (int j = 0; j < D; ++j) lw[j] = rg.oo[j] >= 0 ? max(j == 0 ? le_last : (int)rg.le[j - 1], (int)rg.le[j]) : 0;
-        const bool dry0 = k0 >= lemin, dry1 = k0 + 1 >= lemin;
-        // ring nodes outer, tracers inner (groups of TG): one memory round
-        // trip per ring node carries the LO and tracer pairs of TG tracers
+        dry0 = k0 >= lemin;
+        dry1 = k0 + 1 >= lemin;
+    }
+    const R flux_eps = (R)1e-16;
 #pragma unroll
-        for (int t0 = 0; t0 < NT; t0 += TG) {
+    for (int t0 = 0; t0 < NT; t0 += TG) {
+        // bounds: ring nodes outer, the group's tracers inner, one memory
+        // round trip per ring node carrying the LO and tracer pairs of TG tracers
+        double tvmax0[TG], tvmin0[TG], tvmax1[TG], tvmin1[TG];
+#pragma unroll
+        for (int u = 0; u < TG; ++u) { tvmax0[u] = 0; tvmin0[u] = 0; tvmax1[u] = 0; tvmin1[u] = 0; }
+        if (lay0) {
             double vmax0[TG], vmin0[TG], vmax1[TG], vmin1[TG];
 #pragma unroll
             for (int u = 0; u < TG; ++u) {
@@ -765,21 +776,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
             }
 #pragma unroll
             for (int u = 0; u < TG; ++u) {
-                tvmax0[t0 + u] = dry0 ? fmx(vmax0[u], -bignumber) : vmax0[u];
-                tvmin0[t0 + u] = dry0 ? fmn(vmin0[u], bignumber) : vmin0[u];
-                tvmax1[t0 + u] = lay1 ? (dry1 ? fmx(vmax1[u], -bignumber) : vmax1[u]) : 0.0;
-                tvmin1[t0 + u] = lay1 ? (dry1 ? fmn(vmin1[u], bignumber) : vmin1[u]) : 0.0;
+                tvmax0[u] = dry0 ? fmx(vmax0[u], -bignumber) : vmax0[u];
+                tvmin0[u] = dry0 ? fmn(vmin0[u], bignumber) : vmin0[u];
+                tvmax1[u] = lay1 ? (dry1 ? fmx(vmax1[u], -bignumber) : vmax1[u]) : 0.0;
+                tvmin1[u] = lay1 ? (dry1 ? fmn(vmin1[u], bignumber) : vmin1[u]) : 0.0;
             }
         }
-    }
-    int oe[D], oo[D];
-    unsigned info[D];
-    load_node_edges<D, true>(m, sr, slice, p.col, lay0, oe, info, oo);
-    V2<R> av{1, 1};
-    if (lay0) av = ld2(m.areasvol + c);
-    const R flux_eps = (R)1e-16;
-#pragma unroll
-    for (int t0 = 0; t0 < NT; t0 += TG) {
+        // the node->edge records (from the LDS slice when staged), loaded
+        // here so that they are not live during the bounds
+        int oe[D], oo[D];
+        unsigned info[D];
+        load_node_edges<D, true>(m, sr, slice, p.col, lay0, oe, info, oo);
+        V2<R> av{1, 1};
+        if (lay0) av = ld2(m.areasvol + c);
         // P+/P-: the vertical HO-LO fluxes of interfaces k0+1, k0+2 (layer k0)
         // and k0+2, k0+3 (layer k0+1; the last one in the next lane: where it
         // is used the layer below exists, so that lane holds the same column),
@@ -821,8 +830,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
 #pragma unroll
         for (int u = 0; u < TG; ++u) {
             const int t = t0 + u;
-            const double upx0 = from_prev(tvmax1[t]), upn0 = from_prev(tvmin1[t]);
-            const double dnx1 = from_next(tvmax0[t]), dnn1 = from_next(tvmin0[t]);
+            const double upx0 = from_prev(tvmax1[u]), upn0 = from_prev(tvmin1[u]);
+            const double dnx1 = from_next(tvmax0[u]), dnn1 = from_next(tvmin0[u]);
             if (!lay0) continue;
             const size_t tn = t * tsn;
             const V2<double> lo = ld2(LO + tn + c);
@@ -833,11 +842,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
                 const int nz = k0 + 1;
                 double qp, qm;
                 if (nz == 1 || nz == L) {
-                    qp = tvmax0[t] - lo.x;
-                    qm = tvmin0[t] - lo.x;
+                    qp = tvmax0[u] - lo.x;
+                    qm = tvmin0[u] - lo.x;
                 } else {
-                    qp = fmx(fmx(upx0, tvmax0[t]), tvmax1[t]) - lo.x;
-                    qm = fmn(fmn(upn0, tvmin0[t]), tvmin1[t]) - lo.x;
+                    qp = fmx(fmx(upx0, tvmax0[u]), tvmax1[u]) - lo.x;
+                    qm = fmn(fmn(upn0, tvmin0[u]), tvmin1[u]) - lo.x;
                 }
                 R flux = P.x * (R)dt / av.x + flux_eps;
                 rp0 = fmn((R)1, (R)qp / flux);
@@ -850,11 +859,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
                 const int nz = k0 + 2;
                 double qp, qm;
                 if (nz == L) {
-                    qp = tvmax1[t] - lo.y;
-                    qm = tvmin1[t] - lo.y;
+                    qp = tvmax1[u] - lo.y;
+                    qm = tvmin1[u] - lo.y;
                 } else {
-                    qp = fmx(fmx(tvmax0[t], tvmax1[t]), dnx1) - lo.y;
-                    qm = fmn(fmn(tvmin0[t], tvmin1[t]), dnn1) - lo.y;
+                    qp = fmx(fmx(tvmax0[u], tvmax1[u]), dnx1) - lo.y;
+                    qm = fmn(fmn(tvmin0[u], tvmin1[u]), dnn1) - lo.y;
                 }
                 R flux = P.y * (R)dt / av.y + flux_eps;
                 rp1 = fmn((R)1, (R)qp / flux);

@@ -45,3 +45,26 @@ def test_batch_graph_matches_eager_and_conserves():
     ms = b.time(3)
     assert ms > 0.0
     b.close()
+
+
+def test_batch_schedules_give_identical_tracers():
+    """The batch autotuner (each mesh picks its own waves per workgroup and
+    tracers per launch) and a fixed schedule change only the launch shape:
+    the tracers after a few steps are bitwise those of the default schedule."""
+    print("[gpu] batch: default vs autotuned vs (2 waves, 1 tracer) schedules", flush=True)
+    results = []
+    for mode in ("default", "tuned", "fixed"):
+        b = _build()
+        if mode == "tuned":
+            sched = b.autotune(1)
+            assert len(sched) == len(MESHES)
+            assert all(w in (2, 4, 8) and nt in (1, 2, 4) for w, nt in sched)
+        elif mode == "fixed":
+            b.set_schedule(2, 1)
+        b.run(3, use_graph=True)
+        results.append([[b.values(i, t + 1) for t in range(kw["ntracers"])] for i, kw in enumerate(MESHES)])
+        b.close()
+    for other in results[1:]:
+        for i, kw in enumerate(MESHES):
+            for t in range(kw["ntracers"]):
+                assert np.array_equal(other[i][t], results[0][i][t]), f"mesh {i} tracer {t + 1}"
