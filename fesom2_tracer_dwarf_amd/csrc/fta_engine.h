@@ -7,7 +7,7 @@
 // edges follow a space-filling curve when the mesh was generated with one),
 // into 128-cell chunks; a wavefront handles one chunk, two cells per lane, so
 // a sweep over consecutive columns streams through contiguous memory
-// (requires nl <= 64).
+// (requires nl <= 128 level interfaces).
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -41,9 +41,9 @@ void set_error(const std::string &s) { g_last_error = s; }
                                      " at " #call);                                           \
     } while (0)
 
-// node->edge slot info: nlmax | 64 if the node is edges(1) | other node's layers << 7
-__device__ __forceinline__ int info_nlmax(unsigned i) { return (int)(i & 63u); }
-__device__ __forceinline__ bool info_pos(unsigned i) { return (i & 64u) != 0; }
+// node->edge slot info: nlmax (7 bits) | 128 if the node is edges(1) | other node's layers << 8
+__device__ __forceinline__ int info_nlmax(unsigned i) { return (int)(i & 127u); }
+__device__ __forceinline__ bool info_pos(unsigned i) { return (i & 128u) != 0; }
 
 // node -> edge records from their 16-byte units (global memory or LDS)
 template <int D>
@@ -1750,7 +1750,7 @@ static void build_node_records(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, 
             const int other = se > 0 ? m.edges[2 * e + 1] : m.edges[2 * e];
             ne[n].oe[j] = (int)I.le.off[e];
             ne[n].oo[j] = (int)I.ln.off[other];
-            ni[n].info[j] = (unsigned short)(enlmax[e] | (se > 0 ? 64 : 0) | ((m.nlevels_nod2D[other] - 1) << 7));
+            ni[n].info[j] = (unsigned short)(enlmax[e] | (se > 0 ? 128 : 0) | ((m.nlevels_nod2D[other] - 1) << 8));
         }
         for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) {
             const int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
@@ -1920,7 +1920,8 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
 Engine::Engine(const HostMesh &m, int ntracers, int precision)
     : p_(new Impl), N_(m.nod2D), T_(m.elem2D), E_(m.edge2D), nl_(m.nl), ntr_(ntracers), prec_(precision)
 {
-    if (nl_ > kWave) throw std::runtime_error("the device path supports nl <= 64 level interfaces");
+    // a column's interfaces fill at most one 128-cell chunk (two per lane of one wave)
+    if (nl_ > kMaxLevels) throw std::runtime_error("the device path supports nl <= 128 level interfaces");
     if (ntracers < 1) throw std::runtime_error("need at least one tracer");
     if (precision < PREC_FP64 || precision > PREC_FP32_H16) throw std::runtime_error("unknown precision");
     for (int u : m.ulevels) if (u != 1) throw std::runtime_error("cavities (ulevels > 1) are not supported");

@@ -38,7 +38,8 @@ constexpr int kMaxBlock = 512;                     // chunk kernels take 64..512
 constexpr int kChunk = 2 * kWave;   // cells per wave chunk
 constexpr int kDeg = 8;             // node record width (incident edges / elements / 1-ring) of ordinary meshes
 constexpr int kDegWide = 16;        // the width used when some node has more than kDeg
-constexpr int kColBits = 26;        // slot word: column | pair index << kColBits
+constexpr int kColBits = 25;        // slot word: column | pair index << kColBits (pairs 0..63, sign bit clear)
+constexpr int kMaxLevels = 2 * kWave;  // level interfaces a column may have: a whole column in one chunk
 constexpr double kRE = 6367500.0;   // r_earth, lib/oce_modules.F90:24
 
 // per-edge record: layer offsets of both nodes, of both elements (t2 = -1 on
@@ -62,7 +63,7 @@ struct alignas(16) ElemRec {
 //
 // node -> incident edges (sorted by edge index), D slots:
 // oe = edge offset (-1 = empty), oo = layer offset of the edge's other node,
-// info = nlmax | 64 if the node is edges(1) | (layers of the other node) << 7
+// info = nlmax (7 bits) | 128 if the node is edges(1) | (layers of the other node) << 8
 template <int D>
 struct alignas(16) NodeEdges {
     int oe[D];

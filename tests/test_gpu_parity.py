@@ -23,9 +23,14 @@ MESHES = {
                                           stretch=0.3, lev_lo=3, lev_hi=10, seed=5),
     "periodic_20x20x10_flat": dict(nx=20, ny=20, nl=10, periodic=True, Lx=1.0e5, Ly=1.0e5, dz=10.0,
                                    stretch=0.0, lev_lo=10, lev_hi=10, seed=0),
-    # 64 interfaces: the deepest column the device layout takes (a whole column in one 128-cell chunk)
+    # 64 interfaces: the benchmark's depth
     "periodic_20x16x64_random_bottom": dict(nx=20, ny=16, nl=64, periodic=True, Lx=1.0e5, Ly=0.8e5, dz=8.0,
                                             stretch=0.5, lev_lo=3, lev_hi=64, seed=13),
+    # 128 interfaces: the deepest column the device layout takes (a whole column in one 128-cell chunk,
+    # pair index 63 in the slot word, 127 layers in the node->edge records)
+    # (vertical outflow Courant number 0.72: FCT's no-new-extrema bound needs a monotone LO step, CFL <= 1)
+    "periodic_16x12x128_random_bottom": dict(nx=16, ny=12, nl=128, periodic=True, Lx=1.6e5, Ly=1.2e5, dz=8.0,
+                                             stretch=0.3, lev_lo=3, lev_hi=128, seed=17),
 }
 
 SCHEMES = [("UPW1", "UPW1", "NONE"), ("UPW1", "UPW1", "FCT"), ("MUSCL", "QR4C", "FCT"), ("MFCT", "PPM", "FCT"),
