@@ -115,14 +115,16 @@ __device__ __forceinline__ void load_node_edges(const DMesh<R> &m, const StageRa
     }
 }
 
-// records without staging (kernels that do not stage)
+// records without staging (kernels that do not stage); the other nodes are
+// decoded too and dropped: leaving them out makes the compiler keep one slot
+// of the partly written arrays in scratch memory
 template <int D, class R>
 __device__ __forceinline__ void load_node_edges(const DMesh<R> &m, int n, bool live, int (&oe)[D],
                                                 unsigned (&info)[D])
 {
     int oo[D];
     const StageRange none{0, 0, false};
-    load_node_edges<D, false>(m, none, nullptr, n, live, oe, info, oo);
+    load_node_edges<D, true>(m, none, nullptr, n, live, oe, info, oo);
 }
 
 // ===========================================================================
