@@ -874,8 +874,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
                 qp1 = qp;
                 qm1 = qm;
             }
-            st2(Rp + tn + c, rp0, rp1);
-            st2(Rm + tn + c, rm0, rm1);
+            st_rpm(Rp, Rm, tn + c, rp0, rp1, rm0, rm1);
             if constexpr (STOREQ) {  // Q+/Q- and P+/P- for the learned limiter's features and bound check
                 st2(Qp_out + tn + c, (R)qp0, (R)qp1);
                 st2(Qm_out + tn + c, (R)qm0, (R)qm1);
@@ -989,7 +988,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             us[u] = V2<R>{0, 0};
             if (if0) fv = ld2(adf_v + tn + c);
             if (!if1) fv.y = 0;
-            if (lay0) { rp[u] = ld2(Rp + tn + c); rm[u] = ld2(Rm + tn + c); }
+            if (lay0) ld_rpm(Rp, Rm, tn + c, rp[u], rm[u]);
             if (!lay1) { rp[u].y = 0; rm[u].y = 0; }
             const R rpu0 = from_prev(rp[u].y), rmu0 = from_prev(rm[u].y);
             fvl0[u] = limit_vertical(k0 + 1, lay0, fv.x, rp[u].x, rm[u].x, rpu0, rmu0);
@@ -1015,8 +1014,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             for (int u = 0; u < TG; ++u) {
                 const size_t tn = (t0 + u) * tsn, te = (t0 + u) * tse;
                 f[u] = ld2(adf_h + te + (size_t)oe[j] + k0);
-                rpo[u] = ld2(Rp + tn + (size_t)oo[j] + k0);
-                rmo[u] = ld2(Rm + tn + (size_t)oo[j] + k0);
+                ld_rpm(Rp, Rm, tn + (size_t)oo[j] + k0, rpo[u], rmo[u]);
             }
 #pragma unroll
             for (int u = 0; u < TG; ++u) {
@@ -1188,8 +1186,9 @@ __device__ __forceinline__ void edge_features(const DMesh<R> &m, const EdgeRec &
         y[q] = 0.0f;
     }
     if (S.Rp) {  // training rows: Zalesak's coefficient (b3) as the target
-        const V2<R> rp1 = ld2(S.Rp + er.o1 + k0), rm1 = ld2(S.Rm + er.o1 + k0);
-        const V2<R> rp2 = ld2(S.Rp + er.o2 + k0), rm2 = ld2(S.Rm + er.o2 + k0);
+        V2<R> rp1, rm1, rp2, rm2;
+        ld_rpm(S.Rp, S.Rm, (size_t)er.o1 + k0, rp1, rm1);
+        ld_rpm(S.Rp, S.Rm, (size_t)er.o2 + k0, rp2, rm2);
         y[0] = (float)edge_alpha(true, a.x, rp1.x, rm1.x, rp2.x, rm2.x);
         y[1] = (float)edge_alpha(true, a.y, rp1.y, rm1.y, rp2.y, rm2.y);
     }
@@ -1206,7 +1205,8 @@ template <class R, int NT>
 __device__ __forceinline__ LearnSrc<R> learn_src(const LearnArgs<R, NT> &a, int t)
 {
     return LearnSrc<R>{a.ttf + t * a.tsn, a.LO + t * a.tsn, a.adf + t * a.tse, a.vfl,
-                       a.Qp + t * a.tsn, a.Qm + t * a.tsn, a.Rp + t * a.tsn, a.Rm + t * a.tsn,
+                       a.Qp + t * a.tsn, a.Qm + t * a.tsn, a.Rp + t * a.tsn * kRpmPerCell<R>,
+                       a.Rm + t * a.tsn * kRpmPerCell<R>,
                        a.Pp + t * a.tsn, a.Pm + t * a.tsn};
 }
 
@@ -1364,7 +1364,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_check(DMesh<R> m, const R *
         V2<R> fv{0, 0}, rp{0, 0}, rm{0, 0};
         if (if0) fv = ld2(adf_v + tn + c);
         if (!if1) fv.y = 0;
-        if (lay0) { rp = ld2(Rp + tn + c); rm = ld2(Rm + tn + c); }
+        if (lay0) ld_rpm(Rp, Rm, tn + c, rp, rm);
         if (!lay1) { rp.y = 0; rm.y = 0; }
         const R rpu0 = from_prev(rp.y), rmu0 = from_prev(rm.y);
         const R fvl0 = limit_vertical(k0 + 1, lay0, fv.x, rp.x, rm.x, rpu0, rmu0);
@@ -1965,8 +1965,9 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     I.vfl = zalloc(I.tse * rs);
     I.adf_h = zalloc(I.tse * I.wslots * rs);
     I.adf_v = zalloc(I.tsn * I.wslots * rs);
-    I.Pp = zalloc(I.tsn * I.wslots * rs);
-    I.Pm = zalloc(I.tsn * I.wslots * rs);
+    // R+/R-: two arrays in FP64, one interleaved array in FP32 (ld_rpm)
+    I.Pp = zalloc(I.tsn * I.wslots * 2 * rs);
+    I.Pm = rs == 8 ? (char *)I.Pp + I.tsn * I.wslots * rs : I.Pp;
     I.LO = (double *)zalloc(I.tsn * I.wslots * sizeof(double));
     I.gnod = zalloc(2 * I.tsn * I.wslots * gs);
     I.dpart = I.track(dalloc<DiagAcc>(I.ndiag));
@@ -2846,6 +2847,22 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
     }
     const int slot = work_slot(spec_, tr, p_->tune_nt);
     const size_t per = (size_t)lay->total * comps;
+    // FP32 R+/R- sit interleaved per cell pair (ld_rpm): fetch the slot's
+    // pairs and take every other two values
+    const bool rpm32 = es == 4 && (name == "fct_plus" || name == "fct_minus");
+    if (rpm32) {
+        std::vector<float> pr(2 * per);
+        HIP_OK(hipMemcpy(pr.data(), (const char *)I.Pp + slot * 2 * per * es, pr.size() * es,
+                         hipMemcpyDeviceToHost));
+        const size_t half = name == "fct_plus" ? 0 : 2;
+        std::fill(out, out + need, 0.0);
+        for (size_t c = 0; c < lay->len.size(); ++c)
+            for (int k = 0; k < std::min(lay->len[c] - sub, L); ++k) {
+                const size_t i = (size_t)lay->off[c] + k;
+                out[(size_t)c * L + k] = pr[2 * (i & ~(size_t)1) + half + (i & 1)];
+            }
+        return 0;
+    }
     std::vector<char> buf(per * es);
     HIP_OK(hipMemcpy(buf.data(), src + slot * per * es, buf.size(), hipMemcpyDeviceToHost));
     std::fill(out, out + need, 0.0);

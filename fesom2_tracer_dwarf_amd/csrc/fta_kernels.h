@@ -306,6 +306,35 @@ __device__ __forceinline__ V2<T> ld_same(const T *b, const T *a, size_t i, V2<T>
     return ld2(b + i);
 }
 
+// Zalesak's R+/R- of the cell pair at i (i even, tracer offset included).
+// FP64: two arrays, each pair one 16-byte access. FP32: one array holding
+// (R+ k, R+ k+1, R- k, R- k+1) per pair, so that both are one 16-byte access
+// (the array is twice as long per tracer; Rm is not used).
+__device__ __forceinline__ void ld_rpm(const double *Rp, const double *Rm, size_t i, V2<double> &rp,
+                                       V2<double> &rm)
+{
+    rp = ld2(Rp + i);
+    rm = ld2(Rm + i);
+}
+__device__ __forceinline__ void ld_rpm(const float *Rp, const float *, size_t i, V2<float> &rp, V2<float> &rm)
+{
+    const float4 v = *reinterpret_cast<const float4 *>(Rp + 2 * i);
+    rp = {v.x, v.y};
+    rm = {v.z, v.w};
+}
+__device__ __forceinline__ void st_rpm(double *Rp, double *Rm, size_t i, double p0, double p1, double m0, double m1)
+{
+    st2(Rp + i, p0, p1);
+    st2(Rm + i, m0, m1);
+}
+__device__ __forceinline__ void st_rpm(float *Rp, float *, size_t i, float p0, float p1, float m0, float m1)
+{
+    *reinterpret_cast<float4 *>(Rp + 2 * i) = make_float4(p0, p1, m0, m1);
+}
+// elements of R+/R- storage per cell (see ld_rpm)
+template <class R>
+constexpr size_t kRpmPerCell = sizeof(R) == 4 ? 2 : 1;
+
 // two cells x two components (x0 y0 x1 y1), gradients and velocity
 template <class T>
 struct V4 {

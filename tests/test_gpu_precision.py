@@ -90,3 +90,35 @@ def test_precision_error_of_fp64_against_itself_is_zero(api):
     d, dt, trs = setup_case(api, MESH, SCHEMES, ntr=2, seed=2)
     err = api.precision_error(64, 3, dt)
     assert np.all(err == 0.0)
+
+
+def test_fp32_limiter_factors_against_fp64(api):
+    """Zalesak's R+/R- (work arrays fct_plus / fct_minus) after one step from
+    the same state: in [0, 1] in both precisions (min(1, Q/P) with Q and P of
+    one sign), and the FP32 factors -- which the device keeps interleaved per
+    cell pair -- equal the FP64 ones to rounding except where the limiter
+    sits on a switching point: a swapped or shifted read-out would differ on
+    the limited cells, which the second bar looks at."""
+    print("[gpu] FP32 R+/R- against FP64", flush=True)
+    d, dt, trs = setup_case(api, MESH, SCHEMES, ntr=2)
+    m = mesh_dict(api)
+    L, N = d["nl"] - 1, d["nod2D"]
+    act = np.arange(L)[None, :] < (m["nlevels_nod2D"] - 1)[:, None]   # (N, L)
+    out = {}
+    for prec in (64, 32):
+        api.set_option("precision", prec)
+        for i, t in enumerate(trs):
+            api.set_values(i + 1, t)
+        api.advect_steps(1, dt)
+        out[prec] = [api.work_array(k, L * N, tracer_id=1).reshape(N, L)[act] for k in ("fct_plus", "fct_minus")]
+    for name, a64, a32 in zip(("R+", "R-"), out[64], out[32]):
+        for a in (a64, a32):
+            assert a.min() >= 0.0 and a.max() <= 1.0, name
+        diff = np.abs(a32 - a64)
+        lim = a64 < 0.5
+        far = np.mean(diff > 1e-2)
+        med = np.median(diff[lim]) if lim.any() else 0.0
+        print(f"[gpu]   {name}: {lim.sum()} limited layers, median |d| there {med:.2e}, share |d| > 1e-2 {far:.2e}",
+              flush=True)
+        assert lim.sum() > 0, name
+        assert far <= 1e-2 and med <= 1e-4, name
