@@ -522,7 +522,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
                 ho1 = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux1 * num_ord * (R)0.5 * (Tmean1 + Tmean2);
             }
         }
-        st2(flux + t * tse + o, (R)((double)ho0 - lo0), valid1 ? (R)((double)ho1 - lo1) : (R)0);
+        st2(flux + adf_at<R>(t, tse, o), (R)((double)ho0 - lo0), valid1 ? (R)((double)ho1 - lo1) : (R)0);
     }
 }
 
@@ -815,8 +815,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
             const bool on1 = lay1 && k0 + 1 < nlm;
             const bool pos = info_pos(info[j]);
             V2<R> a[TG];
-#pragma unroll
-            for (int u = 0; u < TG; ++u) a[u] = ld2(adf_h + (t0 + u) * tse + (size_t)oe[j] + k0);
+            if constexpr (TG == 2) ld_adf2(adf_h, t0, tse, (size_t)oe[j] + k0, a[0], a[1]);
+            else a[0] = ld2(adf_h + adf_at<R>(t0, tse, (size_t)oe[j] + k0));
 #pragma unroll
             for (int u = 0; u < TG; ++u) {
                 const R ax = pos ? a[u].x : -a[u].x;
@@ -1010,12 +1010,10 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             const bool on1 = lay1 && k0 + 1 < nlm;
             const bool pos = info_pos(info[j]);  // this node is edges(1)
             V2<R> f[TG], rpo[TG], rmo[TG];
+            if constexpr (TG == 2) ld_adf2(adf_h, t0, tse, (size_t)oe[j] + k0, f[0], f[1]);
+            else f[0] = ld2(adf_h + adf_at<R>(t0, tse, (size_t)oe[j] + k0));
 #pragma unroll
-            for (int u = 0; u < TG; ++u) {
-                const size_t tn = (t0 + u) * tsn, te = (t0 + u) * tse;
-                f[u] = ld2(adf_h + te + (size_t)oe[j] + k0);
-                ld_rpm(Rp, Rm, tn + (size_t)oo[j] + k0, rpo[u], rmo[u]);
-            }
+            for (int u = 0; u < TG; ++u) ld_rpm(Rp, Rm, (t0 + u) * tsn + (size_t)oo[j] + k0, rpo[u], rmo[u]);
 #pragma unroll
             for (int u = 0; u < TG; ++u) {
                 // both cells evaluated unconditionally, the second one masked
@@ -1102,7 +1100,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *_
             if (oe[j] < 0 || k0 >= nlm) continue;
             const bool on1 = lay1 && k0 + 1 < nlm;
             const bool pos = info_pos(info[j]);
-            const V2<R> f = ld2(flux_h + te + (size_t)oe[j] + k0);
+            const V2<R> f = ld2(flux_h + adf_at<R>(t, tse, (size_t)oe[j] + k0));
             const double f0 = (double)f.x, f1 = (double)f.y;
             sh0 = pos ? sh0 + f0 : sh0 - f0;
             const double s1 = pos ? sh1 + f1 : sh1 - f1;
@@ -1133,7 +1131,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *_
 template <class R>
 struct LearnSrc {
     const double *T, *LO;  // node tracer and LO solution (this tracer)
-    const R *adf, *vfl;    // edge HO-LO flux (this tracer) and edge transport
+    const R *adf, *vfl;    // edge HO-LO flux (this tracer: cell i at adf[kAdfStride * i]) and edge transport
     const R *Qp, *Qm;      // node headroom (this tracer)
     const R *Rp, *Rm;      // node R+/R- (this tracer)
     const R *Pp, *Pm;      // node P+/P- (this tracer)
@@ -1153,7 +1151,7 @@ __device__ __forceinline__ void edge_features(const DMesh<R> &m, const EdgeRec &
     const V2<double> L1 = ld2(S.LO + er.o1 + k0), L2 = ld2(S.LO + er.o2 + k0);
     const V2<R> qp1 = ld2(S.Qp + er.o1 + k0), qm1 = ld2(S.Qm + er.o1 + k0);
     const V2<R> qp2 = ld2(S.Qp + er.o2 + k0), qm2 = ld2(S.Qm + er.o2 + k0);
-    const V2<R> a = ld2(S.adf + c), v = ld2(S.vfl + c);
+    const V2<R> a = ld2(S.adf + kAdfStride<R> * c), v = ld2(S.vfl + c);
     const V2<double> av1 = ld2(m.av_d + er.o1 + k0), hn1 = ld2(m.hn_d + er.o1 + k0);
     const V2<double> av2 = ld2(m.av_d + er.o2 + k0);
     const V2<R> pp1 = ld2(S.Pp + er.o1 + k0), pm1 = ld2(S.Pm + er.o1 + k0);
@@ -1204,7 +1202,7 @@ struct LearnArgs {
 template <class R, int NT>
 __device__ __forceinline__ LearnSrc<R> learn_src(const LearnArgs<R, NT> &a, int t)
 {
-    return LearnSrc<R>{a.ttf + t * a.tsn, a.LO + t * a.tsn, a.adf + t * a.tse, a.vfl,
+    return LearnSrc<R>{a.ttf + t * a.tsn, a.LO + t * a.tsn, a.adf + adf_slot<R>(t, a.tse), a.vfl,
                        a.Qp + t * a.tsn, a.Qm + t * a.tsn, a.Rp + t * a.tsn * kRpmPerCell<R>,
                        a.Rm + t * a.tsn * kRpmPerCell<R>,
                        a.Pp + t * a.tsn, a.Pm + t * a.tsn};
@@ -1381,7 +1379,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_check(DMesh<R> m, const R *
             if (oe[j] < 0 || k0 >= nlm) continue;
             const bool on1 = lay1 && k0 + 1 < nlm;
             const bool pos = info_pos(info[j]);
-            const V2<R> f = ld2(adf_h + te + (size_t)oe[j] + k0), al = ld2(alpha + te + (size_t)oe[j] + k0);
+            const V2<R> f = ld2(adf_h + adf_at<R>(t, tse, (size_t)oe[j] + k0));
+            const V2<R> al = ld2(alpha + te + (size_t)oe[j] + k0);
             const double c0 = (double)(al.x * f.x) * (pos ? 1.0 : -1.0);
             const double c1 = (double)(al.y * f.y) * (pos ? 1.0 : -1.0);
             sp0 += fmax(0.0, c0);
@@ -1963,7 +1962,8 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     I.values = (double *)zalloc(I.tsn * ntr_ * sizeof(double));
     // valuesAB only exists once a caller sets it (the default refresh mode reads values)
     I.vfl = zalloc(I.tse * rs);
-    I.adf_h = zalloc(I.tse * I.wslots * rs);
+    // HO-LO edge fluxes: FP32 interleaves work slots in pairs (adf_at), so an odd count is rounded up
+    I.adf_h = zalloc(I.tse * (rs == 8 ? I.wslots : (I.wslots + 1) / 2 * 2) * rs);
     I.adf_v = zalloc(I.tsn * I.wslots * rs);
     // R+/R-: two arrays in FP64, one interleaved array in FP32 (ld_rpm)
     I.Pp = zalloc(I.tsn * I.wslots * 2 * rs);
@@ -2847,6 +2847,21 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
     }
     const int slot = work_slot(spec_, tr, p_->tune_nt);
     const size_t per = (size_t)lay->total * comps;
+    // FP32 HO-LO fluxes sit interleaved with the neighbouring work slot's
+    // (adf_at): fetch the pair's block and take this slot's values
+    if (es == 4 && name == "adv_flux_hor") {
+        const size_t tse = (size_t)lay->total;
+        std::vector<float> pr(2 * tse);
+        HIP_OK(hipMemcpy(pr.data(), (const char *)I.adf_h + (size_t)(slot >> 1) * 2 * tse * es, pr.size() * es,
+                         hipMemcpyDeviceToHost));
+        std::fill(out, out + need, 0.0);
+        for (size_t c = 0; c < lay->len.size(); ++c)
+            for (int k = 0; k < std::min(lay->len[c] - sub, L); ++k) {
+                const size_t i = (size_t)lay->off[c] + k;
+                out[(size_t)c * L + k] = pr[adf_at<float>((size_t)(slot & 1), tse, i & ~(size_t)1) + (i & 1)];
+            }
+        return 0;
+    }
     // FP32 R+/R- sit interleaved per cell pair (ld_rpm): fetch the slot's
     // pairs and take every other two values
     const bool rpm32 = es == 4 && (name == "fct_plus" || name == "fct_minus");

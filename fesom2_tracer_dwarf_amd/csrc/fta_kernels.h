@@ -331,6 +331,37 @@ __device__ __forceinline__ void st_rpm(float *Rp, float *, size_t i, float p0, f
 {
     *reinterpret_cast<float4 *>(Rp + 2 * i) = make_float4(p0, p1, m0, m1);
 }
+// HO-LO (or HO) edge fluxes of work slot t at the cell pair i (i even).
+// FP64: slot t is the block [t*tse, (t+1)*tse). FP32: slots 2s and 2s+1 share
+// the block of 2*tse floats at 2s*tse, interleaved per cell pair (slot 2s's
+// k, k+1, then slot 2s+1's), so that both slots' pairs are one 16-byte access.
+template <class R>
+constexpr size_t kAdfStride = sizeof(R) == 4 ? 2 : 1;
+template <class R>
+__host__ __device__ __forceinline__ size_t adf_slot(size_t t, size_t tse)
+{
+    return sizeof(R) == 4 ? (t >> 1) * 2 * tse + 2 * (t & 1) : t * tse;
+}
+// (i must start a pair; its second cell follows at + 1)
+template <class R>
+__host__ __device__ __forceinline__ size_t adf_at(size_t t, size_t tse, size_t i)
+{
+    return adf_slot<R>(t, tse) + kAdfStride<R> * i;
+}
+// slots t and t + 1 (t even) of the cell pair i
+__device__ __forceinline__ void ld_adf2(const double *a, size_t t, size_t tse, size_t i, V2<double> &x,
+                                        V2<double> &y)
+{
+    x = ld2(a + t * tse + i);
+    y = ld2(a + (t + 1) * tse + i);
+}
+__device__ __forceinline__ void ld_adf2(const float *a, size_t t, size_t tse, size_t i, V2<float> &x, V2<float> &y)
+{
+    const float4 v = *reinterpret_cast<const float4 *>(a + (t >> 1) * 2 * tse + 2 * i);
+    x = {v.x, v.y};
+    y = {v.z, v.w};
+}
+
 // elements of R+/R- storage per cell (see ld_rpm)
 template <class R>
 constexpr size_t kRpmPerCell = sizeof(R) == 4 ? 2 : 1;

@@ -122,3 +122,36 @@ def test_fp32_limiter_factors_against_fp64(api):
               flush=True)
         assert lim.sum() > 0, name
         assert far <= 1e-2 and med <= 1e-4, name
+
+
+def test_fp32_edge_fluxes_against_fp64(api):
+    """The antidiffusive edge fluxes (work array adv_flux_hor, HO - LO) of
+    both tracers of a two-tracer launch after one step from the same state.
+    The device keeps FP32 fluxes of two work slots interleaved per cell pair,
+    so tracer 2 reads the second half. Tracer 2 is tracer 1 doubled: HO and LO
+    are linear in the tracer and doubling is exact in floating point, so its
+    fluxes are exactly twice tracer 1's in both precisions; tracer 1's FP32
+    fluxes equal the FP64 ones to a small relative error where there is flux."""
+    print("[gpu] FP32 HO-LO edge fluxes: both work slots, against FP64", flush=True)
+    d, dt, trs = setup_case(api, MESH, SCHEMES, ntr=2)
+    trs[1] = 2.0 * trs[0]
+    m = mesh_dict(api)
+    L, E = d["nl"] - 1, d["edge2D"]
+    el = m["edge_tri"].reshape(E, 2)
+    nlv = m["nlevels"]
+    nle = np.maximum(nlv[el[:, 0]], np.where(el[:, 1] >= 0, nlv[np.maximum(el[:, 1], 0)], 0)) - 1
+    act = np.arange(L)[None, :] < nle[:, None]                       # (E, L) wet edge levels
+    out = {}
+    for prec in (64, 32):
+        api.set_option("precision", prec)
+        for i, t in enumerate(trs):
+            api.set_values(i + 1, t)
+        api.advect_steps(1, dt)
+        out[prec] = [api.work_array("adv_flux_hor", L * E, tracer_id=i + 1).reshape(E, L)[act] for i in range(2)]
+        assert np.array_equal(out[prec][1], 2.0 * out[prec][0]), f"FP{prec}: slot 2 is not slot 1 doubled"
+    a64, a32 = out[64][0], out[32][0]
+    big = np.abs(a64) > 1e-3 * np.abs(a64).max()
+    rel = np.abs(a32 - a64)[big] / np.abs(a64)[big]
+    print(f"[gpu]   {big.sum()} edge levels with flux, median relative FP32 difference {np.median(rel):.2e}",
+          flush=True)
+    assert big.sum() > 0 and np.median(rel) <= 1e-4
