@@ -34,7 +34,8 @@ MESHES = {
 }
 
 SCHEMES = [("UPW1", "UPW1", "NONE"), ("UPW1", "UPW1", "FCT"), ("MUSCL", "QR4C", "FCT"), ("MFCT", "PPM", "FCT"),
-           ("MUSCL", "CDIFF", "FCT"), ("MUSCL", "QR4C", "NONE"), ("MFCT", "UPW1", "FCT"), ("MUSCL", "PPM", "NONE")]
+           ("MUSCL", "CDIFF", "FCT"), ("MUSCL", "QR4C", "NONE"), ("MFCT", "UPW1", "FCT"), ("MUSCL", "PPM", "NONE"),
+           ("MFCT", "CDIFF", "NONE")]
 
 
 def setup_case(api, mesh_kw, schemes, ntr=4, seed=1, ph=1.0, pv=1.0):
