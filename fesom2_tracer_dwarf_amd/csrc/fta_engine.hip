@@ -876,10 +876,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
             }
             st_rpm(Rp, Rm, tn + c, rp0, rp1, rm0, rm1);
             if constexpr (STOREQ) {  // Q+/Q- and P+/P- for the learned limiter's features and bound check
-                st2(Qp_out + tn + c, (R)qp0, (R)qp1);
-                st2(Qm_out + tn + c, (R)qm0, (R)qm1);
-                st2(Pp_out + tn + c, P.x, P.y);
-                st2(Pm_out + tn + c, M.x, M.y);
+                st_rpm(Qp_out, Qm_out, tn + c, (R)qp0, (R)qp1, (R)qm0, (R)qm1);
+                st_rpm(Pp_out, Pm_out, tn + c, P.x, P.y, M.x, M.y);
             }
         }
     }
@@ -950,7 +948,6 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
                                                         const R *__restrict__ Rp,
                                                         const R *__restrict__ Rm, double dt,
                                                         const R *__restrict__ alpha,
-                                                        const R *__restrict__ unsafe,
                                                         double *__restrict__ acc_h, double *__restrict__ acc_v)
 {
     const int ch = wave_chunk();
@@ -977,7 +974,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
     constexpr int TG = NT < 2 ? NT : 2;  // tracers whose gathers share a round trip
 #pragma unroll
     for (int t0 = 0; t0 < NT; t0 += TG) {
-        V2<R> rp[TG], rm[TG], us[TG];
+        V2<R> rp[TG], rm[TG];
+        bool us0[TG], us1[TG];  // learned: this node level failed its bound check
         R fvl0[TG], fvl1[TG], fvl2[TG];
 #pragma unroll
         for (int u = 0; u < TG; ++u) {
@@ -985,16 +983,17 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             V2<R> fv{0, 0};
             rp[u] = V2<R>{0, 0};
             rm[u] = V2<R>{0, 0};
-            us[u] = V2<R>{0, 0};
             if (if0) fv = ld2(adf_v + tn + c);
             if (!if1) fv.y = 0;
             if (lay0) ld_rpm(Rp, Rm, tn + c, rp[u], rm[u]);
+            us0[u] = LEARN && has_flag(rp[u].x);
+            us1[u] = LEARN && has_flag(rp[u].y);
+            if constexpr (LEARN) { rp[u].x = fabs(rp[u].x); rp[u].y = fabs(rp[u].y); }
             if (!lay1) { rp[u].y = 0; rm[u].y = 0; }
             const R rpu0 = from_prev(rp[u].y), rmu0 = from_prev(rm[u].y);
             fvl0[u] = limit_vertical(k0 + 1, lay0, fv.x, rp[u].x, rm[u].x, rpu0, rmu0);
             fvl1[u] = limit_vertical(k0 + 2, lay1, fv.y, rp[u].y, rm[u].y, rp[u].x, rm[u].x);
             fvl2[u] = from_next(fvl0[u]);
-            if constexpr (LEARN) { if (lay0) us[u] = ld2(unsafe + tn + c); }
         }
         if (!lay0) continue;
 
@@ -1009,13 +1008,24 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             if (oe[j] < 0 || k0 >= nlm) continue;
             const bool on1 = lay1 && k0 + 1 < nlm;
             const bool pos = info_pos(info[j]);  // this node is edges(1)
-            V2<R> f[TG], rpo[TG], rmo[TG];
+            V2<R> f[TG], rpo[TG], rmo[TG], al[TG];
             if constexpr (TG == 2) ld_adf2(adf_h, t0, tse, (size_t)oe[j] + k0, f[0], f[1]);
             else f[0] = ld2(adf_h + adf_at<R>(t0, tse, (size_t)oe[j] + k0));
+            if constexpr (LEARN) {
+                if constexpr (TG == 2) ld_adf2(alpha, t0, tse, (size_t)oe[j] + k0, al[0], al[1]);
+                else al[0] = ld2(alpha + adf_at<R>(t0, tse, (size_t)oe[j] + k0));
+            }
 #pragma unroll
             for (int u = 0; u < TG; ++u) ld_rpm(Rp, Rm, (t0 + u) * tsn + (size_t)oo[j] + k0, rpo[u], rmo[u]);
 #pragma unroll
             for (int u = 0; u < TG; ++u) {
+                bool uo0 = false, uo1 = false;  // learned: the other node level failed its bound check
+                if constexpr (LEARN) {
+                    uo0 = has_flag(rpo[u].x);
+                    uo1 = has_flag(rpo[u].y);
+                    rpo[u].x = fabs(rpo[u].x);
+                    rpo[u].y = fabs(rpo[u].y);
+                }
                 // both cells evaluated unconditionally, the second one masked
                 // (keeps every pair load one 16-byte access). Oriented as the flux
                 // entering this node, fs = +-f, Zalesak's coefficient (b3) is
@@ -1028,11 +1038,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
                 R a1 = fmn(fmn((R)1, in1 ? rp[u].y : rm[u].y), in1 ? rmo[u].y : rpo[u].y);
                 if constexpr (LEARN) {
                     // the network's coefficient, Zalesak's where either node failed its bound check
-                    const size_t tn = (t0 + u) * tsn, te = (t0 + u) * tse;
-                    const V2<R> al = ld2(alpha + te + (size_t)oe[j] + k0);
-                    const V2<R> uo = ld2(unsafe + tn + (size_t)oo[j] + k0);
-                    a0 = (us[u].x != (R)0 || uo.x != (R)0) ? fmn(al.x, a0) : al.x;
-                    a1 = (us[u].y != (R)0 || uo.y != (R)0) ? fmn(al.y, a1) : al.y;
+                    a0 = (us0[u] || uo0) ? fmn(al[u].x, a0) : al[u].x;
+                    a1 = (us1[u] || uo1) ? fmn(al[u].y, a1) : al[u].y;
                 }
                 const double fl0 = (double)(a0 * fx);
                 const double fl1 = (double)(a1 * fy);
@@ -1131,11 +1138,31 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *_
 template <class R>
 struct LearnSrc {
     const double *T, *LO;  // node tracer and LO solution (this tracer)
-    const R *adf, *vfl;    // edge HO-LO flux (this tracer: cell i at adf[kAdfStride * i]) and edge transport
-    const R *Qp, *Qm;      // node headroom (this tracer)
-    const R *Rp, *Rm;      // node R+/R- (this tracer)
-    const R *Pp, *Pm;      // node P+/P- (this tracer)
+    const R *adf;          // edge HO-LO flux (this tracer: cell i at adf[kAdfStride * i])
+    // node headroom Q+/Q-, R+/R- and P+/P- (this tracer), each pair of
+    // arrays stored as ld_rpm reads it
+    const R *Qp, *Qm, *Rp, *Rm, *Pp, *Pm;
 };
+
+// the tracer-independent part of an edge level's features: the cell volumes
+// of the end nodes as the scalings use them, and the edge transport
+template <class R>
+struct EdgeGeo {
+    float vh1[2], v1[2], v2[2];  // (float)(areasvol * hnode_new) of node 1, (float)areasvol of nodes 1 and 2
+    R vf[2];
+};
+template <class R>
+__device__ __forceinline__ EdgeGeo<R> edge_geo(const DMesh<R> &m, const EdgeRec &er, int k0, size_t c,
+                                               const R *vfl)
+{
+    const V2<double> av1 = ld2(m.av_d + er.o1 + k0), hn1 = ld2(m.hn_d + er.o1 + k0);
+    const V2<double> av2 = ld2(m.av_d + er.o2 + k0);
+    const V2<R> v = ld2(vfl + c);
+    return EdgeGeo<R>{{(float)(av1.x * hn1.x), (float)(av1.y * hn1.y)},
+                      {(float)av1.x, (float)av1.y},
+                      {(float)av2.x, (float)av2.y},
+                      {v.x, v.y}};
+}
 
 __device__ __forceinline__ float clip8(float v) { return fminf(8.0f, fmaxf(-8.0f, v)); }
 
@@ -1144,23 +1171,21 @@ __device__ __forceinline__ float clip8(float v) { return fminf(8.0f, fmaxf(-8.0f
 // the scalings by the local range and the cell volume are float reciprocals:
 // the features are clipped to [-8, 8] and rounded to bf16 afterwards.
 template <class R>
-__device__ __forceinline__ void edge_features(const DMesh<R> &m, const EdgeRec &er, int k0, size_t c, double dt,
+__device__ __forceinline__ void edge_features(const EdgeRec &er, int k0, size_t c, double dt, const EdgeGeo<R> &G,
                                               const LearnSrc<R> &S, float (&x)[2][kMlpIn], float (&y)[2])
 {
     const V2<double> T1 = ld2(S.T + er.o1 + k0), T2 = ld2(S.T + er.o2 + k0);
     const V2<double> L1 = ld2(S.LO + er.o1 + k0), L2 = ld2(S.LO + er.o2 + k0);
-    const V2<R> qp1 = ld2(S.Qp + er.o1 + k0), qm1 = ld2(S.Qm + er.o1 + k0);
-    const V2<R> qp2 = ld2(S.Qp + er.o2 + k0), qm2 = ld2(S.Qm + er.o2 + k0);
-    const V2<R> a = ld2(S.adf + kAdfStride<R> * c), v = ld2(S.vfl + c);
-    const V2<double> av1 = ld2(m.av_d + er.o1 + k0), hn1 = ld2(m.hn_d + er.o1 + k0);
-    const V2<double> av2 = ld2(m.av_d + er.o2 + k0);
-    const V2<R> pp1 = ld2(S.Pp + er.o1 + k0), pm1 = ld2(S.Pm + er.o1 + k0);
-    const V2<R> pp2 = ld2(S.Pp + er.o2 + k0), pm2 = ld2(S.Pm + er.o2 + k0);
+    V2<R> qp1, qm1, qp2, qm2, pp1, pm1, pp2, pm2;
+    ld_rpm(S.Qp, S.Qm, (size_t)er.o1 + k0, qp1, qm1);
+    ld_rpm(S.Qp, S.Qm, (size_t)er.o2 + k0, qp2, qm2);
+    const V2<R> a = ld2(S.adf + kAdfStride<R> * c);
+    ld_rpm(S.Pp, S.Pm, (size_t)er.o1 + k0, pp1, pm1);
+    ld_rpm(S.Pp, S.Pm, (size_t)er.o2 + k0, pp2, pm2);
     const double t1[2] = {T1.x, T1.y}, t2[2] = {T2.x, T2.y}, l1[2] = {L1.x, L1.y}, l2[2] = {L2.x, L2.y};
     const R Qp1[2] = {qp1.x, qp1.y}, Qm1[2] = {qm1.x, qm1.y}, Qp2[2] = {qp2.x, qp2.y}, Qm2[2] = {qm2.x, qm2.y};
     const R Pp1[2] = {pp1.x, pp1.y}, Pm1[2] = {pm1.x, pm1.y}, Pp2[2] = {pp2.x, pp2.y}, Pm2[2] = {pm2.x, pm2.y};
-    const double AV1[2] = {av1.x, av1.y}, AV2[2] = {av2.x, av2.y}, HN1[2] = {hn1.x, hn1.y};
-    const R A[2] = {a.x, a.y}, Vf[2] = {v.x, v.y};
+    const R A[2] = {a.x, a.y};
     const float fdt = (float)dt;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -1168,8 +1193,8 @@ __device__ __forceinline__ void edge_features(const DMesh<R> &m, const EdgeRec &
         const double mn = fmin(l1[q] + (double)Qm1[q], l2[q] + (double)Qm2[q]);
         const float sc = mx - mn > 1e-30 ? (float)(mx - mn) : 1.0f;
         const float isc = __builtin_amdgcn_rcpf(sc);
-        const double lo = lo_flux((double)Vf[q], t1[q], t2[q]);
-        const float toT = fdt * __builtin_amdgcn_rcpf((float)(AV1[q] * HN1[q]) * sc);
+        const double lo = lo_flux((double)G.vf[q], t1[q], t2[q]);
+        const float toT = fdt * __builtin_amdgcn_rcpf(G.vh1[q] * sc);
         x[q][0] = clip8((float)lo * toT);
         x[q][1] = clip8((float)(lo + (double)A[q]) * toT);
         x[q][2] = clip8((float)(t1[q] - mn) * isc);
@@ -1179,8 +1204,8 @@ __device__ __forceinline__ void edge_features(const DMesh<R> &m, const EdgeRec &
         x[q][6] = clip8((float)Qp2[q] * isc);
         x[q][7] = clip8(-(float)Qm2[q] * isc);
         const bool in1 = A[q] >= (R)0;  // flux into node 1
-        x[q][8] = clip8((float)(in1 ? Pp1[q] : -Pm1[q]) * (fdt * __builtin_amdgcn_rcpf((float)AV1[q] * sc)));
-        x[q][9] = clip8((float)(in1 ? -Pm2[q] : Pp2[q]) * (fdt * __builtin_amdgcn_rcpf((float)AV2[q] * sc)));
+        x[q][8] = clip8((float)(in1 ? Pp1[q] : -Pm1[q]) * (fdt * __builtin_amdgcn_rcpf(G.v1[q] * sc)));
+        x[q][9] = clip8((float)(in1 ? -Pm2[q] : Pp2[q]) * (fdt * __builtin_amdgcn_rcpf(G.v2[q] * sc)));
         y[q] = 0.0f;
     }
     if (S.Rp) {  // training rows: Zalesak's coefficient (b3) as the target
@@ -1202,10 +1227,9 @@ struct LearnArgs {
 template <class R, int NT>
 __device__ __forceinline__ LearnSrc<R> learn_src(const LearnArgs<R, NT> &a, int t)
 {
-    return LearnSrc<R>{a.ttf + t * a.tsn, a.LO + t * a.tsn, a.adf + adf_slot<R>(t, a.tse), a.vfl,
-                       a.Qp + t * a.tsn, a.Qm + t * a.tsn, a.Rp + t * a.tsn * kRpmPerCell<R>,
-                       a.Rm + t * a.tsn * kRpmPerCell<R>,
-                       a.Pp + t * a.tsn, a.Pm + t * a.tsn};
+    const size_t tq = t * a.tsn * kRpmPerCell<R>;
+    return LearnSrc<R>{a.ttf + t * a.tsn, a.LO + t * a.tsn, a.adf + adf_slot<R>(t, a.tse),
+                       a.Qp + tq, a.Qm + tq, a.Rp + tq, a.Rm + tq, a.Pp + tq, a.Pm + tq};
 }
 
 // training rows: features (bf16, 16 per row) and Zalesak target of every
@@ -1219,15 +1243,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_dump(DMesh<R> m, LearnArgs<
     if (ch >= m.nch_e) return;
     const Pair p = lane_pair(m.slot_e, ch);
     EdgeRec er{};
+    EdgeGeo<R> geo{};
     int nlmax = 0;
     if (p.live) {
         er = m.erec[p.col];
         nlmax = (int)((unsigned)er.nl >> 24);
+        geo = edge_geo(m, er, p.k0, (size_t)p.c, a.vfl);
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         float x[2][kMlpIn], y[2] = {-1.0f, -1.0f};
-        if (p.live) edge_features(m, er, p.k0, (size_t)p.c, dt, learn_src(a, t), x, y);
+        if (p.live) edge_features(er, p.k0, (size_t)p.c, dt, geo, learn_src(a, t), x, y);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const bool ok = p.live && p.k0 + q < nlmax;
@@ -1246,15 +1272,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_dump(DMesh<R> m, LearnArgs<
     }
 }
 
-// the network's coefficient of every horizontal edge level: each wave packs
-// its chunk's 128 feature rows into LDS and runs them through the matrix
-// cores as eight 16-row tiles. Persistent waves (weights loaded once);
-// block-uniform trip counts.
 // the network's coefficient of every edge level: a persistent grid; each
-// wave forms its chunk's 128 feature rows (bf16) in its own LDS slice, runs
-// eight 16-row tiles through the matrix cores and reads its two outputs back.
-// The lanes of one wave exchange data only with each other, so a wave-scope
-// fence replaces block barriers and waves leave the loop independently.
+// wave forms the feature rows (bf16) of its chunk's edge levels packed to the
+// front of its own LDS slice (a padded column's last cell and the chunk's
+// tail get no row), runs them through the matrix cores 16 rows at a time and
+// reads its outputs back. The lanes of one wave exchange data only with each
+// other, so a wave-scope fence replaces block barriers and waves leave the
+// loop independently.
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1262,17 +1286,20 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// number of set bits of b below this lane
+__device__ __forceinline__ int lanes_below(unsigned long long b)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+}
+
 template <class R, int NT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_learn_alpha(DMesh<R> m, MlpDev w, LearnArgs<R, NT> a, double dt,
                                                         R *__restrict__ alpha, int iters)
 {
-    // one tracer staged at a time: a block's LDS (18 KB) then lets six waves
-    // per SIMD stay resident (the 84-VGPR limit), whose gathers hide each
-    // other's latency better than two tracers' gathers in one wave did
-    // (measured: 4.74 -> 4.56 ms at 4 tracers)
-    constexpr int G = 1;
-    __shared__ uint4 sX[kWavesPerBlock][G][kChunk][2];  // 16 bf16 features per row
-    __shared__ float sY[kWavesPerBlock][G][kChunk];
+    // one tracer staged at a time (18 KB of LDS per block); the weights held
+    // in registers (80 VGPRs) leave room for three waves per SIMD
+    __shared__ uint4 sX[kWavesPerBlock][kChunk][2];  // 16 bf16 features per row
+    __shared__ float sY[kWavesPerBlock][kChunk];
     const int wv = (int)(threadIdx.x >> 6), l = lane_id(), i = l & 15, g = l >> 4;
     MlpInfer r;
     mlp_infer_load(w, r);
@@ -1281,51 +1308,53 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         if (ch >= m.nch_e) break;  // wave-uniform
         const Pair p = lane_pair(m.slot_e, ch);
         EdgeRec er{};
+        EdgeGeo<R> geo{};
         int nlmax = 0;
         if (p.live) {
             er = m.erec[p.col];
             nlmax = (int)((unsigned)er.nl >> 24);
+            geo = edge_geo(m, er, p.k0, (size_t)p.c, a.vfl);
         }
+        // the lane's rows in the packed order: lane by lane, first cell first
+        const bool r0 = p.live && p.k0 < nlmax, r1 = p.live && p.k0 + 1 < nlmax;
+        const unsigned long long b0 = __ballot(r0), b1 = __ballot(r1);
+        const int pos0 = lanes_below(b0) + lanes_below(b1), pos1 = pos0 + (r0 ? 1 : 0);
+        const int ntile = (__popcll(b0) + __popcll(b1) + 15) >> 4;  // wave-uniform
 #pragma unroll 1
-        for (int t0 = 0; t0 < NT; t0 += G) {
+        for (int t = 0; t < NT; ++t) {
+            float x[2][kMlpIn], y[2];
+            LearnSrc<R> src = learn_src(a, t);
+            src.Rp = nullptr;  // no target needed
+            if (p.live) edge_features(er, p.k0, (size_t)p.c, dt, geo, src, x, y);
 #pragma unroll
-            for (int s = 0; s < G; ++s) {
-                float x[2][kMlpIn], y[2];
-                LearnSrc<R> src = learn_src(a, t0 + s);
-                src.Rp = nullptr;  // no target needed
-                if (p.live) edge_features(m, er, p.k0, (size_t)p.c, dt, src, x, y);
+            for (int q = 0; q < 2; ++q) {
+                unsigned u[8];
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    unsigned u[8];
-#pragma unroll
-                    for (int f = 0; f < 8; ++f) {
-                        // v_cvt_pk_bf16_f32: two features per instruction, nearest even
-                        const mlp_v2f v2 = {2 * f < kMlpIn ? x[q][2 * f] : 0.0f,
-                                            2 * f + 1 < kMlpIn ? x[q][2 * f + 1] : 0.0f};
-                        const unsigned b = __builtin_bit_cast(unsigned, __builtin_convertvector(v2, mlp_v2b));
-                        u[f] = (p.live && 2 * f < kMlpIn) ? b : 0u;
-                    }
-                    sX[wv][s][2 * l + q][0] = make_uint4(u[0], u[1], u[2], u[3]);
-                    sX[wv][s][2 * l + q][1] = make_uint4(u[4], u[5], u[6], u[7]);
+                for (int f = 0; f < 8; ++f) {
+                    // v_cvt_pk_bf16_f32: two features per instruction, nearest even
+                    const mlp_v2f v2 = {2 * f < kMlpIn ? x[q][2 * f] : 0.0f,
+                                        2 * f + 1 < kMlpIn ? x[q][2 * f + 1] : 0.0f};
+                    const unsigned b = __builtin_bit_cast(unsigned, __builtin_convertvector(v2, mlp_v2b));
+                    u[f] = 2 * f < kMlpIn ? b : 0u;
+                }
+                if (q == 0 ? r0 : r1) {
+                    const int row = q == 0 ? pos0 : pos1;
+                    sX[wv][row][0] = make_uint4(u[0], u[1], u[2], u[3]);
+                    sX[wv][row][1] = make_uint4(u[4], u[5], u[6], u[7]);
                 }
             }
             wave_lds_sync();
-#pragma unroll 1
-            for (int s = 0; s < G; ++s) {
+            // rows past the last one in the last tile hold stale features;
+            // their outputs are never read
 #pragma unroll 4
-                for (int tile = 0; tile < kChunk / 16; ++tile) {
-                    const mlp_v4s xb =
-                        ld_bf4(reinterpret_cast<const unsigned short *>(&sX[wv][s][16 * tile + i][0]) + 4 * g);
-                    const float z = mlp_infer(r, xb);
-                    if (g == 0) sY[wv][s][16 * tile + i] = mlp_sigmoid_fast(z);
-                }
+            for (int tile = 0; tile < ntile; ++tile) {
+                const mlp_v4s xb = ld_bf4(reinterpret_cast<const unsigned short *>(&sX[wv][16 * tile + i][0]) + 4 * g);
+                const float z = mlp_infer(r, xb);
+                if (g == 0) sY[wv][16 * tile + i] = mlp_sigmoid_fast(z);
             }
             wave_lds_sync();
-#pragma unroll
-            for (int s = 0; s < G; ++s) {
-                const float y0 = sY[wv][s][2 * l], y1 = sY[wv][s][2 * l + 1];
-                if (p.live) st2(alpha + (t0 + s) * a.tse + p.c, (R)y0, p.k0 + 1 < nlmax ? (R)y1 : (R)0);
-            }
+            const float y0 = r0 ? sY[wv][pos0] : 0.0f, y1 = r1 ? sY[wv][pos1] : 0.0f;
+            if (p.live) st2(alpha + adf_at<R>(t, a.tse, p.c), (R)y0, (R)y1);
             wave_lds_sync();
         }
     }
@@ -1334,12 +1363,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // per node level: would the network's horizontal coefficients (vertical
 // ones as Zalesak's) keep the node inside its bounds? Checked as Zalesak's
 // own sufficient condition, positive and negative contributions apart:
-// sum(max(0, c)) * dt / areasvol <= Q+ and sum(min(0, c)) * dt / areasvol >= Q-
+// sum(max(0, c)) * dt / areasvol <= Q+ and sum(min(0, c)) * dt / areasvol >= Q-.
+// The flag goes to the tracer's own record (unsafe) and onto the sign bit of
+// the level's R+ (with_flag), where the update reads it with the R+/R- it
+// gathers anyway.
 template <class R, int NT, int D>
 __global__ __launch_bounds__(kMaxBlock) void k_learn_check(DMesh<R> m, const R *__restrict__ adf_h,
                                                         const R *__restrict__ alpha, size_t tse,
-                                                        const R *__restrict__ adf_v, const R *__restrict__ Rp,
-                                                        const R *__restrict__ Rm, const R *__restrict__ Qp,
+                                                        const R *__restrict__ adf_v, R *__restrict__ Rp,
+                                                        R *__restrict__ Rm, const R *__restrict__ Qp,
                                                         const R *__restrict__ Qm, size_t tsn, double dt,
                                                         R *__restrict__ unsafe)
 {
@@ -1356,45 +1388,71 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_check(DMesh<R> m, const R *
     load_node_edges<D>(m, p.col, lay0, oe, info);
     V2<R> av{1, 1};
     if (lay0) av = ld2(m.areasvol + c);
+    constexpr int TG = NT < 2 ? NT : 2;  // tracers whose gathers share a round trip
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const size_t tn = t * tsn, te = t * tse;
-        V2<R> fv{0, 0}, rp{0, 0}, rm{0, 0};
-        if (if0) fv = ld2(adf_v + tn + c);
-        if (!if1) fv.y = 0;
-        if (lay0) ld_rpm(Rp, Rm, tn + c, rp, rm);
-        if (!lay1) { rp.y = 0; rm.y = 0; }
-        const R rpu0 = from_prev(rp.y), rmu0 = from_prev(rm.y);
-        const R fvl0 = limit_vertical(k0 + 1, lay0, fv.x, rp.x, rm.x, rpu0, rmu0);
-        const R fvl1 = limit_vertical(k0 + 2, lay1, fv.y, rp.y, rm.y, rp.x, rm.x);
-        const R fvl2 = from_next(fvl0);
+    for (int t0 = 0; t0 < NT; t0 += TG) {
+        V2<R> rp[TG], rm[TG];
+        double sp0[TG], sn0[TG], sp1[TG], sn1[TG];
+#pragma unroll
+        for (int u = 0; u < TG; ++u) {
+            const size_t tn = (t0 + u) * tsn;
+            V2<R> fv{0, 0};
+            rp[u] = V2<R>{0, 0};
+            rm[u] = V2<R>{0, 0};
+            if (if0) fv = ld2(adf_v + tn + c);
+            if (!if1) fv.y = 0;
+            if (lay0) ld_rpm(Rp, Rm, tn + c, rp[u], rm[u]);
+            const R rpy = lay1 ? rp[u].y : (R)0, rmy = lay1 ? rm[u].y : (R)0;
+            const R rpu0 = from_prev(rpy), rmu0 = from_prev(rmy);
+            const R fvl0 = limit_vertical(k0 + 1, lay0, fv.x, rp[u].x, rm[u].x, rpu0, rmu0);
+            const R fvl1 = limit_vertical(k0 + 2, lay1, fv.y, rpy, rmy, rp[u].x, rm[u].x);
+            const R fvl2 = from_next(fvl0);
+            sp0[u] = fmax(0.0, (double)fvl0) + fmax(0.0, -(double)fvl1);
+            sn0[u] = fmin(0.0, (double)fvl0) + fmin(0.0, -(double)fvl1);
+            sp1[u] = fmax(0.0, (double)fvl1) + fmax(0.0, -(double)fvl2);
+            sn1[u] = fmin(0.0, (double)fvl1) + fmin(0.0, -(double)fvl2);
+        }
         if (!lay0) continue;
-        double sp0 = fmax(0.0, (double)fvl0) + fmax(0.0, -(double)fvl1);
-        double sn0 = fmin(0.0, (double)fvl0) + fmin(0.0, -(double)fvl1);
-        double sp1 = fmax(0.0, (double)fvl1) + fmax(0.0, -(double)fvl2);
-        double sn1 = fmin(0.0, (double)fvl1) + fmin(0.0, -(double)fvl2);
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             const int nlm = info_nlmax(info[j]);
             if (oe[j] < 0 || k0 >= nlm) continue;
             const bool on1 = lay1 && k0 + 1 < nlm;
-            const bool pos = info_pos(info[j]);
-            const V2<R> f = ld2(adf_h + adf_at<R>(t, tse, (size_t)oe[j] + k0));
-            const V2<R> al = ld2(alpha + te + (size_t)oe[j] + k0);
-            const double c0 = (double)(al.x * f.x) * (pos ? 1.0 : -1.0);
-            const double c1 = (double)(al.y * f.y) * (pos ? 1.0 : -1.0);
-            sp0 += fmax(0.0, c0);
-            sn0 += fmin(0.0, c0);
-            sp1 = on1 ? sp1 + fmax(0.0, c1) : sp1;
-            sn1 = on1 ? sn1 + fmin(0.0, c1) : sn1;
+            const double sg = info_pos(info[j]) ? 1.0 : -1.0;
+            const size_t i = (size_t)oe[j] + k0;
+            V2<R> f[TG], al[TG];
+            if constexpr (TG == 2) {
+                ld_adf2(adf_h, t0, tse, i, f[0], f[1]);
+                ld_adf2(alpha, t0, tse, i, al[0], al[1]);
+            } else {
+                f[0] = ld2(adf_h + adf_at<R>(t0, tse, i));
+                al[0] = ld2(alpha + adf_at<R>(t0, tse, i));
+            }
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                const double c0 = (double)(al[u].x * f[u].x) * sg;
+                const double c1 = (double)(al[u].y * f[u].y) * sg;
+                sp0[u] += fmax(0.0, c0);
+                sn0[u] += fmin(0.0, c0);
+                sp1[u] = on1 ? sp1[u] + fmax(0.0, c1) : sp1[u];
+                sn1[u] = on1 ? sn1[u] + fmin(0.0, c1) : sn1[u];
+            }
         }
-        const V2<R> qp = ld2(Qp + tn + c), qm = ld2(Qm + tn + c);
         const double s0 = dt / (double)av.x, s1 = dt / (double)av.y;
-        const bool bad0 = sp0 * s0 > (double)qp.x + 1e-7 * fabs((double)qp.x) ||
-                          sn0 * s0 < (double)qm.x - 1e-7 * fabs((double)qm.x);
-        const bool bad1 = lay1 && (sp1 * s1 > (double)qp.y + 1e-7 * fabs((double)qp.y) ||
-                                   sn1 * s1 < (double)qm.y - 1e-7 * fabs((double)qm.y));
-        st2(unsafe + tn + c, bad0 ? (R)1 : (R)0, bad1 ? (R)1 : (R)0);
+#pragma unroll
+        for (int u = 0; u < TG; ++u) {
+            const size_t tn = (t0 + u) * tsn;
+            V2<R> qp, qm;
+            ld_rpm(Qp, Qm, tn + c, qp, qm);
+            const bool bad0 = sp0[u] * s0 > (double)qp.x + 1e-7 * fabs((double)qp.x) ||
+                              sn0[u] * s0 < (double)qm.x - 1e-7 * fabs((double)qm.x);
+            const bool bad1 = lay1 && (sp1[u] * s1 > (double)qp.y + 1e-7 * fabs((double)qp.y) ||
+                                       sn1[u] * s1 < (double)qm.y - 1e-7 * fabs((double)qm.y));
+            st2(unsafe + tn + c, bad0 ? (R)1 : (R)0, bad1 ? (R)1 : (R)0);
+            // R+ of the pair only (FP32: the first half of its 16-byte record), only where a flag is set
+            if (bad0 || bad1)
+                st2(Rp + kRpmPerCell<R> * (tn + c), with_flag(rp[u].x, bad0), with_flag(rp[u].y, bad1));
+        }
     }
 }
 
@@ -2027,6 +2085,20 @@ Engine::~Engine()
     delete p_;
 }
 
+// Q+/Q- and P+/P- of every work slot for the learned limiter (features and
+// bound check), each pair stored as R+/R- are (ld_rpm): two arrays in FP64,
+// one interleaved array of twice the length in FP32
+static void alloc_learn_bounds(Engine::Impl &I, size_t rs)
+{
+    const size_t n = I.tsn * I.wslots;
+    for (void **q : {&I.Qp, &I.Sp}) {
+        *q = I.track((void *)dalloc<char>(2 * n * rs));
+        HIP_OK(hipMemset(*q, 0, 2 * n * rs));
+    }
+    I.Qm = rs == 8 ? (char *)I.Qp + n * rs : I.Qp;
+    I.Sm = rs == 8 ? (char *)I.Sp + n * rs : I.Sp;
+}
+
 void Engine::set_spec(int tr, const TracerSpec &sp)
 {
     Impl &I = *p_;
@@ -2042,15 +2114,11 @@ void Engine::set_spec(int tr, const TracerSpec &sp)
             HIP_OK(hipMemset(p, 0, bytes));
             return p;
         };
-        I.alpha = zalloc(I.tse * I.wslots * rs);
+        // coefficients of the edge levels, stored as the HO-LO fluxes are (adf_at)
+        I.alpha = zalloc(I.tse * (rs == 8 ? I.wslots : (I.wslots + 1) / 2 * 2) * rs);
         // the fallback flags stay per tracer (tracer_learn_fallback reads them after the step)
         I.unsafe = zalloc(I.tsn * ntr_ * rs);
-        if (!I.Qp) {
-            I.Qp = zalloc(I.tsn * I.wslots * rs);
-            I.Qm = zalloc(I.tsn * I.wslots * rs);
-            I.Sp = zalloc(I.tsn * I.wslots * rs);
-            I.Sm = zalloc(I.tsn * I.wslots * rs);
-        }
+        if (!I.Qp) alloc_learn_bounds(I, rs);
     }
 }
 
@@ -2379,7 +2447,7 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
                 dump->row += (long long)NT * m.CE;
             }
             hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_UPDATE>), gn, blk, nlds, st, m, vals, LO, tsn, adf_h,
-                               tse, adf_v, Pp, Pm, dt, alpha, unsafe, nullptr, nullptr);
+                               tse, adf_v, Pp, Pm, dt, alpha, nullptr, nullptr);
         } else if (learned) {
             const MlpDev w = mlp_dev(I);
             const int gb = std::min((m.nch_e + kWavesPerBlock - 1) / kWavesPerBlock, 2048);
@@ -2390,17 +2458,17 @@ static void launch_batch(Engine::Impl &I, const DMesh<R> &m, const TracerSpec &s
                                tsn, dt, unsafe);
             hk.mark("k_learn_check", st);
             hipLaunchKernelGGL((k_node_update<R, NT, true, D, OUT_UPDATE>), gn, blk, nlds, st, m, vals, LO, tsn, adf_h,
-                               tse, adf_v, Pp, Pm, dt, alpha, unsafe, nullptr, nullptr);
+                               tse, adf_v, Pp, Pm, dt, alpha, nullptr, nullptr);
         } else if (out == OUT_ACCUM) {
             hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_ACCUM>), gn, blk, nlds, st, m, vals, LO, tsn, adf_h,
-                               tse, adf_v, Pp, Pm, dt, alpha, unsafe, acc_h, nullptr);
+                               tse, adf_v, Pp, Pm, dt, alpha, acc_h, nullptr);
         } else if (out == OUT_TEND) {
             if constexpr (NT == 1)
                 hipLaunchKernelGGL((k_node_update<R, 1, false, D, OUT_TEND>), gn, blk, nlds, st, m, vals, LO, tsn,
-                                   adf_h, tse, adf_v, Pp, Pm, dt, alpha, unsafe, acc_h, acc_v);
+                                   adf_h, tse, adf_v, Pp, Pm, dt, alpha, acc_h, acc_v);
         } else {
             hipLaunchKernelGGL((k_node_update<R, NT, false, D, OUT_UPDATE>), gn, blk, nlds, st, m, vals, LO, tsn, adf_h,
-                               tse, adf_v, Pp, Pm, dt, alpha, unsafe, nullptr, nullptr);
+                               tse, adf_v, Pp, Pm, dt, alpha, nullptr, nullptr);
         }
         hk.mark("k_node_update", st);
     } else {
@@ -2566,14 +2634,7 @@ long long Engine::learn_collect(int nsteps, double dt, long long max_rows)
         I.dsY = Y;
         I.ds_cap = max_rows;
     }
-    if (!I.Qp) {
-        const size_t rs = prec_ == PREC_FP64 ? sizeof(double) : sizeof(float);
-        for (void **q : {&I.Qp, &I.Qm, &I.Sp, &I.Sm}) {
-            HIP_OK(hipMalloc(q, I.tsn * I.wslots * rs));
-            HIP_OK(hipMemset(*q, 0, I.tsn * I.wslots * rs));
-            I.allocs.push_back(*q);
-        }
-    }
+    if (!I.Qp) alloc_learn_bounds(I, prec_ == PREC_FP64 ? sizeof(double) : sizeof(float));
     Impl::Dump d{I.dsX, I.dsY, I.ds_rows, I.ds_cap};
     I.dump = &d;
     try {
@@ -2874,7 +2935,8 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
         for (size_t c = 0; c < lay->len.size(); ++c)
             for (int k = 0; k < std::min(lay->len[c] - sub, L); ++k) {
                 const size_t i = (size_t)lay->off[c] + k;
-                out[(size_t)c * L + k] = pr[2 * (i & ~(size_t)1) + half + (i & 1)];
+                // fabs: the learned limiter's bound-check flags ride on R+'s sign bit
+                out[(size_t)c * L + k] = std::fabs(pr[2 * (i & ~(size_t)1) + half + (i & 1)]);
             }
         return 0;
     }
@@ -2889,7 +2951,7 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
                 if (es == 8) v = ((const double *)buf.data())[i];
                 else if (es == 4) v = ((const float *)buf.data())[i];
                 else v = (double)__half2float(((const __half *)buf.data())[i]) * (double)I.ginv;
-                out[((size_t)c * L + k) * comps + q] = v;
+                out[((size_t)c * L + k) * comps + q] = name == "fct_plus" ? std::fabs(v) : v;
             }
     return 0;
 }

@@ -331,6 +331,20 @@ __device__ __forceinline__ void st_rpm(float *Rp, float *, size_t i, float p0, f
 {
     *reinterpret_cast<float4 *>(Rp + 2 * i) = make_float4(p0, p1, m0, m1);
 }
+// The learned limiter's bound-check flag of a node level rides on the sign
+// bit of its R+ (R+ >= 0, so the bit is otherwise clear): with_flag sets it,
+// has_flag reads it, and fabs gives the value back.
+__device__ __forceinline__ float with_flag(float r, bool f)
+{
+    return __uint_as_float(__float_as_uint(r) | (f ? 0x80000000u : 0u));
+}
+__device__ __forceinline__ double with_flag(double r, bool f)
+{
+    return __longlong_as_double(__double_as_longlong(r) | (f ? (long long)(1ull << 63) : 0ll));
+}
+__device__ __forceinline__ bool has_flag(float r) { return (__float_as_uint(r) >> 31) != 0u; }
+__device__ __forceinline__ bool has_flag(double r) { return __double_as_longlong(r) < 0; }
+
 // HO-LO (or HO) edge fluxes of work slot t at the cell pair i (i even).
 // FP64: slot t is the block [t*tse, (t+1)*tse). FP32: slots 2s and 2s+1 share
 // the block of 2*tse floats at 2s*tse, interleaved per cell pair (slot 2s's
