@@ -378,10 +378,11 @@ __global__ __launch_bounds__(256) void k_fill_f32(float *__restrict__ a, long lo
 // the B operand of K block c (hidden 32c..32c+31) is simply the two layer-1
 // accumulator tiles 2c and 2c+1 of the lane (lane l, element e: hidden
 // 32c + 16*(e/4) + 4*(l/16) + e%4), and the weights are loaded in that same
-// order. Layer 3 (64 -> 1) is a VALU dot product over the lane's 16 hidden
-// values and a sum over the four lane groups -- one useful row out of 16 did
-// not pay for 4 more matrix instructions. Roundings are those of the
-// training pass: bf16 inputs, weights and activations, FP32 sums.
+// order. Layer 3 (64 -> 1) takes two more of these instructions with w3 as
+// row 0 of the A operand (rows 1-15 zero) and the layer-2 activations as B,
+// permuted the same way: its output is D row 0, lanes 0..15, with no
+// cross-lane sum. Roundings are those of the training pass: bf16 inputs,
+// weights and activations, FP32 sums.
 // ---------------------------------------------------------------------------
 typedef __bf16 mlp_v8b __attribute__((ext_vector_type(8)));
 typedef short mlp_v8s __attribute__((ext_vector_type(8)));
@@ -417,7 +418,7 @@ struct MlpInfer {
     mlp_v4s w1[4];        // W1^T tile t (A operand, K = features)
     mlp_v8b w2[4][2];     // W2^T row tile t2, K block c (A operand, permuted K)
     float b1[4][4], b2[4][4];
-    mlp_v2b w3[4][2];     // w3[16t + 4*(l/16) + 2p .. +1] as bf16 pairs
+    mlp_v8b w3[2];        // w3 as row 0 of the A operand, K block c (permuted K), zero in rows 1-15
     float b3;
 };
 
@@ -437,16 +438,15 @@ __device__ __forceinline__ void mlp_infer_load(const MlpDev &w, MlpInfer &r)
             r.b1[t][e] = w.b1[16 * t + 4 * g + e];
             r.b2[t][e] = w.b2[16 * t + 4 * g + e];
         }
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const unsigned u = (unsigned)w.w3[16 * t + 4 * g + 2 * q] | ((unsigned)w.w3[16 * t + 4 * g + 2 * q + 1] << 16);
-            r.w3[t][q] = __builtin_bit_cast(mlp_v2b, u);
-        }
     }
+    const mlp_v4s z4 = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+        r.w3[c] = i == 0 ? cat8(ld_bf4(w.w3 + 32 * c + 4 * g), ld_bf4(w.w3 + 32 * c + 16 + 4 * g)) : cat8(z4, z4);
     r.b3 = w.b3[0];
 }
 
-// z3 (pre-sigmoid) of tile row l%16, in every lane
+// z3 (pre-sigmoid) of tile row l%16, in lanes 0..15
 __device__ __forceinline__ float mlp_infer(const MlpInfer &r, mlp_v4s x)
 {
     mlp_v4s a1[4];
@@ -456,18 +456,18 @@ __device__ __forceinline__ float mlp_infer(const MlpInfer &r, mlp_v4s x)
         a1[t] = relu_bf4(acc);
     }
     const mlp_v8b b0 = cat8(a1[0], a1[1]), b1 = cat8(a1[2], a1[3]);
-    float z = 0.0f;
+    mlp_v4s a2[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         mlp_v4f acc = {r.b2[t][0], r.b2[t][1], r.b2[t][2], r.b2[t][3]};
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w2[t][0], b0, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w2[t][1], b1, acc, 0, 0, 0);
-        z = __builtin_amdgcn_fdot2_f32_bf16(relu_bf2(acc[0], acc[1]), r.w3[t][0], z, false);
-        z = __builtin_amdgcn_fdot2_f32_bf16(relu_bf2(acc[2], acc[3]), r.w3[t][1], z, false);
+        a2[t] = relu_bf4(acc);
     }
-    z += __shfl_xor(z, 16, 64);
-    z += __shfl_xor(z, 32, 64);
-    return z + r.b3;
+    mlp_v4f z = {0.0f, 0.0f, 0.0f, 0.0f};
+    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w3[0], cat8(a2[0], a2[1]), z, 0, 0, 0);
+    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w3[1], cat8(a2[2], a2[3]), z, 0, 0, 0);
+    return z[0] + r.b3;
 }
 
 __device__ __forceinline__ float mlp_sigmoid_fast(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
