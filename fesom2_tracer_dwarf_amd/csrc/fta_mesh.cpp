@@ -25,17 +25,34 @@ uint64_t splitmix64(uint64_t &s)
     return z ^ (z >> 31);
 }
 
-uint32_t morton2(uint32_t i, uint32_t j)
+// position of grid point (x, y) along the Hilbert curve that fills the
+// square of side 2^bits: quadrant by quadrant from the top bit down, each
+// quadrant's sub-square turned so that the curve enters and leaves it next
+// to its neighbours (unlike a Morton curve, no jumps between quadrants)
+uint64_t hilbert2(uint32_t x, uint32_t y, int bits)
 {
-    auto spread = [](uint32_t v) {
-        v &= 0xFFFF;
-        v = (v | (v << 8)) & 0x00FF00FF;
-        v = (v | (v << 4)) & 0x0F0F0F0F;
-        v = (v | (v << 2)) & 0x33333333;
-        v = (v | (v << 1)) & 0x55555555;
-        return v;
-    };
-    return spread(i) | (spread(j) << 1);
+    const uint32_t n = 1u << bits;
+    uint64_t d = 0;
+    for (uint32_t s = n >> 1; s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {  // lower quadrants: reflect (right one) and transpose
+            if (rx == 1) {
+                x = n - 1 - x;
+                y = n - 1 - y;
+            }
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+// bits of a square side that covers an nx x ny grid
+int side_bits(int nx, int ny)
+{
+    int b = 0;
+    while ((1 << b) < std::max(nx, ny)) ++b;
+    return b;
 }
 
 // minimum-image trim of a coordinate difference (trim_cyclic,
@@ -124,9 +141,10 @@ void rect_nodes(HostMesh &m, int nx, int ny, double Lx, double Ly, int order, bo
     std::vector<int> perm(m.nod2D);
     std::iota(perm.begin(), perm.end(), 0);
     if (order == 1) {
-        std::vector<uint32_t> key(m.nod2D);
+        std::vector<uint64_t> key(m.nod2D);
+        const int bits = side_bits(nx, ny);
         for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i) key[j * nx + i] = morton2(i, j);
+            for (int i = 0; i < nx; ++i) key[j * nx + i] = hilbert2(i, j, bits);
         std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
     }
     node_id.assign(m.nod2D, 0);
@@ -151,9 +169,10 @@ void rect_elements(HostMesh &m, int ncx, int ncy, int nx, int order, bool period
     std::vector<int> cells(ncx * ncy);
     std::iota(cells.begin(), cells.end(), 0);
     if (order == 1) {
-        std::vector<uint32_t> key(cells.size());
+        std::vector<uint64_t> key(cells.size());
+        const int bits = side_bits(ncx, ncy);
         for (int j = 0; j < ncy; ++j)
-            for (int i = 0; i < ncx; ++i) key[j * ncx + i] = morton2(i, j);
+            for (int i = 0; i < ncx; ++i) key[j * ncx + i] = hilbert2(i, j, bits);
         std::stable_sort(cells.begin(), cells.end(), [&](int a, int b) { return key[a] < key[b]; });
     }
     m.elem2D = 2 * ncx * ncy;

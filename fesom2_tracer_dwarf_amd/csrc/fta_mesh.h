@@ -56,7 +56,7 @@ struct HostMesh {
 
 // Doubly periodic nx x ny node rectangle of Lx x Ly metres split into right
 // triangles (a re-design of generate_analytic_mesh, lib/analytic_mesh.F90:114,
-// with FESOM's edge orientation). order: 0 = row-major, 1 = Morton (Z) curve.
+// with FESOM's edge orientation). order: 0 = row-major, 1 = Hilbert curve.
 void generate_periodic(HostMesh &m, int nx, int ny, double Lx, double Ly, int order);
 
 // Closed (non-periodic) rectangle: boundary edges have edge_tri(2) = -1.

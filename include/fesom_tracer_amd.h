@@ -141,7 +141,7 @@ int tracer_init_for_advection(void);
  * clockwise elements), nl interfaces of thickness dz (stretched by
  * 'stretch'), per-node bottom levels drawn from [lev_lo, lev_hi] with
  * 'seed' (elements = min of nodes, nodes = max of elements), nodes and
- * elements ordered along a Morton curve when sfc != 0. Replaces the
+ * elements ordered along a Hilbert curve when sfc != 0. Replaces the
  * reference's generate_analytic_mesh (lib/analytic_mesh.F90:114). */
 int tracer_generate_mesh(int nx, int ny, int periodic, double Lx, double Ly, int nl, double dz,
                          double stretch, int lev_lo, int lev_hi, unsigned long long seed, int sfc);

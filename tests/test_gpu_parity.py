@@ -26,11 +26,14 @@ MESHES = {
     # 64 interfaces: the benchmark's depth
     "periodic_20x16x64_random_bottom": dict(nx=20, ny=16, nl=64, periodic=True, Lx=1.0e5, Ly=0.8e5, dz=8.0,
                                             stretch=0.5, lev_lo=3, lev_hi=64, seed=13),
-    # 128 interfaces: the deepest column the device layout takes (a whole column in one 128-cell chunk,
-    # pair index 63 in the slot word, 127 layers in the node->edge records)
-    # (vertical outflow Courant number 0.72: FCT's no-new-extrema bound needs a monotone LO step, CFL <= 1)
+    # 128 interfaces, the deepest column the device layout takes: bottoms from 3 to 113 interfaces, and
+    # bottoms of 126-128 with 31 columns of all 128 (a whole column in one 128-cell chunk, pair index 63
+    # in the slot word, 127 layers in the node->edge records). Vertical outflow Courant numbers 0.73 and
+    # 0.35: FCT's no-new-extrema bound needs a monotone LO step, CFL <= 1.
     "periodic_16x12x128_random_bottom": dict(nx=16, ny=12, nl=128, periodic=True, Lx=1.6e5, Ly=1.2e5, dz=8.0,
-                                             stretch=0.3, lev_lo=3, lev_hi=128, seed=17),
+                                             stretch=0.3, lev_lo=3, lev_hi=128, seed=18),
+    "periodic_16x12x128_full_depth": dict(nx=16, ny=12, nl=128, periodic=True, Lx=1.6e5, Ly=1.2e5, dz=8.0,
+                                          stretch=0.3, lev_lo=126, lev_hi=128, seed=17),
 }
 
 SCHEMES = [("UPW1", "UPW1", "NONE"), ("UPW1", "UPW1", "FCT"), ("MUSCL", "QR4C", "FCT"), ("MFCT", "PPM", "FCT"),
