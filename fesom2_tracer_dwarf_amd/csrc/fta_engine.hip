@@ -1166,20 +1166,31 @@ __device__ __forceinline__ EdgeGeo<R> edge_geo(const DMesh<R> &m, const EdgeRec 
 
 __device__ __forceinline__ float clip8(float v) { return fminf(8.0f, fmaxf(-8.0f, v)); }
 
-// features (and Zalesak target) of the lane's two edge levels. Differences of
-// tracer values are formed in double (they may be small against the values);
-// the scalings by the local range and the cell volume are float reciprocals:
-// the features are clipped to [-8, 8] and rounded to bf16 afterwards.
+// Zalesak's coefficient (b3) of the lane's two edge levels with HO-LO flux
+// f: the training target
 template <class R>
-__device__ __forceinline__ void edge_features(const EdgeRec &er, int k0, size_t c, double dt, const EdgeGeo<R> &G,
-                                              const LearnSrc<R> &S, float (&x)[2][kMlpIn], float (&y)[2])
+__device__ __forceinline__ V2<R> edge_zalesak(const EdgeRec &er, int k0, const LearnSrc<R> &S, V2<R> f)
+{
+    V2<R> rp1, rm1, rp2, rm2;
+    ld_rpm(S.Rp, S.Rm, (size_t)er.o1 + k0, rp1, rm1);
+    ld_rpm(S.Rp, S.Rm, (size_t)er.o2 + k0, rp2, rm2);
+    return V2<R>{edge_alpha(true, f.x, rp1.x, rm1.x, rp2.x, rm2.x), edge_alpha(true, f.y, rp1.y, rm1.y, rp2.y, rm2.y)};
+}
+
+// features of the lane's two edge levels (a: their HO-LO flux).
+// Differences of tracer values are formed in double (they may be small
+// against the values); the scalings by the local range and the cell volume
+// are float reciprocals: the features are clipped to [-8, 8] and rounded to
+// bf16 afterwards.
+template <class R>
+__device__ __forceinline__ void edge_features(const EdgeRec &er, int k0, double dt, const EdgeGeo<R> &G,
+                                              const LearnSrc<R> &S, V2<R> a, float (&x)[2][kMlpIn])
 {
     const V2<double> T1 = ld2(S.T + er.o1 + k0), T2 = ld2(S.T + er.o2 + k0);
     const V2<double> L1 = ld2(S.LO + er.o1 + k0), L2 = ld2(S.LO + er.o2 + k0);
     V2<R> qp1, qm1, qp2, qm2, pp1, pm1, pp2, pm2;
     ld_rpm(S.Qp, S.Qm, (size_t)er.o1 + k0, qp1, qm1);
     ld_rpm(S.Qp, S.Qm, (size_t)er.o2 + k0, qp2, qm2);
-    const V2<R> a = ld2(S.adf + kAdfStride<R> * c);
     ld_rpm(S.Pp, S.Pm, (size_t)er.o1 + k0, pp1, pm1);
     ld_rpm(S.Pp, S.Pm, (size_t)er.o2 + k0, pp2, pm2);
     const double t1[2] = {T1.x, T1.y}, t2[2] = {T2.x, T2.y}, l1[2] = {L1.x, L1.y}, l2[2] = {L2.x, L2.y};
@@ -1206,14 +1217,6 @@ __device__ __forceinline__ void edge_features(const EdgeRec &er, int k0, size_t 
         const bool in1 = A[q] >= (R)0;  // flux into node 1
         x[q][8] = clip8((float)(in1 ? Pp1[q] : -Pm1[q]) * (fdt * __builtin_amdgcn_rcpf(G.v1[q] * sc)));
         x[q][9] = clip8((float)(in1 ? -Pm2[q] : Pp2[q]) * (fdt * __builtin_amdgcn_rcpf(G.v2[q] * sc)));
-        y[q] = 0.0f;
-    }
-    if (S.Rp) {  // training rows: Zalesak's coefficient (b3) as the target
-        V2<R> rp1, rm1, rp2, rm2;
-        ld_rpm(S.Rp, S.Rm, (size_t)er.o1 + k0, rp1, rm1);
-        ld_rpm(S.Rp, S.Rm, (size_t)er.o2 + k0, rp2, rm2);
-        y[0] = (float)edge_alpha(true, a.x, rp1.x, rm1.x, rp2.x, rm2.x);
-        y[1] = (float)edge_alpha(true, a.y, rp1.y, rm1.y, rp2.y, rm2.y);
     }
 }
 
@@ -1252,8 +1255,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_dump(DMesh<R> m, LearnArgs<
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        float x[2][kMlpIn], y[2] = {-1.0f, -1.0f};
-        if (p.live) edge_features(er, p.k0, (size_t)p.c, dt, geo, learn_src(a, t), x, y);
+        float x[2][kMlpIn] = {}, y[2] = {-1.0f, -1.0f};
+        const LearnSrc<R> src = learn_src(a, t);
+        if (p.live) {
+            const V2<R> f = ld2(src.adf + kAdfStride<R> * p.c);
+            edge_features(er, p.k0, dt, geo, src, f, x);
+            const V2<R> az = edge_zalesak(er, p.k0, src, f);
+            y[0] = (float)az.x;
+            y[1] = (float)az.y;
+        }
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const bool ok = p.live && p.k0 + q < nlmax;
@@ -1272,11 +1282,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_dump(DMesh<R> m, LearnArgs<
     }
 }
 
-// the network's coefficient of every edge level: a persistent grid; each
-// wave forms the feature rows (bf16) of its chunk's edge levels packed to the
-// front of its own LDS slice (a padded column's last cell and the chunk's
-// tail get no row), runs them through the matrix cores 16 rows at a time and
-// reads its outputs back. The lanes of one wave exchange data only with each
+// the learned coefficient of every edge level: a persistent grid; each wave
+// forms the feature rows (bf16) of the edge levels of its chunk that carry a
+// nonzero HO-LO flux, packed to the front of its own LDS slice, runs them
+// through the matrix cores 16 rows at a time and reads its outputs back; the
+// other levels get 1. The lanes of one wave exchange data only with each
 // other, so a wave-scope fence replaces block barriers and waves leave the
 // loop independently.
 __device__ __forceinline__ void wave_lds_sync()
@@ -1315,32 +1325,38 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             nlmax = (int)((unsigned)er.nl >> 24);
             geo = edge_geo(m, er, p.k0, (size_t)p.c, a.vfl);
         }
-        // the lane's rows in the packed order: lane by lane, first cell first
-        const bool r0 = p.live && p.k0 < nlmax, r1 = p.live && p.k0 + 1 < nlmax;
-        const unsigned long long b0 = __ballot(r0), b1 = __ballot(r1);
-        const int pos0 = lanes_below(b0) + lanes_below(b1), pos1 = pos0 + (r0 ? 1 : 0);
-        const int ntile = (__popcll(b0) + __popcll(b1) + 15) >> 4;  // wave-uniform
 #pragma unroll 1
         for (int t = 0; t < NT; ++t) {
-            float x[2][kMlpIn], y[2];
-            LearnSrc<R> src = learn_src(a, t);
-            src.Rp = nullptr;  // no target needed
-            if (p.live) edge_features(er, p.k0, (size_t)p.c, dt, geo, src, x, y);
+            const LearnSrc<R> src = learn_src(a, t);
+            V2<R> f{0, 0};
+            if (p.live) f = ld2(src.adf + kAdfStride<R> * p.c);
+            // the rows the network is asked about, packed in LDS lane by lane, first cell first:
+            // edge levels with a nonzero HO-LO flux (where it is zero, every coefficient gives the
+            // same limited flux, zero, and the level gets 1)
+            const bool r0 = p.live && p.k0 < nlmax && f.x != (R)0;
+            const bool r1 = p.live && p.k0 + 1 < nlmax && f.y != (R)0;
+            const unsigned long long b0 = __ballot(r0), b1 = __ballot(r1);
+            const int pos0 = lanes_below(b0) + lanes_below(b1), pos1 = pos0 + (r0 ? 1 : 0);
+            const int ntile = (__popcll(b0) + __popcll(b1) + 15) >> 4;  // wave-uniform
+            if (r0 || r1) {
+                float x[2][kMlpIn];
+                edge_features(er, p.k0, dt, geo, src, f, x);
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                unsigned u[8];
+                for (int q = 0; q < 2; ++q) {
+                    unsigned u[8];
 #pragma unroll
-                for (int f = 0; f < 8; ++f) {
-                    // v_cvt_pk_bf16_f32: two features per instruction, nearest even
-                    const mlp_v2f v2 = {2 * f < kMlpIn ? x[q][2 * f] : 0.0f,
-                                        2 * f + 1 < kMlpIn ? x[q][2 * f + 1] : 0.0f};
-                    const unsigned b = __builtin_bit_cast(unsigned, __builtin_convertvector(v2, mlp_v2b));
-                    u[f] = 2 * f < kMlpIn ? b : 0u;
-                }
-                if (q == 0 ? r0 : r1) {
-                    const int row = q == 0 ? pos0 : pos1;
-                    sX[wv][row][0] = make_uint4(u[0], u[1], u[2], u[3]);
-                    sX[wv][row][1] = make_uint4(u[4], u[5], u[6], u[7]);
+                    for (int e = 0; e < 8; ++e) {
+                        // v_cvt_pk_bf16_f32: two features per instruction, nearest even
+                        const mlp_v2f v2 = {2 * e < kMlpIn ? x[q][2 * e] : 0.0f,
+                                            2 * e + 1 < kMlpIn ? x[q][2 * e + 1] : 0.0f};
+                        const unsigned b = __builtin_bit_cast(unsigned, __builtin_convertvector(v2, mlp_v2b));
+                        u[e] = 2 * e < kMlpIn ? b : 0u;
+                    }
+                    if (q == 0 ? r0 : r1) {
+                        const int row = q == 0 ? pos0 : pos1;
+                        sX[wv][row][0] = make_uint4(u[0], u[1], u[2], u[3]);
+                        sX[wv][row][1] = make_uint4(u[4], u[5], u[6], u[7]);
+                    }
                 }
             }
             wave_lds_sync();
@@ -1353,8 +1369,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                 if (g == 0) sY[wv][16 * tile + i] = mlp_sigmoid_fast(z);
             }
             wave_lds_sync();
-            const float y0 = r0 ? sY[wv][pos0] : 0.0f, y1 = r1 ? sY[wv][pos1] : 0.0f;
-            if (p.live) st2(alpha + adf_at<R>(t, a.tse, p.c), (R)y0, (R)y1);
+            const float y0 = r0 ? sY[wv][pos0] : 1.0f, y1 = r1 ? sY[wv][pos1] : 1.0f;
+            if (p.live) st2(alpha + adf_at<R>(t, a.tse, p.c), (R)y0, p.k0 + 1 < nlmax ? (R)y1 : (R)0);
             wave_lds_sync();
         }
     }
