@@ -1510,17 +1510,17 @@ __device__ inline void diag_merge(DiagAcc &a, const DiagAcc &b)
 // unpacking writes only the wet layers, so the dense copy keeps the levels
 // below the bottom as last set.
 __global__ __launch_bounds__(256) void k_pack_layers(const int *__restrict__ slot, const int *__restrict__ nlev,
-                                                     long long npair, const double *__restrict__ dense, int L,
-                                                     double *__restrict__ out)
+                                                     const int *__restrict__ ncol, long long npair,
+                                                     const double *__restrict__ dense, int L, double *__restrict__ out)
 {
     const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
     if (s >= npair) return;
     const int w = slot[s];
     double a = 0.0, b = 0.0;
     if (w >= 0) {
-        const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
-        const int Lc = nlev[n] - 1;
-        const double *src = dense + (size_t)n * L;
+        const int q = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
+        const int Lc = nlev[q] - 1;
+        const double *src = dense + (size_t)ncol[q] * L;
         if (k0 < Lc) a = src[k0];
         if (k0 + 1 < Lc) b = src[k0 + 1];
     }
@@ -1529,16 +1529,16 @@ __global__ __launch_bounds__(256) void k_pack_layers(const int *__restrict__ slo
 }
 
 __global__ __launch_bounds__(256) void k_unpack_layers(const int *__restrict__ slot, const int *__restrict__ nlev,
-                                                       long long npair, const double *__restrict__ in, int L,
-                                                       double *__restrict__ dense)
+                                                       const int *__restrict__ ncol, long long npair,
+                                                       const double *__restrict__ in, int L, double *__restrict__ dense)
 {
     const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
     if (s >= npair) return;
     const int w = slot[s];
     if (w < 0) return;
-    const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
-    const int Lc = nlev[n] - 1;
-    double *dst = dense + (size_t)n * L;
+    const int q = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
+    const int Lc = nlev[q] - 1;
+    double *dst = dense + (size_t)ncol[q] * L;
     if (k0 < Lc) dst[k0] = in[2 * s];
     if (k0 + 1 < Lc) dst[k0 + 1] = in[2 * s + 1];
 }
@@ -1546,16 +1546,17 @@ __global__ __launch_bounds__(256) void k_unpack_layers(const int *__restrict__ s
 // dense += the wet layers of a chunk-packed node array (the caller's tendency
 // arrays accumulate what a step returns, as do_oce_adv_tra adds into them)
 __global__ __launch_bounds__(256) void k_unpack_add_layers(const int *__restrict__ slot, const int *__restrict__ nlev,
-                                                           long long npair, const double *__restrict__ in, int L,
+                                                           const int *__restrict__ ncol, long long npair,
+                                                           const double *__restrict__ in, int L,
                                                            double *__restrict__ dense)
 {
     const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
     if (s >= npair) return;
     const int w = slot[s];
     if (w < 0) return;
-    const int n = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
-    const int Lc = nlev[n] - 1;
-    double *dst = dense + (size_t)n * L;
+    const int q = w & ((1 << kColBits) - 1), k0 = 2 * (int)((unsigned)w >> kColBits);
+    const int Lc = nlev[q] - 1;
+    double *dst = dense + (size_t)ncol[q] * L;
     if (k0 < Lc) dst[k0] = dst[k0] + in[2 * s];
     if (k0 + 1 < Lc) dst[k0 + 1] = dst[k0 + 1] + in[2 * s + 1];
 }
@@ -1600,7 +1601,7 @@ __global__ __launch_bounds__(256) void k_diag_partial(DMesh<R> m, const double *
         long long k = i - (long long)n * L;
         if (k < 0) { --n; k += L; }
         else if (k >= L) { ++n; k -= L; }
-        if (k < m.nlev[n] - 1) continue;
+        if (k < m.nlev_c[n] - 1) continue;
         const double x = dense[i];
         if (is_nan_bits(x)) { a.nan += 1.0; continue; }
         a.vmin = fmin(a.vmin, x); a.vmax = fmax(a.vmax, x); a.sum += x;
@@ -1695,32 +1696,57 @@ inline unsigned grid_for(int nchunks, int waves = kWavesPerBlock)
 }  // namespace
 
 // chunked layout of a set of columns: column c uses len[c] cells from off[c]
-// (padded to an even count); columns are packed whole into kChunk-cell chunks
+// (padded to an even count); columns are packed whole into kChunk-cell chunks,
+// in the caller's order except that a column that no longer fits the current
+// chunk may be passed over by up to `window` later ones that do (so chunks
+// waste less of their tail while neighbours stay close), and a chunk takes at
+// most `maxcols` columns (what a wave stages in LDS). order[q] is the column
+// at place q; the slot words and the per-column records use q.
 struct Layout {
     std::vector<long long> off;
     std::vector<int> len;
-    std::vector<int> slot;  // per cell pair: column | pair << kColBits, -1 = padding
+    std::vector<int> order;
+    std::vector<int> slot;  // per cell pair: place | pair << kColBits, -1 = padding
     long long total = 0;    // cells, nch * kChunk
     int nch = 0;
-    void build(const std::vector<int> &lens)
+    void build(const std::vector<int> &lens, int window = 0, int maxcols = kChunk)
     {
+        const size_t n = lens.size();
         len = lens;
-        off.assign(lens.size(), 0);
-        if (lens.size() >= (size_t)1 << kColBits) throw std::runtime_error("too many columns for the slot word");
-        long long chunk = 0, pos = 0;
-        for (size_t c = 0; c < lens.size(); ++c) {
-            const int plen = lens[c] + (lens[c] & 1);
-            if (plen > kChunk) throw std::runtime_error("column longer than a wave chunk");
-            if (pos + plen > kChunk) { ++chunk; pos = 0; }
+        off.assign(n, 0);
+        order.clear();
+        order.reserve(n);
+        if (n >= (size_t)1 << kColBits) throw std::runtime_error("too many columns for the slot word");
+        auto plen = [&](size_t c) { return lens[c] + (lens[c] & 1); };
+        for (size_t c = 0; c < n; ++c)
+            if (plen(c) > kChunk) throw std::runtime_error("column longer than a wave chunk");
+        std::vector<char> placed(n, 0);
+        long long chunk = 0;
+        int pos = 0, ncols = 0;
+        auto place = [&](size_t c) {
             off[c] = chunk * kChunk + pos;
-            pos += plen;
+            pos += plen(c);
+            ++ncols;
+            placed[c] = 1;
+            order.push_back((int)c);
+        };
+        size_t head = 0;
+        while (head < n) {
+            if (placed[head]) { ++head; continue; }
+            if (ncols < maxcols && pos + plen(head) <= kChunk) { place(head); continue; }
+            // the next column in order does not fit: fill the tail from the window
+            for (size_t j = head + 1; j < std::min(n, head + 1 + (size_t)window) && ncols < maxcols; ++j)
+                if (!placed[j] && pos + plen(j) <= kChunk) place(j);
+            ++chunk;
+            pos = 0;
+            ncols = 0;
         }
-        nch = lens.empty() ? 0 : (int)(chunk + 1);
+        nch = n == 0 ? 0 : (int)(pos > 0 ? chunk + 1 : chunk);
         total = (long long)nch * kChunk;
         slot.assign((size_t)total / 2, -1);
-        for (size_t c = 0; c < lens.size(); ++c) {
-            const int pairs = (lens[c] + 1) / 2;
-            for (int q = 0; q < pairs; ++q) slot[(size_t)(off[c] / 2 + q)] = (int)c | (q << kColBits);
+        for (size_t q = 0; q < n; ++q) {
+            const int c = order[q], pairs = (lens[c] + 1) / 2;
+            for (int k = 0; k < pairs; ++k) slot[(size_t)(off[c] / 2 + k)] = (int)q | (k << kColBits);
         }
     }
     // unpadded column-major (ld, ncol) with comps interleaved components ->
@@ -1801,7 +1827,8 @@ struct Engine::Impl {
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
-// the per-node records (incident edges, 1-ring fan, elements) at width D
+// the per-node records (incident edges, 1-ring fan, elements) at width D, in
+// the node layout's order (record q belongs to node ln.order[q])
 template <int D, class R>
 static void build_node_records(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, const std::vector<int> &enlmax)
 {
@@ -1812,31 +1839,32 @@ static void build_node_records(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, 
     std::vector<NodeElems<D>> nt(N);
     std::vector<R> gw((size_t)8 * D * N, (R)0);
     std::vector<int> ring, fan;
-    for (int n = 0; n < N; ++n) {
+    for (int q = 0; q < N; ++q) {
+        const int n = I.ln.order[q];
         const int deg = m.n2e_ptr[n + 1] - m.n2e_ptr[n];
         for (int j = 0; j < D; ++j) {
-            ne[n].oe[j] = -1; ne[n].oo[j] = 0; ni[n].info[j] = 0;
-            rg[n].oo[j] = -1; rg[n].lr[j] = 0; rg[n].le[j] = 0;
-            nt[n].ei[j] = -1; nt[n].oa[j] = 0; nt[n].ob[j] = 0; nt[n].le[j] = 0; nt[n].pos[j] = 0;
+            ne[q].oe[j] = -1; ne[q].oo[j] = 0; ni[q].info[j] = 0;
+            rg[q].oo[j] = -1; rg[q].lr[j] = 0; rg[q].le[j] = 0;
+            nt[q].ei[j] = -1; nt[q].oa[j] = 0; nt[q].ob[j] = 0; nt[q].le[j] = 0; nt[q].pos[j] = 0;
         }
         for (int j = 0; j < deg; ++j) {
             const int se = m.n2e[m.n2e_ptr[n] + j];
             const int e = (se > 0 ? se : -se) - 1;
             const int other = se > 0 ? m.edges[2 * e + 1] : m.edges[2 * e];
-            ne[n].oe[j] = (int)I.le.off[e];
-            ne[n].oo[j] = (int)I.ln.off[other];
-            ni[n].info[j] = (unsigned short)(enlmax[e] | (se > 0 ? 128 : 0) | ((m.nlevels_nod2D[other] - 1) << 8));
+            ne[q].oe[j] = (int)I.le.off[e];
+            ne[q].oo[j] = (int)I.ln.off[other];
+            ni[q].info[j] = (unsigned short)(enlmax[e] | (se > 0 ? 128 : 0) | ((m.nlevels_nod2D[other] - 1) << 8));
         }
         for (int j = 0; j < m.nod_in_elem2D_num[n]; ++j) {
             const int el = m.nod_in_elem2D[(size_t)n * m.maxAdj + j];
             const int *en = &m.elem2D_nodes[3 * (size_t)el];
             const int ps = en[0] == n ? 0 : (en[1] == n ? 1 : 2);
             if (en[ps] != n) throw std::runtime_error("nod_in_elem2D lists an element without the node");
-            nt[n].ei[j] = el;
-            nt[n].oa[j] = (int)I.ln.off[en[ps == 0 ? 1 : 0]];
-            nt[n].ob[j] = (int)I.ln.off[en[ps == 2 ? 1 : 2]];
-            nt[n].le[j] = (unsigned char)(m.nlevels[el] - 1);
-            nt[n].pos[j] = (unsigned char)ps;
+            nt[q].ei[j] = el;
+            nt[q].oa[j] = (int)I.ln.off[en[ps == 0 ? 1 : 0]];
+            nt[q].ob[j] = (int)I.ln.off[en[ps == 2 ? 1 : 2]];
+            nt[q].le[j] = (unsigned char)(m.nlevels[el] - 1);
+            nt[q].pos[j] = (unsigned char)ps;
         }
         // the cluster bounds of FCT walk the fan; a node without one (a
         // caller-supplied element list that is not a fan) leaves its ring
@@ -1847,9 +1875,9 @@ static void build_node_records(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, 
         }
         if ((int)ring.size() > D) throw std::runtime_error("1-ring above " + std::to_string(D) + " nodes");
         for (size_t j = 0; j < ring.size(); ++j) {
-            rg[n].oo[j] = (int)I.ln.off[ring[j]];
-            rg[n].lr[j] = (unsigned char)(m.nlevels_nod2D[ring[j]] - 1);
-            rg[n].le[j] = fan[j] >= 0 ? (unsigned char)(m.nlevels[fan[j]] - 1) : 0;
+            rg[q].oo[j] = (int)I.ln.off[ring[j]];
+            rg[q].lr[j] = (unsigned char)(m.nlevels_nod2D[ring[j]] - 1);
+            rg[q].le[j] = fan[j] >= 0 ? (unsigned char)(m.nlevels[fan[j]] - 1) : 0;
             // fan element j between ring j and ring j+1: its area-weighted
             // gradient coefficients of those two nodes
             const int el = fan[j];
@@ -1858,7 +1886,7 @@ static void build_node_records(Engine::Impl &I, DMesh<R> &d, const HostMesh &m, 
             const int *en = &m.elem2D_nodes[3 * (size_t)el];
             const int pa = en[0] == ra ? 0 : (en[1] == ra ? 1 : 2), pb = en[0] == rb ? 0 : (en[1] == rb ? 1 : 2);
             const double A = m.elem_area[el], *gs = &m.gradient_sca[6 * (size_t)el];
-            R *w = &gw[(size_t)8 * D * n + 8 * j];
+            R *w = &gw[(size_t)8 * D * q + 8 * j];
             w[0] = (R)(A * gs[pa]);
             w[1] = (R)(A * gs[pb]);
             w[2] = (R)(A * gs[3 + pa]);
@@ -1884,13 +1912,21 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
     d.slot_n = I.track(dupload(I.ln.slot));
     d.slot_t = I.track(dupload(I.lt.slot));
     d.slot_e = I.track(dupload(I.le.slot));
-    d.nlev = I.track(dupload(m.nlevels_nod2D));
+    // per-column arrays in the layouts' orders (see Layout)
+    {
+        std::vector<int> nlq(N);
+        for (int q = 0; q < N; ++q) nlq[q] = m.nlevels_nod2D[I.ln.order[q]];
+        d.nlev = I.track(dupload(nlq));
+        d.ncol = I.track(dupload(I.ln.order));
+        d.nlev_c = I.track(dupload(m.nlevels_nod2D));
+    }
     std::vector<EdgeRec> er(E);
-    std::vector<int> enlmax(E);
+    std::vector<int> enlmax(E);  // the caller's edge order
     std::vector<R> eg(8 * (size_t)E, (R)0), egud(4 * (size_t)E, (R)0);
     std::vector<int4> eud(E, make_int4(0, 0, 0, 0));
-    for (int e = 0; e < E; ++e) {
-        EdgeRec &r = er[e];
+    for (int qe = 0; qe < E; ++qe) {
+        const int e = I.le.order[qe];
+        EdgeRec &r = er[qe];
         const int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
         const int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
         r.o1 = (int)I.ln.off[n1]; r.o2 = (int)I.ln.off[n2];
@@ -1906,7 +1942,7 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
         r.nb = (m.nboundary_lay[n1] & 0xFFFF) | ((m.nboundary_lay[n2] & 0xFFFF) << 16);
         double a = kRE * m.elem_cos[el1];
         if (el2 >= 0) a = 0.5 * (a + kRE * m.elem_cos[el2]);
-        R *g = &eg[8 * (size_t)e];
+        R *g = &eg[8 * (size_t)qe];
         for (int q = 0; q < 4; ++q) g[q] = (R)m.edge_cross_dxdy[4 * e + q];
         g[4] = (R)m.edge_dxdy[2 * e];
         g[5] = (R)m.edge_dxdy[2 * e + 1];
@@ -1932,8 +1968,8 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
                 off[2 * side] = (int)I.ln.off[en[ia]];
                 off[2 * side + 1] = (int)I.ln.off[en[ib]];
             }
-            eud[e] = make_int4(off[0], off[1], off[2], off[3]);
-            for (int q = 0; q < 4; ++q) egud[4 * (size_t)e + q] = (R)wt[q];
+            eud[qe] = make_int4(off[0], off[1], off[2], off[3]);
+            for (int q = 0; q < 4; ++q) egud[4 * (size_t)qe + q] = (R)wt[q];
         }
     }
     d.erec = I.track(dupload(er));
@@ -1944,12 +1980,12 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
         constexpr int U = kEdgeUnits<R>;
         static_assert(sizeof(EdgeRec) == 32, "EdgeRec is two 16-byte units");
         std::vector<uint4> pk((size_t)U * E);
-        for (int e = 0; e < E; ++e) {
-            char *q = reinterpret_cast<char *>(&pk[(size_t)U * e]);
-            std::memcpy(q, &er[e], 32);
-            std::memcpy(q + 32, &eg[8 * (size_t)e], 8 * sizeof(R));
-            std::memcpy(q + 32 + 8 * sizeof(R), &eud[e], 16);
-            std::memcpy(q + 48 + 8 * sizeof(R), &egud[4 * (size_t)e], 4 * sizeof(R));
+        for (int qe = 0; qe < E; ++qe) {
+            char *q = reinterpret_cast<char *>(&pk[(size_t)U * qe]);
+            std::memcpy(q, &er[qe], 32);
+            std::memcpy(q + 32, &eg[8 * (size_t)qe], 8 * sizeof(R));
+            std::memcpy(q + 32 + 8 * sizeof(R), &eud[qe], 16);
+            std::memcpy(q + 48 + 8 * sizeof(R), &egud[4 * (size_t)qe], 4 * sizeof(R));
         }
         d.epack = I.track(dupload(pk));
     }
@@ -2011,7 +2047,15 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
             int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
             d[e] = std::max(m.nlevels[el1] - 1, el2 >= 0 ? m.nlevels[el2] - 1 : 0);
         }
-        I.ln.build(a); I.lt.build(c); I.le.build(d);
+        // node and edge columns may be passed over by up to 32 later ones that
+        // fill a chunk's tail (a chunk then wastes about 2% instead of 11-13% of
+        // its cells on the benchmark's random bottoms); a node chunk stays within
+        // the 8 columns whose FP64 fan weights a wave stages, an edge chunk within
+        // the 16 whose records it stages
+        constexpr int kPackWindow = 32;
+        I.ln.build(a, kPackWindow, std::min(kStageNodes, FanW<double, kDeg>::kCols));
+        I.lt.build(c);
+        I.le.build(d, kPackWindow, kStageEdges);
         if (I.le.total >= (1ll << 31) || I.ln.total >= (1ll << 31) || I.lt.total >= (1ll << 31))
             throw std::runtime_error("mesh too large for 32-bit column offsets");
     }
@@ -2158,11 +2202,13 @@ void Engine::set_velocity(const double *uv, const double *w, const double *we)
     I.we_is_w = same;
 }
 
-// the node layout's pair slots and column heights (the same in both precisions)
-static void node_slots(const Engine::Impl &I, int prec, const int *&slot, const int *&nlev, long long &npair)
+// the node layout's pair slots, column heights and caller node of each
+// column (the same in both precisions)
+static void node_slots(const Engine::Impl &I, int prec, const int *&slot, const int *&nlev, const int *&ncol,
+                       long long &npair)
 {
-    if (prec == PREC_FP64) { slot = I.m64.slot_n; nlev = I.m64.nlev; npair = I.m64.CN / 2; }
-    else { slot = I.m32.slot_n; nlev = I.m32.nlev; npair = I.m32.CN / 2; }
+    if (prec == PREC_FP64) { slot = I.m64.slot_n; nlev = I.m64.nlev; ncol = I.m64.ncol; npair = I.m64.CN / 2; }
+    else { slot = I.m32.slot_n; nlev = I.m32.nlev; ncol = I.m32.ncol; npair = I.m32.CN / 2; }
 }
 
 static void ensure_staging(Engine::Impl &I, hipStream_t st, size_t array_bytes)
@@ -2219,13 +2265,13 @@ static void d2h_staged(Engine::Impl &I, hipStream_t st, void *dst, const void *s
 static void upload_layers(Engine::Impl &I, int prec, hipStream_t st, int N, int L, const double *v, double *dense,
                           double *dst)
 {
-    const int *slot, *nlev;
+    const int *slot, *nlev, *ncol;
     long long npair;
-    node_slots(I, prec, slot, nlev, npair);
+    node_slots(I, prec, slot, nlev, ncol, npair);
     HIP_OK(hipStreamSynchronize(st));  // no step or earlier transfer still reads dense or dst
     h2d_staged(I, st, dense, v, (size_t)L * N * sizeof(double));
-    hipLaunchKernelGGL(k_pack_layers, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, st, slot, nlev, npair,
-                       dense, L, dst);
+    hipLaunchKernelGGL(k_pack_layers, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, st, slot, nlev, ncol,
+                       npair, dense, L, dst);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
 }
@@ -2255,13 +2301,13 @@ void Engine::get_values(int tr, double *v)
 {
     if (tr < 0 || tr >= ntr_) throw std::runtime_error("tracer index out of range");
     Impl &I = *p_;
-    const int *slot, *nlev;
+    const int *slot, *nlev, *ncol;
     long long npair;
-    node_slots(I, prec_, slot, nlev, npair);
+    node_slots(I, prec_, slot, nlev, ncol, npair);
     const int L = nl_ - 1;
     double *dense = I.dense + tr * (size_t)L * N_;
     hipLaunchKernelGGL(k_unpack_layers, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, stream_, slot, nlev,
-                       npair, I.values + tr * I.tsn, L, dense);
+                       ncol, npair, I.values + tr * I.tsn, L, dense);
     HIP_OK(hipGetLastError());
     d2h_staged(I, stream_, v, dense, (size_t)L * N_ * sizeof(double));
 }
@@ -2300,14 +2346,14 @@ void Engine::tendencies(int tr, double dt, double *dttf_h, double *dttf_v)
         throw;
     }
     I.out_mode = saved;
-    const int *slot, *nlev;
+    const int *slot, *nlev, *ncol;
     long long npair;
-    node_slots(I, prec_, slot, nlev, npair);
+    node_slots(I, prec_, slot, nlev, ncol, npair);
     for (int q = 0; q < 2; ++q) {
         double *host = q == 0 ? dttf_h : dttf_v;
         h2d_staged(I, stream_, I.dense_t, host, L * N_ * sizeof(double));
         hipLaunchKernelGGL(k_unpack_add_layers, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, stream_, slot,
-                           nlev, npair, q == 0 ? I.tend_h : I.tend_v, (int)L, I.dense_t);
+                           nlev, ncol, npair, q == 0 ? I.tend_h : I.tend_v, (int)L, I.dense_t);
         HIP_OK(hipGetLastError());
         d2h_staged(I, stream_, host, I.dense_t, L * N_ * sizeof(double));
     }
@@ -2338,12 +2384,12 @@ void Engine::tendencies_device(int tr, double dt, double *dttf_h, double *dttf_v
         throw;
     }
     I.tend_h = own_h; I.tend_v = own_v; I.out_mode = saved;
-    const int *slot, *nlev;
+    const int *slot, *nlev, *ncol;
     long long npair;
-    node_slots(I, prec_, slot, nlev, npair);
+    node_slots(I, prec_, slot, nlev, ncol, npair);
     for (int q = 0; q < 2; ++q)
         hipLaunchKernelGGL(k_unpack_add_layers, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, stream_, slot,
-                           nlev, npair, scratch + (size_t)q * I.tsn, nl_ - 1, q == 0 ? dttf_h : dttf_v);
+                           nlev, ncol, npair, scratch + (size_t)q * I.tsn, nl_ - 1, q == 0 ? dttf_h : dttf_v);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(stream_));
 }

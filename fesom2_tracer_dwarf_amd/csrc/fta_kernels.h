@@ -3,8 +3,9 @@
 // Cell-packed mapping. Every level-indexed array (node layers and
 // interfaces, element layers, edge levels) stores the wet levels of each
 // column back to back, each column padded to an even length and columns
-// packed whole, in mesh order, into chunks of kChunk = 128 cells; a column
-// never straddles two chunks. One 64-lane wavefront processes one chunk and
+// packed whole, in mesh order (nodes and edges with a short look-ahead that
+// fills chunk tails), into chunks of kChunk = 128 cells; a column never
+// straddles two chunks. One 64-lane wavefront processes one chunk and
 // each lane two consecutive cells of one column (levels k0 = 2p, k0+1), so
 // every tracer/flux/geometry access of a lane is one 16-byte load for FP64
 // data (8 bytes for FP32). On gfx950 a vector-memory instruction of 64- or
@@ -116,9 +117,15 @@ struct DMesh {
     int N, T, E, nl;
     int nch_n, nch_t, nch_e;      // wave chunks of the node / element / edge layouts
     long long CN, CT, CE;         // padded cell counts (nch * kChunk)
-    const int *slot_n, *slot_t, *slot_e;   // per cell pair: column | pair << kColBits, -1 = pad
-    const int *nlev;              // (N) nlevels_nod2D
-    const EdgeRec *erec;          // (E)
+    // per cell pair: column | pair << kColBits, -1 = pad. A column is named by
+    // its place in the layout (the order its columns were packed in), and the
+    // per-column records below are stored in that order, so that the columns
+    // of one chunk are consecutive records
+    const int *slot_n, *slot_t, *slot_e;
+    const int *nlev;              // (N) nlevels_nod2D, node layout order
+    const int *ncol;              // (N) node layout column -> the caller's node index
+    const int *nlev_c;            // (N) nlevels_nod2D, the caller's node order
+    const EdgeRec *erec;          // (E) edge layout order
     const R *egeo;                // (8,E): dx1 dy1 dx2 dy2 (edge_cross_dxdy) ex ey (edge_dxdy) a
     const ElemRec *trec;          // (T)
     const R *tgeo;                // (8,T): gradient_sca(1:6), elem_area, 0
