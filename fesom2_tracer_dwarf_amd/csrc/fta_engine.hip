@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -2049,13 +2050,23 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
         }
         // node and edge columns may be passed over by up to 32 later ones that
         // fill a chunk's tail (a chunk then wastes about 2% instead of 11-13% of
-        // its cells on the benchmark's random bottoms); a node chunk stays within
-        // the 8 columns whose FP64 fan weights a wave stages, an edge chunk within
-        // the 16 whose records it stages
+        // its cells on the benchmark's random bottoms). Where the columns are
+        // long enough that a full chunk holds few of them, a chunk also stays
+        // within what a wave stages in LDS: 8 node columns (FP64 fan weights) or
+        // 16 (node->edge records), 16 edge columns; shallower meshes, whose
+        // chunks hold more columns than that anyway, keep chunks full instead
         constexpr int kPackWindow = 32;
-        I.ln.build(a, kPackWindow, std::min(kStageNodes, FanW<double, kDeg>::kCols));
+        auto col_cap = [](const std::vector<int> &lens, std::initializer_list<int> caps) {
+            double cells = 0.0;
+            for (int l : lens) cells += l + (l & 1);
+            const double mean = lens.empty() ? (double)kChunk : cells / (double)lens.size();
+            for (int cap : caps)  // the tightest cap whose columns still fill a chunk
+                if (mean * cap >= kChunk) return cap;
+            return kChunk;
+        };
+        I.ln.build(a, kPackWindow, col_cap(a, {FanW<double, kDeg>::kCols, kStageNodes}));
         I.lt.build(c);
-        I.le.build(d, kPackWindow, kStageEdges);
+        I.le.build(d, kPackWindow, col_cap(d, {kStageEdges}));
         if (I.le.total >= (1ll << 31) || I.ln.total >= (1ll << 31) || I.lt.total >= (1ll << 31))
             throw std::runtime_error("mesh too large for 32-bit column offsets");
     }
