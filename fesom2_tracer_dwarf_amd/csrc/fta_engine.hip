@@ -1363,8 +1363,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             wave_lds_sync();
             // rows past the last one in the last tile hold stale features;
             // their outputs are never read
-#pragma unroll 4
-            for (int tile = 0; tile < ntile; ++tile) {
+            int tile = 0;
+#pragma unroll 1
+            for (; tile + 1 < ntile; tile += 2) {  // two tiles at a time (mlp_infer_tiles)
+                const mlp_v4s xb[2] = {
+                    ld_bf4(reinterpret_cast<const unsigned short *>(&sX[wv][16 * tile + i][0]) + 4 * g),
+                    ld_bf4(reinterpret_cast<const unsigned short *>(&sX[wv][16 * tile + 16 + i][0]) + 4 * g)};
+                float z[2];
+                mlp_infer_tiles<2>(r, xb, z);
+                if (g == 0) {
+                    sY[wv][16 * tile + i] = mlp_sigmoid_fast(z[0]);
+                    sY[wv][16 * tile + 16 + i] = mlp_sigmoid_fast(z[1]);
+                }
+            }
+            if (tile < ntile) {
                 const mlp_v4s xb = ld_bf4(reinterpret_cast<const unsigned short *>(&sX[wv][16 * tile + i][0]) + 4 * g);
                 const float z = mlp_infer(r, xb);
                 if (g == 0) sY[wv][16 * tile + i] = mlp_sigmoid_fast(z);

@@ -446,28 +446,59 @@ __device__ __forceinline__ void mlp_infer_load(const MlpDev &w, MlpInfer &r)
     r.b3 = w.b3[0];
 }
 
-// z3 (pre-sigmoid) of tile row l%16, in lanes 0..15
+// z3 (pre-sigmoid) of tile row l%16, in lanes 0..15, for NTILE tiles at once:
+// each layer's instructions of the tiles are interleaved, so that one tile's
+// matrix instructions overlap another's epilogue
+template <int NTILE>
+__device__ __forceinline__ void mlp_infer_tiles(const MlpInfer &r, const mlp_v4s (&x)[NTILE], float (&z3)[NTILE])
+{
+    mlp_v4s a1[NTILE][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const mlp_v4f bias = {r.b1[t][0], r.b1[t][1], r.b1[t][2], r.b1[t][3]};
+#pragma unroll
+        for (int u = 0; u < NTILE; ++u) a1[u][t] = relu_bf4(mfma16(r.w1[t], x[u], bias));
+    }
+    mlp_v8b b0[NTILE], b1[NTILE];
+#pragma unroll
+    for (int u = 0; u < NTILE; ++u) {
+        b0[u] = cat8(a1[u][0], a1[u][1]);
+        b1[u] = cat8(a1[u][2], a1[u][3]);
+    }
+    mlp_v4s a2[NTILE][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        mlp_v4f acc[NTILE];
+#pragma unroll
+        for (int u = 0; u < NTILE; ++u) {
+            acc[u] = mlp_v4f{r.b2[t][0], r.b2[t][1], r.b2[t][2], r.b2[t][3]};
+            acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w2[t][0], b0[u], acc[u], 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < NTILE; ++u) {
+            acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w2[t][1], b1[u], acc[u], 0, 0, 0);
+            a2[u][t] = relu_bf4(acc[u]);
+        }
+    }
+    mlp_v4f z[NTILE];
+#pragma unroll
+    for (int u = 0; u < NTILE; ++u) {
+        z[u] = mlp_v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        z[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w3[0], cat8(a2[u][0], a2[u][1]), z[u], 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < NTILE; ++u) {
+        z[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w3[1], cat8(a2[u][2], a2[u][3]), z[u], 0, 0, 0);
+        z3[u] = z[u][0] + r.b3;
+    }
+}
+
 __device__ __forceinline__ float mlp_infer(const MlpInfer &r, mlp_v4s x)
 {
-    mlp_v4s a1[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const mlp_v4f acc = mfma16(r.w1[t], x, mlp_v4f{r.b1[t][0], r.b1[t][1], r.b1[t][2], r.b1[t][3]});
-        a1[t] = relu_bf4(acc);
-    }
-    const mlp_v8b b0 = cat8(a1[0], a1[1]), b1 = cat8(a1[2], a1[3]);
-    mlp_v4s a2[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        mlp_v4f acc = {r.b2[t][0], r.b2[t][1], r.b2[t][2], r.b2[t][3]};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w2[t][0], b0, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w2[t][1], b1, acc, 0, 0, 0);
-        a2[t] = relu_bf4(acc);
-    }
-    mlp_v4f z = {0.0f, 0.0f, 0.0f, 0.0f};
-    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w3[0], cat8(a2[0], a2[1]), z, 0, 0, 0);
-    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.w3[1], cat8(a2[2], a2[3]), z, 0, 0, 0);
-    return z[0] + r.b3;
+    const mlp_v4s xs[1] = {x};
+    float z[1];
+    mlp_infer_tiles<1>(r, xs, z);
+    return z[0];
 }
 
 __device__ __forceinline__ float mlp_sigmoid_fast(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
