@@ -608,7 +608,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
                 // LO vertical flux (UPW1 with w_e and ttf) and HO-LO with w and ttfAB
                 fl0[u] = vflux_upw1<0>(lc, 0.0);
                 fl1[u] = vflux_upw1<1>(lc, 0.0);
-                fl2[u] = from_next(fl0[u]);
+                fl2[u] = below(from_next(fl0[u]), k0, L);
                 if constexpr (SAME) {
                     hc.A = tracer_window<R>(T[u], k0, nlev);
                 } else {
@@ -803,7 +803,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
         for (int u = 0; u < TG; ++u) {
             V2<R> ad{0, 0};
             if (p.live) ad = ld2(adf_v + (t0 + u) * tsn + c);
-            const R ad0 = ad.x, ad1 = ad.y, ad2 = from_next(ad0);
+            const R ad0 = ad.x, ad1 = ad.y, ad2 = below(from_next(ad0), k0, L);
             pp0[u] = (R)0 + (fmx((R)0, ad0) + fmx((R)0, -ad1));
             pm0[u] = (R)0 + (fmn((R)0, ad0) + fmn((R)0, -ad1));
             pp1[u] = (R)0 + (fmx((R)0, ad1) + fmx((R)0, -ad2));
@@ -994,7 +994,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             const R rpu0 = from_prev(rp[u].y), rmu0 = from_prev(rm[u].y);
             fvl0[u] = limit_vertical(k0 + 1, lay0, fv.x, rp[u].x, rm[u].x, rpu0, rmu0);
             fvl1[u] = limit_vertical(k0 + 2, lay1, fv.y, rp[u].y, rm[u].y, rp[u].x, rm[u].x);
-            fvl2[u] = from_next(fvl0[u]);
+            fvl2[u] = below(from_next(fvl0[u]), k0, L);
         }
         if (!lay0) continue;
 
@@ -1099,7 +1099,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *_
         hc.A = tracer_window<R>(B, k0, nlev);
         R fv0, fv1;
         ver_flux<R, VER>(hc, num_ord, (R)dt, (R)0, (R)0, fv0, fv1);
-        const R fv2 = from_next(fv0);
+        const R fv2 = below(from_next(fv0), k0, L);
         if (!lay0) continue;
         double sh0 = 0.0, sh1 = 0.0;
 #pragma unroll
@@ -1435,7 +1435,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_check(DMesh<R> m, const R *
             const R rpu0 = from_prev(rpy), rmu0 = from_prev(rmy);
             const R fvl0 = limit_vertical(k0 + 1, lay0, fv.x, rp[u].x, rm[u].x, rpu0, rmu0);
             const R fvl1 = limit_vertical(k0 + 2, lay1, fv.y, rpy, rmy, rp[u].x, rm[u].x);
-            const R fvl2 = from_next(fvl0);
+            const R fvl2 = below(from_next(fvl0), k0, L);
             sp0[u] = fmax(0.0, (double)fvl0) + fmax(0.0, -(double)fvl1);
             sn0[u] = fmin(0.0, (double)fvl0) + fmin(0.0, -(double)fvl1);
             sp1[u] = fmax(0.0, (double)fvl1) + fmax(0.0, -(double)fvl2);
@@ -2023,21 +2023,21 @@ static void build_dmesh(Engine::Impl &I, DMesh<R> &d, const HostMesh &m)
     else build_node_records<kDeg>(I, d, m, enlmax);
     d.area = I.track(dupload(I.ln.pack<R>(m.area.data(), nl, 0)));
     d.zbarn = I.track(dupload(I.ln.pack<R>(m.zbar_3d_n.data(), nl, 0)));
-    d.areasvol = I.track(dupload(I.ln.pack<R>(m.areasvol.data(), nl, 1)));
-    d.hnode = I.track(dupload(I.ln.pack<R>(m.hnode.data(), nl - 1, 1)));
+    d.areasvol = I.track(dupload(I.ln.pack<R>(m.areasvol.data(), nl, 0)));
+    d.hnode = I.track(dupload(I.ln.pack<R>(m.hnode.data(), nl - 1, 0)));
     // layers that keep their thickness over the step (hnode_new == hnode, no
     // tracer_set_thickness): one device copy, which the node kernels then
     // read once (ld_same)
     const bool same_h = m.hnode_new == m.hnode;
-    d.hnode_new = same_h ? d.hnode : I.track(dupload(I.ln.pack<R>(m.hnode_new.data(), nl - 1, 1)));
-    d.Zn = I.track(dupload(I.ln.pack<R>(m.Z_3d_n.data(), nl - 1, 1)));
+    d.hnode_new = same_h ? d.hnode : I.track(dupload(I.ln.pack<R>(m.hnode_new.data(), nl - 1, 0)));
+    d.Zn = I.track(dupload(I.ln.pack<R>(m.Z_3d_n.data(), nl - 1, 0)));
     d.helem = I.track(dupload(I.lt.pack<R>(m.helem.data(), nl - 1, 0)));
     if constexpr (std::is_same<R, double>::value) {
         d.av_d = d.areasvol; d.h_d = d.hnode; d.hn_d = d.hnode_new;
     } else {
-        d.av_d = I.track(dupload(I.ln.pack<double>(m.areasvol.data(), nl, 1)));
-        d.h_d = I.track(dupload(I.ln.pack<double>(m.hnode.data(), nl - 1, 1)));
-        d.hn_d = same_h ? d.h_d : I.track(dupload(I.ln.pack<double>(m.hnode_new.data(), nl - 1, 1)));
+        d.av_d = I.track(dupload(I.ln.pack<double>(m.areasvol.data(), nl, 0)));
+        d.h_d = I.track(dupload(I.ln.pack<double>(m.hnode.data(), nl - 1, 0)));
+        d.hn_d = same_h ? d.h_d : I.track(dupload(I.ln.pack<double>(m.hnode_new.data(), nl - 1, 0)));
     }
 }
 
@@ -2054,7 +2054,9 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
     Impl &I = *p_;
     {
         std::vector<int> a(N_), c(T_), d(E_);
-        for (int n = 0; n < N_; ++n) a[n] = m.nlevels_nod2D[n];
+        // node columns hold the layers; interfaces 1..L share their cells, and
+        // the bottom interface (zero flux in every scheme) has none
+        for (int n = 0; n < N_; ++n) a[n] = m.nlevels_nod2D[n] - 1;
         for (int el = 0; el < T_; ++el) c[el] = m.nlevels[el] - 1;
         for (int e = 0; e < E_; ++e) {
             int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
@@ -2980,13 +2982,13 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
     HIP_OK(hipStreamSynchronize(stream_));
     const char *src = nullptr;
     const Layout *lay = &I.ln;
-    int L = nl_ - 1, comps = 1, sub = 1;
+    int L = nl_ - 1, comps = 1;
     size_t es = prec_ == PREC_FP64 ? 8 : 4;
     if (name == "fct_LO") { src = (const char *)I.LO; es = 8; }
     else if (name == "fct_plus") src = (const char *)I.Pp;
     else if (name == "fct_minus") src = (const char *)I.Pm;
-    else if (name == "adv_flux_hor") { src = (const char *)I.adf_h; lay = &I.le; sub = 0; }
-    else if (name == "adv_flux_ver") { src = (const char *)I.adf_v; L = nl_; sub = 0; }
+    else if (name == "adv_flux_hor") { src = (const char *)I.adf_h; lay = &I.le; }
+    else if (name == "adv_flux_ver") { src = (const char *)I.adf_v; L = nl_; }  // the bottom interface stays 0
     else if (name == "gnod") {
         src = (const char *)I.gnod; comps = 2;
         es = prec_ == PREC_FP64 ? 8 : (prec_ == PREC_FP32 ? 4 : 2);
@@ -3002,7 +3004,7 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
                          hipMemcpyDeviceToHost));
         std::fill(out, out + need, 0.0);
         for (size_t c = 0; c < lay->len.size(); ++c)
-            for (int k = 0; k < std::min(lay->len[c] - sub, L); ++k) {
+            for (int k = 0; k < std::min(lay->len[c], L); ++k) {
                 const size_t i = (size_t)lay->off[c] + k;
                 out[(size_t)c * L + k] = pr[adf_at<float>((size_t)(slot & 1), tse, i & ~(size_t)1) + (i & 1)];
             }
@@ -3018,7 +3020,7 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
         const size_t half = name == "fct_plus" ? 0 : 2;
         std::fill(out, out + need, 0.0);
         for (size_t c = 0; c < lay->len.size(); ++c)
-            for (int k = 0; k < std::min(lay->len[c] - sub, L); ++k) {
+            for (int k = 0; k < std::min(lay->len[c], L); ++k) {
                 const size_t i = (size_t)lay->off[c] + k;
                 // fabs: the learned limiter's bound-check flags ride on R+'s sign bit
                 out[(size_t)c * L + k] = std::fabs(pr[2 * (i & ~(size_t)1) + half + (i & 1)]);
@@ -3029,7 +3031,7 @@ int Engine::get_work(const std::string &name, int tr, double *out, size_t count)
     HIP_OK(hipMemcpy(buf.data(), src + slot * per * es, buf.size(), hipMemcpyDeviceToHost));
     std::fill(out, out + need, 0.0);
     for (size_t c = 0; c < lay->len.size(); ++c)
-        for (int k = 0; k < std::min(lay->len[c] - sub, L); ++k)
+        for (int k = 0; k < std::min(lay->len[c], L); ++k)
             for (int q = 0; q < comps; ++q) {
                 size_t i = ((size_t)lay->off[c] + k) * comps + q;
                 double v;

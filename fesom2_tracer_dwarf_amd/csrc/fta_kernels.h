@@ -277,6 +277,10 @@ __host__ __device__ __forceinline__ bool is_nan_bits(double x)
 // lane l receives lane l-1's / l+1's value (own value at the wave ends)
 template <class T> __device__ __forceinline__ T from_prev(T v) { return __shfl_up(v, 1, kWave); }
 template <class T> __device__ __forceinline__ T from_next(T v) { return __shfl_down(v, 1, kWave); }
+// an interface quantity of cell k0+2 taken from the next lane (nx), or zero
+// where that is the column's bottom interface (k0 + 2 == L layers): node
+// columns keep no cell for it, and its flux is zero in every scheme
+template <class T> __device__ __forceinline__ T below(T nx, int k0, int L) { return k0 + 2 < L ? nx : (T)0; }
 
 // two consecutive cells
 template <class T>

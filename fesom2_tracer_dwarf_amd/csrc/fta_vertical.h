@@ -7,7 +7,9 @@
 // scheme reads is widened into a window of six cells k0-2 .. k0+3 by
 // cross-lane moves. Window cells outside the column belong to another column
 // or to padding and are masked by the level logic exactly as the reference's
-// loop bounds exclude them.
+// loop bounds exclude them. A column's cells are its layers: the bottom
+// interface nz = nlev (zero flux) has no cell, so what a scheme would read
+// there is supplied by the level logic (ppm's tv(nzmax), the callers' below()).
 #pragma once
 
 #include "fta_engine.h"
@@ -211,7 +213,10 @@ template <class R>
 __device__ __forceinline__ void vflux_ppm(const VCol2<R> &c, R dt, R fin0, R fin1, R &f0, R &f1)
 {
     const R tv0 = ppm_tv<0>(c), tv1 = ppm_tv<1>(c);
-    const R tv2 = from_next(tv0);  // tv(k0+3)
+    // tv(k0+3); at the bottom interface (no cell of its own) ttf(nzmax-1), the
+    // lane's second layer
+    const R tvn = from_next(tv0);
+    const R tv2 = c.k0 + 3 == c.nlev ? c.A.v[3] : tvn;
     R aL0 = tv0, aR0 = tv1, aj0, aL1 = tv1, aR1 = tv2, aj1;
     ppm_parabola(c.A.v[2], aL0, aR0, aj0);
     ppm_parabola(c.A.v[3], aL1, aR1, aj1);
