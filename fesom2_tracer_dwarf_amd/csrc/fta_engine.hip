@@ -2062,14 +2062,14 @@ Engine::Engine(const HostMesh &m, int ntracers, int precision)
             int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
             d[e] = std::max(m.nlevels[el1] - 1, el2 >= 0 ? m.nlevels[el2] - 1 : 0);
         }
-        // node and edge columns may be passed over by up to 32 later ones that
-        // fill a chunk's tail (a chunk then wastes about 2% instead of 11-13% of
-        // its cells on the benchmark's random bottoms). Where the columns are
+        // node and edge columns may be passed over by up to 128 later ones that
+        // fill a chunk's tail (a chunk then wastes about 1-2% instead of 11-13%
+        // of its cells on the benchmark's random bottoms). Where the columns are
         // long enough that a full chunk holds few of them, a chunk also stays
         // within what a wave stages in LDS: 8 node columns (FP64 fan weights) or
         // 16 (node->edge records), 16 edge columns; shallower meshes, whose
         // chunks hold more columns than that anyway, keep chunks full instead
-        constexpr int kPackWindow = 32;
+        constexpr int kPackWindow = 128;
         auto col_cap = [](const std::vector<int> &lens, std::initializer_list<int> caps) {
             double cells = 0.0;
             for (int l : lens) cells += l + (l & 1);
