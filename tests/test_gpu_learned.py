@@ -5,9 +5,11 @@
     accumulation): outputs to 2e-3, gradient to 1e-2 relative 2-norm;
   * the limited step in flux form: FP64 mass conserved to 1e-13 and no new
     extrema beyond rounding, whatever the network (untrained here);
-  * training on the project's own FCT pairs lowers the loss, and the trained
-    network gets closer to the Zalesak solution than the untrained one and
-    than the low-order solution (coefficient 0), with fewer fallbacks.
+  * training on the project's own FCT pairs lowers the loss; on fresh pairs
+    (later steps it was not trained on) the trained network predicts Zalesak's
+    coefficient with well under half the untrained network's error, and its
+    step is closer to the Zalesak solution than the low-order one
+    (coefficient 0), with fewer fallbacks than the untrained network.
 """
 import numpy as np
 import pytest
@@ -87,6 +89,13 @@ def _run(api, trs, schemes, dt, nsteps, params=None):
 
 
 def test_training_gains_over_untrained_and_low_order(api):
+    """The gain is measured where training acts: the error of the predicted
+    coefficient on pairs the network has not seen. The step's distance to
+    the Zalesak solution is compared with the low-order solution's; for the
+    untrained network it is printed only: wherever a network's coefficients
+    would break a bound the step takes Zalesak's, so a random network that
+    breaks bounds often can land as close to the Zalesak solution as a
+    trained one."""
     from oracle import mlp
     print("[gpu] learned limiter training", flush=True)
     d, dt, trs = setup_case(api, MESH, ("MUSCL", "QR4C", "FCT"), ntr=5, seed=9)
@@ -95,10 +104,21 @@ def test_training_gains_over_untrained_and_low_order(api):
     api.learn_clear()
     rows = api.learn_collect(6, dt, 40_000_000)
     assert rows > 0
-    losses = api.learn_train(6, batch=16384, lr=2e-3, seed=1)
+    losses = api.learn_train(20, batch=16384, lr=2e-3, seed=1)
     print(f"[gpu]   {rows} rows, loss per epoch {np.round(losses, 5).tolist()}", flush=True)
     assert losses[-1] < 0.5 * losses[0]
     trained = api.mlp_params().copy()
+    # fresh pairs: two further FCT steps from where the collection left the tracers;
+    # a pass with learning rate 0 returns the mean loss without changing the parameters
+    api.learn_clear()
+    api.learn_collect(2, dt, 40_000_000)
+    held = {}
+    for name, prm in (("trained", trained), ("untrained", untrained)):
+        api.set_mlp_params(prm)
+        held[name] = float(api.learn_train(1, batch=16384, lr=0.0, seed=0)[0])
+        assert np.array_equal(api.mlp_params(), prm)
+        print(f"[gpu]   {name}: loss on fresh pairs {held[name]:.4e}", flush=True)
+    assert held["trained"] < 0.5 * held["untrained"]
     low_order = untrained.copy()
     low_order[mlp.OFF["W1"][0]:mlp.OFF["b3"][1]] = 0.0
     low_order[mlp.OFF["b3"][0]] = -40.0  # coefficient ~0: the LO solution where the check allows it
@@ -109,8 +129,8 @@ def test_training_gains_over_untrained_and_low_order(api):
         err = float(np.mean([np.linalg.norm(v - r) / np.linalg.norm(r) for v, r in zip(vals, ref)]))
         res[name] = (err, float(np.mean(fb)))
         print(f"[gpu]   {name}: mean rel L2 distance to Zalesak {err:.3e}, fallback {np.mean(fb):.3f}", flush=True)
-    assert res["trained"][0] < res["untrained"][0]
     assert res["trained"][0] < res["low_order"][0]
+    assert res["trained"][1] < res["untrained"][1]
     api.set_mlp_params(untrained)
     api.set_option("precision", 64)
 
