@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const double *__
     const Pair p = lane_pair(m.slot_n, ch);
     if (!p.live) return;
     const int L = m.nlev[p.col] - 1;
-    if (p.k0 >= L) return;  // the pair holds only the bottom interface
+    if (p.k0 >= L) return;  // (every live pair starts on a layer; kept as a guard)
     const bool v1 = p.k0 + 1 < L;
     int ei[D], oa[D], ob[D], le[D], pos[D];
     {
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_gnod_fan(DMesh<R> m, const double
     }
     if (!p.live) return;
     const int L = m.nlev[p.col] - 1, k0 = p.k0;
-    if (k0 >= L) return;  // the pair holds only the bottom interface
+    if (k0 >= L) return;  // (every live pair starts on a layer; kept as a guard)
     const bool v1 = k0 + 1 < L;
     const NodeRing<D> rg = node_ring<D>(m)[p.col];
     int nr = 0;
