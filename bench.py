@@ -432,7 +432,7 @@ def main():
                    "bottom_levels": "U{3..64} seed 0", "tracers": ntr, "schemes": "MUSCL+QR4C+FCT",
                    "active_node_cells": c["C"], "dense_node_cells": c["N"] * (cfg["nl"] - 1),
                    "dt_s": dt, "cfl": 0.3},
-        # BASELINE.md's definition: the step's compulsory bytes (each semantic
+        # the metric's roofline: the step's compulsory bytes (each semantic
         # input read once, each output written once) over the step time
         "roofline": {"bound": "hbm", "achieved": f64["step_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": f64["step_gbs"] / HBM_PEAK_GBS, "traffic": step_traffic,
