@@ -345,7 +345,7 @@ def main():
         print(json.dumps({"copy_bytes_each_way": a.numel() * 8, "launches": 5}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (the mesh would shard by "
+        print("bench.py measures one GPU only: the library has no multi-GPU path (the mesh would shard by "
               "space-filling-curve ranges with halo exchanges; see the design notes).", file=sys.stderr)
         sys.exit(2)
 

@@ -497,7 +497,7 @@ void read_restart(const std::string &npdir, RestartState &st, bool with_tracers,
     const int eDim = find(part, "eDim_nod2D") ? find(part, "eDim_nod2D")->i[0] : 0;
     if (st.npes != 1 || eDim != 0)
         throw err(npdir + ": a restart of " + std::to_string(st.npes) +
-                  " ranks; this build reads one-rank restarts (npes = 1, no halo)");
+                  " ranks; the library reads one-rank restarts (npes = 1, no halo)");
 
     const RRecord mr = read_restart_file(rank_file(npdir, "t_mesh"), "t_mesh");
     HostMesh m;
