@@ -149,7 +149,7 @@ template <class R, class G, int NT, int D>
 __global__ __launch_bounds__(kMaxBlock) void k_gnod(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
                                                  G *__restrict__ gnod, size_t tsg, float gscale)
 {
-    const int ch = wave_chunk();
+    const int ch = wave_chunk(true);  // from the end of each XCD's range (wave_chunk)
     if (ch >= m.nch_n) return;
     const Pair p = lane_pair(m.slot_n, ch);
     if (!p.live) return;
@@ -277,7 +277,7 @@ template <class R, class G, int NT, int D>
 __global__ __launch_bounds__(kMaxBlock) void k_gnod_fan(DMesh<R> m, const double *__restrict__ ab, size_t tsn,
                                                      G *__restrict__ gnod, size_t tsg, float gscale)
 {
-    const int ch = wave_chunk();
+    const int ch = wave_chunk(true);  // from the end of each XCD's range (wave_chunk)
     if (ch >= m.nch_n) return;
     const Pair p = lane_pair(m.slot_n, ch);
     constexpr int WU = FanW<R, D>::kUnits, WC = FanW<R, D>::kCols;
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
                                                     double dt, R num_ord,
                                                     double *__restrict__ LO, R *__restrict__ adf_v)
 {
-    const int ch = wave_chunk();
+    const int ch = wave_chunk(true);  // from the end of each XCD's range (wave_chunk)
     if (ch >= m.nch_n) return;
     const Pair p = lane_pair(m.slot_n, ch);
     const int nlev = p.live ? m.nlev[p.col] : 0;
@@ -951,7 +951,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
                                                         const R *__restrict__ alpha,
                                                         double *__restrict__ acc_h, double *__restrict__ acc_v)
 {
-    const int ch = wave_chunk();
+    const int ch = wave_chunk(true);  // from the end of each XCD's range (wave_chunk)
     if (ch >= m.nch_n) return;
     const Pair p = lane_pair(m.slot_n, ch);
     const int nlev = p.live ? m.nlev[p.col] : 0;
@@ -1075,7 +1075,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_nolim(DMesh<R> m, const R *_
                                                        double dt, R num_ord,
                                                        double *__restrict__ acc_h, double *__restrict__ acc_v)
 {
-    const int ch = wave_chunk();
+    const int ch = wave_chunk(true);  // from the end of each XCD's range (wave_chunk)
     if (ch >= m.nch_n) return;
     const Pair p = lane_pair(m.slot_n, ch);
     const int nlev = p.live ? m.nlev[p.col] : 0;

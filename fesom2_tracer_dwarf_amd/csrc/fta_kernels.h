@@ -184,12 +184,17 @@ __device__ __forceinline__ const NodeElems<D> *node_elems(const DMesh<R> &m)
     return static_cast<const NodeElems<D> *>(m.nelem);
 }
 
-__device__ __forceinline__ int wave_chunk()
+__device__ __forceinline__ int wave_chunk(bool rev = false)
 {
     // hardware dispatches workgroup b to XCD b % 8; give XCD x the
-    // contiguous block range [x*per, (x+1)*per) (gridDim.x is a multiple of 8)
-    const unsigned nb = gridDim.x, b = blockIdx.x, per = nb >> 3;
-    const unsigned lb = (b & 7u) * per + (b >> 3);
+    // contiguous block range [x*per, (x+1)*per) (gridDim.x is a multiple of 8),
+    // swept from its end when rev is set. The kernels of a step alternate
+    // (k_gnod_fan, k_node_lo, k_node_update and k_node_nolim sweep from the
+    // end, k_edge and k_node_limit from the start), so that each kernel
+    // begins on the chunks its predecessor touched last, whose lines the
+    // memory-side cache in front of HBM (256 MB) still holds
+    const unsigned nb = gridDim.x, b = blockIdx.x, per = nb >> 3, i = b >> 3;
+    const unsigned lb = (b & 7u) * per + (rev ? per - 1u - i : i);
     int ch = (int)(lb * (blockDim.x >> 6) + (threadIdx.x >> 6));
     return __builtin_amdgcn_readfirstlane(ch);
 }
