@@ -303,9 +303,17 @@ __device__ __forceinline__ V2<float> ld2(const float *p)
     const float2 v = *reinterpret_cast<const float2 *>(p);
     return {v.x, v.y};
 }
+// FP64 pairs are stored with the nontemporal hint: each lane writes 16
+// contiguous bytes, the lanes of a column whole lines, and nothing is read
+// back within the kernel, so the writes need not hold L2 lines that the
+// kernel's gathers use (FP64 step -1.1%, bitwise equal). FP32 pairs are 8
+// bytes, interleaved per tracer pair for some arrays, and keep ordinary
+// stores (the hint made k_edge 13% slower there)
+typedef double st_d2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void st2(double *p, double a, double b)
 {
-    *reinterpret_cast<double2 *>(p) = make_double2(a, b);
+    const st_d2 v = {a, b};
+    __builtin_nontemporal_store(v, reinterpret_cast<st_d2 *>(p));
 }
 __device__ __forceinline__ void st2(float *p, float a, float b)
 {
