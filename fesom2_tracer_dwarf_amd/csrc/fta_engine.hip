@@ -523,7 +523,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
                 ho1 = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux1 * num_ord * (R)0.5 * (Tmean1 + Tmean2);
             }
         }
-        st2(flux + adf_at<R>(t, tse, o), (R)((double)ho0 - lo0), valid1 ? (R)((double)ho1 - lo1) : (R)0);
+        st2_mixed(flux + adf_at<R>(t, tse, o), (R)((double)ho0 - lo0), valid1 ? (R)((double)ho1 - lo1) : (R)0);
     }
 }
 
@@ -1383,7 +1383,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             }
             wave_lds_sync();
             const float y0 = r0 ? sY[wv][pos0] : 1.0f, y1 = r1 ? sY[wv][pos1] : 1.0f;
-            if (p.live) st2(alpha + adf_at<R>(t, a.tse, p.c), (R)y0, p.k0 + 1 < nlmax ? (R)y1 : (R)0);
+            if (p.live) st2_mixed(alpha + adf_at<R>(t, a.tse, p.c), (R)y0, p.k0 + 1 < nlmax ? (R)y1 : (R)0);
             wave_lds_sync();
         }
     }
@@ -1480,7 +1480,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_learn_check(DMesh<R> m, const R *
             st2(unsafe + tn + c, bad0 ? (R)1 : (R)0, bad1 ? (R)1 : (R)0);
             // R+ of the pair only (FP32: the first half of its 16-byte record), only where a flag is set
             if (bad0 || bad1)
-                st2(Rp + kRpmPerCell<R> * (tn + c), with_flag(rp[u].x, bad0), with_flag(rp[u].y, bad1));
+                st2_mixed(Rp + kRpmPerCell<R> * (tn + c), with_flag(rp[u].x, bad0), with_flag(rp[u].y, bad1));
         }
     }
 }

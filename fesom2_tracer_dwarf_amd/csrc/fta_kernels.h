@@ -303,19 +303,34 @@ __device__ __forceinline__ V2<float> ld2(const float *p)
     const float2 v = *reinterpret_cast<const float2 *>(p);
     return {v.x, v.y};
 }
-// FP64 pairs are stored with the nontemporal hint: each lane writes 16
-// contiguous bytes, the lanes of a column whole lines, and nothing is read
-// back within the kernel, so the writes need not hold L2 lines that the
-// kernel's gathers use (FP64 step -1.1%, bitwise equal). FP32 pairs are 8
-// bytes, interleaved per tracer pair for some arrays, and keep ordinary
-// stores (the hint made k_edge 13% slower there)
+// Pair stores carry the nontemporal hint: the lanes of a column write whole
+// lines, and no kernel reads back what it writes, so the writes need not
+// hold L2 lines that the kernel's gathers use (FP64 step -1.6%, FP32 -0.5%,
+// bitwise equal). The exception is an FP32 store that covers every other 8
+// bytes of an interleaved array (adf_at's tracer pairs, R+ alone of an R+/R-
+// group), which keeps ordinary stores through st2_mixed (the hint made k_edge
+// 13% slower there)
 typedef double st_d2 __attribute__((ext_vector_type(2)));
+typedef float st_f2 __attribute__((ext_vector_type(2)));
+typedef float st_f4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void st2(double *p, double a, double b)
 {
     const st_d2 v = {a, b};
     __builtin_nontemporal_store(v, reinterpret_cast<st_d2 *>(p));
 }
 __device__ __forceinline__ void st2(float *p, float a, float b)
+{
+    const st_f2 v = {a, b};
+    __builtin_nontemporal_store(v, reinterpret_cast<st_f2 *>(p));
+}
+__device__ __forceinline__ void st4(float *p, float a, float b, float c, float d)
+{
+    const st_f4 v = {a, b, c, d};
+    __builtin_nontemporal_store(v, reinterpret_cast<st_f4 *>(p));
+}
+// a pair of an interleaved array (FP32: every other 8 bytes)
+__device__ __forceinline__ void st2_mixed(double *p, double a, double b) { st2(p, a, b); }
+__device__ __forceinline__ void st2_mixed(float *p, float a, float b)
 {
     *reinterpret_cast<float2 *>(p) = make_float2(a, b);
 }
@@ -353,7 +368,7 @@ __device__ __forceinline__ void st_rpm(double *Rp, double *Rm, size_t i, double 
 }
 __device__ __forceinline__ void st_rpm(float *Rp, float *, size_t i, float p0, float p1, float m0, float m1)
 {
-    *reinterpret_cast<float4 *>(Rp + 2 * i) = make_float4(p0, p1, m0, m1);
+    st4(Rp + 2 * i, p0, p1, m0, m1);
 }
 // The learned limiter's bound-check flag of a node level rides on the sign
 // bit of its R+ (R+ >= 0, so the bit is otherwise clear): with_flag sets it,
@@ -429,15 +444,15 @@ template <> struct GradIO<double> {
     __device__ static V4<double> ld(const double *p, double) { return ld4(p); }
     __device__ static void st(double *p, double a, double b, double c, double d, double)
     {
-        reinterpret_cast<double2 *>(p)[0] = make_double2(a, b);
-        reinterpret_cast<double2 *>(p)[1] = make_double2(c, d);
+        st2(p, a, b);
+        st2(p + 2, c, d);
     }
 };
 template <> struct GradIO<float> {
     __device__ static V4<float> ld(const float *p, float) { return ld4(p); }
     __device__ static void st(float *p, float a, float b, float c, float d, float)
     {
-        *reinterpret_cast<float4 *>(p) = make_float4(a, b, c, d);
+        st4(p, a, b, c, d);
     }
 };
 template <> struct GradIO<__half> {
