@@ -791,7 +791,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
         unsigned info[D];
         load_node_edges<D, true>(m, sr, slice, p.col, lay0, oe, info, oo);
         V2<R> av{1, 1};
-        if (lay0) av = ld2(m.areasvol + c);
+        if (lay0) av = ld2_nt(m.areasvol + c);
         // P+/P-: the vertical HO-LO fluxes of interfaces k0+1, k0+2 (layer k0)
         // and k0+2, k0+3 (layer k0+1; the last one in the next lane: where it
         // is used the layer below exists, so that lane holds the same column),
@@ -802,7 +802,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_node_limit(DMesh<R> m, const doub
 #pragma unroll
         for (int u = 0; u < TG; ++u) {
             V2<R> ad{0, 0};
-            if (p.live) ad = ld2(adf_v + (t0 + u) * tsn + c);
+            if (p.live) ad = ld2_nt(adf_v + (t0 + u) * tsn + c);
             const R ad0 = ad.x, ad1 = ad.y, ad2 = below(from_next(ad0), k0, L);
             pp0[u] = (R)0 + (fmx((R)0, ad0) + fmx((R)0, -ad1));
             pm0[u] = (R)0 + (fmn((R)0, ad0) + fmn((R)0, -ad1));
@@ -967,8 +967,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
     load_node_edges<D, true>(m, sr, slice, p.col, lay0, oe, info, oo);
     V2<double> av{1, 1}, h{0, 0}, hn{1, 1};  // FP64 geometry: conservative accumulation
     if (lay0) {
-        av = ld2(m.av_d + c);
-        h = ld2(m.h_d + c);
+        av = ld2_nt(m.av_d + c);
+        h = ld2_nt(m.h_d + c);
         hn = ld_same(m.hn_d, m.h_d, c, h);
     }
 
@@ -984,7 +984,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             V2<R> fv{0, 0};
             rp[u] = V2<R>{0, 0};
             rm[u] = V2<R>{0, 0};
-            if (if0) fv = ld2(adf_v + tn + c);
+            if (if0) fv = ld2_nt(adf_v + tn + c);
             if (!if1) fv.y = 0;
             if (lay0) ld_rpm(Rp, Rm, tn + c, rp[u], rm[u]);
             us0[u] = LEARN && has_flag(rp[u].x);
@@ -1056,7 +1056,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
             // flux before summing; the difference is rounding)
             const double s0 = sh0[u] * dt / (double)av.x;
             const double s1 = sh1[u] * dt / (double)av.y;
-            const V2<double> T = ld2(ttf + tn + c), lo = ld2(LO + tn + c);
+            const V2<double> T = ld2_nt(ttf + tn + c), lo = ld2_nt(LO + tn + c);
             double dv0 = 0.0 - T.x * (double)h.x + lo.x * (double)hn.x;
             dv0 = dv0 + ((double)fvl0[u] - (double)fvl1[u]) * dt / (double)av.x;
             double dv1 = 0.0 - T.y * (double)h.y + lo.y * (double)hn.y;

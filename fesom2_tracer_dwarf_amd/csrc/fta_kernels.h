@@ -303,6 +303,21 @@ __device__ __forceinline__ V2<float> ld2(const float *p)
     const float2 v = *reinterpret_cast<const float2 *>(p);
     return {v.x, v.y};
 }
+// a pair that no other lane of the launch reads (the own layers of a node in
+// the node kernels), loaded with the nontemporal hint so that it does not
+// keep L2 lines from the gathered data (FP64 step -0.4%, FP32 -0.8%)
+typedef double ld_d2 __attribute__((ext_vector_type(2)));
+typedef float ld_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ V2<double> ld2_nt(const double *p)
+{
+    const ld_d2 v = __builtin_nontemporal_load(reinterpret_cast<const ld_d2 *>(p));
+    return {v.x, v.y};
+}
+__device__ __forceinline__ V2<float> ld2_nt(const float *p)
+{
+    const ld_f2 v = __builtin_nontemporal_load(reinterpret_cast<const ld_f2 *>(p));
+    return {v.x, v.y};
+}
 // Pair stores carry the nontemporal hint: the lanes of a column write whole
 // lines, and no kernel reads back what it writes, so the writes need not
 // hold L2 lines that the kernel's gathers use (FP64 step -1.6%, FP32 -0.5%,
