@@ -319,9 +319,10 @@ __device__ __forceinline__ V2<float> ld2_nt(const float *p)
     return {v.x, v.y};
 }
 // Pair stores carry the nontemporal hint: the lanes of a column write whole
-// lines, and no kernel reads back what it writes, so the writes need not
-// hold L2 lines that the kernel's gathers use (FP64 step -1.6%, FP32 -0.5%,
-// bitwise equal). The exception is an FP32 store that covers every other 8
+// lines, which the step reads again only in a later kernel, so the writes
+// need not hold L2 lines that the kernel's gathers use (FP64 step -1.6%,
+// FP32 -0.5%). The hint marks lines for early eviction and changes no value
+// a later read sees. The exception is an FP32 store that covers every other 8
 // bytes of an interleaved array (adf_at's tracer pairs, R+ alone of an R+/R-
 // group), which keeps ordinary stores through st2_mixed (the hint made k_edge
 // 13% slower there)
