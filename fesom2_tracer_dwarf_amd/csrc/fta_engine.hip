@@ -460,6 +460,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
         sh1 = updn && nz1 <= nzmax_e - 1;
     }
     const size_t o = (size_t)p.c;
+    R q0 = 0, q1 = 0;  // FP32: the even tracer's pair, stored with the odd one's
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const size_t tn = t * tsn;
@@ -523,7 +524,14 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(sizeo
                 ho1 = (R)(-0.5) * ((R)1 - num_ord) * cHO - vflux1 * num_ord * (R)0.5 * (Tmean1 + Tmean2);
             }
         }
-        st2_mixed(flux + adf_at<R>(t, tse, o), (R)((double)ho0 - lo0), valid1 ? (R)((double)ho1 - lo1) : (R)0);
+        const R v0 = (R)((double)ho0 - lo0), v1 = valid1 ? (R)((double)ho1 - lo1) : (R)0;
+        if constexpr (sizeof(R) == 4 && NT % 2 == 0) {
+            // the two slots of a tracer pair share 16 bytes per cell pair: one store
+            if (t & 1) st4(reinterpret_cast<float *>(flux + adf_at<R>(t - 1, tse, o)), q0, q1, v0, v1);
+            else { q0 = v0; q1 = v1; }
+        } else {
+            st2_mixed(flux + adf_at<R>(t, tse, o), v0, v1);
+        }
     }
 }
 
